@@ -1,0 +1,228 @@
+"""Python mirror of the reference's GBM3D call surface, backed by libgbm3d.so.
+
+    U, out = GBM3D(I, sigma, opts)          # source/GBM3D.m:1
+    opts = setBM3Dopts('default')           # utilities/setBM3Dopts.m:1
+    opts = checkBM3Dopts(opts)              # utilities/checkBM3Dopts.m:1
+
+Images are numpy arrays of shape (m, n) holding values on any scale (the
+path is scale-equivariant; sigma must be on the image's scale).  opts is a
+dict playing the MATLAB struct: absent keys are absent fields.  Every call runs
+the HIP kernels on the GPU; there is no CPU path.
+
+Stage functions mirror the reference stages for stage-isolated use:
+matchBlocksCPU, denoiseAndAggregateGPU (ht filter + aggregation),
+matchBlocksWie, denoiseAndAggregateWieFast.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import Opts, check, load_library, opts_to_struct, ptr, struct_to_opts
+
+
+def setBM3Dopts(profile: str = "default") -> dict:
+    """utilities/setBM3Dopts.m: profile table; unknown names mean 'default'."""
+    lib = load_library()
+    o = Opts()
+    check(lib.gbm3d_set_opts(C.c_char_p(str(profile).encode()), C.byref(o)))
+    return struct_to_opts(o)
+
+
+def checkBM3Dopts(opts: dict) -> dict:
+    """utilities/checkBM3Dopts.m: fill absent fields with defaults."""
+    lib = load_library()
+    o = opts_to_struct(opts)
+    check(lib.gbm3d_check_opts(C.byref(o)))
+    out = dict(opts)
+    out.update(struct_to_opts(o))
+    return out
+
+
+def resolve_opts(opts: dict | None) -> dict:
+    """The option rule at the top of GBM3D (source/GBM3D.m:15-19)."""
+    lib = load_library()
+    src = opts_to_struct(opts)
+    o = Opts()
+    check(lib.gbm3d_resolve_opts(C.byref(src) if src is not None else None, C.byref(o)))
+    return struct_to_opts(o)
+
+
+def getTau(levels: int, mode: int, sigma: float = 1.0) -> np.ndarray:
+    """sigma*getTau(levels, mode) as a (levels, 7) array."""
+    lib = load_library()
+    t = np.zeros((levels, 7), dtype=np.float64, order="F")
+    check(lib.gbm3d_get_tau(levels, mode, C.c_double(sigma), ptr(t)))
+    return t
+
+
+def myPSNR(Iref, Irec, maxI=None) -> float:
+    """utilities/myPSNR.m: 10*log10(maxI^2/MSE), maxI defaults to max(Iref)."""
+    Iref = np.asarray(Iref, dtype=np.float64)
+    Irec = np.asarray(Irec, dtype=np.float64)
+    if maxI is None:
+        maxI = Iref.max()
+    mse = np.sum((Iref - Irec) ** 2) / Iref.size
+    return float(10 * np.log10(maxI ** 2 / mse))
+
+
+class Context:
+    """One gbm3d_ctx: a device, a HIP stream and cached device workspace."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        check(self.lib.gbm3d_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            self.lib.gbm3d_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr: int | None):
+        check(self.lib.gbm3d_set_stream(self.handle, C.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def synchronize(self):
+        check(self.lib.gbm3d_synchronize(self.handle))
+
+    def set_timing(self, on: bool):
+        check(self.lib.gbm3d_set_timing(self.handle, int(bool(on))))
+
+    def stage_times(self) -> dict:
+        ms = np.zeros(len(_abi.STAGES), dtype=np.float64)
+        cnt = np.zeros(len(_abi.STAGES), dtype=np.int32)
+        check(self.lib.gbm3d_get_stage_times(self.handle, ptr(ms), ptr(cnt)))
+        return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(_abi.STAGES)}
+
+    # ------------------------------------------------------------ full path
+    def denoise(self, I, sigma: float, opts: dict | None = None):
+        I = np.asfortranarray(I, dtype=np.float32)
+        if I.ndim != 2:
+            raise ValueError("GBM3D expects a 2-D grayscale image")
+        m, n = I.shape
+        U = np.zeros((m, n), dtype=np.float32, order="F")
+        U1 = np.zeros_like(U)
+        o = opts_to_struct(opts)
+        check(self.lib.gbm3d_denoise(self.handle, ptr(I), m, n, C.c_float(sigma),
+                                     C.byref(o) if o is not None else None, ptr(U), ptr(U1)))
+        return U, {"U1": U1}
+
+    def denoise_device(self, dI: int, m: int, n: int, sigma: float, opts, dU: int, dU1: int | None = None):
+        """Device pointers (ints, column-major float32); asynchronous."""
+        o = opts_to_struct(opts)
+        check(self.lib.gbm3d_denoise_device(self.handle, C.c_void_p(dI), m, n, C.c_float(sigma),
+                                            C.byref(o) if o is not None else None, C.c_void_p(dU),
+                                            C.c_void_p(dU1) if dU1 else None))
+
+    # ------------------------------------------------------------ stages
+    def match(self, I, d: int, d2: int, K: int, want_volume: bool = True):
+        I = np.asfortranarray(I, dtype=np.float32)
+        m, n = I.shape
+        S = np.zeros((K, m // d, n // d), dtype=np.uint32, order="F")
+        V = np.zeros((m, n, K), dtype=np.float32, order="F") if want_volume else None
+        check(self.lib.gbm3d_match(self.handle, ptr(I), m, n, d, d2, K, ptr(S), ptr(V)))
+        return S, V
+
+    def ht_filter(self, V, levels: int, cycle_spin: int, tau):
+        V = np.asfortranarray(V, dtype=np.float32)
+        m, n, K = V.shape
+        tau = np.asfortranarray(tau, dtype=np.float64)
+        out = np.zeros_like(V)
+        check(self.lib.gbm3d_ht_filter(self.handle, ptr(V), m, n, K, levels, cycle_spin, ptr(tau), ptr(out)))
+        return out
+
+    def aggregate_ht(self, Vavg, S, d: int, d2: int):
+        Vavg = np.asfortranarray(Vavg, dtype=np.float32)
+        S = np.asfortranarray(S, dtype=np.uint32)
+        m, n, K = Vavg.shape
+        U = np.zeros((m, n), dtype=np.float32, order="F")
+        W = np.zeros_like(U)
+        check(self.lib.gbm3d_aggregate_ht(self.handle, ptr(Vavg), ptr(S), m, n, d, d2, K, ptr(U), ptr(W)))
+        return U, W
+
+    def match_wie(self, I, In, d: int, d2: int, K: int, want_groups: bool = True):
+        I = np.asfortranarray(I, dtype=np.float32)
+        In = np.asfortranarray(In, dtype=np.float32)
+        m, n = I.shape
+        S = np.zeros((K, m // d, n // d), dtype=np.uint32, order="F")
+        V1 = V2 = None
+        if want_groups:
+            V1 = np.zeros((d, d, K, m // d, n // d), dtype=np.float32, order="F")
+            V2 = np.zeros_like(V1)
+        check(self.lib.gbm3d_match_wie(self.handle, ptr(I), ptr(In), m, n, d, d2, K, ptr(S), ptr(V1), ptr(V2)))
+        return S, V1, V2
+
+    def wiener(self, V1, V2, sigma: float):
+        V1 = np.asfortranarray(V1, dtype=np.float32)
+        V2 = np.asfortranarray(V2, dtype=np.float32)
+        d, K = V1.shape[0], V1.shape[2]
+        G = V1.size // (d * d * K)
+        V2f = np.zeros_like(V2)
+        g2 = np.zeros(V1.shape[3:], dtype=np.float32, order="F")
+        check(self.lib.gbm3d_wiener(self.handle, ptr(V1), ptr(V2), d, K, G, C.c_float(sigma), ptr(V2f), ptr(g2)))
+        return V2f, g2
+
+    def aggregate_wie(self, V2f, gain2, S, m: int, n: int, d2: int):
+        V2f = np.asfortranarray(V2f, dtype=np.float32)
+        gain2 = np.asfortranarray(gain2, dtype=np.float32)
+        S = np.asfortranarray(S, dtype=np.uint32)
+        d, K = V2f.shape[0], V2f.shape[2]
+        U = np.zeros((m, n), dtype=np.float32, order="F")
+        W = np.zeros_like(U)
+        check(self.lib.gbm3d_aggregate_wie(self.handle, ptr(V2f), ptr(gain2), ptr(S), m, n, d, d2, K, ptr(U),
+                                           ptr(W)))
+        return U, W
+
+
+_default: Context | None = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def GBM3D(I, sigma: float, opts: dict | None = None):
+    """[U, out] = GBM3D(I, sigma, opts)  (source/GBM3D.m:1-84); out['U1'] is the pilot."""
+    return default_context().denoise(I, sigma, opts)
+
+
+def matchBlocksCPU(I, sigma: float, opts: dict):
+    """[S, Volume] = matchBlocksCPU(I, sigma, opts) on the GPU (numMax == numMin)."""
+    o = checkBM3Dopts(opts)
+    return default_context().match(I, o["blockSize"], o["matchSize"], o["numMax"])
+
+
+def denoiseAndAggregateGPU(Volume, S, tau, opts: dict):
+    """[U, W] = denoiseAndAggregateGPU(Volume, S, tau, Psi, Psi2, opts) ('ht')."""
+    o = checkBM3Dopts(opts)
+    ctx = default_context()
+    Vavg = ctx.ht_filter(Volume, o["levels"], o["cycleSpin"], tau)
+    return ctx.aggregate_ht(Vavg, S, o["blockSize"], o["matchSize"])
+
+
+def matchBlocksWie(I, In, sigma: float, opts: dict):
+    """[S, V1, V2] = matchBlocksWie(I, In, sigma, opts)."""
+    o = checkBM3Dopts(opts)
+    return default_context().match_wie(I, In, o["blockSizeWie"], o["matchSize"], o["numMax"])
+
+
+def denoiseAndAggregateWieFast(V1, V2, S, sigma: float, opts: dict):
+    """[U, W] = denoiseAndAggregateWieFast(V1, V2, S, sigma, opts)."""
+    o = checkBM3Dopts(opts)
+    ctx = default_context()
+    V2f, g2 = ctx.wiener(V1, V2, sigma)
+    d = o["blockSizeWie"]
+    return ctx.aggregate_wie(V2f, g2, S, S.shape[1] * d, S.shape[2] * d, o["matchSize"])

@@ -1,0 +1,608 @@
+// api.hip -- the C ABI of libgbm3d: contexts, the two-step pipeline of
+// source/GBM3D.m:21-83, and the stage entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace gbm3d;
+
+struct gbm3d_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  char* io = nullptr;  // staging for the host-buffer entry points
+  size_t io_bytes = 0;
+  float* masks = nullptr;  // [4 block sizes][2][32*32]
+  bool timing = false;
+  struct Mark {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+};
+
+namespace {
+
+constexpr int kMaskStride = 2 * 32 * 32;
+int mask_slot(int d) { return d == 4 ? 0 : d == 8 ? 1 : d == 16 ? 2 : d == 32 ? 3 : -1; }
+
+// MATLAB linspace(a,b,n) with the last element pinned to b.
+std::vector<double> linspace(double a, double b, int n) {
+  std::vector<double> y(n);
+  const double step = n > 1 ? (b - a) / (n - 1) : 0.0;
+  for (int k = 0; k < n; ++k) y[k] = a + k * step;
+  y[n - 1] = b;
+  return y;
+}
+
+// Bump allocator over the context arena; a first pass with base == nullptr
+// measures the layout.
+struct Carve {
+  char* base;
+  size_t off = 0;
+  template <class T>
+  T* take(size_t n) {
+    size_t a = (off + 255) & ~(size_t)255;
+    off = a + n * sizeof(T);
+    return base ? reinterpret_cast<T*>(base + a) : nullptr;
+  }
+};
+
+int ensure(char** buf, size_t* have, size_t need) {
+  if (need <= *have) return GBM3D_OK;
+  if (*buf) GBM3D_HIP(hipFree(*buf));
+  *buf = nullptr;
+  *have = 0;
+  size_t want = need + need / 8;
+  if (hipMalloc(reinterpret_cast<void**>(buf), want) != hipSuccess) {
+    (void)hipGetLastError();
+    *buf = nullptr;
+    return fail(GBM3D_ERR_NOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+  }
+  *have = want;
+  return GBM3D_OK;
+}
+
+struct StageTimer {
+  gbm3d_ctx* c;
+  int stage;
+  hipEvent_t b = nullptr;
+  StageTimer(gbm3d_ctx* ctx, int s) : c(ctx), stage(s) {
+    if (!c->timing) return;
+    if (c->pool_used + 2 > c->pool.size()) {
+      hipEvent_t e1, e2;
+      if (hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess) {
+        c->timing = false;
+        return;
+      }
+      c->pool.push_back(e1);
+      c->pool.push_back(e2);
+    }
+    hipEvent_t a = c->pool[c->pool_used++];
+    b = c->pool[c->pool_used++];
+    (void)hipEventRecord(a, c->stream);
+    c->marks.push_back({stage, a, b});
+  }
+  ~StageTimer() {
+    if (c->timing && b) (void)hipEventRecord(b, c->stream);
+  }
+};
+
+int validate(const gbm3d_opts& o, int m, int n, int* M, int* N) {
+  if (m < 1 || n < 1) return fail(GBM3D_ERR_ARG, "image must be non-empty");
+  if (o.filtType != GBM3D_FILT_HT) {
+    if (o.filtType == GBM3D_FILT_MEDIAN) return fail(GBM3D_ERR_UNSUPPORTED, "filtType 'median' is not built");
+    return fail(GBM3D_ERR_ARG, "unrecognized filtering");
+  }
+  if (o.numMax != o.numMin) return fail(GBM3D_ERR_UNSUPPORTED, "numMax ~= numMin (tolerance matching) is not built");
+  if (o.nMatchSpins < 1 || o.nMatchSpins > GBM3D_MAX_SPINS) return fail(GBM3D_ERR_ARG, "matchSpins: 1..16 entries");
+  if (o.Wiener) {
+    if (o.nMatchSpinsWie == GBM3D_UNSET) return fail(GBM3D_ERR_MISSING_FIELD, "Reference to non-existent field 'matchSpinsWie'");
+    if (o.nMatchSpinsWie < 1 || o.nMatchSpinsWie > GBM3D_MAX_SPINS) return fail(GBM3D_ERR_ARG, "matchSpinsWie: 1..16 entries");
+  }
+  const int bs = o.blockSize, d2 = o.matchSize, K = o.numMax;
+  if (mask_slot(bs) < 0) return fail(GBM3D_ERR_UNSUPPORTED, "blockSize must be 4, 8, 16 or 32");
+  if (o.Wiener && (o.blockSizeWie != 4 && o.blockSizeWie != 8 && o.blockSizeWie != 16))
+    return fail(GBM3D_ERR_UNSUPPORTED, "blockSizeWie must be 4, 8 or 16");
+  if (d2 != 32 && d2 != 48) return fail(GBM3D_ERR_UNSUPPORTED, "matchSize must be 32 or 48");
+  if (o.levels < 1 || o.levels > 4) return fail(GBM3D_ERR_UNSUPPORTED, "levels must be 1..4");
+  if (o.cycleSpin < 1 || o.cycleSpin > 8) return fail(GBM3D_ERR_ARG, "cycleSpin must be 1..8");
+  if (K < 1 || K > 64 || K % (1 << o.levels)) return fail(GBM3D_ERR_UNSUPPORTED, "numMax must be <= 64 and a multiple of 2^levels");
+  *M = (m + bs - 1) / bs * bs;
+  *N = (n + bs - 1) / bs * bs;
+  if (*M > 2 * m || *N > 2 * n) return fail(GBM3D_ERR_ARG, "image smaller than half a block");
+  if (*M < d2 || *N < d2) return fail(GBM3D_ERR_ARG, "padded image smaller than matchSize");
+  if (*M % (1 << o.levels) || *N % (1 << o.levels)) return fail(GBM3D_ERR_UNSUPPORTED, "image size not a multiple of 2^levels");
+  if (o.Wiener && (*M % o.blockSizeWie || *N % o.blockSizeWie))
+    return fail(GBM3D_ERR_UNSUPPORTED, "padded size must be a multiple of blockSizeWie");
+  return GBM3D_OK;
+}
+
+struct Plan {
+  int m, n, M, N, bs, bw, d2, K, n1, n2;
+  size_t MN() const { return (size_t)M * N; }
+};
+
+// All device buffers of one denoise call.
+struct Bufs {
+  float *Ip, *pad1, *pad2, *bn, *vol, *vavg, *w1, *U1s, *W1s, *pilot, *V2f, *g2, *U2s, *W2s, *htw;
+  uint32_t *S1, *S2;
+};
+
+void carve(Carve& cv, const Plan& p, const gbm3d_opts& o, Bufs* b) {
+  const size_t MN = p.MN(), P = (size_t)p.M + p.d2, Q = (size_t)p.N + p.d2;
+  const int dmin = std::min(p.bs, o.Wiener ? p.bw : p.bs);
+  const size_t G1 = MN / ((size_t)p.bs * p.bs), G2 = o.Wiener ? MN / ((size_t)p.bw * p.bw) : 0;
+  b->Ip = cv.take<float>(MN);
+  b->pad1 = cv.take<float>(P * Q);
+  b->pad2 = cv.take<float>(P * Q);
+  b->bn = cv.take<float>((P - dmin + 1) * (Q - dmin + 1));
+  b->S1 = cv.take<uint32_t>((size_t)p.K * G1);
+  b->vol = cv.take<float>(MN * p.K);
+  b->htw = cv.take<float>(ht_work_floats(p.M, p.N, p.K, o.levels, o.cycleSpin));
+  b->vavg = cv.take<float>(MN * p.K);
+  b->w1 = cv.take<float>(G1);
+  b->U1s = cv.take<float>(MN * p.n1);
+  b->W1s = cv.take<float>(MN * p.n1);
+  b->pilot = cv.take<float>(MN);
+  b->S2 = cv.take<uint32_t>((size_t)p.K * G2);
+  b->V2f = cv.take<float>(G2 ? MN * p.K : 0);
+  b->g2 = cv.take<float>(G2);
+  b->U2s = cv.take<float>(MN * p.n2);
+  b->W2s = cv.take<float>(MN * p.n2);
+}
+
+int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const gbm3d_opts& o, float* dU,
+                 float* dU1) {
+  Plan p;
+  p.m = m;
+  p.n = n;
+  GBM3D_TRY(validate(o, m, n, &p.M, &p.N));
+  p.bs = o.blockSize;
+  p.bw = o.blockSizeWie;
+  p.d2 = o.matchSize;
+  p.K = o.numMax;
+  p.n1 = o.nMatchSpins;
+  p.n2 = o.Wiener ? o.nMatchSpinsWie : 0;
+  Carve measure{nullptr};
+  Bufs b;
+  carve(measure, p, o, &b);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, measure.off));
+  Carve cv{c->arena};
+  carve(cv, p, o, &b);
+  hipStream_t st = c->stream;
+  const size_t MN = p.MN();
+  const int P = p.M + p.d2, Q = p.N + p.d2;
+  std::vector<double> tau((size_t)o.levels * 7);
+  tau_table(o.levels, o.tauMode, (double)sigma, tau.data());
+  HtWork hw;
+  ht_work_bind(b.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
+  const float* masks1 = c->masks + kMaskStride * mask_slot(p.bs);
+
+  {
+    StageTimer t(c, GBM3D_STAGE_PAD);
+    GBM3D_TRY(launch_mirror_pad(dI, m, n, p.M, p.N, b.Ip, st));  // source/GBM3D.m:22-25
+  }
+  // ---- step 1: source/GBM3D.m:50-57
+  for (int ii = 0; ii < p.n1; ++ii) {
+    const int sp = o.matchSpins[ii];
+    {
+      StageTimer t(c, GBM3D_STAGE_PAD);
+      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, b.pad1, st));
+      GBM3D_TRY(launch_norm_map(b.pad1, P, Q, p.bs, b.bn, st));
+    }
+    {
+      StageTimer t(c, GBM3D_STAGE_MATCH1);
+      GBM3D_TRY(launch_match(b.pad1, b.bn, p.M, p.N, p.bs, p.d2, p.K, b.S1, b.vol, st));
+    }
+    {
+      StageTimer t(c, GBM3D_STAGE_HT);
+      GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, st));
+    }
+    {
+      StageTimer t(c, GBM3D_STAGE_AGG1);
+      GBM3D_TRY(launch_agg1_prep(hw.vrec, o.cycleSpin, nullptr, p.M, p.N, p.K, p.bs, b.vavg, b.w1, st));
+      GBM3D_TRY(launch_aggregate_m(1, b.vavg, b.w1, b.S1, masks1, p.M, p.N, p.bs, p.d2, p.K, sp, b.U1s + MN * ii,
+                                   b.W1s + MN * ii, st));
+    }
+  }
+  {
+    StageTimer t(c, GBM3D_STAGE_NORM);
+    GBM3D_TRY(launch_normalize(b.U1s, p.n1, b.W1s, p.n1, nullptr, 0, p.M, p.N, p.M, p.N, b.pilot, st));
+    if (dU1) GBM3D_TRY(launch_normalize(b.U1s, p.n1, b.W1s, p.n1, nullptr, 0, p.M, p.N, m, n, dU1, st));
+    if (!o.Wiener) {  // :59-63
+      GBM3D_TRY(launch_normalize(b.U1s, p.n1, b.W1s, p.n1, nullptr, 0, p.M, p.N, m, n, dU, st));
+      return GBM3D_OK;
+    }
+  }
+  // ---- step 2: source/GBM3D.m:70-82
+  const float* masks2 = c->masks + kMaskStride * mask_slot(p.bw);
+  const long G2 = (long)(p.M / p.bw) * (p.N / p.bw);
+  for (int ii = 0; ii < p.n2; ++ii) {
+    const int sp = o.matchSpinsWie[ii];
+    {
+      StageTimer t(c, GBM3D_STAGE_PAD);
+      GBM3D_TRY(launch_search_pad(b.pilot, p.M, p.N, sp, p.d2, b.pad1, st));
+      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, b.pad2, st));
+      GBM3D_TRY(launch_norm_map(b.pad1, P, Q, p.bw, b.bn, st));
+    }
+    {
+      StageTimer t(c, GBM3D_STAGE_MATCH2);
+      GBM3D_TRY(launch_match(b.pad1, b.bn, p.M, p.N, p.bw, p.d2, p.K, b.S2, nullptr, st));
+    }
+    {
+      StageTimer t(c, GBM3D_STAGE_WIENER);
+      GBM3D_TRY(launch_wiener(b.pad1, b.pad2, b.S2, nullptr, nullptr, p.M, p.N, p.bw, p.d2, p.K, G2, sigma, b.V2f,
+                              b.g2, st));
+    }
+    {
+      StageTimer t(c, GBM3D_STAGE_AGG2);
+      GBM3D_TRY(launch_aggregate_m(2, b.V2f, b.g2, b.S2, masks2, p.M, p.N, p.bw, p.d2, p.K, sp, b.U2s + MN * ii,
+                                   b.W2s + MN * ii, st));
+    }
+  }
+  {
+    // U = sum(U,3)./sum(W,3): slices past numel(matchSpinsWie) still hold the
+    // step-1 weights (U, W are sized by numel(matchSpins), :26-27).
+    StageTimer t(c, GBM3D_STAGE_NORM);
+    const int extra = std::max(0, p.n1 - p.n2);
+    GBM3D_TRY(launch_normalize(b.U2s, p.n2, b.W2s, p.n2, b.W1s + MN * p.n2, extra, p.M, p.N, m, n, dU, st));
+  }
+  return GBM3D_OK;
+}
+
+int set_dev(gbm3d_ctx* c) {
+  if (!c) return fail(GBM3D_ERR_ARG, "NULL context");
+  GBM3D_HIP(hipSetDevice(c->device));
+  clear_error();
+  return GBM3D_OK;
+}
+
+template <class T>
+int upload(gbm3d_ctx* c, T* dst, const T* src, size_t n) {
+  GBM3D_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return GBM3D_OK;
+}
+template <class T>
+int download(gbm3d_ctx* c, T* dst, const T* src, size_t n) {
+  if (!dst) return GBM3D_OK;
+  GBM3D_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  return GBM3D_OK;
+}
+
+int resolve_checked(const gbm3d_opts* in, gbm3d_opts* out) {
+  GBM3D_TRY(resolve(in, out));
+  return GBM3D_OK;
+}
+
+bool supported_block(int d) { return d == 4 || d == 8 || d == 16 || d == 32; }
+
+int stage_sizes(int m, int n, int d, int d2, int K) {
+  if (m < 1 || n < 1 || !supported_block(d)) return fail(GBM3D_ERR_ARG, "stage: bad size or block size");
+  if (m % d || n % d) return fail(GBM3D_ERR_ARG, "stage: image size must be a multiple of the block size");
+  if (m < d2 || n < d2 || (d2 != 32 && d2 != 48)) return fail(GBM3D_ERR_ARG, "stage: matchSize must be 32 or 48 and <= image size");
+  if (K < 1 || K > 64) return fail(GBM3D_ERR_ARG, "stage: numMax must be 1..64");
+  return GBM3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gbm3d_create(int device, gbm3d_ctx** out) {
+  clear_error();
+  if (!out) return fail(GBM3D_ERR_ARG, "NULL output");
+  *out = nullptr;
+  int count = 0;
+  GBM3D_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(GBM3D_ERR_ARG, "no such device");
+  GBM3D_HIP(hipSetDevice(device));
+  gbm3d_ctx* c = new gbm3d_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(GBM3D_ERR_HIP, "hipStreamCreate failed");
+  }
+  c->stream = c->own;
+  // Aggregation masks for every supported block size (computed in double,
+  // stored as float): step 1 sin(pi*linspace(.1,.9,d)) outer product,
+  // step 2 sin(pi*linspace(.2,.8,d)) outer product.
+  std::vector<float> h(4 * kMaskStride, 0.f);
+  const int ds[4] = {4, 8, 16, 32};
+  for (int i = 0; i < 4; ++i) {
+    const int d = ds[i];
+    for (int which = 0; which < 2; ++which) {
+      std::vector<double> s = which == 0 ? linspace(0.1, 0.9, d) : linspace(0.2, 0.8, d);
+      for (auto& x : s) x = std::sin(M_PI * x);
+      for (int v = 0; v < d; ++v)
+        for (int u = 0; u < d; ++u) h[i * kMaskStride + which * d * d + u + d * v] = (float)(s[u] * s[v]);
+    }
+  }
+  if (hipMalloc(reinterpret_cast<void**>(&c->masks), h.size() * sizeof(float)) != hipSuccess ||
+      hipMemcpy(c->masks, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return fail(GBM3D_ERR_HIP, "mask table upload failed");
+  }
+  *out = c;
+  return GBM3D_OK;
+}
+
+int gbm3d_destroy(gbm3d_ctx* c) {
+  if (!c) return GBM3D_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->io) (void)hipFree(c->io);
+  if (c->masks) (void)hipFree(c->masks);
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+  return GBM3D_OK;
+}
+
+int gbm3d_set_stream(gbm3d_ctx* c, void* s) {
+  GBM3D_TRY(set_dev(c));
+  c->stream = s ? reinterpret_cast<hipStream_t>(s) : c->own;
+  return GBM3D_OK;
+}
+
+int gbm3d_synchronize(gbm3d_ctx* c) {
+  GBM3D_TRY(set_dev(c));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_set_timing(gbm3d_ctx* c, int enable) {
+  GBM3D_TRY(set_dev(c));
+  c->timing = enable != 0;
+  return GBM3D_OK;
+}
+
+int gbm3d_get_stage_times(gbm3d_ctx* c, double* ms, int* launches) {
+  GBM3D_TRY(set_dev(c));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> acc(GBM3D_NUM_STAGES, 0.0);
+  std::vector<int> cnt(GBM3D_NUM_STAGES, 0);
+  for (auto& mk : c->marks) {
+    float t = 0.f;
+    GBM3D_HIP(hipEventElapsedTime(&t, mk.a, mk.b));
+    acc[mk.stage] += t;
+    cnt[mk.stage] += 1;
+  }
+  for (int s = 0; s < GBM3D_NUM_STAGES; ++s) {
+    if (ms) ms[s] = acc[s];
+    if (launches) launches[s] = cnt[s];
+  }
+  return GBM3D_OK;
+}
+
+int gbm3d_denoise_device(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const gbm3d_opts* opts, float* dU,
+                         float* dU1) {
+  GBM3D_TRY(set_dev(c));
+  if (!dI || !dU) return fail(GBM3D_ERR_ARG, "NULL image");
+  gbm3d_opts o;
+  GBM3D_TRY(resolve_checked(opts, &o));
+  if (c->timing) {
+    c->marks.clear();
+    c->pool_used = 0;
+  }
+  return run_pipeline(c, dI, m, n, sigma, o, dU, dU1);
+}
+
+int gbm3d_denoise(gbm3d_ctx* c, const float* I, int m, int n, float sigma, const gbm3d_opts* opts, float* U,
+                  float* U1) {
+  GBM3D_TRY(set_dev(c));
+  if (!I || !U || m < 1 || n < 1) return fail(GBM3D_ERR_ARG, "NULL or empty image");
+  const size_t mn = (size_t)m * n;
+  GBM3D_TRY(ensure(&c->io, &c->io_bytes, 3 * mn * sizeof(float) + 1024));
+  float* dI = reinterpret_cast<float*>(c->io);
+  float* dU = dI + mn;
+  float* dU1 = dU + mn;
+  GBM3D_TRY(upload(c, dI, I, mn));
+  GBM3D_TRY(gbm3d_denoise_device(c, dI, m, n, sigma, opts, dU, U1 ? dU1 : nullptr));
+  GBM3D_TRY(download(c, U, dU, mn));
+  if (U1) GBM3D_TRY(download(c, U1, dU1, mn));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+// ---------------------------------------------------------- stage entry points
+
+int gbm3d_match(gbm3d_ctx* c, const float* I, int m, int n, int d, int d2, int K, uint32_t* S, float* Volume) {
+  GBM3D_TRY(set_dev(c));
+  if (!I || !S) return fail(GBM3D_ERR_ARG, "NULL argument");
+  GBM3D_TRY(stage_sizes(m, n, d, d2, K));
+  const size_t mn = (size_t)m * n, P = (size_t)m + d2, Q = (size_t)n + d2, G = mn / ((size_t)d * d);
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** img, float** pad, float** bn, uint32_t** dS, float** vol) {
+    *img = k.take<float>(mn);
+    *pad = k.take<float>(P * Q);
+    *bn = k.take<float>((P - d + 1) * (Q - d + 1));
+    *dS = k.take<uint32_t>(G * K);
+    *vol = k.take<float>(Volume ? mn * K : 0);
+  };
+  float *img, *pad, *bn, *vol;
+  uint32_t* dS;
+  lay(cv, &img, &pad, &bn, &dS, &vol);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &img, &pad, &bn, &dS, &vol);
+  GBM3D_TRY(upload(c, img, I, mn));
+  GBM3D_TRY(launch_search_pad(img, m, n, 0, d2, pad, c->stream));
+  GBM3D_TRY(launch_norm_map(pad, (int)P, (int)Q, d, bn, c->stream));
+  GBM3D_TRY(launch_match(pad, bn, m, n, d, d2, K, dS, Volume ? vol : nullptr, c->stream));
+  GBM3D_TRY(download(c, S, dS, G * K));
+  if (Volume) GBM3D_TRY(download(c, Volume, vol, mn * K));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_ht_filter(gbm3d_ctx* c, const float* Volume, int m, int n, int K, int levels, int cycleSpin,
+                    const double* tau, float* Vavg) {
+  GBM3D_TRY(set_dev(c));
+  if (!Volume || !tau || !Vavg) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (m < 2 || n < 2 || K < 2) return fail(GBM3D_ERR_ARG, "ht_filter: size");
+  if (levels < 1 || levels > 4 || cycleSpin < 1 || cycleSpin > 8) return fail(GBM3D_ERR_ARG, "ht_filter: levels/cycleSpin");
+  const size_t vol = (size_t)m * n * K;
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** in, float** out, float** htw) {
+    *in = k.take<float>(vol);
+    *out = k.take<float>(vol);
+    *htw = k.take<float>(ht_work_floats(m, n, K, levels, cycleSpin));
+  };
+  float *in, *out, *htw;
+  lay(cv, &in, &out, &htw);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &in, &out, &htw);
+  HtWork hw;
+  ht_work_bind(htw, m, n, K, levels, cycleSpin, &hw);
+  GBM3D_TRY(upload(c, in, Volume, vol));
+  GBM3D_TRY(run_ht(in, m, n, K, levels, cycleSpin, tau, hw, c->stream));
+  GBM3D_TRY(launch_combine_spins(hw.vrec, cycleSpin, vol, out, c->stream));
+  GBM3D_TRY(download(c, Vavg, out, vol));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_aggregate_ht(gbm3d_ctx* c, const float* Vavg, const uint32_t* S, int m, int n, int d, int d2, int K,
+                       float* U, float* W) {
+  GBM3D_TRY(set_dev(c));
+  if (!Vavg || !S || !U || !W) return fail(GBM3D_ERR_ARG, "NULL argument");
+  GBM3D_TRY(stage_sizes(m, n, d, d2, K));
+  const size_t mn = (size_t)m * n, G = mn / ((size_t)d * d);
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** va, uint32_t** dS, float** w, float** dU, float** dW) {
+    *va = k.take<float>(mn * K);
+    *dS = k.take<uint32_t>(G * K);
+    *w = k.take<float>(G);
+    *dU = k.take<float>(mn);
+    *dW = k.take<float>(mn);
+  };
+  float *va, *w, *dU, *dW;
+  uint32_t* dS;
+  lay(cv, &va, &dS, &w, &dU, &dW);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &va, &dS, &w, &dU, &dW);
+  GBM3D_TRY(upload(c, va, Vavg, mn * K));
+  GBM3D_TRY(upload(c, dS, S, G * K));
+  GBM3D_TRY(launch_agg1_prep(nullptr, 0, va, m, n, K, d, nullptr, w, c->stream));
+  GBM3D_TRY(launch_aggregate_m(1, va, w, dS, c->masks + kMaskStride * mask_slot(d), m, n, d, d2, K, 0, dU, dW,
+                               c->stream));
+  GBM3D_TRY(download(c, U, dU, mn));
+  GBM3D_TRY(download(c, W, dW, mn));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_match_wie(gbm3d_ctx* c, const float* I, const float* In, int m, int n, int d, int d2, int K, uint32_t* S,
+                    float* V1, float* V2) {
+  GBM3D_TRY(set_dev(c));
+  if (!I || !In || !S) return fail(GBM3D_ERR_ARG, "NULL argument");
+  GBM3D_TRY(stage_sizes(m, n, d, d2, K));
+  const size_t mn = (size_t)m * n, P = (size_t)m + d2, Q = (size_t)n + d2, G = mn / ((size_t)d * d);
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** i1, float** i2, float** p1, float** p2, float** bn, uint32_t** dS, float** v) {
+    *i1 = k.take<float>(mn);
+    *i2 = k.take<float>(mn);
+    *p1 = k.take<float>(P * Q);
+    *p2 = k.take<float>(P * Q);
+    *bn = k.take<float>((P - d + 1) * (Q - d + 1));
+    *dS = k.take<uint32_t>(G * K);
+    *v = k.take<float>(mn * K);
+  };
+  float *i1, *i2, *p1, *p2, *bn, *v;
+  uint32_t* dS;
+  lay(cv, &i1, &i2, &p1, &p2, &bn, &dS, &v);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &i1, &i2, &p1, &p2, &bn, &dS, &v);
+  GBM3D_TRY(upload(c, i1, I, mn));
+  GBM3D_TRY(upload(c, i2, In, mn));
+  GBM3D_TRY(launch_search_pad(i1, m, n, 0, d2, p1, c->stream));
+  GBM3D_TRY(launch_search_pad(i2, m, n, 0, d2, p2, c->stream));
+  GBM3D_TRY(launch_norm_map(p1, (int)P, (int)Q, d, bn, c->stream));
+  GBM3D_TRY(launch_match(p1, bn, m, n, d, d2, K, dS, nullptr, c->stream));
+  GBM3D_TRY(download(c, S, dS, G * K));
+  if (V1) {
+    GBM3D_TRY(launch_gather_groups(p1, dS, m, n, d, d2, K, v, c->stream));
+    GBM3D_TRY(download(c, V1, v, mn * K));
+    GBM3D_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (V2) {
+    GBM3D_TRY(launch_gather_groups(p2, dS, m, n, d, d2, K, v, c->stream));
+    GBM3D_TRY(download(c, V2, v, mn * K));
+  }
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_wiener(gbm3d_ctx* c, const float* V1, const float* V2, int d, int K, int G, float sigma, float* V2f,
+                 float* gain2) {
+  GBM3D_TRY(set_dev(c));
+  if (!V1 || !V2 || !V2f || !gain2 || G < 0) return fail(GBM3D_ERR_ARG, "NULL argument");
+  const size_t n = (size_t)d * d * K * G;
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** a, float** b, float** o, float** g) {
+    *a = k.take<float>(n);
+    *b = k.take<float>(n);
+    *o = k.take<float>(n);
+    *g = k.take<float>(G);
+  };
+  float *a, *bb, *o, *g;
+  lay(cv, &a, &bb, &o, &g);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &a, &bb, &o, &g);
+  GBM3D_TRY(upload(c, a, V1, n));
+  GBM3D_TRY(upload(c, bb, V2, n));
+  GBM3D_TRY(launch_wiener(nullptr, nullptr, nullptr, a, bb, 0, 0, d, 0, K, G, sigma, o, g, c->stream));
+  GBM3D_TRY(download(c, V2f, o, n));
+  GBM3D_TRY(download(c, gain2, g, (size_t)G));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_aggregate_wie(gbm3d_ctx* c, const float* V2f, const float* gain2, const uint32_t* S, int m, int n, int d,
+                        int d2, int K, float* U, float* W) {
+  GBM3D_TRY(set_dev(c));
+  if (!V2f || !gain2 || !S || !U || !W) return fail(GBM3D_ERR_ARG, "NULL argument");
+  GBM3D_TRY(stage_sizes(m, n, d, d2, K));
+  const size_t mn = (size_t)m * n, G = mn / ((size_t)d * d);
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** v, float** g, uint32_t** dS, float** dU, float** dW) {
+    *v = k.take<float>(mn * K);
+    *g = k.take<float>(G);
+    *dS = k.take<uint32_t>(G * K);
+    *dU = k.take<float>(mn);
+    *dW = k.take<float>(mn);
+  };
+  float *v, *g, *dU, *dW;
+  uint32_t* dS;
+  lay(cv, &v, &g, &dS, &dU, &dW);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &v, &g, &dS, &dU, &dW);
+  GBM3D_TRY(upload(c, v, V2f, mn * K));
+  GBM3D_TRY(upload(c, g, gain2, G));
+  GBM3D_TRY(upload(c, dS, S, G * K));
+  GBM3D_TRY(launch_aggregate_m(2, v, g, dS, c->masks + kMaskStride * mask_slot(d), m, n, d, d2, K, 0, dU, dW,
+                               c->stream));
+  GBM3D_TRY(download(c, U, dU, mn));
+  GBM3D_TRY(download(c, W, dW, mn));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+}  // extern "C"

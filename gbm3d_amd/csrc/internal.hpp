@@ -1,0 +1,106 @@
+// internal.hpp -- shared host/device declarations of libgbm3d (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gbm3d.h"
+
+namespace gbm3d {
+
+// ------------------------------------------------------------ error plumbing
+void set_error(const std::string& msg);
+void clear_error();
+
+struct Status {
+  int code = GBM3D_OK;
+  Status() = default;
+  Status(int c) : code(c) {}
+  bool ok() const { return code == GBM3D_OK; }
+};
+
+int fail(int code, const std::string& msg);
+
+#define GBM3D_HIP(expr)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::gbm3d::fail(GBM3D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define GBM3D_TRY(expr)               \
+  do {                                \
+    int s_ = (expr);                  \
+    if (s_ != GBM3D_OK) return s_;    \
+  } while (0)
+
+// Launch-error check right after a kernel launch.
+int check_launch(const char* what);
+
+// ------------------------------------------------------------------ options
+int resolve(const gbm3d_opts* in, gbm3d_opts* out);
+void tau_table(int levels, int mode, double sigma, double* tau);
+
+// ------------------------------------------------------------ the filters
+// bior1.5 (x, y) x Haar (z) of waveTransforms3D/getWaveFilters3Dbior.m:16-27,49-50.
+constexpr double kCa = 0.016572815184060;
+constexpr double kCb = 0.121533978016438;
+constexpr double kCc = 0.70710678118654752440;  // sqrt(2)/2
+
+// ------------------------------------------------------------ kernel launchers
+// All launchers validate their own grid assumptions and return a status.
+struct PadGeom {
+  int M, N;   // image size (block multiple)
+  int d2;     // match size; padded image is (M+d2) x (N+d2)
+  int P() const { return M + d2; }
+  int Q() const { return N + d2; }
+};
+
+int launch_mirror_pad(const float* src, int m, int n, int M, int N, float* dst, hipStream_t st);
+int launch_search_pad(const float* img, int M, int N, int spin, int d2, float* pad, hipStream_t st);
+int launch_norm_map(const float* pad, int P, int Q, int d, float* bn, hipStream_t st);
+// Block matching: S (K x Mt x Nt, 1-based window indices) and optionally the
+// mosaic volume (M x N x K).
+int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S,
+                 float* vol, hipStream_t st);
+// Gather V (d x d x K x G) from a padded image through S.
+int launch_gather_groups(const float* pad, const uint32_t* S, int M, int N, int d, int d2, int K,
+                         float* V, hipStream_t st);
+
+// Wavelet hard threshold (all levels, all cycle spins) on the mosaic volume.
+struct HtWork {
+  float* vrec;                 // cycleSpin x (M*N*K)
+  std::vector<float*> lo[4];   // lo[l][spin]: level-(l+1) approximation
+  std::vector<float*> rec[4];  // rec[l][spin]: reconstructed level-(l+1) approximation
+};
+size_t ht_work_floats(int M, int N, int K, int levels, int cycleSpin);
+void ht_work_bind(float* base, int M, int N, int K, int levels, int cycleSpin, HtWork* w);
+int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau,
+           const HtWork& w, hipStream_t st);
+
+// Step-1 aggregation prep: Vavg = max(mean over spins of vrec, 0) (or a given
+// Vavg when nspin == 0) and per-group weights 1/(TV(Vavg)+eps).
+int launch_agg1_prep(const float* vrec, int nspin, const float* vavg_in, int M, int N, int K, int d,
+                     float* vavg, float* wgt, hipStream_t st);
+// Vavg = max(mean over spins of vrec, 0), elementwise.
+int launch_combine_spins(const float* vrec, int nspin, size_t vol, float* out, hipStream_t st);
+// Gather aggregation. step 1: blocks from the mosaic Vavg, weight wgt[g] * sine mask (.1,.9);
+// step 2: blocks from V2f (d x d x K x G), weight mask(.2,.8)/gain2[g].
+// masks: device array [2][d*d] (step-1 mask, then step-2 mask).
+// Writes U, W for one match spin in the unshifted frame (shift 'spin').
+int launch_aggregate_m(int step, const float* blocks, const float* wgt, const uint32_t* S, const float* masks,
+                       int M, int N, int d, int d2, int K, int spin, float* U, float* W, hipStream_t st);
+// Step-2 Wiener filter on groups gathered from the padded pilot / noisy images
+// (pads != nullptr) or read from V1/V2 arrays.
+int launch_wiener(const float* pad_pilot, const float* pad_noisy, const uint32_t* S, const float* V1,
+                  const float* V2, int M, int N, int d, int d2, int K, long G, float sigma, float* V2f,
+                  float* gain2, hipStream_t st);
+// out = sum_s U_s / (sum_s W_s + sum_t Wextra_t), cropped to m x n, double math.
+int launch_normalize(const float* U, int nU, const float* W, int nW, const float* Wextra, int nWextra,
+                     int M, int N, int m, int n, float* out, hipStream_t st);
+
+}  // namespace gbm3d

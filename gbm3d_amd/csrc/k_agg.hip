@@ -1,0 +1,242 @@
+// k_agg.hip -- step-1 aggregation weights and the deterministic gather
+// aggregation shared by both steps, plus the spin-sum normalisation.
+//
+// Reference: source/denoiseAndAggregateGPU.m:58-92 (weights 1/(TV+eps) and
+// a sine mask), source/denoiseAndAggregateWieFast.m:68-100 (weight
+// mask/|filt|^2), utilities/getBM3DAggWeights.m:15-19, source/GBM3D.m:57,82.
+//
+// The reference scatters every (tile y, tile x, slot k) block onto a padded
+// canvas in (y outer, x inner, k) order and crops it.  Here one wave owns a
+// 16x16 output tile: it enumerates the candidate (y, x, k) triples in that
+// same order, keeps those whose block meets its tile, and adds each block's
+// pixels into an LDS accumulator one block after another.  Every pixel thus
+// receives its contributions in the reference's order, in FP32, with no
+// atomics: results are bitwise reproducible run to run.
+#include "internal.hpp"
+#include "wave.hpp"
+
+namespace gbm3d {
+
+// -------------------------------------------------------- step-1 prep
+// One 256-thread workgroup per group (tile) g: Vavg = max(mean_s vrec_s, 0)
+// over the tile's d x d x K stack, then w_g = 1/(D + eps) with
+// D = sum|diff3_v|/4 + sum|diff3_u|/4 + sum|diff1_k| (getBM3DAggWeights(3,1)).
+__device__ __forceinline__ float third_diff(float x0, float x1, float x2, float x3) {
+  const float a = x1 - x0, b = x2 - x1, c = x3 - x2;
+  const float e = b - a, f = c - b;
+  return f - e;
+}
+
+__global__ __launch_bounds__(256) void agg1_prep_kernel(const float* __restrict__ vrec, int nspin,
+                                                        const float* __restrict__ vavg_in, int M, int N, int K,
+                                                        int d, float* __restrict__ vavg, float* __restrict__ wgt) {
+  extern __shared__ float sl[];  // two d x d slices, [v][u]
+  __shared__ float red[3][256];
+  const int Mt = M / d;
+  const int g = blockIdx.x;
+  const int ti = g % Mt, tj = g / Mt;
+  const size_t MN = (size_t)M * N, vol = MN * K;
+  const int dd = d * d;
+  float s2 = 0.f, s1 = 0.f, s3 = 0.f;
+  for (int k = 0; k < K; ++k) {
+    float* cur = sl + (k & 1) * dd;
+    const float* prev = sl + ((k + 1) & 1) * dd;
+    for (int e = threadIdx.x; e < dd; e += 256) {
+      const int u = e % d, v = e / d;
+      const size_t o = (size_t)(ti * d + u) + (size_t)M * (tj * d + v) + MN * k;
+      float x;
+      if (nspin > 0) {
+        float acc = 0.f;
+        for (int s = 0; s < nspin; ++s) acc += vrec[vol * s + o];
+        x = acc / (float)nspin;
+        x = x > 0.f ? x : 0.f;  // max(mean(Vrec,4), 0)
+        vavg[o] = x;
+      } else {
+        x = vavg_in[o];
+      }
+      if (k > 0) s3 += fabsf(x - prev[e]);  // |diff(U,1,3)|, same thread wrote prev[e]
+      cur[e] = x;
+    }
+    __syncthreads();
+    // third differences along dim 2 (v) and dim 1 (u) of slice k
+    for (int e = threadIdx.x; e < d * (d - 3); e += 256) {
+      const int u = e % d, v = e / d;
+      const float* p = cur + u + d * v;
+      s2 += fabsf(third_diff(p[0], p[d], p[2 * d], p[3 * d]));
+      const int uu = e % (d - 3), vv = e / (d - 3);
+      const float* q = cur + uu + d * vv;
+      s1 += fabsf(third_diff(q[0], q[1], q[2], q[3]));
+    }
+    __syncthreads();
+  }
+  red[0][threadIdx.x] = s2;
+  red[1][threadIdx.x] = s1;
+  red[2][threadIdx.x] = s3;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float D = red[0][0] * 0.25f;
+    D = D + red[1][0] * 0.25f;
+    D = D + red[2][0];
+    wgt[g] = 1.0f / (D + 2.220446049250313e-16f);  // 1/(D(blocks)+eps)
+  }
+}
+
+int launch_agg1_prep(const float* vrec, int nspin, const float* vavg_in, int M, int N, int K, int d, float* vavg,
+                     float* wgt, hipStream_t st) {
+  if (d < 4 || M % d || N % d) return fail(GBM3D_ERR_ARG, "agg1 prep: block size");
+  const int G = (M / d) * (N / d);
+  const size_t lds = sizeof(float) * 2 * (size_t)d * d;
+  if (G == 0) return GBM3D_OK;
+  hipLaunchKernelGGL(agg1_prep_kernel, dim3(G), dim3(256), lds, st, vrec, nspin, vavg_in, M, N, K, d, vavg, wgt);
+  return check_launch("agg1_prep");
+}
+
+// Vavg = max(mean over spins, 0) elementwise (denoiseAndAggregateGPU.m:45).
+__global__ void combine_spins_kernel(const float* __restrict__ vrec, int nspin, size_t vol, float* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= vol) return;
+  float acc = 0.f;
+  for (int s = 0; s < nspin; ++s) acc += vrec[vol * s + e];
+  const float x = acc / (float)nspin;
+  out[e] = x > 0.f ? x : 0.f;
+}
+
+int launch_combine_spins(const float* vrec, int nspin, size_t vol, float* out, hipStream_t st) {
+  if (vol == 0) return GBM3D_OK;
+  hipLaunchKernelGGL(combine_spins_kernel, dim3((unsigned)((vol + 255) / 256)), dim3(256), 0, st, vrec, nspin, vol, out);
+  return check_launch("combine_spins");
+}
+
+// ---------------------------------------------------------- aggregation
+constexpr int AT = 16;  // output tile edge per wave
+
+// masks: [0, d*d) step-1 sine mask (.1,.9), [d*d, 2*d*d) step-2 mask (.2,.8)
+template <int STEP>
+__global__ __launch_bounds__(256) void aggregate_kernel(const float* __restrict__ blocks,
+                                                        const float* __restrict__ wgt,
+                                                        const uint32_t* __restrict__ S,
+                                                        const float* __restrict__ masks, int M, int N, int d,
+                                                        int d2, int K, int spin, float* __restrict__ U,
+                                                        float* __restrict__ W) {
+  __shared__ float accU[4][AT * AT], accW[4][AT * AT];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tilesR = (M + AT - 1) / AT, tilesC = (N + AT - 1) / AT;
+  const int tile = blockIdx.x * 4 + wave;
+  if (tile >= tilesR * tilesC) return;
+  const int R0 = (tile % tilesR) * AT, C0 = (tile / tilesR) * AT;
+  float* aU = accU[wave];
+  float* aW = accW[wave];
+  for (int e = lane; e < AT * AT; e += 64) {
+    aU[e] = 0.f;
+    aW[e] = 0.f;
+  }
+  wave_sync();
+  const int s = d2 / 2, Mt = M / d, Nt = N / d;
+  const int dd = d * d;
+  // tile rows y whose blocks can reach rows [R0, R0+AT): y*d in [R0+s-d2-d+2, R0+AT-1+s]
+  auto first = [&](int r) {
+    const int x = r + s - d2 - d + 2;
+    return x > 0 ? (x + d - 1) / d : 0;
+  };
+  const int ylo = first(R0), yhi = min(Mt - 1, (R0 + AT - 1 + s) / d);
+  const int xlo = first(C0), xhi = min(Nt - 1, (C0 + AT - 1 + s) / d);
+  const int ny = yhi - ylo + 1, nx = xhi - xlo + 1;
+  const int ncand = (ny > 0 && nx > 0) ? ny * nx * K : 0;
+  const size_t MN = (size_t)M * N;
+  for (int base = 0; base < ncand; base += 64) {
+    // candidate t = ((y - ylo) * nx + (x - xlo)) * K + k: lexicographic (y, x, k)
+    const int t = base + lane;
+    int br = 0, bc = 0, g = 0, k = 0;
+    bool hit = false;
+    if (t < ncand) {
+      k = t % K;
+      const int yx = t / K;
+      const int y = ylo + yx / nx, x = xlo + yx % nx;
+      g = y + Mt * x;
+      const uint32_t idx = S[(size_t)K * g + k] - 1u;
+      br = y * d + (int)(idx % (uint32_t)d2) - s;
+      bc = x * d + (int)(idx / (uint32_t)d2) - s;
+      hit = br < R0 + AT && br + d > R0 && bc < C0 + AT && bc + d > C0;
+    }
+    uint64_t mask = __ballot(hit);
+    while (mask) {
+      const int src = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const int gb = __shfl(g, src, 64), kb = __shfl(k, src, 64);
+      const int rr = __shfl(br, src, 64), cc = __shfl(bc, src, 64);
+      const float wg = wgt[gb];
+      for (int e = lane; e < dd; e += 64) {
+        const int u = e % d, v = e / d;
+        const int pr = rr + u - R0, pc = cc + v - C0;
+        if (pr < 0 || pr >= AT || pc < 0 || pc >= AT) continue;
+        float b;
+        if (STEP == 1) {
+          const int ti = gb % Mt, tj = gb / Mt;
+          b = blocks[(size_t)(ti * d + u) + (size_t)M * (tj * d + v) + MN * kb];
+          const float mk = masks[e];
+          const float c = (wg * b) * mk;  // weight*blocks(:,:,i).*mask in single
+          aU[pr + AT * pc] = aU[pr + AT * pc] + c;
+          aW[pr + AT * pc] = aW[pr + AT * pc] + wg * mk;
+        } else {
+          b = blocks[(size_t)e + (size_t)dd * (kb + (size_t)K * gb)];
+          const float wt = masks[dd + e] / wg;  // w / norm(filt)^2
+          aU[pr + AT * pc] = aU[pr + AT * pc] + wt * b;
+          aW[pr + AT * pc] = aW[pr + AT * pc] + wt;
+        }
+      }
+      wave_sync();
+    }
+  }
+  // write the tile to the unshifted frame (circshift back by -spin)
+  for (int e = lane; e < AT * AT; e += 64) {
+    const int pr = R0 + (e % AT), pc = C0 + (e / AT);
+    if (pr >= M || pc >= N) continue;
+    const size_t o = (size_t)pmod_i(pr - spin, M) + (size_t)M * pmod_i(pc - spin, N);
+    U[o] = aU[e];
+    W[o] = aW[e];
+  }
+}
+
+int launch_aggregate_m(int step, const float* blocks, const float* wgt, const uint32_t* S, const float* masks, int M,
+                       int N, int d, int d2, int K, int spin, float* U, float* W, hipStream_t st) {
+  if (M % d || N % d) return fail(GBM3D_ERR_ARG, "aggregate: size");
+  const int tiles = ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
+  dim3 grid((tiles + 3) / 4);
+  if (step == 1)
+    hipLaunchKernelGGL(aggregate_kernel<1>, grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d, d2, K, spin, U, W);
+  else
+    hipLaunchKernelGGL(aggregate_kernel<2>, grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d, d2, K, spin, U, W);
+  return check_launch("aggregate");
+}
+
+// --------------------------------------------------------- normalisation
+// out(r,c) = sum_s U_s(r,c) / (sum_s W_s(r,c) + sum_t Wx_t(r,c)), double
+// arithmetic (sum(U,3)./sum(W,3) of source/GBM3D.m:57,82), cropped to m x n.
+__global__ void normalize_kernel(const float* __restrict__ U, int nU, const float* __restrict__ W, int nW,
+                                 const float* __restrict__ Wx, int nWx, int M, int N, int m, int n,
+                                 float* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= m) return;
+  const size_t MN = (size_t)M * N, o = (size_t)r + (size_t)M * c;
+  double a = 0.0, b = 0.0;
+  for (int s = 0; s < nU; ++s) a += (double)U[MN * s + o];
+  for (int s = 0; s < nW; ++s) b += (double)W[MN * s + o];
+  for (int s = 0; s < nWx; ++s) b += (double)Wx[MN * s + o];
+  out[(size_t)r + (size_t)m * c] = (float)(a / b);
+}
+
+int launch_normalize(const float* U, int nU, const float* W, int nW, const float* Wextra, int nWextra, int M, int N,
+                     int m, int n, float* out, hipStream_t st) {
+  if (m > M || n > N) return fail(GBM3D_ERR_ARG, "normalize: crop");
+  dim3 grid((m + 255) / 256, n);
+  hipLaunchKernelGGL(normalize_kernel, grid, dim3(256), 0, st, U, nU, W, nW, Wextra, nWextra, M, N, m, n, out);
+  return check_launch("normalize");
+}
+
+}  // namespace gbm3d

@@ -1,0 +1,77 @@
+// wave.hpp -- wavefront (64-lane) primitives for gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gbm3d {
+
+__device__ __forceinline__ int pmod_i(int a, int m) {
+  int r = a % m;
+  return r < 0 ? r + m : r;
+}
+
+// Make LDS writes of this wave visible to its other lanes (a wave executes in
+// lockstep; the fences stop the compiler from moving LDS accesses across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t shfl_xor_u32(uint32_t v, int m) {
+  return (uint32_t)__shfl_xor((int)v, m, 64);
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = shfl_xor_u32((uint32_t)v, m);
+  uint32_t hi = shfl_xor_u32((uint32_t)(v >> 32), m);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t shfl_xor_any(uint32_t v, int m) { return shfl_xor_u32(v, m); }
+__device__ __forceinline__ uint64_t shfl_xor_any(uint64_t v, int m) { return shfl_xor_u64(v, m); }
+
+// Ascending bitonic sort of one value per lane across the 64 lanes.
+template <class T>
+__device__ __forceinline__ T wave_sort64(T v, int lane) {
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      T p = shfl_xor_any(v, j);
+      bool asc = (lane & k) == 0;
+      bool low = (lane & j) == 0;
+      T mn = v < p ? v : p;
+      T mx = v < p ? p : v;
+      v = (asc == low) ? mn : mx;
+    }
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    uint64_t p = shfl_xor_u64(v, o);
+    v = p < v ? p : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int wave_excl_scan(int x, int lane) {
+  int incl = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  return incl - x;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace gbm3d

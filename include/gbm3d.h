@@ -1,0 +1,183 @@
+/*
+ * gbm3d.h -- C ABI of libgbm3d.so, the MI355X (gfx950) implementation of the
+ * GBM3D two-step denoiser (block matching -> global 3-D wavelet hard
+ * threshold -> aggregation -> re-matching -> empirical Wiener -> aggregation).
+ *
+ * Conventions shared by every entry point:
+ *   - Images are single precision, COLUMN-MAJOR (MATLAB layout): element
+ *     (r, c) of an m x n image lives at r + c*m.
+ *   - Block-match index sets S use the reference's encoding: a 1-based
+ *     column-major index into the matchSize x matchSize search window,
+ *     idx = a + matchSize*b + 1 with (a, b) the 0-based window offset; the
+ *     block's image offset from its reference tile is (a - matchSize/2,
+ *     b - matchSize/2).  S is stored K x Mt x Nt column-major (the cell array
+ *     S{i,j} of uint32 K-vectors, flattened).
+ *   - Every function returns 0 on success or a gbm3d_status code; the message
+ *     of the last failure on the calling thread is gbm3d_last_error().  No
+ *     function throws or exits across the ABI.
+ *   - Host-buffer entry points copy in, compute on the context's stream and
+ *     copy out (synchronous).  *_device entry points take device pointers and
+ *     are asynchronous on the context's stream.
+ */
+#ifndef GBM3D_H
+#define GBM3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBM3D_MAX_SPINS 16
+/* Field value meaning "absent" (MATLAB isfield(opts, f) == false). */
+#define GBM3D_UNSET (-2147483647 - 1)
+
+enum gbm3d_status {
+  GBM3D_OK = 0,
+  GBM3D_ERR_ARG = 1,           /* bad argument (size, NULL, range)             */
+  GBM3D_ERR_UNSUPPORTED = 2,   /* option combination this build does not run   */
+  GBM3D_ERR_HIP = 3,           /* HIP runtime failure                          */
+  GBM3D_ERR_NOMEM = 4,         /* device allocation failed                     */
+  GBM3D_ERR_MISSING_FIELD = 5  /* a required opts field is absent              */
+};
+
+enum gbm3d_filt { GBM3D_FILT_HT = 0, GBM3D_FILT_MEDIAN = 1 };
+
+/*
+ * Mirror of the reference's opts struct (utilities/setBM3Dopts.m:19-34,
+ * utilities/checkBM3Dopts.m:10-24).  Integer fields set to GBM3D_UNSET and
+ * spin lists with n* == GBM3D_UNSET are "absent".  profile[0] == 0 means no
+ * profile field; a non-empty profile replaces the whole struct
+ * (source/GBM3D.m:16-18).  wname/wnamez/order/orderz exist in the reference
+ * but are never read on the GBM3D path (source/GBM3D.m:38-40), so they are
+ * not carried.
+ */
+typedef struct gbm3d_opts {
+  char profile[16];
+  int32_t numMax, numMin;
+  int32_t levels, cycleSpin;
+  int32_t nMatchSpins;
+  int32_t matchSpins[GBM3D_MAX_SPINS];
+  int32_t nMatchSpinsWie;
+  int32_t matchSpinsWie[GBM3D_MAX_SPINS];
+  int32_t tauMode;
+  int32_t blockSize, blockSizeWie, matchSize;
+  int32_t Wiener;   /* 0 / 1 */
+  int32_t filtType; /* enum gbm3d_filt */
+} gbm3d_opts;
+
+/* ---------------------------------------------------------------- options */
+
+/* Every field absent (an empty MATLAB struct). */
+int gbm3d_opts_clear(gbm3d_opts* opts);
+
+/* opts = setBM3Dopts(profile)   -- utilities/setBM3Dopts.m:1-86.
+ * Unknown profile names fall back to 'default' (:12-14). */
+int gbm3d_set_opts(const char* profile, gbm3d_opts* opts);
+
+/* opts = checkBM3Dopts(opts)    -- utilities/checkBM3Dopts.m:10-24.
+ * Fills absent fields with the reference defaults in place. */
+int gbm3d_check_opts(gbm3d_opts* opts);
+
+/* The option handling at the top of GBM3D: opts == NULL behaves as
+ * nargin<3 (profile 'default'); a profile replaces the struct; then
+ * checkBM3Dopts.  source/GBM3D.m:15-19. */
+int gbm3d_resolve_opts(const gbm3d_opts* in, gbm3d_opts* out);
+
+/* tau = sigma*getTau(levels, mode): levels x 7, column-major
+ * (tau[i + levels*j], i = level-1, j = subband-1). utilities/getTau.m:9-51. */
+int gbm3d_get_tau(int levels, int mode, double sigma, double* tau);
+
+/* Thread-local message of the last failure ("" when none). */
+const char* gbm3d_last_error(void);
+
+/* ---------------------------------------------------------------- context */
+
+typedef struct gbm3d_ctx gbm3d_ctx;
+
+/* One context per (device, stream).  Device workspace is owned by the
+ * context, grows on demand and is released by gbm3d_destroy. */
+int gbm3d_create(int device, gbm3d_ctx** ctx);
+int gbm3d_destroy(gbm3d_ctx* ctx);
+/* Use a caller-owned hipStream_t (NULL = the context's own stream). */
+int gbm3d_set_stream(gbm3d_ctx* ctx, void* hip_stream);
+int gbm3d_synchronize(gbm3d_ctx* ctx);
+
+/* --------------------------------------------------------------- hot path */
+
+/*
+ * [U, out] = GBM3D(I, sigma, opts)            -- source/GBM3D.m:1-84
+ * I: m x n input (host), U: m x n output, U1 (nullable): m x n pilot
+ * (out.U1, source/GBM3D.m:81).  opts may be NULL (profile 'default').
+ */
+int gbm3d_denoise(gbm3d_ctx* ctx, const float* I, int m, int n, float sigma,
+                  const gbm3d_opts* opts, float* U, float* U1);
+
+/* Same on device buffers, asynchronous on the context's stream. */
+int gbm3d_denoise_device(gbm3d_ctx* ctx, const float* dI, int m, int n,
+                         float sigma, const gbm3d_opts* opts, float* dU,
+                         float* dU1);
+
+/* ------------------------------------------------- stage entry points
+ * Each replaces one reference stage for stage-isolated use; m, n are the
+ * (already block-multiple) image sizes the stage sees. */
+
+/* [S, Volume] = matchBlocksCPU(I, sigma, opts)   -- source/matchBlocksCPU.m:1-116
+ * (numMax == numMin branch).  d = blockSize, d2 = matchSize, K = numMax.
+ * S: K x (m/d) x (n/d) uint32; Volume (nullable): m x n x K. */
+int gbm3d_match(gbm3d_ctx* ctx, const float* I, int m, int n, int d, int d2,
+                int K, uint32_t* S, float* Volume);
+
+/* Vavg = the 'ht' filter of denoiseAndAggregateGPU (cycle-spun 3-D
+ * bior1.5 x Haar hard threshold, mean over spins, clip at 0)
+ * -- source/denoiseAndAggregateGPU.m:19-47.  tau: levels x 7 (see
+ * gbm3d_get_tau).  Volume, Vavg: m x n x K. */
+int gbm3d_ht_filter(gbm3d_ctx* ctx, const float* Volume, int m, int n, int K,
+                    int levels, int cycleSpin, const double* tau, float* Vavg);
+
+/* [U, W] = aggregation of denoiseAndAggregateGPU   -- :59-92.
+ * U, W: m x n (crop of the padded canvas). */
+int gbm3d_aggregate_ht(gbm3d_ctx* ctx, const float* Vavg, const uint32_t* S,
+                       int m, int n, int d, int d2, int K, float* U, float* W);
+
+/* [S, V1, V2] = matchBlocksWie(I, In, sigma, opts) -- source/matchBlocksWie.m:1-92
+ * I = pilot, In = noisy.  V1, V2 (nullable): d x d x K x (m/d) x (n/d). */
+int gbm3d_match_wie(gbm3d_ctx* ctx, const float* I, const float* In, int m,
+                    int n, int d, int d2, int K, uint32_t* S, float* V1,
+                    float* V2);
+
+/* Filter part of denoiseAndAggregateWieFast: 2-D DCT x Haar-K empirical
+ * Wiener, clip at 0 -- source/denoiseAndAggregateWieFast.m:19-66.
+ * V1, V2, V2f: d x d x K x G.  gain2 (G): squared Frobenius norm of each
+ * group's Wiener gains (the aggregation weight denominator, :81). */
+int gbm3d_wiener(gbm3d_ctx* ctx, const float* V1, const float* V2, int d,
+                 int K, int G, float sigma, float* V2f, float* gain2);
+
+/* [U, W] = aggregation of denoiseAndAggregateWieFast -- :68-100. */
+int gbm3d_aggregate_wie(gbm3d_ctx* ctx, const float* V2f, const float* gain2,
+                        const uint32_t* S, int m, int n, int d, int d2, int K,
+                        float* U, float* W);
+
+/* ------------------------------------------------------- instrumentation
+ * Per-stage device time of the last gbm3d_denoise* call, measured with HIP
+ * events on the stream the kernels ran on (enabled by gbm3d_set_timing). */
+enum gbm3d_stage {
+  GBM3D_STAGE_PAD = 0,      /* padded search images + block-norm maps   */
+  GBM3D_STAGE_MATCH1 = 1,   /* step-1 distances + top-K + volume        */
+  GBM3D_STAGE_HT = 2,       /* step-1 wavelet hard threshold (all levels) */
+  GBM3D_STAGE_AGG1 = 3,     /* step-1 weights + gather aggregation      */
+  GBM3D_STAGE_MATCH2 = 4,   /* step-2 distances + top-K                 */
+  GBM3D_STAGE_WIENER = 5,   /* step-2 Wiener filter                     */
+  GBM3D_STAGE_AGG2 = 6,     /* step-2 gather aggregation                */
+  GBM3D_STAGE_NORM = 7,     /* spin sums / normalisation / crop         */
+  GBM3D_NUM_STAGES = 8
+};
+int gbm3d_set_timing(gbm3d_ctx* ctx, int enable);
+/* ms[GBM3D_NUM_STAGES], launches[GBM3D_NUM_STAGES] (either may be NULL). */
+int gbm3d_get_stage_times(gbm3d_ctx* ctx, double* ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GBM3D_H */

@@ -1,0 +1,230 @@
+"""ctypes wrapper of the oracle library (oracle/liboracle_gbm3d.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker / CPU baseline, never by the
+product path (gbm3d_amd/ does not import this package).
+
+Arrays are numpy arrays with MATLAB semantics: an m x n image is a numpy array
+of shape (m, n) and is passed column-major; volumes are (m, n, K); S is
+(K, Mt, Nt) uint32 with 1-based window indices.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from gbm3d_amd._abi import Opts, opts_to_struct, struct_to_opts
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_gbm3d.so")
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f(a, dt):
+    return np.asfortranarray(a, dtype=dt)
+
+
+def _check(st):
+    if st != 0:
+        raise RuntimeError(f"oracle status {st}")
+
+
+def _dt(precision):
+    return (np.float64, "f64") if precision == 64 else (np.float32, "f32")
+
+
+def set_threads(n: int) -> int:
+    return lib().orc_set_threads(C.c_int(n))
+
+
+def resolve_opts(opts: dict | None) -> dict:
+    o = Opts()
+    src = opts_to_struct(opts)
+    lib().orc_resolve_opts(C.byref(src) if src is not None else None, C.byref(o))
+    return struct_to_opts(o)
+
+
+def set_opts(profile: str) -> dict:
+    o = Opts()
+    lib().orc_set_opts(C.c_char_p(profile.encode()), C.byref(o))
+    return struct_to_opts(o)
+
+
+def check_opts(opts: dict) -> dict:
+    o = opts_to_struct(opts)
+    lib().orc_check_opts(C.byref(o))
+    return struct_to_opts(o)
+
+
+def get_tau(levels: int, mode: int, sigma: float) -> np.ndarray:
+    t = np.zeros((levels, 7), dtype=np.float64, order="F")
+    _check(lib().orc_get_tau(C.c_int(levels), C.c_int(mode), C.c_double(sigma), _p(t)))
+    return t
+
+
+def mirror_pad(I, M, N):
+    I = _f(I, np.float64)
+    out = np.zeros((M, N), dtype=np.float64, order="F")
+    _check(lib().orc_mirror_pad_f64(_p(I), I.shape[0], I.shape[1], M, N, _p(out)))
+    return out
+
+
+def search_pad(X, d2):
+    X = _f(X, np.float64)
+    m, n = X.shape
+    out = np.zeros((m + d2, n + d2), dtype=np.float64, order="F")
+    _check(lib().orc_search_pad_f64(_p(X), m, n, d2, _p(out)))
+    return out
+
+
+def match(I, d, d2, K, precision=64, want_dist=False):
+    """matchBlocksCPU -> (S (K,Mt,Nt) uint32, Volume (m,n,K), dist (d2*d2,Mt,Nt) or None)."""
+    dt, suf = _dt(precision)
+    I = _f(I, dt)
+    m, n = I.shape
+    Mt, Nt = m // d, n // d
+    S = np.zeros((K, Mt, Nt), dtype=np.uint32, order="F")
+    V = np.zeros((m, n, K), dtype=dt, order="F")
+    dist = np.zeros((d2 * d2, Mt, Nt), dtype=dt, order="F") if want_dist else None
+    fn = getattr(lib(), "orc_match_" + suf)
+    _check(fn(_p(I), m, n, d, d2, K, _p(S), _p(V), _p(dist)))
+    return S, V, dist
+
+
+def match_wie(I, In, d, d2, K, precision=64, want_dist=False):
+    """matchBlocksWie -> (S, V1, V2, dist); V: (d, d, K, Mt, Nt)."""
+    dt, suf = _dt(precision)
+    I = _f(I, dt)
+    In = _f(In, dt)
+    m, n = I.shape
+    Mt, Nt = m // d, n // d
+    S = np.zeros((K, Mt, Nt), dtype=np.uint32, order="F")
+    V1 = np.zeros((d, d, K, Mt, Nt), dtype=dt, order="F")
+    V2 = np.zeros_like(V1)
+    dist = np.zeros((d2 * d2, Mt, Nt), dtype=dt, order="F") if want_dist else None
+    fn = getattr(lib(), "orc_match_wie_" + suf)
+    _check(fn(_p(I), _p(In), m, n, d, d2, K, _p(S), _p(V1), _p(V2), _p(dist)))
+    return S, V1, V2, dist
+
+
+def ht_filter(V, levels, cycle_spin, tau, precision=64, want_margin=False):
+    dt, suf = _dt(precision)
+    V = _f(V, dt)
+    m, n, K = V.shape
+    out = np.zeros_like(V)
+    tau = np.asfortranarray(tau, dtype=np.float64)
+    mm = C.c_double(0.0)
+    fn = getattr(lib(), "orc_ht_filter_" + suf)
+    _check(fn(_p(V), m, n, K, levels, cycle_spin, _p(tau), _p(out), C.byref(mm) if want_margin else None))
+    return (out, mm.value) if want_margin else out
+
+
+def agg_weights(Vavg, d, precision=64):
+    dt, suf = _dt(precision)
+    Vavg = _f(Vavg, dt)
+    m, n, K = Vavg.shape
+    w = np.zeros((m // d, n // d), dtype=dt, order="F")
+    _check(getattr(lib(), "orc_agg_weights_" + suf)(_p(Vavg), m, n, d, K, _p(w)))
+    return w
+
+
+def aggregate_ht(Vavg, S, d, d2, precision=64, gather=True):
+    dt, suf = _dt(precision)
+    Vavg = _f(Vavg, dt)
+    S = np.asfortranarray(S, dtype=np.uint32)
+    m, n, K = Vavg.shape
+    U = np.zeros((m, n), dtype=dt, order="F")
+    W = np.zeros_like(U)
+    _check(getattr(lib(), "orc_agg_ht_" + suf)(_p(Vavg), _p(S), m, n, d, d2, K, _p(U), _p(W), int(gather)))
+    return U, W
+
+
+def wiener(V1, V2, sigma, precision=64):
+    dt, suf = _dt(precision)
+    V1 = _f(V1, dt)
+    V2 = _f(V2, dt)
+    d, K = V1.shape[0], V1.shape[2]
+    G = V1.size // (d * d * K)
+    V2f = np.zeros_like(V2)
+    g2 = np.zeros(V1.shape[3:], dtype=dt, order="F")
+    _check(getattr(lib(), "orc_wiener_" + suf)(_p(V1), _p(V2), d, K, C.c_long(G), C.c_double(sigma), _p(V2f), _p(g2)))
+    return V2f, g2
+
+
+def aggregate_wie(V2f, gain2, S, m, n, d2, precision=64, gather=True):
+    dt, suf = _dt(precision)
+    V2f = _f(V2f, dt)
+    gain2 = _f(gain2, dt)
+    S = np.asfortranarray(S, dtype=np.uint32)
+    d, K = V2f.shape[0], V2f.shape[2]
+    U = np.zeros((m, n), dtype=dt, order="F")
+    W = np.zeros_like(U)
+    _check(getattr(lib(), "orc_agg_wie_" + suf)(_p(V2f), _p(gain2), _p(S), m, n, d, d2, K, _p(U), _p(W), int(gather)))
+    return U, W
+
+
+def wav_dec_level(X):
+    X = _f(X, np.float64)
+    p, q, r = X.shape
+    C8 = np.zeros((p // 2, q // 2, r // 2, 8), dtype=np.float64, order="F")
+    _check(lib().orc_wav_dec_level_f64(_p(X), p, q, r, _p(C8)))
+    return C8
+
+
+def wav_rec_level(C8):
+    C8 = _f(C8, np.float64)
+    p, q, r = 2 * C8.shape[0], 2 * C8.shape[1], 2 * C8.shape[2]
+    X = np.zeros((p, q, r), dtype=np.float64, order="F")
+    _check(lib().orc_wav_rec_level_f64(_p(C8), p, q, r, _p(X)))
+    return X
+
+
+def haar_dec(x, levels):
+    x = np.array(x, dtype=np.float64)
+    lib().orc_haar_dec_f64(_p(x), x.size, levels)
+    return x
+
+
+def haar_rec(x, levels):
+    x = np.array(x, dtype=np.float64)
+    lib().orc_haar_rec_f64(_p(x), x.size, levels)
+    return x
+
+
+def gbm3d(I, sigma, opts=None, precision=64, nthreads=0):
+    """[U, out.U1] = GBM3D(I, sigma, opts) on the CPU restatement."""
+    dt, suf = _dt(precision)
+    I = _f(I, dt)
+    m, n = I.shape
+    o = Opts()
+    src = opts_to_struct(opts)
+    lib().orc_resolve_opts(C.byref(src) if src is not None else None, C.byref(o))
+    U = np.zeros((m, n), dtype=dt, order="F")
+    U1 = np.zeros_like(U)
+    if precision == 64:
+        st = lib().orc_gbm3d_f64(_p(I), m, n, C.c_double(sigma), C.byref(o), _p(U), _p(U1))
+    else:
+        st = lib().orc_gbm3d_f32(_p(I), m, n, C.c_float(sigma), C.byref(o), _p(U), _p(U1), nthreads)
+    _check(st)
+    return U, U1

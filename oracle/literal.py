@@ -1,0 +1,394 @@
+"""Literal numpy transcription of the reference MATLAB path (FFT formulation).
+
+TEST INFRASTRUCTURE ONLY (a pin for the C++ oracle, not a product path).
+
+The C++ oracle (gbm3d_oracle.hpp) restates the algorithm with direct index
+formulas (separable circular filters, direct block sums).  This module follows
+the reference's own formulation instead -- fftn of zero-padded filters,
+2x2-tiled + circshifted search images, fft2 correlations, ind2sub, 1-based
+slicing turned into 0-based -- so that an index, sign or alignment slip in
+either formulation makes the two disagree.  Everything runs in float64 and is
+meant for small inputs (tens of pixels).
+
+MATLAB -> numpy: A(1:2:end) -> A[::2]; circshift(A,[s,s]) -> np.roll(A,(s,s),(0,1));
+fftn(P, siz) zero-pads/truncates like np.fft.fftn(P, s=siz); column-major
+linear indices via order='F'.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.fft
+
+CA = 0.016572815184060
+CB = 0.121533978016438
+CC = np.sqrt(2) / 2
+
+
+def getWaveFilters3Dbior():
+    """waveTransforms3D/getWaveFilters3Dbior.m:12-58 -> Psi, Psi2 (10,10,2,8)."""
+    lo = np.array([CA, -CA, -CB, CB, CC, CC, CB, -CB, -CA, CA])
+    hi = np.zeros(10)
+    hi[4] = -CC
+    hi[5] = CC
+    lo_d = np.array([CC, CC])
+    hi_d = np.array([-CC, CC])
+
+    def build(Lo, Hi):
+        P = np.zeros((10, 10, 2, 8))
+        combos = [(Lo, Lo, lo_d), (Hi, Lo, lo_d), (Lo, Hi, lo_d), (Lo, Lo, hi_d),
+                  (Hi, Hi, lo_d), (Hi, Lo, hi_d), (Lo, Hi, hi_d), (Hi, Hi, hi_d)]
+        for j, (a, b, z) in enumerate(combos):
+            P[:, :, :, j] = np.outer(a, b)[:, :, None] * z[None, None, :]
+        return P
+
+    Psi = build(lo, hi)
+    Lo2 = np.abs(hi)
+    Hi2 = np.array([-CA, -CA, CB, CB, -CC, CC, -CB, -CB, CA, CA])
+    Psi2 = build(Lo2, Hi2)
+    return Psi, Psi2
+
+
+def myWavDec3DFFT(Psi, levels, U):
+    """waveTransforms3D/myWavDec3DFFT.m:10-23.  C[i][j] (j = 0..6), C[levels][0]."""
+    C = [[None] * 7 for _ in range(levels + 1)]
+    for i in range(levels):
+        FU = np.fft.fftn(U)
+        for j in range(7):
+            c = np.fft.ifftn(FU * np.fft.fftn(Psi[:, :, :, j + 1], s=U.shape))
+            C[i][j] = np.real(c[::2, ::2, ::2])
+        U = np.real(np.fft.ifftn(FU * np.fft.fftn(Psi[:, :, :, 0], s=U.shape)))
+        U = U[::2, ::2, ::2]
+    C[levels][0] = U
+    return C
+
+
+def myWavRec3DGPUfast(Psi, levels, C):
+    """waveTransforms3D/myWavRec3DGPUfast.m:10-40."""
+    Cup = [None] * (levels + 1)
+    for i in range(levels):
+        sh = tuple(2 * s for s in C[i][0].shape)
+        Cup[i] = np.zeros(sh)
+        tmp = np.zeros(sh)
+        for j in range(7):
+            tmp[::2, ::2, ::2] = C[i][j]
+            Cup[i] = Cup[i] + np.real(np.fft.ifftn(np.fft.fftn(tmp) * np.conj(np.fft.fftn(Psi[:, :, :, j + 1], s=sh))))
+    Cup[levels] = C[levels][0]
+    for i in range(levels, 0, -1):
+        tmp = np.zeros(Cup[i - 1].shape)
+        tmp[::2, ::2, ::2] = Cup[i]
+        FPsi = np.conj(np.fft.fftn(Psi[:, :, :, 0], s=tmp.shape))
+        Cup[i - 1] = Cup[i - 1] + np.real(np.fft.ifftn(np.fft.fftn(tmp) * FPsi))
+    return Cup[0]
+
+
+def getTau(levels, mode):
+    """utilities/getTau.m:9-51 (mode 1 only is used on the path)."""
+    tau = np.zeros((levels, 7))
+    for i in range(levels):
+        for j in range(7):
+            base = 3.3 - 0.3 * i
+            if mode == 1:
+                tau[i, j] = base
+            elif mode == 2:
+                tau[i, j] = base if j < 3 else (base + 0.5 if j < 6 else base + 1)
+            elif mode == 3:
+                tau[i, j] = base if j < 3 else (base + 0.25 if j < 6 else base + 1.5)
+            elif mode == 4:
+                tau[i, j] = 2.9 if j < 3 else (2.9 + 0.5 if j < 6 else 2.9 + 1)
+    return tau
+
+
+def ht_filter(Volume, tau, levels, cycleSpin):
+    """source/denoiseAndAggregateGPU.m:19-47 ('ht' branch)."""
+    Psi, Psi2 = getWaveFilters3Dbior()
+    Vrec = []
+    for spin in range(cycleSpin):
+        V = np.roll(Volume, (spin, spin, spin), axis=(0, 1, 2))
+        C = myWavDec3DFFT(Psi, levels, V)
+        for i in range(levels):
+            for j in range(7):
+                C[i][j] = C[i][j] * (np.abs(C[i][j]) > tau[i, j])
+        R = myWavRec3DGPUfast(Psi2, levels, C)
+        Vrec.append(np.roll(R, (-spin, -spin, -spin), axis=(0, 1, 2)))
+    return np.maximum(np.mean(np.stack(Vrec, axis=3), axis=3), 0)
+
+
+def _ind2sub(d2, idx1):
+    """[a,b] = ind2sub([d2,d2], idx) (1-based in and out)."""
+    i0 = int(idx1) - 1
+    return i0 % d2 + 1, i0 // d2 + 1
+
+
+def _mink_idx(x, k):
+    """[~, idx] = mink(x, k): ascending, NaN last, ties by position (1-based)."""
+    key = np.where(np.isnan(x), np.inf, x)
+    nan_last = np.isnan(x).astype(np.int8)
+    order = np.lexsort((np.arange(x.size), key, nan_last))
+    return order[:k] + 1
+
+
+def _search_pad(I, d2):
+    m, n = I.shape
+    Ipad = np.zeros((2 * m, 2 * n))
+    Ipad[:m, :n] = I
+    Ipad[:m, n:] = I
+    Ipad[m:, :n] = I
+    Ipad[m:, n:] = I
+    Ipad = Ipad[:m + d2, :n + d2]
+    return np.roll(Ipad, (d2 // 2, d2 // 2), axis=(0, 1))
+
+
+def _corr_fft(a, b_shape_src, shape):
+    """real(ifft2(conj(fft2(a, shape)) .* fft2(b)))."""
+    return np.real(np.fft.ifft2(np.conj(np.fft.fft2(a, s=shape)) * np.fft.fft2(b_shape_src)))
+
+
+def matchBlocksCPU(I, d, d2, K):
+    """source/matchBlocksCPU.m:25-111 (numMax == numMin).  S: list of lists."""
+    m, n = I.shape
+    M, N = m // d, n // d
+    Ipad = _search_pad(I, d2)
+    g = np.ones((d, d))
+    blockNorms = _corr_fft(g, I ** 2, (m, n))
+    blockNormsPad = _corr_fft(g, Ipad ** 2, (m + d2, n + d2))
+    Volume = np.zeros((m, n, K))
+    S = [[None] * N for _ in range(M)]
+    dists = {}
+    for i in range(1, M + 1):
+        indI = (i - 1) * d
+        for j in range(1, N + 1):
+            indJ = (j - 1) * d
+            xR = I[(i - 1) * d:i * d, (j - 1) * d:j * d]
+            Itmp = Ipad[indI:indI + d + d2, indJ:indJ + d + d2]
+            bd = _corr_fft(xR, Itmp, (d + d2, d + d2))
+            nb = blockNorms[(i - 1) * d, (j - 1) * d]
+            bd = (nb + blockNormsPad[indI:indI + d2, indJ:indJ + d2] - 2 * bd[:d2, :d2]) / nb
+            dists[(i - 1, j - 1)] = bd
+            s = _mink_idx(bd.flatten(order="F"), K)
+            s[0] = d2 * d2 // 2 + d2 // 2 + 1
+            S[i - 1][j - 1] = s
+            for k in range(K):
+                a, b = _ind2sub(d2, s[k])
+                Volume[(i - 1) * d:i * d, (j - 1) * d:j * d, k] = Itmp[a - 1:a - 1 + d, b - 1:b - 1 + d]
+    return S, Volume, dists
+
+
+def matchBlocksWie(I, In, d, d2, K):
+    """source/matchBlocksWie.m:24-89.  V1, V2: (d, d, K, M, N)."""
+    m, n = I.shape
+    M, N = m // d, n // d
+    Ipad = _search_pad(I, d2)
+    Ipad2 = _search_pad(In, d2)
+    g = np.zeros((m, n))
+    g[:d, :d] = 1
+    blockNorms = _corr_fft(g, I ** 2, (m, n))
+    gp = np.zeros((d, d)) + 1
+    blockNormsPad = _corr_fft(gp, Ipad ** 2, (m + d2, n + d2))
+    V1 = np.zeros((d, d, K, M, N))
+    V2 = np.zeros_like(V1)
+    S = [[None] * N for _ in range(M)]
+    for i in range(1, M + 1):
+        indI = (i - 1) * d
+        for j in range(1, N + 1):
+            indJ = (j - 1) * d
+            xR = I[(i - 1) * d:i * d, (j - 1) * d:j * d]
+            Itmp = Ipad[indI:indI + d + d2, indJ:indJ + d + d2]
+            Itmp2 = Ipad2[indI:indI + d + d2, indJ:indJ + d + d2]
+            bd = _corr_fft(xR, Itmp, (d + d2, d + d2))
+            nb = blockNorms[(i - 1) * d, (j - 1) * d]
+            bd = (nb + blockNormsPad[indI:indI + d2, indJ:indJ + d2] - 2 * bd[:d2, :d2]) / nb
+            s = _mink_idx(bd.flatten(order="F"), K)
+            s[0] = d2 * d2 // 2 + d2 // 2 + 1
+            S[i - 1][j - 1] = s
+            for k in range(K):
+                a, b = _ind2sub(d2, s[k])
+                V1[:, :, k, i - 1, j - 1] = Itmp[a - 1:a - 1 + d, b - 1:b - 1 + d]
+                V2[:, :, k, i - 1, j - 1] = Itmp2[a - 1:a - 1 + d, b - 1:b - 1 + d]
+    return S, V1, V2
+
+
+def getBM3DAggWeights(U):
+    """utilities/getBM3DAggWeights.m (kx = 3, kz = 1)."""
+    W = np.sum(np.abs(np.diff(U, 3, axis=1))) * 2.0 ** (1 - 3)
+    W = W + np.sum(np.abs(np.diff(U, 3, axis=0))) * 2.0 ** (1 - 3)
+    W = W + np.sum(np.abs(np.diff(U, 1, axis=2))) * 2.0 ** (1 - 1)
+    return W
+
+
+def _linspace(a, b, n):
+    y = a + np.arange(n) * ((b - a) / (n - 1))
+    y[-1] = b
+    return y
+
+
+def aggregate_ht(Vavg, S, d, d2):
+    """source/denoiseAndAggregateGPU.m:58-92."""
+    M, N = len(S), len(S[0])
+    m, n = M * d, N * d
+    U = np.zeros((m + d2, n + d2))
+    W = np.zeros((m + d2, n + d2))
+    s = np.sin(np.pi * _linspace(0.1, 0.9, d))
+    mask = np.outer(s, s)
+    for y in range(1, M + 1):
+        for x in range(1, N + 1):
+            Nm = len(S[y - 1][x - 1])
+            blocks = Vavg[(y - 1) * d:y * d, (x - 1) * d:x * d, :Nm]
+            weight = 1 / (getBM3DAggWeights(blocks) + np.finfo(float).eps)
+            for i in range(Nm):
+                a, b = _ind2sub(d2, S[y - 1][x - 1][i])
+                a = a + (y - 1) * d
+                b = b + (x - 1) * d
+                U[a - 1:a - 1 + d, b - 1:b - 1 + d] += weight * blocks[:, :, i] * mask
+                W[a - 1:a - 1 + d, b - 1:b - 1 + d] += weight * mask
+    h = d2 // 2
+    return U[h:m + h, h:n + h], W[h:m + h, h:n + h]
+
+
+def myWavDec1D(Psi, Phi, levels, U):
+    """waveTransforms3D/myWavDec1D.m (dim = 3) on a (p,q,r,...) array."""
+    C = np.zeros(U.shape)
+    cnt = 0
+    for _ in range(levels):
+        r = U.shape[2]
+        FU = np.fft.fft(U, r, axis=2)
+        dd = np.fft.ifft(FU * np.fft.fft(Psi, r).reshape(1, 1, r, *([1] * (U.ndim - 3))), r, axis=2)
+        C[:, :, cnt:cnt + r // 2] = np.real(dd[:, :, ::2])
+        cnt += r // 2
+        U = np.real(np.fft.ifft(FU * np.fft.fft(Phi, r).reshape(1, 1, r, *([1] * (U.ndim - 3))), r, axis=2))
+        U = U[:, :, ::2]
+    C[:, :, cnt:] = U
+    return C
+
+
+def myWavRec1D(Psi, Phi, levels, C):
+    """waveTransforms3D/myWavRec1D.m (dim = 3)."""
+    r = C.shape[2]
+    Cup = [None] * (levels + 1)
+    cnt = 0
+    for i in range(levels):
+        R = r // 2 ** i
+        Cup[i] = np.zeros(C.shape[:2] + (R,) + C.shape[3:])
+        Cup[i][:, :, ::2] = C[:, :, cnt:cnt + r // 2 ** (i + 1)]
+        cnt += r // 2 ** (i + 1)
+        FPsi = np.conj(np.fft.fft(Psi, R)).reshape(1, 1, R, *([1] * (C.ndim - 3)))
+        Cup[i] = np.real(np.fft.ifft(np.fft.fft(Cup[i], R, axis=2) * FPsi, R, axis=2))
+    Cup[levels] = C[:, :, cnt:]
+    for i in range(levels, 0, -1):
+        R = Cup[i - 1].shape[2]
+        tmp = np.zeros(Cup[i - 1].shape)
+        tmp[:, :, ::2] = Cup[i]
+        FPhi = np.conj(np.fft.fft(Phi, R)).reshape(1, 1, R, *([1] * (C.ndim - 3)))
+        Cup[i - 1] = Cup[i - 1] + np.real(np.fft.ifft(np.fft.fft(tmp, R, axis=2) * FPhi, R, axis=2))
+    return Cup[0]
+
+
+def wiener_filter(V1, V2, sigma):
+    """source/denoiseAndAggregateWieFast.m:19-66 -> (V2 filtered, filt)."""
+    d, K = V1.shape[0], V1.shape[2]
+    levels = 3 if K >= 16 else 2
+    Psi = np.array([-CC, CC])
+    Phi = np.abs(Psi)
+    T = scipy.fft.dct(np.eye(d), axis=0, norm="ortho")  # dct(eye(d))
+    Tinv = np.linalg.inv(T)
+    V1 = np.einsum("pu,uv...->pv...", T, V1)
+    V1 = np.einsum("pv...,qv->pq...", V1, T)
+    V1 = myWavDec1D(Psi, Phi, levels, V1)
+    filt = np.abs(V1) ** 2 / (np.abs(V1) ** 2 + sigma ** 2)
+    filt[0, 0, 14:] = 1
+    V2 = np.einsum("pu,uv...->pv...", T, V2)
+    V2 = np.einsum("pv...,qv->pq...", V2, T)
+    V2 = myWavDec1D(Psi, Phi, levels, V2)
+    V2 = V2 * filt
+    V2 = myWavRec1D(Psi, Phi, levels, V2)
+    V2 = np.einsum("pv...,qv->pq...", V2, Tinv)
+    V2 = np.einsum("pu,uv...->pv...", Tinv, V2)
+    return np.maximum(V2, 0), filt
+
+
+def aggregate_wie(V2, filt, S, d, d2):
+    """source/denoiseAndAggregateWieFast.m:68-100."""
+    M, N = len(S), len(S[0])
+    m, n = M * d, N * d
+    U = np.zeros((m + d2, n + d2))
+    W = np.zeros((m + d2, n + d2))
+    s = np.sin(np.pi * _linspace(0.2, 0.8, d))
+    w = np.outer(s, s)
+    for y in range(1, M + 1):
+        for x in range(1, N + 1):
+            blocks = V2[:, :, :, y - 1, x - 1]
+            weight = w / np.linalg.norm(filt[:, :, :, y - 1, x - 1].flatten(order="F")) ** 2
+            for i in range(len(S[y - 1][x - 1])):
+                a, b = _ind2sub(d2, S[y - 1][x - 1][i])
+                a = a + (y - 1) * d
+                b = b + (x - 1) * d
+                U[a - 1:a - 1 + d, b - 1:b - 1 + d] += weight * blocks[:, :, i]
+                W[a - 1:a - 1 + d, b - 1:b - 1 + d] += weight
+    h = d2 // 2
+    return U[h:m + h, h:n + h], W[h:m + h, h:n + h]
+
+
+def myMirrorPadImage(I, M, N):
+    """utilities/myMirrorPadImage.m:3-8."""
+    m, n = I.shape
+    I2 = np.zeros((M, N))
+    I2[:m, :n] = I
+    I2[m:M, :n] = np.flipud(I[m - (M - m):m, :n])
+    I2[:m, n:N] = np.fliplr(I[:m, n - (N - n):n])
+    I2[m:M, n:N] = np.fliplr(np.flipud(I[m - (M - m):m, n - (N - n):n]))
+    return I2
+
+
+def GBM3D(I, sigma, opts):
+    """source/GBM3D.m:21-83 on a resolved opts dict."""
+    m, n = I.shape
+    bs = opts["blockSize"]
+    M = int(np.ceil(m / bs) * bs)
+    N = int(np.ceil(n / bs) * bs)
+    I = myMirrorPadImage(np.asarray(I, dtype=np.float64), M, N)
+    ns = len(opts["matchSpins"])
+    U = np.zeros((M, N, ns))
+    W = U.copy()
+    U1 = U.copy()
+    tau = sigma * getTau(opts["levels"], opts["tauMode"])
+    K = opts["numMax"]
+
+    def setslice(A, ii, val):
+        if ii >= A.shape[2]:
+            A = np.concatenate([A, np.zeros((M, N, ii + 1 - A.shape[2]))], axis=2)
+        A[:, :, ii] = val
+        return A
+
+    for ii, sp in enumerate(opts["matchSpins"]):
+        S, Vol, _ = matchBlocksCPU(np.roll(I, (sp, sp), (0, 1)), bs, opts["matchSize"], K)
+        Vavg = ht_filter(Vol, tau, opts["levels"], opts["cycleSpin"])
+        u, w = aggregate_ht(Vavg, S, bs, opts["matchSize"])
+        U1 = setslice(U1, ii, np.roll(u, (-sp, -sp), (0, 1)))
+        W = setslice(W, ii, np.roll(w, (-sp, -sp), (0, 1)))
+    U1 = np.sum(U1, 2) / np.sum(W, 2)
+    if not opts["Wiener"]:
+        return U1[:m, :n], U1[:m, :n]
+    d = opts["blockSizeWie"]
+    for ii, sp in enumerate(opts["matchSpinsWie"]):
+        S, V1, V2 = matchBlocksWie(np.roll(U1, (sp, sp), (0, 1)), np.roll(I, (sp, sp), (0, 1)), d,
+                                   opts["matchSize"], K)
+        V2f, filt = wiener_filter(V1, V2, sigma)
+        u, w = aggregate_wie(V2f, filt, S, d, opts["matchSize"])
+        U = setslice(U, ii, np.roll(u, (-sp, -sp), (0, 1)))
+        W = setslice(W, ii, np.roll(w, (-sp, -sp), (0, 1)))
+    Uo = np.sum(U, 2) / np.sum(W, 2)
+    return Uo[:m, :n], U1[:m, :n]
+
+
+def S_to_array(S, K):
+    """list-of-lists S -> (K, M, N) uint32 array."""
+    M, N = len(S), len(S[0])
+    out = np.zeros((K, M, N), dtype=np.uint32)
+    for i in range(M):
+        for j in range(N):
+            out[:, i, j] = S[i][j]
+    return out
+
+
+def S_from_array(Sa):
+    K, M, N = Sa.shape
+    return [[Sa[:, i, j] for j in range(N)] for i in range(M)]

@@ -1,0 +1,186 @@
+// oracle_capi.cpp -- extern "C" entry points of the oracle (liboracle_gbm3d.so)
+// for the Python test harness.  TEST INFRASTRUCTURE ONLY: never linked into the
+// product library.  Every function returns 0 on success, a gbm3d_status code
+// otherwise.  Layouts are the reference's (column-major), see gbm3d_oracle.hpp.
+#include "gbm3d_oracle.hpp"
+
+using namespace gbm3d_oracle;
+
+namespace {
+bool ok_sizes(int m, int n, int d, int d2) {
+  return m > 0 && n > 0 && d > 0 && d2 > 0 && (d2 % 2) == 0 && m >= d2 && n >= d2 && m % d == 0 && n % d == 0;
+}
+template <class F>
+int guarded(F f) {
+  try {
+    return f();
+  } catch (...) {
+    return GBM3D_ERR_NOMEM;
+  }
+}
+void threads(int nt) {
+#ifdef _OPENMP
+  if (nt > 0) omp_set_num_threads(nt);
+#else
+  (void)nt;
+#endif
+}
+}  // namespace
+
+extern "C" {
+
+int orc_set_threads(int nt) {
+  threads(nt);
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+int orc_set_opts(const char* profile, gbm3d_opts* out) {
+  set_opts(profile, out);
+  return 0;
+}
+int orc_check_opts(gbm3d_opts* o) {
+  check_opts(o);
+  return 0;
+}
+int orc_resolve_opts(const gbm3d_opts* in, gbm3d_opts* out) {
+  resolve_opts(in, out);
+  return 0;
+}
+int orc_get_tau(int levels, int mode, double sigma, double* tau) {
+  get_tau(levels, mode, sigma, tau);
+  return 0;
+}
+
+int orc_mirror_pad_f64(const double* I, int m, int n, int M, int N, double* out) {
+  if (M < m || N < n || M > 2 * m || N > 2 * n) return GBM3D_ERR_ARG;
+  auto v = mirror_pad<double>(I, m, n, M, N);
+  std::copy(v.begin(), v.end(), out);
+  return 0;
+}
+
+int orc_search_pad_f64(const double* X, int m, int n, int d2, double* out) {
+  if (m < d2 || n < d2) return GBM3D_ERR_ARG;
+  auto v = search_pad<double>(X, m, n, d2);
+  std::copy(v.begin(), v.end(), out);
+  return 0;
+}
+
+#define MATCH1(SUF, T)                                                                               \
+  int orc_match_##SUF(const T* I, int m, int n, int d, int d2, int K, uint32_t* S, T* Volume,      \
+                      T* dist) {                                                                     \
+    if (!ok_sizes(m, n, d, d2) || K < 1 || K > d2 * d2) return GBM3D_ERR_ARG;                     \
+    return guarded([&] {                                                                             \
+      match_step1<T>(I, m, n, d, d2, K, S, Volume, dist);                                            \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }                                                                                                  \
+  int orc_match_wie_##SUF(const T* I, const T* In, int m, int n, int d, int d2, int K, uint32_t* S, \
+                          T* V1, T* V2, T* dist) {                                                   \
+    if (!ok_sizes(m, n, d, d2) || K < 1 || K > d2 * d2) return GBM3D_ERR_ARG;                     \
+    return guarded([&] {                                                                             \
+      match_step2<T>(I, In, m, n, d, d2, K, S, V1, V2, dist);                                        \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }
+MATCH1(f64, double)
+MATCH1(f32, float)
+
+#define HT(SUF, T)                                                                                   \
+  int orc_ht_filter_##SUF(const T* V, int m, int n, int K, int levels, int cycleSpin,               \
+                          const double* tau, T* Vavg, double* min_margin) {                          \
+    if (levels < 1 || (m % (1 << levels)) || (n % (1 << levels)) || (K % (1 << levels)) ||          \
+        cycleSpin < 1)                                                                               \
+      return GBM3D_ERR_ARG;                                                                          \
+    return guarded([&] {                                                                             \
+      if (min_margin) *min_margin = 1e300;                                                           \
+      ht_filter<T>(V, m, n, K, levels, cycleSpin, tau, Vavg, min_margin);                            \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }
+HT(f64, double)
+HT(f32, float)
+
+#define AGG(SUF, T)                                                                                  \
+  int orc_agg_ht_##SUF(const T* Vavg, const uint32_t* S, int m, int n, int d, int d2, int K, T* U,  \
+                       T* W, int gather) {                                                           \
+    if (!ok_sizes(m, n, d, d2)) return GBM3D_ERR_ARG;                                                \
+    return guarded([&] {                                                                             \
+      aggregate_ht<T>(Vavg, S, m, n, d, d2, K, U, W, gather != 0);                                   \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }                                                                                                  \
+  int orc_agg_wie_##SUF(const T* V2f, const T* gain2, const uint32_t* S, int m, int n, int d, int d2, \
+                        int K, T* U, T* W, int gather) {                                             \
+    if (!ok_sizes(m, n, d, d2)) return GBM3D_ERR_ARG;                                                \
+    return guarded([&] {                                                                             \
+      aggregate_wie<T>(V2f, gain2, S, m, n, d, d2, K, U, W, gather != 0);                            \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }                                                                                                  \
+  int orc_agg_weights_##SUF(const T* Vavg, int m, int n, int d, int K, T* w) {                       \
+    if (m % d || n % d) return GBM3D_ERR_ARG;                                                        \
+    auto v = agg1_weights<T>(Vavg, m, n, d, K);                                                      \
+    std::copy(v.begin(), v.end(), w);                                                                \
+    return 0;                                                                                        \
+  }                                                                                                  \
+  int orc_wiener_##SUF(const T* V1, const T* V2, int d, int K, long G, double sigma, T* V2f,        \
+                       T* gain2) {                                                                   \
+    if (d < 1 || K < 4 || (K % 4)) return GBM3D_ERR_ARG;                                             \
+    return guarded([&] {                                                                             \
+      wiener<T>(V1, V2, d, K, G, sigma, V2f, gain2);                                                 \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }
+AGG(f64, double)
+AGG(f32, float)
+
+int orc_haar_dec_f64(double* x, int R, int levels) {
+  haar_dec<double>(x, R, levels);
+  return 0;
+}
+int orc_haar_rec_f64(double* x, int R, int levels) {
+  haar_rec<double>(x, R, levels);
+  return 0;
+}
+
+// One level of the 3-D analysis (8 subbands, b = dim1_hi | dim2_hi<<1 | z_hi<<2),
+// each (p/2) x (q/2) x (r/2), written consecutively to C.
+int orc_wav_dec_level_f64(const double* X, int p, int q, int r, double* C) {
+  if (p % 2 || q % 2 || r % 2) return GBM3D_ERR_ARG;
+  Vol<double> V(p, q, r);
+  std::copy(X, X + (size_t)p * q * r, V.v.begin());
+  Vol<double> B[8];
+  analysis_level<double>(V, B, Bank<double>());
+  size_t sz = (size_t)p * q * r / 8;
+  for (int b = 0; b < 8; ++b) std::copy(B[b].v.begin(), B[b].v.end(), C + sz * b);
+  return 0;
+}
+int orc_wav_rec_level_f64(const double* C, int p, int q, int r, double* X) {
+  Vol<double> B[8];
+  size_t sz = (size_t)p * q * r / 8;
+  for (int b = 0; b < 8; ++b) {
+    B[b] = Vol<double>(p / 2, q / 2, r / 2);
+    std::copy(C + sz * b, C + sz * (b + 1), B[b].v.begin());
+  }
+  Vol<double> Y = synthesis_level<double>(B, Bank<double>());
+  std::copy(Y.v.begin(), Y.v.end(), X);
+  return 0;
+}
+
+int orc_gbm3d_f64(const double* I, int m, int n, double sigma, const gbm3d_opts* resolved, double* U,
+                  double* U1) {
+  if (!I || !U || !resolved || m < 1 || n < 1) return GBM3D_ERR_ARG;
+  return guarded([&] { return gbm3d<double>(I, m, n, sigma, *resolved, U, U1, false); });
+}
+int orc_gbm3d_f32(const float* I, int m, int n, float sigma, const gbm3d_opts* resolved, float* U,
+                  float* U1, int nthreads) {
+  if (!I || !U || !resolved || m < 1 || n < 1) return GBM3D_ERR_ARG;
+  threads(nthreads);
+  return guarded([&] { return gbm3d<float>(I, m, n, (double)sigma, *resolved, U, U1, true); });
+}
+
+}  // extern "C"

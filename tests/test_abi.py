@@ -1,0 +1,91 @@
+"""The C ABI (include/gbm3d.h) of libgbm3d.so: symbols, struct layout and the
+host-only option logic, checked against the oracle's restatement.  CPU only:
+no call here touches a GPU."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gbm3d_amd
+import oracle as O
+from gbm3d_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gbm3d.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gbm3d_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gbm3d_amd.load_library()
+    names = declared_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (gbm3d_\w+)", out))
+    assert set(names) <= exported
+    assert set(_abi.SIGNATURES) == set(names)
+
+
+def test_opts_struct_layout_matches_header():
+    # profile[16] + 4 ints + n + 16 + n + 16 + 6 ints
+    assert C.sizeof(_abi.Opts) == 16 + 4 * (4 + 1 + 16 + 1 + 16 + 6)
+    src = "#include \"gbm3d.h\"\n#include <stdio.h>\n#include <stddef.h>\nint main(){printf(\"%zu %zu %zu\\n\"," \
+          "sizeof(gbm3d_opts), offsetof(gbm3d_opts, matchSpinsWie), offsetof(gbm3d_opts, filtType));}\n"
+    exe = os.path.join(ROOT, "gbm3d_amd", "_build", "abi_probe")
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    r = subprocess.run(["gcc", "-x", "c", "-I", os.path.join(ROOT, "include"), "-", "-o", exe], input=src,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    size, off_wie, off_ft = map(int, subprocess.run([exe], capture_output=True, text=True).stdout.split())
+    assert size == C.sizeof(_abi.Opts)
+    assert off_wie == _abi.Opts.matchSpinsWie.offset
+    assert off_ft == _abi.Opts.filtType.offset
+
+
+@pytest.mark.parametrize("profile", ["default", "fast", "superFast", "accuracy", "unknown"])
+def test_set_opts_matches_oracle(profile):
+    assert gbm3d_amd.setBM3Dopts(profile) == O.set_opts(profile)
+
+
+@pytest.mark.parametrize("opts", [{}, {"blockSize": 8}, {"blockSize": 32, "blockSizeWie": 16},
+                                  {"matchSpins": [0, 2, 4], "Wiener": False}, {"numMax": 8, "numMin": 8}])
+def test_check_opts_matches_oracle(opts):
+    assert gbm3d_amd.checkBM3Dopts(opts) == O.check_opts(opts)
+
+
+@pytest.mark.parametrize("opts", [None, {"profile": "fast", "blockSize": 8}, {"blockSize": 8},
+                                  {"profile": "superFast"}])
+def test_resolve_opts_matches_oracle(opts):
+    assert gbm3d_amd.resolve_opts(opts) == O.resolve_opts(opts)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+def test_get_tau_matches_oracle(mode):
+    np.testing.assert_array_equal(gbm3d_amd.getTau(3, mode, 25.0), O.get_tau(3, mode, 25.0))
+
+
+def test_unrecognized_filtering_raises():
+    with pytest.raises(ValueError, match="unrecognized filtering"):
+        _abi.opts_to_struct({"filtType": "gauss"})
+
+
+def test_null_arguments_return_status_not_crash():
+    lib = gbm3d_amd.load_library()
+    assert lib.gbm3d_set_opts(b"default", None) == _abi.GBM3D_ERR_ARG
+    assert b"NULL" in lib.gbm3d_last_error()
+    assert lib.gbm3d_get_tau(0, 1, C.c_double(1.0), None) == _abi.GBM3D_ERR_ARG
+    assert lib.gbm3d_destroy(None) == 0
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(ImportError):
+        _abi.load_library(str(tmp_path / "libgbm3d.so"))

@@ -1,0 +1,230 @@
+"""Pins of the oracle (oracle/gbm3d_oracle.hpp) against checks that do not
+reuse its formulas: the reference's own filter data file, the FFT-literal
+numpy transcription (oracle/literal.py), brute force, closed forms and
+invariants.  CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.fft
+
+import oracle as O
+from gbm3d_amd import synth
+from oracle import literal as L
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+BITS_TO_J = {1: 0, 2: 1, 4: 2, 3: 3, 5: 4, 6: 5, 7: 6}  # subband bits -> C{i, j} column (0-based)
+
+
+def test_filter_taps_match_reference_data_file():
+    """archive/waveletFilters/bior15Filters.mat (as JSON) pins the 1-D factors.
+
+    Psi(:,:,:,1) = Lo'*Lo.*Lo_z and Psi(:,:,:,2) = Hi'*Lo.*Lo_z with
+    Lo(5) = Lo_z(1) = sqrt(2)/2, so column 5 of slice 1 is factor/2.
+    """
+    taps = json.load(open(os.path.join(GOLD, "bior15_taps.json")))
+    Psi, Psi2 = L.getWaveFilters3Dbior()
+    np.testing.assert_allclose(2 * Psi[:, 4, 0, 0], taps["LoD"], atol=1e-14)
+    np.testing.assert_allclose(2 * Psi[:, 4, 0, 1], taps["HiD"], atol=1e-14)
+    np.testing.assert_allclose(2 * Psi2[:, 4, 0, 0], taps["LoR"], atol=1e-14)
+    # the reference's synthesis high pass is the data file's HiR with the opposite sign
+    np.testing.assert_allclose(2 * Psi2[:, 4, 0, 1], -np.asarray(taps["HiR"]), atol=1e-14)
+
+
+@pytest.mark.parametrize("shape", [(16, 20, 8), (12, 12, 4), (8, 6, 2)])
+def test_one_level_analysis_matches_fft_literal(shape):
+    rng = np.random.default_rng(sum(shape))
+    X = rng.standard_normal(shape)
+    C8 = O.wav_dec_level(X)
+    Psi, _ = L.getWaveFilters3Dbior()
+    Cl = L.myWavDec3DFFT(Psi, 1, X)
+    for b, j in BITS_TO_J.items():
+        np.testing.assert_allclose(C8[..., b], Cl[0][j], atol=1e-12)
+    np.testing.assert_allclose(C8[..., 0], Cl[1][0], atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(16, 20, 8), (8, 6, 2)])
+def test_one_level_synthesis_matches_fft_literal_and_reconstructs(shape):
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal(shape)
+    C8 = O.wav_dec_level(X)
+    _, Psi2 = L.getWaveFilters3Dbior()
+    C = [[None] * 7 for _ in range(2)]
+    for b, j in BITS_TO_J.items():
+        C[0][j] = C8[..., b]
+    C[1][0] = C8[..., 0]
+    np.testing.assert_allclose(O.wav_rec_level(C8), L.myWavRec3DGPUfast(Psi2, 1, C), atol=1e-12)
+    if min(shape[:2]) >= 10:  # filters longer than an axis are truncated (fftn) and PR is lost
+        np.testing.assert_allclose(O.wav_rec_level(C8), X, atol=1e-12)  # perfect reconstruction
+
+
+def test_ht_filter_matches_fft_literal():
+    rng = np.random.default_rng(11)
+    V = rng.standard_normal((32, 48, 16)) * 40 + 60
+    tau = L.getTau(3, 1) * 25
+    np.testing.assert_allclose(O.ht_filter(V, 3, 2, tau), L.ht_filter(V, tau, 3, 2), atol=1e-10)
+
+
+def test_ht_filter_tau_zero_is_identity_then_clip():
+    rng = np.random.default_rng(12)
+    V = rng.standard_normal((96, 80, 16)) * 30  # every level >= 10 samples: no filter truncation
+    out = O.ht_filter(V, 3, 2, np.zeros((3, 7)))
+    np.testing.assert_allclose(out, np.maximum(V, 0), atol=1e-11)
+
+
+def test_ht_filter_tau_infinite_keeps_only_the_approximation():
+    rng = np.random.default_rng(13)
+    V = rng.standard_normal((32, 32, 16)) * 30 + 100
+    tau = np.full((3, 7), np.inf)
+    np.testing.assert_allclose(O.ht_filter(V, 3, 1, tau), L.ht_filter(V, tau, 3, 1), atol=1e-10)
+    # the approximation of a constant volume is exact
+    Vc = np.full((96, 80, 16), 7.5)
+    np.testing.assert_allclose(O.ht_filter(Vc, 3, 2, tau), Vc, atol=1e-12)
+
+
+@pytest.mark.parametrize("tau_mode", [1, 2, 3, 4])
+def test_get_tau_matches_literal(tau_mode):
+    np.testing.assert_allclose(O.get_tau(3, tau_mode, 25.0), 25.0 * L.getTau(3, tau_mode), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("d,d2", [(16, 32), (8, 32), (8, 48)])
+def test_match_matches_fft_literal(d, d2):
+    clean, noisy = synth.make_pair(3, 64 if d2 == 32 else 96, 64 if d2 == 32 else 96, 25.0)
+    I = noisy.astype(np.float64)
+    S, V, dist = O.match(I, d, d2, 16, want_dist=True)
+    Sl, Vl, dl = L.matchBlocksCPU(I, d, d2, 16)
+    np.testing.assert_array_equal(S, L.S_to_array(Sl, 16))
+    np.testing.assert_array_equal(V, Vl)
+    for (i, j), bd in dl.items():
+        np.testing.assert_allclose(dist[:, i, j].reshape(d2, d2, order="F"), bd, atol=1e-9)
+
+
+def test_distance_expansion_equals_direct_squared_difference():
+    """(|x|^2+|y|^2-2<x,y>)/|x|^2 == |x-y|^2/|x|^2 on brute force."""
+    rng = np.random.default_rng(5)
+    I = rng.uniform(0, 255, (32, 32))
+    d, d2 = 8, 32
+    S, _, dist = O.match(I, d, d2, 16, want_dist=True)
+    Ipad = O.search_pad(I, d2)
+    i, j = 1, 2
+    x = I[i * d:(i + 1) * d, j * d:(j + 1) * d]
+    for a in (0, 5, 16, 31):
+        for b in (0, 7, 16, 31):
+            y = Ipad[i * d + a:i * d + a + d, j * d + b:j * d + b + d]
+            np.testing.assert_allclose(dist[a + d2 * b, i, j], np.sum((x - y) ** 2) / np.sum(x ** 2), rtol=1e-9)
+    # stable argsort reproduces the ranking; slot 1 is the centre
+    order = np.argsort(dist[:, i, j], kind="stable")[:16] + 1
+    order[0] = d2 * d2 // 2 + d2 // 2 + 1
+    np.testing.assert_array_equal(S[:, i, j], order)
+
+
+def test_search_pad_shifted_copy_quirk():
+    """1-D analogue from matchBlocksCPU.m:57-61: [3 4 | 1..8 | 1 2] for m = 8, d2 = 4."""
+    X = np.arange(1, 9, dtype=np.float64)[:, None] * np.ones((1, 8))
+    P = O.search_pad(X, 4)
+    np.testing.assert_array_equal(P[:, 0], [3, 4, 1, 2, 3, 4, 5, 6, 7, 8, 1, 2])
+    np.testing.assert_array_equal(P, L._search_pad(X, 4))
+
+
+def test_mirror_pad_matches_literal():
+    rng = np.random.default_rng(1)
+    I = rng.standard_normal((37, 45))
+    np.testing.assert_array_equal(O.mirror_pad(I, 48, 48), L.myMirrorPadImage(I, 48, 48))
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_aggregation_gather_equals_scatter_and_literal(precision):
+    clean, noisy = synth.make_pair(9, 64, 64, 25.0)
+    S, V, _ = O.match(noisy, 16, 32, 16, precision=64)
+    Vavg = O.ht_filter(V, 3, 2, O.get_tau(3, 1, 25.0))
+    Ug, Wg = O.aggregate_ht(Vavg, S, 16, 32, precision=precision, gather=True)
+    Us, Ws = O.aggregate_ht(Vavg, S, 16, 32, precision=precision, gather=False)
+    np.testing.assert_array_equal(Ug, Us)
+    np.testing.assert_array_equal(Wg, Ws)
+    Ul, Wl = L.aggregate_ht(Vavg, L.S_from_array(S), 16, 32)
+    tol = 1e-12 if precision == 64 else 1e-5
+    np.testing.assert_allclose(Ug, Ul, rtol=tol, atol=tol * np.abs(Ul).max())
+    np.testing.assert_allclose(Wg, Wl, rtol=tol, atol=tol * np.abs(Wl).max())
+
+
+def test_dct_matrix_is_orthonormal_dct2():
+    T = scipy.fft.dct(np.eye(8), axis=0, norm="ortho")
+    rng = np.random.default_rng(2)
+    V1 = rng.standard_normal((8, 8, 16, 3))
+    V2 = V1 + 0.1 * rng.standard_normal(V1.shape)
+    V2f, g2 = O.wiener(V1, V2, 0.0)  # sigma = 0: all gains 1 except zero coefficients -> identity then clip
+    np.testing.assert_allclose(V2f, np.maximum(V2, 0), atol=1e-11)
+    assert np.allclose(T @ T.T, np.eye(8))
+
+
+def test_haar_1d_matches_fft_literal_and_reconstructs():
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(16)
+    Psi = np.array([-np.sqrt(2) / 2, np.sqrt(2) / 2])
+    lit = L.myWavDec1D(Psi, np.abs(Psi), 3, x.reshape(1, 1, 16))[0, 0]
+    np.testing.assert_allclose(O.haar_dec(x, 3), lit, atol=1e-13)
+    np.testing.assert_allclose(O.haar_rec(O.haar_dec(x, 3), 3), x, atol=1e-13)
+    rec = L.myWavRec1D(Psi, np.abs(Psi), 3, lit.reshape(1, 1, 16))[0, 0]
+    np.testing.assert_allclose(rec, x, atol=1e-13)
+
+
+def test_wiener_matches_literal():
+    clean, noisy = synth.make_pair(21, 64, 64, 25.0)
+    pilot = clean + 3.0 * synth.noise(99, 64, 64)
+    S, V1, V2, _ = O.match_wie(pilot, noisy, 8, 32, 16)
+    Sl, V1l, V2l = L.matchBlocksWie(pilot, noisy.astype(np.float64), 8, 32, 16)
+    np.testing.assert_array_equal(S, L.S_to_array(Sl, 16))
+    V2f, g2 = O.wiener(V1, V2, 25.0)
+    V2fl, filt = L.wiener_filter(V1l, V2l, 25.0)
+    np.testing.assert_allclose(V2f, V2fl, atol=1e-9)
+    np.testing.assert_allclose(g2, np.sum(filt ** 2, axis=(0, 1, 2)), rtol=1e-12)
+    U, W = O.aggregate_wie(V2f, g2, S, 64, 64, 32)
+    Ul, Wl = L.aggregate_wie(V2fl, filt, Sl, 8, 32)
+    np.testing.assert_allclose(U, Ul, rtol=1e-10)
+    np.testing.assert_allclose(W, Wl, rtol=1e-10)
+
+
+@pytest.mark.parametrize("opts", [None, {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]},
+                                  {"profile": "fast"}, {"profile": "superFast"}, {"profile": "accuracy"}])
+def test_end_to_end_matches_fft_literal(opts):
+    clean, noisy = synth.make_pair(3, 64, 64, 25.0)
+    Uo, U1o = O.gbm3d(noisy, 25.0, opts)
+    Ul, U1l = L.GBM3D(noisy, 25.0, O.resolve_opts(opts))
+    np.testing.assert_allclose(Uo, Ul, atol=1e-9)
+    np.testing.assert_allclose(U1o, U1l, atol=1e-9)
+    assert synth.psnr(clean, Uo) > synth.psnr(clean, noisy) + 5
+
+
+def test_end_to_end_non_block_multiple_size():
+    clean, noisy = synth.make_pair(8, 70, 90, 20.0)
+    Uo, U1o = O.gbm3d(noisy, 20.0, None)
+    Ul, U1l = L.GBM3D(noisy, 20.0, O.resolve_opts(None))
+    np.testing.assert_allclose(Uo, Ul, atol=1e-9)
+    assert Uo.shape == (70, 90)
+
+
+def test_scale_equivariance():
+    """U(a*I, a*sigma) = a*U(I, sigma) (thresholds and gains scale with sigma)."""
+    clean, noisy = synth.make_pair(6, 64, 64, 25.0)
+    a = 1.0 / 255.0
+    U, U1 = O.gbm3d(noisy.astype(np.float64), 25.0, None)
+    Us, U1s = O.gbm3d(noisy.astype(np.float64) * a, 25.0 * a, None)
+    np.testing.assert_allclose(Us / a, U, atol=1e-8)
+
+
+def test_fp32_restatement_close_to_fp64():
+    clean, noisy = synth.make_pair(10, 128, 128, 25.0)
+    U64, _ = O.gbm3d(noisy, 25.0, None, precision=64)
+    U32, _ = O.gbm3d(noisy, 25.0, None, precision=32)
+    assert abs(synth.psnr(clean, U64) - synth.psnr(clean, U32)) < 0.01
+
+
+def test_options_restatement():
+    assert O.set_opts("nonsense") == O.set_opts("default")
+    f = O.resolve_opts({"profile": "fast", "blockSize": 8})  # a profile overrides every field
+    assert f["blockSize"] == 16 and f["matchSpins"] == [0, 4, 6] and f["matchSpinsWie"] == [0]
+    c = O.check_opts({"blockSize": 8})
+    assert c["matchSpins"] == [0, 4] and c["matchSize"] == 32 and "matchSpinsWie" not in c
+    c = O.check_opts({"blockSize": 64, "blockSizeWie": 16})
+    assert c["matchSize"] == 64 and c["matchSpins"] == [0, 8]
