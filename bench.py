@@ -1,0 +1,177 @@
+"""Benchmark: GBM3D two-step denoise of a 2048x2048 FP32 image, sigma = 25.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one full two-step GBM3D denoise (source/GBM3D.m) of one synthetic
+2048^2 image whose input is already resident in HBM.  Prints one JSON line.
+The per-kernel roofline figure uses the library's own HIP-event stage timers
+(recorded on the stream the kernels run on).  The CPU baseline is the
+repository's FP32 C++ restatement (the oracle, OpenMP) on a 512^2 sample of
+the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "Mpix/s two-step denoise, sigma=25, 2048^2: plain vs chosen form; dPSNR (dB)"
+SIZE = 2048
+SIGMA = 25.0
+SEED = 2048
+# 8x8 blocks in both steps, K = 16, matchSize 32 (radius 16), spins [0 4]/[0 4],
+# cycleSpin 2, 3 levels, tau = sigma*[3.3 3.0 2.7]
+OPTS = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+
+
+def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2):
+    """Algorithmic work per launch of each single-kernel stage."""
+    MN = M * N
+    vol_bytes = MN * K * 4
+    G2 = MN // (d1 * d1)
+    out = {}
+    flops_match = 2.0 * MN * d2 * d2  # |x|^2+|y|^2-2<x,y>: d^2 FMAs per candidate, d2^2 candidates per d^2 pixels
+    if stage_n["match1"]:
+        out["match1"] = dict(bound="mfma", unit="TFLOP/s", work=flops_match, launches=stage_n["match1"],
+                             ms=stage_ms["match1"] / stage_n["match1"], peak=PEAK_FP32_TFLOPS)
+    if stage_n["match2"]:
+        out["match2"] = dict(bound="mfma", unit="TFLOP/s", work=flops_match, launches=stage_n["match2"],
+                             ms=stage_ms["match2"] / stage_n["match2"], peak=PEAK_FP32_TFLOPS)
+    if stage_n["wiener"]:
+        # V1, V2 in and filtered V2 out, N = M*N*K floats each
+        out["wiener"] = dict(bound="hbm", unit="GB/s", work=3.0 * vol_bytes, launches=stage_n["wiener"],
+                             ms=stage_ms["wiener"] / stage_n["wiener"], peak=PEAK_HBM_GBS)
+    if stage_n["agg2"]:
+        G = MN // (8 * 8)
+        out["agg2"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G * K * 4 + G * 4 + 2 * MN * 4),
+                           launches=stage_n["agg2"], ms=stage_ms["agg2"] / stage_n["agg2"], peak=PEAK_HBM_GBS)
+    if stage_n["ht"]:
+        # read the volume, write the filtered volume, per match spin
+        out["ht"] = dict(bound="hbm", unit="GB/s", work=2.0 * vol_bytes, launches=n1,
+                         ms=stage_ms["ht"] / n1, peak=PEAK_HBM_GBS)
+    for k, v in out.items():
+        s = v["ms"] * 1e-3
+        v["achieved"] = v["work"] / s / (1e12 if v["unit"] == "TFLOP/s" else 1e9)
+        v["frac"] = v["achieved"] / v["peak"]
+    return out
+
+
+def cpu_baseline(threads: int):
+    import oracle
+    from gbm3d_amd import synth
+
+    sz = 512
+    clean, noisy = synth.make_pair(SEED, sz, sz, SIGMA)
+    oracle.set_threads(threads)
+    t0 = time.perf_counter()
+    U, _ = oracle.gbm3d(noisy, SIGMA, OPTS, precision=32, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": sz * sz / dt / 1e6, "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 C++ restatement (OpenMP) of the same two-step GBM3D, one {sz}x{sz} image of the same "
+                      f"synthetic construction, sigma=25, same options; {dt:.2f} s",
+            "psnr_db": synth.psnr(clean, U)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-only", action="store_true", help="warmup + steps only (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (one image per launch); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import gbm3d_amd
+    from gbm3d_amd import device, synth
+
+    clean, noisy = synth.make_pair(SEED, SIZE, SIZE, SIGMA)
+    ctx = gbm3d_amd.Context(0)
+    stream = device.Stream()
+    ctx.set_stream(stream.handle)
+    dI = device.DeviceArray.from_host(noisy.astype(np.float32))
+    dU = device.DeviceArray((SIZE, SIZE))
+    dU1 = device.DeviceArray((SIZE, SIZE))
+
+    def step():
+        ctx.denoise_device(dI.ptr, SIZE, SIZE, SIGMA, OPTS, dU.ptr, dU1.ptr)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    e0, e1 = device.Event(), device.Event()
+    device.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    stream.synchronize()
+    device.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_ms(e1) / args.steps
+    if args.profile_only:
+        print(json.dumps({"ms_per_step": ms}))
+        return
+
+    U = dU.to_host()
+    U1 = dU1.to_host()
+    psnr = synth.psnr(clean, U)
+
+    # one more step with the stage timers on (same work, separate from the timed region)
+    ctx.set_timing(True)
+    step()
+    st = ctx.stage_times()
+    ctx.set_timing(False)
+    stage_ms = {k: v[0] for k, v in st.items()}
+    stage_n = {k: v[1] for k, v in st.items()}
+    ro = gbm3d_amd.resolve_opts(OPTS)
+    rl = kernel_rooflines(stage_ms, stage_n, SIZE, SIZE, ro["numMax"], ro["blockSize"], ro["matchSize"],
+                          len(ro["matchSpins"]), len(ro["matchSpinsWie"]))
+    dom = max(rl, key=lambda k: rl[k]["ms"] * rl[k]["launches"])
+    d = rl[dom]
+    prof = os.path.join(HERE, "profiles", "traffic.json")
+    traffic = None
+    if os.path.exists(prof):
+        traffic = json.load(open(prof)).get(dom)
+    roofline = {"bound": d["bound"], "achieved": round(d["achieved"], 3), "peak": d["peak"], "unit": d["unit"],
+                "frac": round(d["frac"], 4), "traffic": traffic, "kernel": dom,
+                "avg_launch_ms": round(d["ms"], 5)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(min(16, os.cpu_count() or 1))
+
+    mpix = SIZE * SIZE / (ms * 1e-3) / 1e6
+    line = {
+        "metric": METRIC, "value": round(mpix, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain GBM3D two-step denoise, one 2048x2048 FP32 synthetic image (seed 2048), "
+                               "sigma=25, 8x8 blocks both steps, K=16, matchSize 32, spins [0 4]/[0 4], cycleSpin 2, "
+                               "3 levels", "form": "plain FP32 GPU path", "size": SIZE, "sigma": SIGMA},
+        "psnr_db": round(psnr, 4), "psnr_noisy_db": round(synth.psnr(clean, noisy), 4),
+        "psnr_pilot_db": round(synth.psnr(clean, U1), 4), "dpsnr_db": None,
+        "wall_s_timed": round(wall, 4),
+        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "roofline": roofline,
+        "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in rl.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

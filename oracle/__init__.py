@@ -228,3 +228,33 @@ def gbm3d(I, sigma, opts=None, precision=64, nthreads=0):
         st = lib().orc_gbm3d_f32(_p(I), m, n, C.c_float(sigma), C.byref(o), _p(U), _p(U1), nthreads)
     _check(st)
     return U, U1
+
+
+def step2_from_pilot(pilot, noisy, sigma, opts=None, precision=32):
+    """Step 2 of GBM3D (source/GBM3D.m:70-83) run on a given pilot U1.
+
+    pilot, noisy: M x N (already block-multiple sizes).  Composes the oracle's
+    stage functions: circshift, matchBlocksWie, Wiener filter, aggregation,
+    shift back, sum(U,3)./sum(W,3).  Profiles whose step-2 spin list is
+    shorter than the step-1 list (the 'fast' slice quirk) are not covered.
+    """
+    ro = resolve_opts(opts)
+    spins1, spins2 = ro["matchSpins"], ro["matchSpinsWie"]
+    if len(spins2) < len(spins1):
+        raise ValueError("step2_from_pilot: numel(matchSpinsWie) < numel(matchSpins) is not covered")
+    dt = np.float64 if precision == 64 else np.float32
+    pilot = np.asarray(pilot, dtype=dt)
+    noisy = np.asarray(noisy, dtype=dt)
+    M, N = pilot.shape
+    d, d2, K = ro["blockSizeWie"], ro["matchSize"], ro["numMax"]
+    Us = np.zeros((M, N))
+    Ws = np.zeros((M, N))
+    for sp in spins2:
+        P = np.roll(pilot, (sp, sp), (0, 1))
+        I = np.roll(noisy, (sp, sp), (0, 1))
+        S, V1, V2, _ = match_wie(P, I, d, d2, K, precision=precision)
+        V2f, g2 = wiener(V1, V2, sigma, precision=precision)
+        U, W = aggregate_wie(V2f, g2, S, M, N, d2, precision=precision)
+        Us += np.roll(U.astype(np.float64), (-sp, -sp), (0, 1))
+        Ws += np.roll(W.astype(np.float64), (-sp, -sp), (0, 1))
+    return Us / Ws

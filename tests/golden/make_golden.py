@@ -29,10 +29,9 @@ CASES = {
 
 
 def tie_margin(dist, S, K):
-    """Smallest relative gap between consecutive ranks 1..K+1 of every tile."""
+    """Smallest gap between consecutive ranks 1..K+1 of any tile (normalised distance)."""
     d = np.sort(dist, axis=0)[: K + 1]
-    gaps = np.abs(np.diff(d, axis=0)) / np.maximum(np.abs(d[1:]), 1e-30)
-    return float(gaps.min())
+    return float(np.diff(d, axis=0).min())
 
 
 def main():
