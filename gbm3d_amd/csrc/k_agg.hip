@@ -86,13 +86,85 @@ __global__ __launch_bounds__(256) void agg1_prep_kernel(const float* __restrict_
   }
 }
 
+// One wave per group: lane e < d*d owns pixel (u, v) = (e % d, e / d) of every
+// slice; slices go through a 2-slice LDS buffer for the in-plane differences.
+template <int D>
+__global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __restrict__ vrec, int nspin,
+                                                             const float* __restrict__ vavg_in, int M, int N,
+                                                             int K, float* __restrict__ vavg,
+                                                             float* __restrict__ wgt) {
+  constexpr int DD = D * D, PER = (DD + 63) / 64;
+  __shared__ float sl[4][DD];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int Mt = M / D, G = Mt * (N / D);
+  const int g = blockIdx.x * 4 + wave;
+  if (g >= G) return;
+  const int ti = g % Mt, tj = g / Mt;
+  const size_t MN = (size_t)M * N, vol = MN * K;
+  float* cur = sl[wave];
+  float prev[PER];
+  float s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = lane + 64 * i;
+      if (e < DD) {
+        const int u = e % D, v = e / D;
+        const size_t o = (size_t)(ti * D + u) + (size_t)M * (tj * D + v) + MN * k;
+        float x;
+        if (nspin > 0) {
+          float acc = 0.f;
+          for (int s = 0; s < nspin; ++s) acc += vrec[vol * s + o];
+          x = acc / (float)nspin;
+          x = x > 0.f ? x : 0.f;  // max(mean(Vrec,4), 0)
+          vavg[o] = x;
+        } else {
+          x = vavg_in[o];
+        }
+        if (k > 0) s3 += fabsf(x - prev[i]);  // |diff(U,1,3)|
+        prev[i] = x;
+        cur[e] = x;
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = lane + 64 * i;
+      if (e < DD) {
+        const int u = e % D, v = e / D;
+        if (v + 3 < D) s2 += fabsf(third_diff(cur[e], cur[e + D], cur[e + 2 * D], cur[e + 3 * D]));
+        if (u + 3 < D) s1 += fabsf(third_diff(cur[e], cur[e + 1], cur[e + 2], cur[e + 3]));
+      }
+    }
+    wave_sync();
+  }
+  s1 = wave_sum_f32(s1);
+  s2 = wave_sum_f32(s2);
+  s3 = wave_sum_f32(s3);
+  if (lane == 0) {
+    float Dv = s2 * 0.25f;
+    Dv = Dv + s1 * 0.25f;
+    Dv = Dv + s3;
+    wgt[g] = 1.0f / (Dv + 2.220446049250313e-16f);  // 1/(D(blocks)+eps)
+  }
+}
+
 int launch_agg1_prep(const float* vrec, int nspin, const float* vavg_in, int M, int N, int K, int d, float* vavg,
                      float* wgt, hipStream_t st) {
   if (d < 4 || M % d || N % d) return fail(GBM3D_ERR_ARG, "agg1 prep: block size");
   const int G = (M / d) * (N / d);
-  const size_t lds = sizeof(float) * 2 * (size_t)d * d;
   if (G == 0) return GBM3D_OK;
-  hipLaunchKernelGGL(agg1_prep_kernel, dim3(G), dim3(256), lds, st, vrec, nspin, vavg_in, M, N, K, d, vavg, wgt);
+  const dim3 grid((G + 3) / 4);
+  switch (d) {
+    case 4: hipLaunchKernelGGL(agg1_prep_wave_kernel<4>, grid, dim3(256), 0, st, vrec, nspin, vavg_in, M, N, K, vavg, wgt); break;
+    case 8: hipLaunchKernelGGL(agg1_prep_wave_kernel<8>, grid, dim3(256), 0, st, vrec, nspin, vavg_in, M, N, K, vavg, wgt); break;
+    case 16: hipLaunchKernelGGL(agg1_prep_wave_kernel<16>, grid, dim3(256), 0, st, vrec, nspin, vavg_in, M, N, K, vavg, wgt); break;
+    case 32: hipLaunchKernelGGL(agg1_prep_wave_kernel<32>, grid, dim3(256), 0, st, vrec, nspin, vavg_in, M, N, K, vavg, wgt); break;
+    default: {
+      const size_t lds = sizeof(float) * 2 * (size_t)d * d;
+      hipLaunchKernelGGL(agg1_prep_kernel, dim3(G), dim3(256), lds, st, vrec, nspin, vavg_in, M, N, K, d, vavg, wgt);
+    }
+  }
   return check_launch("agg1_prep");
 }
 

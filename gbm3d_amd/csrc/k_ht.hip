@@ -192,7 +192,244 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
   }
 }
 
+// ------------------------------------------------ register-blocked kernels
+// The same level operators for axes of at least 10 samples (no filter
+// truncation).  Each thread filters a run of outputs from one register
+// window, computing the low and the high band from the same loads; zero taps
+// are skipped at compile time; circular indices come from per-tile tables.
+
+// out[i] = sum_k a_lo(k) v[2i+9-k], hi[i] = sum_k a_hi(k) v[2i+9-k]
+template <int R>
+__device__ __forceinline__ void ana_lo_hi(const float (&v)[2 * R + 8], float (&lo)[R], float (&hi)[R]) {
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    float a = 0.f;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) a = fmaf(a_lo(k), v[2 * i + 9 - k], a);
+    lo[i] = a;
+    hi[i] = fCc * v[2 * i + 4] - fCc * v[2 * i + 5];
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void ana_lo(const float (&v)[2 * R + 8], float (&lo)[R]) {
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    float a = 0.f;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) a = fmaf(a_lo(k), v[2 * i + 9 - k], a);
+    lo[i] = a;
+  }
+}
+
+// out[i] = sum_{k: i+k even} s_lo(k) lo[(i+k)/2] + s_hi(k) hi[(i+k)/2], i < R2 (window starts at an even output)
+template <int R2>
+__device__ __forceinline__ void syn_lo_hi(const float (&lo)[R2 / 2 + 5], const float (&hi)[R2 / 2 + 5],
+                                          float (&out)[R2]) {
+#pragma unroll
+  for (int i = 0; i < R2; ++i) {
+    float a = 0.f;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      if (((i + k) & 1) == 0) {
+        const int j = (i + k) >> 1;
+        a = fmaf(s_hi(k), hi[j], a);
+        if (k == 4 || k == 5) a = fmaf(fCc, lo[j], a);
+      }
+    }
+    out[i] = a;
+  }
+}
+
+namespace fs {
+constexpr int P = 32, C = 21, R = 50;    // output tile, coefficients, input rows per edge
+constexpr int ZS = R + 1, AS = R + 1, BS = C + 1, ES = P + 1;
+constexpr int REG0 = 2 * R * ZS;  // Z (later B)
+constexpr int REG1 = 4 * C * AS;  // A (later E)
+static_assert(8 * C * BS <= REG0, "B fits in Z");
+static_assert(4 * C * ES <= REG1, "E fits in A");
+}  // namespace fs
+
+__global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
+                                                          int p, int q, int r, int sh, Tau7 tau,
+                                                          float* __restrict__ Y) {
+  using namespace fs;
+  __shared__ float lds[REG0 + REG1];
+  __shared__ int ridx[R], cidx[R];
+  float* Z = lds;
+  float* B = lds;
+  float* A = lds + REG0;
+  float* E = lds + REG0;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * P, y0 = blockIdx.y * P, t3 = blockIdx.z;
+  if (tid < R)
+    ridx[tid] = pmod_i(x0 - 9 - sh + tid, p);
+  else if (tid < 2 * R)
+    cidx[tid - R] = pmod_i(y0 - 9 - sh + tid - R, q);
+  const size_t pq = (size_t)p * q;
+  const float* X0 = X + pq * pmod_i(2 * t3 - sh, r);
+  const float* X1 = X + pq * pmod_i(2 * t3 - 1 - sh, r);
+  __syncthreads();
+  // z analysis of the pair (2t3-1, 2t3)
+  for (int e = tid; e < R * R; e += 256) {
+    const int rr = e % R, cc = e / R;
+    const size_t o = (size_t)ridx[rr] + (size_t)p * cidx[cc];
+    const float a = X0[o], b = X1[o];
+    Z[cc * ZS + rr] = fCc * a + fCc * b;
+    Z[R * ZS + cc * ZS + rr] = fCc * b - fCc * a;
+  }
+  __syncthreads();
+  // dim-2 analysis: runs of 7 coefficients per (z band, row)
+  for (int e = tid; e < 2 * 3 * R; e += 256) {
+    const int rr = e % R, run = (e / R) % 3, zb = e / (3 * R);
+    const int c0 = run * 7;
+    float v[22];
+#pragma unroll
+    for (int j = 0; j < 22; ++j) v[j] = Z[zb * R * ZS + (2 * c0 + j) * ZS + rr];
+    float lo[7], hi[7];
+    ana_lo_hi<7>(v, lo, hi);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
+      A[((zb * 2 + 1) * C + c0 + i) * AS + rr] = hi[i];
+    }
+  }
+  __syncthreads();
+  // dim-1 analysis + hard threshold (the approximation is the rebuilt coarser level)
+  const int ph = p >> 1, qh = q >> 1;
+  for (int e = tid; e < 2 * 2 * 3 * C; e += 256) {
+    const int c = e % C, run = (e / C) % 3, b2 = (e / (3 * C)) & 1, zb = e / (6 * C);
+    const int t0 = run * 7;
+    float v[22];
+#pragma unroll
+    for (int j = 0; j < 22; ++j) v[j] = A[((zb * 2 + b2) * C + c) * AS + 2 * t0 + j];
+    float lo[7], hi[7];
+    ana_lo_hi<7>(v, lo, hi);
+    const int bl = (b2 << 1) | (zb << 2), bh = bl | 1;
+    const float tl = tau.t[band_j(bl)], th = tau.t[band_j(bh)];
+    float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + t0;
+    float* Bh = Bl + C * BS;
+    if (bl == 0) {
+      const size_t col = (size_t)ph * ((size_t)pmod_i(y0 / 2 + c, qh) + (size_t)qh * t3);
+#pragma unroll
+      for (int i = 0; i < 7; ++i) Bl[i] = Ylo[(size_t)pmod_i(x0 / 2 + t0 + i, ph) + col];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 7; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 7; ++i) Bh[i] = hi[i] * (fabsf(hi[i]) > th ? 1.f : 0.f);
+  }
+  __syncthreads();
+  // dim-1 synthesis: runs of 8 outputs per (z band, dim-2 band, coefficient column)
+  for (int e = tid; e < 2 * 2 * 4 * C; e += 256) {
+    const int c = e % C, xr = (e / C) & 3, b2 = (e / (4 * C)) & 1, zb = e / (8 * C);
+    const int xl = xr * 8;
+    const float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + xl / 2;
+    const float* Bh = Bl + C * BS;
+    float lo[9], hi[9], out[8];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      lo[j] = Bl[j];
+      hi[j] = Bh[j];
+    }
+    syn_lo_hi<8>(lo, hi, out);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) E[((zb * 2 + b2) * C + c) * ES + xl + i] = out[i];
+  }
+  __syncthreads();
+  // dim-2 synthesis (runs of 4), then z synthesis and the store
+  {
+    const int x = tid % P, yl = (tid / P) * 4;
+    float o[2][4];
+#pragma unroll
+    for (int zb = 0; zb < 2; ++zb) {
+      float lo[7], hi[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        lo[j] = E[((zb * 2 + 0) * C + yl / 2 + j) * ES + x];
+        hi[j] = E[((zb * 2 + 1) * C + yl / 2 + j) * ES + x];
+      }
+      syn_lo_hi<4>(lo, hi, o[zb]);
+    }
+    float* Ye = Y + pq * pmod_i(2 * t3 - sh, r);
+    float* Yo = Y + pq * pmod_i(2 * t3 - 1 - sh, r);
+    if (x0 + x < p) {
+      const size_t row = (size_t)ridx[x + 9];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (y0 + yl + i >= q) continue;
+        const size_t off = row + (size_t)p * cidx[yl + i + 9];
+        Ye[off] = fCc * o[0][i] - fCc * o[1][i];
+        Yo[off] = fCc * o[0][i] + fCc * o[1][i];
+      }
+    }
+  }
+}
+
+namespace fl {
+constexpr int P = 32, R = 2 * P + 8, ZS = R + 1, OS = P + 1;
+}
+
+__global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict__ X, int p, int q, int r, int sh,
+                                                         float* __restrict__ Y) {
+  using namespace fl;
+  __shared__ float Z[R * ZS];   // z-lowpass plane [col][row]; later the output tile [c][t]
+  __shared__ float T2[P * ZS];  // dim-2 lowpass [c][row]
+  __shared__ int ridx[R], cidx[R];
+  const int tid = threadIdx.x;
+  const int t1b = blockIdx.x * P, t2b = blockIdx.y * P, t3 = blockIdx.z;
+  if (tid < R)
+    ridx[tid] = pmod_i(2 * t1b - 9 - sh + tid, p);
+  else if (tid < 2 * R)
+    cidx[tid - R] = pmod_i(2 * t2b - 9 - sh + tid - R, q);
+  const size_t pq = (size_t)p * q;
+  const float* X0 = X + pq * pmod_i(2 * t3 - sh, r);
+  const float* X1 = X + pq * pmod_i(2 * t3 - 1 - sh, r);
+  __syncthreads();
+  for (int e = tid; e < R * R; e += 256) {
+    const int rr = e % R, cc = e / R;
+    const size_t o = (size_t)ridx[rr] + (size_t)p * cidx[cc];
+    Z[cc * ZS + rr] = fCc * X0[o] + fCc * X1[o];
+  }
+  __syncthreads();
+  for (int e = tid; e < 4 * R; e += 256) {
+    const int rr = e % R, c0 = (e / R) * 8;
+    float v[24], lo[8];
+#pragma unroll
+    for (int j = 0; j < 24; ++j) v[j] = Z[(2 * c0 + j) * ZS + rr];
+    ana_lo<8>(v, lo);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) T2[(c0 + i) * ZS + rr] = lo[i];
+  }
+  __syncthreads();
+  float* Out = Z;
+  if (tid < 4 * P) {
+    const int c = tid % P, t0 = (tid / P) * 8;
+    float v[24], lo[8];
+#pragma unroll
+    for (int j = 0; j < 24; ++j) v[j] = T2[c * ZS + 2 * t0 + j];
+    ana_lo<8>(v, lo);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Out[c * OS + t0 + i] = lo[i];
+  }
+  __syncthreads();
+  const int ph = p / 2, qh = q / 2;
+  for (int e = tid; e < P * P; e += 256) {
+    const int t = e % P, c = e / P;
+    const int t1 = t1b + t, t2 = t2b + c;
+    if (t1 < ph && t2 < qh) Y[t1 + (size_t)ph * (t2 + (size_t)qh * t3)] = Out[c * OS + t];
+  }
+}
+
 // --------------------------------------------------------------- driver
+// Force the generic (per-tap predicated) kernels everywhere; set through
+// GBM3D_HT_GENERIC=1 for cross-checking the two implementations.
+static const bool g_ht_generic = [] {
+  const char* e = getenv("GBM3D_HT_GENERIC");
+  return e && e[0] == '1';
+}();
+
 size_t ht_work_floats(int M, int N, int K, int levels, int cycleSpin) {
   size_t vol = (size_t)M * N * K, tot = vol * cycleSpin;
   for (int l = 1; l <= levels; ++l) tot += 2 * (vol >> (3 * l)) * cycleSpin;
@@ -235,7 +472,10 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
       const int p = M >> l, q = N >> l, r = K >> l;
       const float* in = l == 0 ? vol : w.lo[l - 1][s];
       dim3 grid((p / 2 + LT - 1) / LT, (q / 2 + LT - 1) / LT, r / 2);
-      hipLaunchKernelGGL(ht_lo_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s]);
+      if (p >= 10 && q >= 10 && !g_ht_generic)
+        hipLaunchKernelGGL(ht_lo_fast_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s]);
+      else
+        hipLaunchKernelGGL(ht_lo_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s]);
       GBM3D_TRY(check_launch("ht_lo"));
     }
     // synthesis chain, coarse to fine
@@ -245,7 +485,10 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
       const float* ylo = (l == levels - 1) ? w.lo[l][s] : w.rec[l][s];
       float* out = l == 0 ? w.vrec + (size_t)M * N * K * s : w.rec[l - 1][s];
       dim3 grid((p + SP - 1) / SP, (q + SP - 1) / SP, r / 2);
-      hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out);
+      if (p >= 10 && q >= 10 && !g_ht_generic)
+        hipLaunchKernelGGL(ht_syn_fast_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out);
+      else
+        hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out);
       GBM3D_TRY(check_launch("ht_syn"));
     }
   }
