@@ -54,10 +54,19 @@ def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2):
         G = MN // (8 * 8)
         out["agg2"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G * K * 4 + G * 4 + 2 * MN * 4),
                            launches=stage_n["agg2"], ms=stage_ms["agg2"] / stage_n["agg2"], peak=PEAK_HBM_GBS)
-    if stage_n["ht"]:
-        # read the volume, write the filtered volume, per match spin
-        out["ht"] = dict(bound="hbm", unit="GB/s", work=2.0 * vol_bytes, launches=n1,
-                         ms=stage_ms["ht"] / n1, peak=PEAK_HBM_GBS)
+    if stage_n["ht_syn0"]:
+        # finest-level threshold + synthesis, one cycle spin per launch: read the
+        # mosaic volume, write the rebuilt volume (the coarse level read is 1/8 of it)
+        out["ht_syn0"] = dict(bound="hbm", unit="GB/s", work=2.0 * vol_bytes, launches=stage_n["ht_syn0"],
+                              ms=stage_ms["ht_syn0"] / stage_n["ht_syn0"], peak=PEAK_HBM_GBS)
+    if stage_n["agg1_prep"]:
+        # read both cycle-spin volumes, write Vavg
+        out["agg1_prep"] = dict(bound="hbm", unit="GB/s", work=3.0 * vol_bytes, launches=stage_n["agg1_prep"],
+                                ms=stage_ms["agg1_prep"] / stage_n["agg1_prep"], peak=PEAK_HBM_GBS)
+    if stage_n["agg1"]:
+        G1 = MN // (d1 * d1)
+        out["agg1"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G1 * K * 4 + G1 * 4 + 2 * MN * 4),
+                           launches=stage_n["agg1"], ms=stage_ms["agg1"] / stage_n["agg1"], peak=PEAK_HBM_GBS)
     for k, v in out.items():
         s = v["ms"] * 1e-3
         v["achieved"] = v["work"] / s / (1e12 if v["unit"] == "TFLOP/s" else 1e9)
@@ -88,6 +97,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-only", action="store_true", help="warmup + steps only (for rocprofv3 runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="with --profile-only: also run one device copy of MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (one image per launch); --gpus must be 1", file=sys.stderr)
@@ -122,6 +133,11 @@ def main():
     wall = time.perf_counter() - t0
     ms = e0.elapsed_ms(e1) / args.steps
     if args.profile_only:
+        if args.calibrate_copy:
+            nb = args.calibrate_copy << 20
+            a, b = device.DeviceArray((nb // 4,)), device.DeviceArray((nb // 4,))
+            ctx.device_copy(b.ptr, a.ptr, nb)
+            stream.synchronize()
         print(json.dumps({"ms_per_step": ms}))
         return
 

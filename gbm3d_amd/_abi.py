@@ -23,7 +23,7 @@ GBM3D_ERR_MISSING_FIELD = 5
 FILT = {"ht": 0, "median": 1}
 FILT_NAME = {v: k for k, v in FILT.items()}
 
-STAGES = ("pad", "match1", "ht", "agg1", "match2", "wiener", "agg2", "norm")
+STAGES = ("pad", "match1", "ht_lo", "ht_syn", "ht_syn0", "agg1_prep", "agg1", "match2", "wiener", "agg2", "norm")
 
 LIB_NAME = "libgbm3d.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -137,6 +137,7 @@ SIGNATURES = {
     "gbm3d_wiener": (_I, [_P, _P, _P, _I, _I, _I, _F, _P, _P]),
     "gbm3d_aggregate_wie": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "gbm3d_set_timing": (_I, [_P, _I]),
+    "gbm3d_device_copy": (_I, [_P, _P, _P, C.c_size_t]),
     "gbm3d_get_stage_times": (_I, [_P, _P, _P]),
 }
 

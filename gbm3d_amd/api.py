@@ -104,6 +104,9 @@ class Context:
         check(self.lib.gbm3d_get_stage_times(self.handle, ptr(ms), ptr(cnt)))
         return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(_abi.STAGES)}
 
+    def device_copy(self, dst: int, src: int, nbytes: int):
+        check(self.lib.gbm3d_device_copy(self.handle, C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes)))
+
     # ------------------------------------------------------------ full path
     def denoise(self, I, sigma: float, opts: dict | None = None):
         I = np.asfortranarray(I, dtype=np.float32)

@@ -72,29 +72,42 @@ int ensure(char** buf, size_t* have, size_t need) {
   return GBM3D_OK;
 }
 
+void mark_begin(gbm3d_ctx* c, int stage) {
+  if (!c->timing) return;
+  if (c->pool_used + 2 > c->pool.size()) {
+    hipEvent_t e1, e2;
+    if (hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess) {
+      c->timing = false;
+      return;
+    }
+    c->pool.push_back(e1);
+    c->pool.push_back(e2);
+  }
+  hipEvent_t a = c->pool[c->pool_used++];
+  hipEvent_t b = c->pool[c->pool_used++];
+  (void)hipEventRecord(a, c->stream);
+  c->marks.push_back({stage, a, b});
+}
+
+void mark_end(gbm3d_ctx* c, int stage) {
+  if (!c->timing || c->marks.empty() || c->marks.back().stage != stage) return;
+  (void)hipEventRecord(c->marks.back().b, c->stream);
+}
+
+void ht_hook(void* user, int stage, int begin) {
+  gbm3d_ctx* c = static_cast<gbm3d_ctx*>(user);
+  if (begin)
+    mark_begin(c, stage);
+  else
+    mark_end(c, stage);
+}
+
+// HIP events around a group of launches on the context stream (when enabled).
 struct StageTimer {
   gbm3d_ctx* c;
   int stage;
-  hipEvent_t b = nullptr;
-  StageTimer(gbm3d_ctx* ctx, int s) : c(ctx), stage(s) {
-    if (!c->timing) return;
-    if (c->pool_used + 2 > c->pool.size()) {
-      hipEvent_t e1, e2;
-      if (hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess) {
-        c->timing = false;
-        return;
-      }
-      c->pool.push_back(e1);
-      c->pool.push_back(e2);
-    }
-    hipEvent_t a = c->pool[c->pool_used++];
-    b = c->pool[c->pool_used++];
-    (void)hipEventRecord(a, c->stream);
-    c->marks.push_back({stage, a, b});
-  }
-  ~StageTimer() {
-    if (c->timing && b) (void)hipEventRecord(b, c->stream);
-  }
+  StageTimer(gbm3d_ctx* ctx, int s) : c(ctx), stage(s) { mark_begin(c, s); }
+  ~StageTimer() { mark_end(c, stage); }
 };
 
 int validate(const gbm3d_opts& o, int m, int n, int* M, int* N) {
@@ -204,13 +217,13 @@ int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const
       StageTimer t(c, GBM3D_STAGE_MATCH1);
       GBM3D_TRY(launch_match(b.pad1, b.bn, p.M, p.N, p.bs, p.d2, p.K, b.S1, b.vol, st));
     }
+    GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, st, ht_hook, c));
     {
-      StageTimer t(c, GBM3D_STAGE_HT);
-      GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, st));
+      StageTimer t(c, GBM3D_STAGE_AGG1_PREP);
+      GBM3D_TRY(launch_agg1_prep(nullptr, 0, b.vavg, p.M, p.N, p.K, p.bs, nullptr, b.w1, st));
     }
     {
       StageTimer t(c, GBM3D_STAGE_AGG1);
-      GBM3D_TRY(launch_agg1_prep(hw.vrec, o.cycleSpin, nullptr, p.M, p.N, p.K, p.bs, b.vavg, b.w1, st));
       GBM3D_TRY(launch_aggregate_m(1, b.vavg, b.w1, b.S1, masks1, p.M, p.N, p.bs, p.d2, p.K, sp, b.U1s + MN * ii,
                                    b.W1s + MN * ii, st));
     }
@@ -368,6 +381,22 @@ int gbm3d_set_timing(gbm3d_ctx* c, int enable) {
   return GBM3D_OK;
 }
 
+__global__ void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int gbm3d_device_copy(gbm3d_ctx* c, void* dst, const void* src, size_t nbytes) {
+  GBM3D_TRY(set_dev(c));
+  if (!dst || !src || nbytes % 16) return fail(GBM3D_ERR_ARG, "device_copy: NULL or size not a multiple of 16");
+  const size_t n = nbytes / 16;
+  if (n == 0) return GBM3D_OK;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, c->stream, static_cast<const float4*>(src),
+                     static_cast<float4*>(dst), n);
+  return check_launch("device_copy");
+}
+
 int gbm3d_get_stage_times(gbm3d_ctx* c, double* ms, int* launches) {
   GBM3D_TRY(set_dev(c));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
@@ -468,8 +497,7 @@ int gbm3d_ht_filter(gbm3d_ctx* c, const float* Volume, int m, int n, int K, int 
   HtWork hw;
   ht_work_bind(htw, m, n, K, levels, cycleSpin, &hw);
   GBM3D_TRY(upload(c, in, Volume, vol));
-  GBM3D_TRY(run_ht(in, m, n, K, levels, cycleSpin, tau, hw, c->stream));
-  GBM3D_TRY(launch_combine_spins(hw.vrec, cycleSpin, vol, out, c->stream));
+  GBM3D_TRY(run_ht(in, m, n, K, levels, cycleSpin, tau, hw, out, c->stream));
   GBM3D_TRY(download(c, Vavg, out, vol));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
   return GBM3D_OK;

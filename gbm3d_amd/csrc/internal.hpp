@@ -73,14 +73,18 @@ int launch_gather_groups(const float* pad, const uint32_t* S, int M, int N, int 
 
 // Wavelet hard threshold (all levels, all cycle spins) on the mosaic volume.
 struct HtWork {
-  float* vrec;                 // cycleSpin x (M*N*K)
+  float* vrec;                 // M*N*K cycle-spin accumulator (nullptr when cycleSpin == 1)
   std::vector<float*> lo[4];   // lo[l][spin]: level-(l+1) approximation
   std::vector<float*> rec[4];  // rec[l][spin]: reconstructed level-(l+1) approximation
 };
 size_t ht_work_floats(int M, int N, int K, int levels, int cycleSpin);
 void ht_work_bind(float* base, int M, int N, int K, int levels, int cycleSpin, HtWork* w);
+// hook(user, stage, 1) before and hook(user, stage, 0) after each kernel group
+// (stages GBM3D_STAGE_HT_LO / _HT_SYN / _HT_SYN0); may be null.
+typedef void (*StageHook)(void* user, int stage, int begin);
+// Writes Vavg = max(mean over cycle spins of the rebuilt volume, 0).
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau,
-           const HtWork& w, hipStream_t st);
+           const HtWork& w, float* vavg, hipStream_t st, StageHook hook = nullptr, void* user = nullptr);
 
 // Step-1 aggregation prep: Vavg = max(mean over spins of vrec, 0) (or a given
 // Vavg when nspin == 0) and per-group weights 1/(TV(Vavg)+eps).

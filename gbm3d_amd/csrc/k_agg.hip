@@ -274,11 +274,145 @@ __global__ __launch_bounds__(256) void aggregate_kernel(const float* __restrict_
   }
 }
 
+// Batched variant.  Lane e of the wave owns pixel (e % D, e / D) (+64i) of
+// every block; overlapping blocks come from the ballot in (y, x, k) order,
+// eight at a time: their loads are issued together, then each block is added
+// into the LDS tile accumulator in order.
+template <int STEP, int D>
+__global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __restrict__ blocks,
+                                                                const float* __restrict__ wgt,
+                                                                const uint32_t* __restrict__ S,
+                                                                const float* __restrict__ masks, int M, int N,
+                                                                int d2, int K, int spin, float* __restrict__ U,
+                                                                float* __restrict__ W) {
+  constexpr int DD = D * D, PER = (DD + 63) / 64, NB = 8;
+  __shared__ float accU[4][AT * AT], accW[4][AT * AT];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tilesR = (M + AT - 1) / AT, tilesC = (N + AT - 1) / AT;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+  if (tile >= tilesR * tilesC) return;
+  const int R0 = (tile % tilesR) * AT, C0 = (tile / tilesR) * AT;
+  float* aU = accU[wave];
+  float* aW = accW[wave];
+  for (int e = lane; e < AT * AT; e += 64) {
+    aU[e] = 0.f;
+    aW[e] = 0.f;
+  }
+  float mk[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = lane + 64 * i;
+    mk[i] = e < DD ? masks[(STEP == 1 ? 0 : DD) + e] : 0.f;
+  }
+  wave_sync();
+  const int s = d2 / 2, Mt = M / D, Nt = N / D;
+  auto first = [&](int r) {
+    const int x = r + s - d2 - D + 2;
+    return x > 0 ? (x + D - 1) / D : 0;
+  };
+  const int ylo = first(R0), yhi = min(Mt - 1, (R0 + AT - 1 + s) / D);
+  const int xlo = first(C0), xhi = min(Nt - 1, (C0 + AT - 1 + s) / D);
+  const int ny = yhi - ylo + 1, nx = xhi - xlo + 1;
+  const int ncand = (ny > 0 && nx > 0) ? ny * nx * K : 0;
+  const size_t MN = (size_t)M * N;
+  for (int base = 0; base < ncand; base += 64) {
+    const int t = base + lane;
+    int br = 0, bc = 0, g = 0, k = 0;
+    float wv = 0.f;
+    bool hit = false;
+    if (t < ncand) {
+      k = t % K;
+      const int yx = t / K;
+      const int y = ylo + yx / nx, x = xlo + yx % nx;
+      g = y + Mt * x;
+      const uint32_t idx = S[(size_t)K * g + k] - 1u;
+      br = y * D + (int)(idx % (uint32_t)d2) - s - R0;  // block origin relative to the tile
+      bc = x * D + (int)(idx / (uint32_t)d2) - s - C0;
+      hit = br < AT && br + D > 0 && bc < AT && bc + D > 0;
+      if (hit) wv = wgt[g];
+    }
+    uint64_t mask = __ballot(hit);
+    while (mask) {
+      int src[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        src[j] = mask ? __builtin_ctzll(mask) : -1;
+        mask &= mask ? mask - 1 : 0;
+      }
+      float val[NB][PER];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (src[j] < 0) continue;
+        const int gb = __builtin_amdgcn_readlane(g, src[j]), kb = __builtin_amdgcn_readlane(k, src[j]);
+        const int rr = __builtin_amdgcn_readlane(br, src[j]), cc = __builtin_amdgcn_readlane(bc, src[j]);
+        const float* bp = STEP == 1 ? blocks + (size_t)((gb % Mt) * D) + (size_t)M * ((gb / Mt) * D) + MN * kb
+                                    : blocks + (size_t)DD * (kb + (size_t)K * gb);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+          const int e = lane + 64 * i;
+          const int u = e % D, v = e / D;
+          const int pr = rr + u, pc = cc + v;
+          val[j][i] = 0.f;
+          if (e < DD && pr >= 0 && pr < AT && pc >= 0 && pc < AT)
+            val[j][i] = STEP == 1 ? bp[u + (size_t)M * v] : bp[e];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (src[j] < 0) continue;
+        const int rr = __builtin_amdgcn_readlane(br, src[j]), cc = __builtin_amdgcn_readlane(bc, src[j]);
+        const float wg = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), src[j]));
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+          const int e = lane + 64 * i;
+          const int u = e % D, v = e / D;
+          const int pr = rr + u, pc = cc + v;
+          if (e < DD && pr >= 0 && pr < AT && pc >= 0 && pc < AT) {
+            const int o = pr + AT * pc;
+            if (STEP == 1) {
+              const float c = (wg * val[j][i]) * mk[i];  // weight*blocks(:,:,i).*mask in single
+              aU[o] = aU[o] + c;
+              aW[o] = aW[o] + wg * mk[i];
+            } else {
+              const float wt = mk[i] / wg;  // w / norm(filt)^2
+              aU[o] = aU[o] + wt * val[j][i];
+              aW[o] = aW[o] + wt;
+            }
+          }
+        }
+        wave_sync();
+      }
+    }
+  }
+  for (int e = lane; e < AT * AT; e += 64) {
+    const int pr = R0 + (e % AT), pc = C0 + (e / AT);
+    if (pr >= M || pc >= N) continue;
+    const size_t o = (size_t)pmod_i(pr - spin, M) + (size_t)M * pmod_i(pc - spin, N);
+    U[o] = aU[e];
+    W[o] = aW[e];
+  }
+}
+
 int launch_aggregate_m(int step, const float* blocks, const float* wgt, const uint32_t* S, const float* masks, int M,
                        int N, int d, int d2, int K, int spin, float* U, float* W, hipStream_t st) {
   if (M % d || N % d) return fail(GBM3D_ERR_ARG, "aggregate: size");
   const int tiles = ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
   dim3 grid((tiles + 3) / 4);
+#define GBM3D_AGG_CASE(DD)                                                                                   \
+  if (d == DD) {                                                                                             \
+    if (step == 1)                                                                                           \
+      hipLaunchKernelGGL((aggregate_batched_kernel<1, DD>), grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d2, \
+                         K, spin, U, W);                                                                     \
+    else                                                                                                     \
+      hipLaunchKernelGGL((aggregate_batched_kernel<2, DD>), grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d2, \
+                         K, spin, U, W);                                                                     \
+    return check_launch("aggregate");                                                                        \
+  }
+  GBM3D_AGG_CASE(4)
+  GBM3D_AGG_CASE(8)
+  GBM3D_AGG_CASE(16)
+  GBM3D_AGG_CASE(32)
+#undef GBM3D_AGG_CASE
   if (step == 1)
     hipLaunchKernelGGL(aggregate_kernel<1>, grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d, d2, K, spin, U, W);
   else

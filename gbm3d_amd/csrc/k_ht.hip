@@ -94,9 +94,29 @@ constexpr int SRP = SR + 1;
 constexpr int CPP = CP + 1;
 constexpr int SPP = SP + 1;
 
+// Store modes of the finest level (cycle spins accumulate into one volume):
+// kStore  Y = v;  kAccum  Y = Y + v;  kFinal  Y = max((Y + v)/cs, 0), with Y
+// read from acc (acc == nullptr reads 0).  Coarser levels always kStore.
+enum { kStore = 0, kAccum = 1, kFinal = 2 };
+
+__device__ __forceinline__ void put(float* Y, const float* acc, size_t o, float v, int mode, float cs) {
+  if (mode == kStore) {
+    Y[o] = v;
+  } else {
+    const float a = acc ? acc[o] : 0.f;
+    const float t = a + v;
+    if (mode == kAccum) {
+      Y[o] = t;
+    } else {
+      const float mean = t / cs;
+      Y[o] = mean > 0.f ? mean : 0.f;  // max(mean(Vrec,4), 0)
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
                                                      int p, int q, int r, int sh, Tau7 tau,
-                                                     float* __restrict__ Y) {
+                                                     float* Y, const float* acc, int mode, float cs) {
   __shared__ float Z[2][SR * SRP];   // z-lo / z-hi planes, [col][row]
   __shared__ float A[2][CP * SRP];   // dim-2 lo / hi, [c][row]
   __shared__ float B[4][CP * CPP];   // bands b1 | b2<<1, [c][t]
@@ -187,8 +207,8 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
     const int x = e % SP, y = e / SP;
     if (x0 + x >= p || y0 + y >= q) continue;
     const size_t o = (size_t)pmod_i(x0 + x - sh, p) + (size_t)p * pmod_i(y0 + y - sh, q);
-    Ye[o] = fCc * out[0][i] - fCc * out[1][i];
-    Yo[o] = fCc * out[0][i] + fCc * out[1][i];
+    put(Ye, acc ? acc + (Ye - Y) : nullptr, o, fCc * out[0][i] - fCc * out[1][i], mode, cs);
+    put(Yo, acc ? acc + (Yo - Y) : nullptr, o, fCc * out[0][i] + fCc * out[1][i], mode, cs);
   }
 }
 
@@ -243,7 +263,7 @@ __device__ __forceinline__ void syn_lo_hi(const float (&lo)[R2 / 2 + 5], const f
 
 namespace fs {
 constexpr int P = 32, C = 21, R = 50;    // output tile, coefficients, input rows per edge
-constexpr int ZS = R + 1, AS = R + 1, BS = C + 1, ES = P + 1;
+constexpr int ZS = R + 1, AS = R + 1, BS = C + 1, ES = P + 16;  // ES: the two half-waves of phase 5 hit disjoint banks
 constexpr int REG0 = 2 * R * ZS;  // Z (later B)
 constexpr int REG1 = 4 * C * AS;  // A (later E)
 static_assert(8 * C * BS <= REG0, "B fits in Z");
@@ -252,7 +272,8 @@ static_assert(4 * C * ES <= REG1, "E fits in A");
 
 __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
                                                           int p, int q, int r, int sh, Tau7 tau,
-                                                          float* __restrict__ Y) {
+                                                          float* Y, const float* acc, int mode,
+                                                          float cs) {
   using namespace fs;
   __shared__ float lds[REG0 + REG1];
   __shared__ int ridx[R], cidx[R];
@@ -261,7 +282,9 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
   float* A = lds + REG0;
   float* E = lds + REG0;
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * P, y0 = blockIdx.y * P, t3 = blockIdx.z;
+  const int nbx = gridDim.x, nby = gridDim.y;
+  const int lb = xcd_remap(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
+  const int x0 = (lb % nbx) * P, y0 = ((lb / nbx) % nby) * P, t3 = lb / (nbx * nby);
   if (tid < R)
     ridx[tid] = pmod_i(x0 - 9 - sh + tid, p);
   else if (tid < 2 * R)
@@ -270,26 +293,42 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
   const float* X0 = X + pq * pmod_i(2 * t3 - sh, r);
   const float* X1 = X + pq * pmod_i(2 * t3 - 1 - sh, r);
   __syncthreads();
-  // z analysis of the pair (2t3-1, 2t3)
-  for (int e = tid; e < R * R; e += 256) {
-    const int rr = e % R, cc = e / R;
-    const size_t o = (size_t)ridx[rr] + (size_t)p * cidx[cc];
-    const float a = X0[o], b = X1[o];
-    Z[cc * ZS + rr] = fCc * a + fCc * b;
-    Z[R * ZS + cc * ZS + rr] = fCc * b - fCc * a;
+  // z analysis of the pair (2t3-1, 2t3); all loads of a thread issued together
+  {
+    constexpr int NIT = (R * R + 255) / 256;
+    float a[NIT], b[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = tid + 256 * it;
+      a[it] = b[it] = 0.f;
+      if (e < R * R) {
+        const size_t o = (size_t)ridx[e % R] + (size_t)p * cidx[e / R];
+        a[it] = X0[o];
+        b[it] = X1[o];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = tid + 256 * it;
+      if (e < R * R) {
+        const int rr = e % R, cc = e / R;
+        Z[cc * ZS + rr] = fCc * a[it] + fCc * b[it];
+        Z[R * ZS + cc * ZS + rr] = fCc * b[it] - fCc * a[it];
+      }
+    }
   }
   __syncthreads();
-  // dim-2 analysis: runs of 7 coefficients per (z band, row)
-  for (int e = tid; e < 2 * 3 * R; e += 256) {
-    const int rr = e % R, run = (e / R) % 3, zb = e / (3 * R);
-    const int c0 = run * 7;
-    float v[22];
+  // dim-2 analysis: two runs of 11 coefficients, [0,11) and [10,21), per (z band, row)
+  if (tid < 2 * 2 * R) {
+    const int rr = tid % R, run = (tid / R) & 1, zb = tid / (2 * R);
+    const int c0 = run * 10;
+    float v[30];
 #pragma unroll
-    for (int j = 0; j < 22; ++j) v[j] = Z[zb * R * ZS + (2 * c0 + j) * ZS + rr];
-    float lo[7], hi[7];
-    ana_lo_hi<7>(v, lo, hi);
+    for (int j = 0; j < 30; ++j) v[j] = Z[zb * R * ZS + (2 * c0 + j) * ZS + rr];
+    float lo[11], hi[11];
+    ana_lo_hi<11>(v, lo, hi);
 #pragma unroll
-    for (int i = 0; i < 7; ++i) {
+    for (int i = 0; i < 11; ++i) {
       A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
       A[((zb * 2 + 1) * C + c0 + i) * AS + rr] = hi[i];
     }
@@ -297,14 +336,14 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
   __syncthreads();
   // dim-1 analysis + hard threshold (the approximation is the rebuilt coarser level)
   const int ph = p >> 1, qh = q >> 1;
-  for (int e = tid; e < 2 * 2 * 3 * C; e += 256) {
-    const int c = e % C, run = (e / C) % 3, b2 = (e / (3 * C)) & 1, zb = e / (6 * C);
-    const int t0 = run * 7;
-    float v[22];
+  if (tid < 2 * 2 * 2 * C) {
+    const int c = tid % C, run = (tid / C) & 1, b2 = (tid / (2 * C)) & 1, zb = tid / (4 * C);
+    const int t0 = run * 10;
+    float v[30];
 #pragma unroll
-    for (int j = 0; j < 22; ++j) v[j] = A[((zb * 2 + b2) * C + c) * AS + 2 * t0 + j];
-    float lo[7], hi[7];
-    ana_lo_hi<7>(v, lo, hi);
+    for (int j = 0; j < 30; ++j) v[j] = A[((zb * 2 + b2) * C + c) * AS + 2 * t0 + j];
+    float lo[11], hi[11];
+    ana_lo_hi<11>(v, lo, hi);
     const int bl = (b2 << 1) | (zb << 2), bh = bl | 1;
     const float tl = tau.t[band_j(bl)], th = tau.t[band_j(bh)];
     float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + t0;
@@ -312,30 +351,30 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
     if (bl == 0) {
       const size_t col = (size_t)ph * ((size_t)pmod_i(y0 / 2 + c, qh) + (size_t)qh * t3);
 #pragma unroll
-      for (int i = 0; i < 7; ++i) Bl[i] = Ylo[(size_t)pmod_i(x0 / 2 + t0 + i, ph) + col];
+      for (int i = 0; i < 11; ++i) Bl[i] = Ylo[(size_t)pmod_i(x0 / 2 + t0 + i, ph) + col];
     } else {
 #pragma unroll
-      for (int i = 0; i < 7; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
+      for (int i = 0; i < 11; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
     }
 #pragma unroll
-    for (int i = 0; i < 7; ++i) Bh[i] = hi[i] * (fabsf(hi[i]) > th ? 1.f : 0.f);
+    for (int i = 0; i < 11; ++i) Bh[i] = hi[i] * (fabsf(hi[i]) > th ? 1.f : 0.f);
   }
   __syncthreads();
-  // dim-1 synthesis: runs of 8 outputs per (z band, dim-2 band, coefficient column)
-  for (int e = tid; e < 2 * 2 * 4 * C; e += 256) {
-    const int c = e % C, xr = (e / C) & 3, b2 = (e / (4 * C)) & 1, zb = e / (8 * C);
-    const int xl = xr * 8;
+  // dim-1 synthesis: runs of 16 outputs per (z band, dim-2 band, coefficient column)
+  if (tid < 2 * 2 * 2 * C) {
+    const int c = tid % C, xr = (tid / C) & 1, b2 = (tid / (2 * C)) & 1, zb = tid / (4 * C);
+    const int xl = xr * 16;
     const float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + xl / 2;
     const float* Bh = Bl + C * BS;
-    float lo[9], hi[9], out[8];
+    float lo[13], hi[13], out[16];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
+    for (int j = 0; j < 13; ++j) {
       lo[j] = Bl[j];
       hi[j] = Bh[j];
     }
-    syn_lo_hi<8>(lo, hi, out);
+    syn_lo_hi<16>(lo, hi, out);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) E[((zb * 2 + b2) * C + c) * ES + xl + i] = out[i];
+    for (int i = 0; i < 16; ++i) E[((zb * 2 + b2) * C + c) * ES + xl + i] = out[i];
   }
   __syncthreads();
   // dim-2 synthesis (runs of 4), then z synthesis and the store
@@ -360,8 +399,8 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
       for (int i = 0; i < 4; ++i) {
         if (y0 + yl + i >= q) continue;
         const size_t off = row + (size_t)p * cidx[yl + i + 9];
-        Ye[off] = fCc * o[0][i] - fCc * o[1][i];
-        Yo[off] = fCc * o[0][i] + fCc * o[1][i];
+        put(Ye, acc ? acc + (Ye - Y) : nullptr, off, fCc * o[0][i] - fCc * o[1][i], mode, cs);
+        put(Yo, acc ? acc + (Yo - Y) : nullptr, off, fCc * o[0][i] + fCc * o[1][i], mode, cs);
       }
     }
   }
@@ -378,7 +417,9 @@ __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict
   __shared__ float T2[P * ZS];  // dim-2 lowpass [c][row]
   __shared__ int ridx[R], cidx[R];
   const int tid = threadIdx.x;
-  const int t1b = blockIdx.x * P, t2b = blockIdx.y * P, t3 = blockIdx.z;
+  const int nbx = gridDim.x, nby = gridDim.y;
+  const int lb = xcd_remap(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
+  const int t1b = (lb % nbx) * P, t2b = ((lb / nbx) % nby) * P, t3 = lb / (nbx * nby);
   if (tid < R)
     ridx[tid] = pmod_i(2 * t1b - 9 - sh + tid, p);
   else if (tid < 2 * R)
@@ -387,10 +428,24 @@ __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict
   const float* X0 = X + pq * pmod_i(2 * t3 - sh, r);
   const float* X1 = X + pq * pmod_i(2 * t3 - 1 - sh, r);
   __syncthreads();
-  for (int e = tid; e < R * R; e += 256) {
-    const int rr = e % R, cc = e / R;
-    const size_t o = (size_t)ridx[rr] + (size_t)p * cidx[cc];
-    Z[cc * ZS + rr] = fCc * X0[o] + fCc * X1[o];
+  {
+    constexpr int NIT = (R * R + 255) / 256;
+    float a[NIT], b[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = tid + 256 * it;
+      a[it] = b[it] = 0.f;
+      if (e < R * R) {
+        const size_t o = (size_t)ridx[e % R] + (size_t)p * cidx[e / R];
+        a[it] = X0[o];
+        b[it] = X1[o];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = tid + 256 * it;
+      if (e < R * R) Z[(e / R) * ZS + e % R] = fCc * a[it] + fCc * b[it];
+    }
   }
   __syncthreads();
   for (int e = tid; e < 4 * R; e += 256) {
@@ -431,15 +486,15 @@ static const bool g_ht_generic = [] {
 }();
 
 size_t ht_work_floats(int M, int N, int K, int levels, int cycleSpin) {
-  size_t vol = (size_t)M * N * K, tot = vol * cycleSpin;
+  size_t vol = (size_t)M * N * K, tot = cycleSpin > 1 ? vol : 0;
   for (int l = 1; l <= levels; ++l) tot += 2 * (vol >> (3 * l)) * cycleSpin;
   return tot;
 }
 
 void ht_work_bind(float* base, int M, int N, int K, int levels, int cycleSpin, HtWork* w) {
   size_t vol = (size_t)M * N * K;
-  w->vrec = base;
-  float* p = base + vol * cycleSpin;
+  w->vrec = cycleSpin > 1 ? base : nullptr;
+  float* p = base + (cycleSpin > 1 ? vol : 0);
   for (int l = 0; l < 4; ++l) {
     w->lo[l].assign(cycleSpin, nullptr);
     w->rec[l].assign(cycleSpin, nullptr);
@@ -456,7 +511,10 @@ void ht_work_bind(float* base, int M, int N, int K, int levels, int cycleSpin, H
 }
 
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau, const HtWork& w,
-           hipStream_t st) {
+           float* vavg, hipStream_t st, StageHook hook, void* user) {
+  auto mark = [&](int stage, int begin) {
+    if (hook) hook(user, stage, begin);
+  };
   if (levels < 1 || levels > 4) return fail(GBM3D_ERR_UNSUPPORTED, "levels must be 1..4");
   const int f = 1 << levels;
   if (M % f || N % f || K % f) return fail(GBM3D_ERR_UNSUPPORTED, "image size and numMax must be multiples of 2^levels");
@@ -468,6 +526,7 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
   }
   for (int s = 0; s < cycleSpin; ++s) {
     // analysis chain (approximations only)
+    mark(GBM3D_STAGE_HT_LO, 1);
     for (int l = 0; l < levels; ++l) {
       const int p = M >> l, q = N >> l, r = K >> l;
       const float* in = l == 0 ? vol : w.lo[l - 1][s];
@@ -478,18 +537,30 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
         hipLaunchKernelGGL(ht_lo_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s]);
       GBM3D_TRY(check_launch("ht_lo"));
     }
+    mark(GBM3D_STAGE_HT_LO, 0);
     // synthesis chain, coarse to fine
     for (int l = levels - 1; l >= 0; --l) {
       const int p = M >> l, q = N >> l, r = K >> l;
       const float* in = l == 0 ? vol : w.lo[l - 1][s];
       const float* ylo = (l == levels - 1) ? w.lo[l][s] : w.rec[l][s];
-      float* out = l == 0 ? w.vrec + (size_t)M * N * K * s : w.rec[l - 1][s];
+      // finest level: spins 0..cs-2 accumulate in w.vrec, the last one writes
+      // Vavg = max(sum/cs, 0) (denoiseAndAggregateGPU.m:45)
+      const bool last = s == cycleSpin - 1;
+      float* out = l == 0 ? (last ? vavg : w.vrec) : w.rec[l - 1][s];
+      const float* acc = (l == 0 && s > 0) ? w.vrec : nullptr;
+      const int mode = l > 0 ? kStore : (last ? kFinal : (s == 0 ? kStore : kAccum));
+      const float cs = (float)cycleSpin;
       dim3 grid((p + SP - 1) / SP, (q + SP - 1) / SP, r / 2);
+      const int stage = l == 0 ? GBM3D_STAGE_HT_SYN0 : GBM3D_STAGE_HT_SYN;
+      mark(stage, 1);
       if (p >= 10 && q >= 10 && !g_ht_generic)
-        hipLaunchKernelGGL(ht_syn_fast_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out);
+        hipLaunchKernelGGL(ht_syn_fast_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out,
+                           acc, mode, cs);
       else
-        hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out);
+        hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out, acc,
+                           mode, cs);
       GBM3D_TRY(check_launch("ht_syn"));
+      mark(stage, 0);
     }
   }
   return GBM3D_OK;

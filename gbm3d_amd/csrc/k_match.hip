@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void match_kernel(const float* __restrict__ pa
   __shared__ uint64_t scr[4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int Mt = M / D, Nt = N / D;
-  const int g = blockIdx.x * 4 + wave;
+  const int g = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
   if (g >= Mt * Nt) return;
   const int ti = g % Mt, tj = g / Mt;
   const int P = M + D2, Pm = P - D + 1;

@@ -12,6 +12,15 @@ __device__ __forceinline__ int pmod_i(int a, int m) {
   return r < 0 ? r + m : r;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Map dispatch index b (of nb) to a logical index so that XCD x works on
+// one contiguous range of logical indices; neighbouring tiles, whose halos
+// overlap, then share an L2.  A bijection for any nb.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Make LDS writes of this wave visible to its other lanes (a wave executes in
 // lockstep; the fences stop the compiler from moving LDS accesses across).
 __device__ __forceinline__ void wave_sync() {

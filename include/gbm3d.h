@@ -163,17 +163,24 @@ int gbm3d_aggregate_wie(gbm3d_ctx* ctx, const float* V2f, const float* gain2,
  * Per-stage device time of the last gbm3d_denoise* call, measured with HIP
  * events on the stream the kernels ran on (enabled by gbm3d_set_timing). */
 enum gbm3d_stage {
-  GBM3D_STAGE_PAD = 0,      /* padded search images + block-norm maps   */
-  GBM3D_STAGE_MATCH1 = 1,   /* step-1 distances + top-K + volume        */
-  GBM3D_STAGE_HT = 2,       /* step-1 wavelet hard threshold (all levels) */
-  GBM3D_STAGE_AGG1 = 3,     /* step-1 weights + gather aggregation      */
-  GBM3D_STAGE_MATCH2 = 4,   /* step-2 distances + top-K                 */
-  GBM3D_STAGE_WIENER = 5,   /* step-2 Wiener filter                     */
-  GBM3D_STAGE_AGG2 = 6,     /* step-2 gather aggregation                */
-  GBM3D_STAGE_NORM = 7,     /* spin sums / normalisation / crop         */
-  GBM3D_NUM_STAGES = 8
+  GBM3D_STAGE_PAD = 0,        /* mirror pad, padded search images, block-norm maps   */
+  GBM3D_STAGE_MATCH1 = 1,     /* step-1 distances + top-K + volume (one kernel)       */
+  GBM3D_STAGE_HT_LO = 2,      /* step-1 wavelet analysis low-pass, all levels         */
+  GBM3D_STAGE_HT_SYN = 3,     /* step-1 threshold + synthesis, levels >= 2            */
+  GBM3D_STAGE_HT_SYN0 = 4,    /* step-1 threshold + synthesis of the finest level     */
+  GBM3D_STAGE_AGG1_PREP = 5,  /* step-1 cycle-spin mean, clip and group weights       */
+  GBM3D_STAGE_AGG1 = 6,       /* step-1 gather aggregation (one kernel)               */
+  GBM3D_STAGE_MATCH2 = 7,     /* step-2 distances + top-K (one kernel)                */
+  GBM3D_STAGE_WIENER = 8,     /* step-2 Wiener filter (one kernel)                    */
+  GBM3D_STAGE_AGG2 = 9,       /* step-2 gather aggregation (one kernel)               */
+  GBM3D_STAGE_NORM = 10,      /* spin sums / normalisation / crop                     */
+  GBM3D_NUM_STAGES = 11
 };
 int gbm3d_set_timing(gbm3d_ctx* ctx, int enable);
+/* Copy nbytes (a multiple of 16) between device buffers with a plain
+ * streaming kernel on the context stream: a kernel of known HBM traffic
+ * (nbytes read + nbytes written) that calibrates hardware byte counters. */
+int gbm3d_device_copy(gbm3d_ctx* ctx, void* dst, const void* src, size_t nbytes);
 /* ms[GBM3D_NUM_STAGES], launches[GBM3D_NUM_STAGES] (either may be NULL). */
 int gbm3d_get_stage_times(gbm3d_ctx* ctx, double* ms, int* launches);
 
