@@ -90,6 +90,47 @@ def cpu_baseline(threads: int):
             "psnr_db": synth.psnr(clean, U)}
 
 
+def precision_report(args):
+    """4096x4096, sigma 25, K=16, matchSize 48 (radius 24): BF16-MFMA distances
+    with FP32 rescoring vs the FP32 path; errors against an FP64 oracle run."""
+    import gbm3d_amd
+    import oracle
+    from gbm3d_amd import synth
+
+    size, opts = args.size or 4096, {"blockSize": 8, "blockSizeWie": 8, "matchSize": 48, "matchSpinsWie": [0, 4]}
+    clean, noisy = synth.make_pair(4096, size, size, SIGMA)
+    c32, cbf = gbm3d_amd.Context(0), gbm3d_amd.Context(0)
+    cbf.set_variant(match_bf16=True)
+    out = {"workload": f"precision: {size}x{size}, sigma 25, K 16, matchSize 48, 8x8 blocks", "size": size}
+    # index sets of step 1, spin 0, on the (block-multiple) noisy image
+    S32, _ = c32.match(noisy, 8, 48, 16, want_volume=False)
+    Sbf, _ = cbf.match(noisy, 8, 48, 16, want_volume=False)
+    out["index_tiles"] = int(S32.shape[1] * S32.shape[2])
+    out["index_identical_frac"] = float(np.mean(np.all(S32 == Sbf, axis=0)))
+    out["index_set_equal_frac"] = float(np.mean(np.all(np.sort(S32, 0) == np.sort(Sbf, 0), axis=0)))
+    times = {}
+    for name, c in (("fp32", c32), ("bf16_rescored", cbf)):
+        c.denoise(noisy, SIGMA, opts)
+        c.set_timing(True)
+        t0 = time.perf_counter()
+        U, o1 = c.denoise(noisy, SIGMA, opts)
+        dt = time.perf_counter() - t0
+        st = c.stage_times()
+        times[name] = (U, dt, st)
+    t0 = time.perf_counter()
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    U64, _ = oracle.gbm3d(noisy, SIGMA, opts, precision=64, nthreads=min(16, os.cpu_count() or 1))
+    out["fp64_oracle_s"] = round(time.perf_counter() - t0, 2)
+    for name, (U, dt, st) in times.items():
+        dev_ms = sum(v[0] for v in st.values())
+        out[name] = {"psnr_db": round(synth.psnr(clean, U), 4), "psnr_vs_fp64_oracle_db": round(synth.psnr(U64, U), 3),
+                     "max_abs_vs_fp64_oracle": float(np.abs(U - U64).max()),
+                     "match_ms": round(st["match1"][0] + st["match2"][0], 3), "device_ms": round(dev_ms, 3),
+                     "mpix_s_device": round(size * size / (dev_ms * 1e-3) / 1e6, 2)}
+    out["fp64_oracle_psnr_db"] = round(synth.psnr(clean, U64), 4)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +140,12 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="warmup + steps only (for rocprofv3 runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="with --profile-only: also run one device copy of MIB MiB (byte-counter calibration)")
+    ap.add_argument("--workload", default="plain", choices=["plain", "precision"])
+    ap.add_argument("--size", type=int, default=0, help="image side for --workload precision")
     args = ap.parse_args()
+    if args.workload == "precision":
+        precision_report(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (one image per launch); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
@@ -165,6 +211,29 @@ def main():
                 "frac": round(d["frac"], 4), "traffic": traffic, "kernel": dom,
                 "avg_launch_ms": round(d["ms"], 5)}
 
+    # precision variant: BF16-MFMA distances + exact FP32 rescoring (same timed loop)
+    variants = {}
+    ctx.set_variant(match_bf16=True)
+    for _ in range(args.warmup):
+        step()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    stream.synchronize()
+    ms_bf = e0.elapsed_ms(e1) / args.steps
+    Ubf = dU.to_host()
+    ctx.set_timing(True)
+    step()
+    st_bf = ctx.stage_times()
+    ctx.set_timing(False)
+    ctx.set_variant(match_bf16=False)
+    variants["bf16_match"] = {
+        "mpix_s": round(SIZE * SIZE / (ms_bf * 1e-3) / 1e6, 3), "ms_per_step": round(ms_bf, 4),
+        "psnr_db": round(synth.psnr(clean, Ubf), 4), "dpsnr_db": round(synth.psnr(clean, Ubf) - psnr, 4),
+        "match_ms": round(st_bf["match1"][0] + st_bf["match2"][0], 4),
+        "max_abs_vs_plain": float(np.abs(Ubf - U).max())}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(min(16, os.cpu_count() or 1))
@@ -184,6 +253,7 @@ def main():
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in rl.items()},
+        "variants": variants,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

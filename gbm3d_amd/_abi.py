@@ -51,6 +51,12 @@ class Opts(C.Structure):
     ]
 
 
+class Ext(C.Structure):
+    """struct gbm3d_ext (include/gbm3d.h)."""
+
+    _fields_ = [("match_bf16", C.c_int32), ("rescore", C.c_int32), ("reserved", C.c_int32 * 6)]
+
+
 _INT_FIELDS = ("numMax", "numMin", "levels", "cycleSpin", "tauMode", "blockSize",
                "blockSizeWie", "matchSize")
 
@@ -125,6 +131,8 @@ SIGNATURES = {
     "gbm3d_get_tau": (_I, [_I, _I, _D, _P]),
     "gbm3d_last_error": (C.c_char_p, []),
     "gbm3d_create": (_I, [_I, C.POINTER(_P)]),
+    "gbm3d_set_ext": (_I, [_P, C.POINTER(Ext)]),
+    "gbm3d_get_ext": (_I, [_P, C.POINTER(Ext)]),
     "gbm3d_destroy": (_I, [_P]),
     "gbm3d_set_stream": (_I, [_P, _P]),
     "gbm3d_synchronize": (_I, [_P]),

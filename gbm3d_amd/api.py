@@ -95,6 +95,14 @@ class Context:
     def synchronize(self):
         check(self.lib.gbm3d_synchronize(self.handle))
 
+    def set_variant(self, match_bf16: bool = False, rescore: int = 0):
+        """Precision variant: BF16-MFMA distances + exact FP32 rescoring of the
+        best `rescore` (>= 2*numMax) candidates per tile.  False = reference."""
+        e = _abi.Ext()
+        e.match_bf16 = int(bool(match_bf16))
+        e.rescore = int(rescore)
+        check(self.lib.gbm3d_set_ext(self.handle, C.byref(e)))
+
     def set_timing(self, on: bool):
         check(self.lib.gbm3d_set_timing(self.handle, int(bool(on))))
 

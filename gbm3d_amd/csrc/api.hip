@@ -20,6 +20,7 @@ struct gbm3d_ctx {
   char* io = nullptr;  // staging for the host-buffer entry points
   size_t io_bytes = 0;
   float* masks = nullptr;  // [4 block sizes][2][32*32]
+  gbm3d_ext ext{};
   bool timing = false;
   struct Mark {
     int stage;
@@ -147,7 +148,7 @@ struct Plan {
 
 // All device buffers of one denoise call.
 struct Bufs {
-  float *Ip, *pad1, *pad2, *bn, *vol, *vavg, *w1, *U1s, *W1s, *pilot, *V2f, *g2, *U2s, *W2s, *htw;
+  float *Ip, *pad1, *pad2, *bn, *smap, *vol, *vavg, *w1, *U1s, *W1s, *pilot, *V2f, *g2, *U2s, *W2s, *htw;
   uint32_t *S1, *S2;
 };
 
@@ -159,6 +160,7 @@ void carve(Carve& cv, const Plan& p, const gbm3d_opts& o, Bufs* b) {
   b->pad1 = cv.take<float>(P * Q);
   b->pad2 = cv.take<float>(P * Q);
   b->bn = cv.take<float>((P - dmin + 1) * (Q - dmin + 1));
+  b->smap = cv.take<float>((P - dmin + 1) * (Q - dmin + 1));
   b->S1 = cv.take<uint32_t>((size_t)p.K * G1);
   b->vol = cv.take<float>(MN * p.K);
   b->htw = cv.take<float>(ht_work_floats(p.M, p.N, p.K, o.levels, o.cycleSpin));
@@ -172,6 +174,23 @@ void carve(Carve& cv, const Plan& p, const gbm3d_opts& o, Bufs* b) {
   b->g2 = cv.take<float>(G2);
   b->U2s = cv.take<float>(MN * p.n2);
   b->W2s = cv.take<float>(MN * p.n2);
+}
+
+// Block matching on a padded image with the context's distance variant.
+int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int N, int d, int d2, int K, uint32_t* S,
+             float* vol, hipStream_t st, int stage) {
+  const int P = M + d2, Q = N + d2;
+  {
+    StageTimer t(c, GBM3D_STAGE_PAD);
+    GBM3D_TRY(launch_norm_map(pad, P, Q, d, bn, st));
+    if (c->ext.match_bf16) GBM3D_TRY(launch_sum_map(pad, P, Q, d, smap, st));
+  }
+  StageTimer t(c, stage);
+  if (c->ext.match_bf16) {
+    const int R = c->ext.rescore > 0 ? std::max(c->ext.rescore, 2 * K) : 2 * K;
+    return launch_match_bf16(pad, bn, smap, M, N, d, d2, K, std::min(R, 64), S, vol, st);
+  }
+  return launch_match(pad, bn, M, N, d, d2, K, S, vol, st);
 }
 
 int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const gbm3d_opts& o, float* dU,
@@ -211,12 +230,8 @@ int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const
     {
       StageTimer t(c, GBM3D_STAGE_PAD);
       GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, b.pad1, st));
-      GBM3D_TRY(launch_norm_map(b.pad1, P, Q, p.bs, b.bn, st));
     }
-    {
-      StageTimer t(c, GBM3D_STAGE_MATCH1);
-      GBM3D_TRY(launch_match(b.pad1, b.bn, p.M, p.N, p.bs, p.d2, p.K, b.S1, b.vol, st));
-    }
+    GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bs, p.d2, p.K, b.S1, b.vol, st, GBM3D_STAGE_MATCH1));
     GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, st, ht_hook, c));
     {
       StageTimer t(c, GBM3D_STAGE_AGG1_PREP);
@@ -246,12 +261,8 @@ int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const
       StageTimer t(c, GBM3D_STAGE_PAD);
       GBM3D_TRY(launch_search_pad(b.pilot, p.M, p.N, sp, p.d2, b.pad1, st));
       GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, b.pad2, st));
-      GBM3D_TRY(launch_norm_map(b.pad1, P, Q, p.bw, b.bn, st));
     }
-    {
-      StageTimer t(c, GBM3D_STAGE_MATCH2);
-      GBM3D_TRY(launch_match(b.pad1, b.bn, p.M, p.N, p.bw, p.d2, p.K, b.S2, nullptr, st));
-    }
+    GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bw, p.d2, p.K, b.S2, nullptr, st, GBM3D_STAGE_MATCH2));
     {
       StageTimer t(c, GBM3D_STAGE_WIENER);
       GBM3D_TRY(launch_wiener(b.pad1, b.pad2, b.S2, nullptr, nullptr, p.M, p.N, p.bw, p.d2, p.K, G2, sigma, b.V2f,
@@ -363,6 +374,20 @@ int gbm3d_destroy(gbm3d_ctx* c) {
   return GBM3D_OK;
 }
 
+int gbm3d_set_ext(gbm3d_ctx* c, const gbm3d_ext* e) {
+  GBM3D_TRY(set_dev(c));
+  if (e && (e->rescore < 0 || e->rescore > 64)) return fail(GBM3D_ERR_ARG, "ext: rescore must be 0..64");
+  c->ext = e ? *e : gbm3d_ext{};
+  return GBM3D_OK;
+}
+
+int gbm3d_get_ext(gbm3d_ctx* c, gbm3d_ext* e) {
+  GBM3D_TRY(set_dev(c));
+  if (!e) return fail(GBM3D_ERR_ARG, "NULL ext");
+  *e = c->ext;
+  return GBM3D_OK;
+}
+
 int gbm3d_set_stream(gbm3d_ctx* c, void* s) {
   GBM3D_TRY(set_dev(c));
   c->stream = s ? reinterpret_cast<hipStream_t>(s) : c->own;
@@ -453,23 +478,23 @@ int gbm3d_match(gbm3d_ctx* c, const float* I, int m, int n, int d, int d2, int K
   GBM3D_TRY(stage_sizes(m, n, d, d2, K));
   const size_t mn = (size_t)m * n, P = (size_t)m + d2, Q = (size_t)n + d2, G = mn / ((size_t)d * d);
   Carve cv{nullptr};
-  auto lay = [&](Carve& k, float** img, float** pad, float** bn, uint32_t** dS, float** vol) {
+  auto lay = [&](Carve& k, float** img, float** pad, float** bn, float** sm, uint32_t** dS, float** vol) {
     *img = k.take<float>(mn);
     *pad = k.take<float>(P * Q);
     *bn = k.take<float>((P - d + 1) * (Q - d + 1));
+    *sm = k.take<float>((P - d + 1) * (Q - d + 1));
     *dS = k.take<uint32_t>(G * K);
     *vol = k.take<float>(Volume ? mn * K : 0);
   };
-  float *img, *pad, *bn, *vol;
+  float *img, *pad, *bn, *sm, *vol;
   uint32_t* dS;
-  lay(cv, &img, &pad, &bn, &dS, &vol);
+  lay(cv, &img, &pad, &bn, &sm, &dS, &vol);
   GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
   Carve k2{c->arena};
-  lay(k2, &img, &pad, &bn, &dS, &vol);
+  lay(k2, &img, &pad, &bn, &sm, &dS, &vol);
   GBM3D_TRY(upload(c, img, I, mn));
   GBM3D_TRY(launch_search_pad(img, m, n, 0, d2, pad, c->stream));
-  GBM3D_TRY(launch_norm_map(pad, (int)P, (int)Q, d, bn, c->stream));
-  GBM3D_TRY(launch_match(pad, bn, m, n, d, d2, K, dS, Volume ? vol : nullptr, c->stream));
+  GBM3D_TRY(do_match(c, pad, bn, sm, m, n, d, d2, K, dS, Volume ? vol : nullptr, c->stream, GBM3D_STAGE_MATCH1));
   GBM3D_TRY(download(c, S, dS, G * K));
   if (Volume) GBM3D_TRY(download(c, Volume, vol, mn * K));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
@@ -541,27 +566,28 @@ int gbm3d_match_wie(gbm3d_ctx* c, const float* I, const float* In, int m, int n,
   GBM3D_TRY(stage_sizes(m, n, d, d2, K));
   const size_t mn = (size_t)m * n, P = (size_t)m + d2, Q = (size_t)n + d2, G = mn / ((size_t)d * d);
   Carve cv{nullptr};
-  auto lay = [&](Carve& k, float** i1, float** i2, float** p1, float** p2, float** bn, uint32_t** dS, float** v) {
+  auto lay = [&](Carve& k, float** i1, float** i2, float** p1, float** p2, float** bn, float** sm, uint32_t** dS,
+                 float** v) {
     *i1 = k.take<float>(mn);
     *i2 = k.take<float>(mn);
     *p1 = k.take<float>(P * Q);
     *p2 = k.take<float>(P * Q);
     *bn = k.take<float>((P - d + 1) * (Q - d + 1));
+    *sm = k.take<float>((P - d + 1) * (Q - d + 1));
     *dS = k.take<uint32_t>(G * K);
     *v = k.take<float>(mn * K);
   };
-  float *i1, *i2, *p1, *p2, *bn, *v;
+  float *i1, *i2, *p1, *p2, *bn, *sm, *v;
   uint32_t* dS;
-  lay(cv, &i1, &i2, &p1, &p2, &bn, &dS, &v);
+  lay(cv, &i1, &i2, &p1, &p2, &bn, &sm, &dS, &v);
   GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
   Carve k2{c->arena};
-  lay(k2, &i1, &i2, &p1, &p2, &bn, &dS, &v);
+  lay(k2, &i1, &i2, &p1, &p2, &bn, &sm, &dS, &v);
   GBM3D_TRY(upload(c, i1, I, mn));
   GBM3D_TRY(upload(c, i2, In, mn));
   GBM3D_TRY(launch_search_pad(i1, m, n, 0, d2, p1, c->stream));
   GBM3D_TRY(launch_search_pad(i2, m, n, 0, d2, p2, c->stream));
-  GBM3D_TRY(launch_norm_map(p1, (int)P, (int)Q, d, bn, c->stream));
-  GBM3D_TRY(launch_match(p1, bn, m, n, d, d2, K, dS, nullptr, c->stream));
+  GBM3D_TRY(do_match(c, p1, bn, sm, m, n, d, d2, K, dS, nullptr, c->stream, GBM3D_STAGE_MATCH2));
   GBM3D_TRY(download(c, S, dS, G * K));
   if (V1) {
     GBM3D_TRY(launch_gather_groups(p1, dS, m, n, d, d2, K, v, c->stream));

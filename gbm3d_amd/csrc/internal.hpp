@@ -67,6 +67,12 @@ int launch_norm_map(const float* pad, int P, int Q, int d, float* bn, hipStream_
 // mosaic volume (M x N x K).
 int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S,
                  float* vol, hipStream_t st);
+// d x d box sums of the padded image (the centring term of the BF16 matcher).
+int launch_sum_map(const float* pad, int P, int Q, int d, float* sm, hipStream_t st);
+// BF16-MFMA matching: approximate scores, exact FP32 rescoring of at least R
+// (>= K) candidates, same S / volume outputs as launch_match.
+int launch_match_bf16(const float* pad, const float* bn, const float* smap, int M, int N, int d, int d2, int K,
+                      int R, uint32_t* S, float* vol, hipStream_t st);
 // Gather V (d x d x K x G) from a padded image through S.
 int launch_gather_groups(const float* pad, const uint32_t* S, int M, int N, int d, int d2, int K,
                          float* V, hipStream_t st);

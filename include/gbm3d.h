@@ -104,6 +104,18 @@ int gbm3d_destroy(gbm3d_ctx* ctx);
 int gbm3d_set_stream(gbm3d_ctx* ctx, void* hip_stream);
 int gbm3d_synchronize(gbm3d_ctx* ctx);
 
+/* ------------------------------------------------------------- variants
+ * Forms beside the reference path, set per context; a zeroed struct (or
+ * NULL) is the reference behaviour.  No reference counterpart. */
+typedef struct gbm3d_ext {
+  int32_t match_bf16; /* 1: block distances on BF16 MFMA (FP32 accumulation),
+                         then exact FP32 rescoring of the best candidates    */
+  int32_t rescore;    /* candidates rescored per tile, >= 2*numMax (0: 2*numMax) */
+  int32_t reserved[6];
+} gbm3d_ext;
+int gbm3d_set_ext(gbm3d_ctx* ctx, const gbm3d_ext* ext);
+int gbm3d_get_ext(gbm3d_ctx* ctx, gbm3d_ext* ext);
+
 /* --------------------------------------------------------------- hot path */
 
 /*

@@ -209,3 +209,40 @@ def test_error_paths(ctx):
         ctx.denoise(noisy[:12, :12], 25.0, None)  # pads to 16x16, smaller than matchSize
     U, _ = ctx.denoise(noisy, 25.0, None)  # the context still works afterwards
     assert np.isfinite(U).all()
+
+
+@pytest.mark.parametrize("m,n,d,d2,seed", [(256, 256, 8, 32, 91), (256, 192, 16, 32, 92), (192, 256, 8, 48, 93),
+                                           (128, 128, 16, 48, 94)])
+def test_match_bf16_rescored_agrees_with_fp32(m, n, d, d2, seed):
+    """BF16-MFMA scores + exact FP32 rescoring of the best 2K: the index sets
+    equal the FP32 path's wherever its top K lie in the rescored set (all
+    tiles here), and the gathered volume follows S."""
+    import gbm3d_amd
+
+    clean, noisy = synth.make_pair(seed, m, n, 25.0)
+    c32 = gbm3d_amd.Context(0)
+    cbf = gbm3d_amd.Context(0)
+    cbf.set_variant(match_bf16=True)
+    S32, V32 = c32.match(noisy, d, d2, 16)
+    Sbf, Vbf = cbf.match(noisy, d, d2, 16)
+    agree = np.mean(np.all(S32 == Sbf, axis=0))
+    assert agree >= 0.999, agree
+    if agree == 1.0:
+        np.testing.assert_array_equal(Vbf, V32)
+    c32.close()
+    cbf.close()
+
+
+def test_bf16_variant_end_to_end():
+    import gbm3d_amd
+
+    clean, noisy = synth.make_pair(95, 256, 256, 25.0)
+    c32 = gbm3d_amd.Context(0)
+    cbf = gbm3d_amd.Context(0)
+    cbf.set_variant(match_bf16=True)
+    for opts in (None, {"blockSize": 8, "matchSpinsWie": [0, 4]}):
+        U32, _ = c32.denoise(noisy, 25.0, opts)
+        Ubf, _ = cbf.denoise(noisy, 25.0, opts)
+        assert abs(synth.psnr(clean, Ubf) - synth.psnr(clean, U32)) < 0.01
+    c32.close()
+    cbf.close()
