@@ -131,6 +131,78 @@ def precision_report(args):
     print(json.dumps(out))
 
 
+def batch_workload(seed=64, count=64):
+    """64 synthetic images: side m ~ U[256,1536], aspect n/m ~ U[0.5,2] (n kept in
+    [256,1536]), sigma ~ U[5,50]; the same construction as the single image."""
+    from gbm3d_amd import synth
+
+    imgs, cleans, sig = [], [], []
+    key = 0xBA7C4 + seed
+    for i in range(count):
+        u = synth.uniform(key, np.arange(3 * i, 3 * i + 3))
+        m = int(256 + u[0] * (1536 - 256))
+        n = int(np.clip(round(m * (0.5 + 1.5 * u[1])), 256, 1536))
+        s = float(5 + 45 * u[2])
+        c, x = synth.make_pair(seed * 1000 + i, m, n, s)
+        imgs.append(x)
+        cleans.append(c)
+        sig.append(s)
+    return imgs, cleans, sig
+
+
+def batched_report(args):
+    """One graph launch for 64 uneven images vs the plain per-image runs."""
+    import gbm3d_amd
+    from gbm3d_amd import device, synth
+
+    imgs, cleans, sig = batch_workload()
+    mp = sum(x.size for x in imgs) / 1e6
+    ctx = gbm3d_amd.Context(0)
+    stream = device.Stream()
+    ctx.set_stream(stream.handle)
+    dI = [device.DeviceArray.from_host(x) for x in imgs]
+    dU = [device.DeviceArray(x.shape) for x in imgs]
+    ms_, ns_ = [x.shape[0] for x in imgs], [x.shape[1] for x in imgs]
+    e0, e1 = device.Event(), device.Event()
+
+    def per_image():
+        for i in range(len(imgs)):
+            ctx.denoise_device(dI[i].ptr, ms_[i], ns_[i], sig[i], OPTS, dU[i].ptr)
+
+    per_image()
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        per_image()
+    e1.record(stream)
+    t_plain = e0.elapsed_ms(e1) / args.steps
+    U_plain = [d.to_host() for d in dU]
+    t0 = time.perf_counter()
+    ctx.denoise_batch_device([d.ptr for d in dI], ms_, ns_, sig, OPTS, [d.ptr for d in dU], graph=True, tune=True)
+    stream.synchronize()
+    tune_s = time.perf_counter() - t0
+    info = ctx.batch_info()
+    e0.record(stream)
+    for _ in range(args.steps):
+        ctx.denoise_batch_device([d.ptr for d in dI], ms_, ns_, sig, OPTS, [d.ptr for d in dU], graph=True,
+                                 tune=True)
+    e1.record(stream)
+    t_batch = e0.elapsed_ms(e1) / args.steps
+    U_b = [d.to_host() for d in dU]
+    p_plain = [synth.psnr(c, u) for c, u in zip(cleans, U_plain)]
+    p_b = [synth.psnr(c, u) for c, u in zip(cleans, U_b)]
+    print(json.dumps({
+        "workload": "batched: 64 synthetic images, sides U[256,1536], aspect U[0.5,2], sigma U[5,50], 8x8 blocks",
+        "images": len(imgs), "mpix": round(mp, 3),
+        "plain_per_image": {"ms": round(t_plain, 3), "mpix_s": round(mp / (t_plain * 1e-3), 2)},
+        "batched_graph": {"ms": round(t_batch, 3), "mpix_s": round(mp / (t_batch * 1e-3), 2), "tuned": info,
+                          "tune_and_capture_s": round(tune_s, 3)},
+        "speedup": round(t_plain / t_batch, 3),
+        "psnr_mean_db": round(float(np.mean(p_b)), 4),
+        "psnr_max_abs_diff_vs_plain_db": float(np.max(np.abs(np.array(p_b) - np.array(p_plain)))),
+        "bitwise_equal_to_plain": all(np.array_equal(a, b) for a, b in zip(U_b, U_plain))}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,11 +212,14 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="warmup + steps only (for rocprofv3 runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="with --profile-only: also run one device copy of MIB MiB (byte-counter calibration)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "precision"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched"])
     ap.add_argument("--size", type=int, default=0, help="image side for --workload precision")
     args = ap.parse_args()
     if args.workload == "precision":
         precision_report(args)
+        return
+    if args.workload == "batched":
+        batched_report(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (one image per launch); --gpus must be 1", file=sys.stderr)

@@ -13,6 +13,9 @@ import numpy as np
 GBM3D_MAX_SPINS = 16
 GBM3D_UNSET = -2147483648
 
+GBM3D_BATCH_GRAPH = 1
+GBM3D_BATCH_TUNE = 2
+
 GBM3D_OK = 0
 GBM3D_ERR_ARG = 1
 GBM3D_ERR_UNSUPPORTED = 2
@@ -138,6 +141,8 @@ SIGNATURES = {
     "gbm3d_synchronize": (_I, [_P]),
     "gbm3d_denoise": (_I, [_P, _P, _I, _I, _F, C.POINTER(Opts), _P, _P]),
     "gbm3d_denoise_device": (_I, [_P, _P, _I, _I, _F, C.POINTER(Opts), _P, _P]),
+    "gbm3d_denoise_batch_device": (_I, [_P, _I, _P, _P, _P, _P, C.POINTER(Opts), _P, _P, _I]),
+    "gbm3d_batch_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "gbm3d_match": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "gbm3d_ht_filter": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "gbm3d_aggregate_ht": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _P, _P]),

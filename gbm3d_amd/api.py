@@ -135,6 +135,36 @@ class Context:
                                             C.byref(o) if o is not None else None, C.c_void_p(dU),
                                             C.c_void_p(dU1) if dU1 else None))
 
+    def denoise_batch_device(self, dIs, ms, ns, sigmas, opts, dUs, dU1s=None, graph=True, tune=False):
+        """Batched GBM3D over device buffers (lists of int pointers); asynchronous."""
+        cnt = len(dIs)
+        P = C.c_void_p * cnt
+        I32 = C.c_int * cnt
+        F32 = C.c_float * cnt
+        o = opts_to_struct(opts)
+        flags = (_abi.GBM3D_BATCH_GRAPH if graph else 0) | (_abi.GBM3D_BATCH_TUNE if tune else 0)
+        check(self.lib.gbm3d_denoise_batch_device(
+            self.handle, cnt, P(*dIs), I32(*ms), I32(*ns), F32(*sigmas), C.byref(o) if o is not None else None,
+            P(*dUs), P(*dU1s) if dU1s else None, flags))
+
+    def batch_info(self):
+        s, w, t = C.c_int(), C.c_int(), C.c_double()
+        check(self.lib.gbm3d_batch_info(self.handle, C.byref(s), C.byref(w), C.byref(t)))
+        return {"streams": s.value, "match_waves": w.value, "tuned_ms": t.value}
+
+    def denoise_batch(self, images, sigmas, opts=None, graph=True, tune=False, repeats=1):
+        """Host convenience: upload, run the batch `repeats` times, download U."""
+        from . import device
+
+        dI = [device.DeviceArray.from_host(np.asarray(I, dtype=np.float32)) for I in images]
+        dU = [device.DeviceArray(np.shape(I)) for I in images]
+        for _ in range(repeats):
+            self.denoise_batch_device([d.ptr for d in dI], [np.shape(I)[0] for I in images],
+                                      [np.shape(I)[1] for I in images], list(sigmas), opts, [d.ptr for d in dU],
+                                      graph=graph, tune=tune)
+        self.synchronize()
+        return [d.to_host() for d in dU]
+
     # ------------------------------------------------------------ stages
     def match(self, I, d: int, d2: int, K: int, want_volume: bool = True):
         I = np.asfortranarray(I, dtype=np.float32)

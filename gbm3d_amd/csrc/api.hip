@@ -22,6 +22,17 @@ struct gbm3d_ctx {
   float* masks = nullptr;  // [4 block sizes][2][32*32]
   gbm3d_ext ext{};
   bool timing = false;
+  bool in_batch = false;  // stage timers off while a batch runs on several streams
+  // batched launches
+  std::vector<hipStream_t> bstreams;
+  std::vector<hipEvent_t> bjoin;
+  hipEvent_t bfork = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<int64_t> gkey;
+  int tuned_streams = 0, tuned_match_waves = 0;
+  double tuned_ms = 0.0;
+  std::vector<int64_t> tkey;
+  int match_waves = 1;  // reference tiles per matching workgroup (single-image path)
   struct Mark {
     int stage;
     hipEvent_t a, b;
@@ -74,7 +85,7 @@ int ensure(char** buf, size_t* have, size_t need) {
 }
 
 void mark_begin(gbm3d_ctx* c, int stage) {
-  if (!c->timing) return;
+  if (!c->timing || c->in_batch) return;
   if (c->pool_used + 2 > c->pool.size()) {
     hipEvent_t e1, e2;
     if (hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess) {
@@ -91,7 +102,7 @@ void mark_begin(gbm3d_ctx* c, int stage) {
 }
 
 void mark_end(gbm3d_ctx* c, int stage) {
-  if (!c->timing || c->marks.empty() || c->marks.back().stage != stage) return;
+  if (!c->timing || c->in_batch || c->marks.empty() || c->marks.back().stage != stage) return;
   (void)hipEventRecord(c->marks.back().b, c->stream);
 }
 
@@ -190,30 +201,38 @@ int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int 
     const int R = c->ext.rescore > 0 ? std::max(c->ext.rescore, 2 * K) : 2 * K;
     return launch_match_bf16(pad, bn, smap, M, N, d, d2, K, std::min(R, 64), S, vol, st);
   }
-  return launch_match(pad, bn, M, N, d, d2, K, S, vol, st);
+  return launch_match_w(pad, bn, M, N, d, d2, K, S, vol, c->match_waves, st);
 }
 
-int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const gbm3d_opts& o, float* dU,
-                 float* dU1) {
-  Plan p;
-  p.m = m;
-  p.n = n;
-  GBM3D_TRY(validate(o, m, n, &p.M, &p.N));
-  p.bs = o.blockSize;
-  p.bw = o.blockSizeWie;
-  p.d2 = o.matchSize;
-  p.K = o.numMax;
-  p.n1 = o.nMatchSpins;
-  p.n2 = o.Wiener ? o.nMatchSpinsWie : 0;
+int make_plan(const gbm3d_opts& o, int m, int n, Plan* p) {
+  p->m = m;
+  p->n = n;
+  GBM3D_TRY(validate(o, m, n, &p->M, &p->N));
+  p->bs = o.blockSize;
+  p->bw = o.blockSizeWie;
+  p->d2 = o.matchSize;
+  p->K = o.numMax;
+  p->n1 = o.nMatchSpins;
+  p->n2 = o.Wiener ? o.nMatchSpinsWie : 0;
+  return GBM3D_OK;
+}
+
+// Device workspace bytes of one image's pipeline.
+size_t pipeline_bytes(const Plan& p, const gbm3d_opts& o) {
   Carve measure{nullptr};
   Bufs b;
   carve(measure, p, o, &b);
-  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, measure.off));
-  Carve cv{c->arena};
+  return measure.off + 256;
+}
+
+// One image through both steps on stream st, workspace at ws (pipeline_bytes).
+int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, const gbm3d_opts& o, float* dU,
+                    float* dU1, hipStream_t st, char* ws) {
+  const int m = p.m, n = p.n;
+  Carve cv{ws};
+  Bufs b;
   carve(cv, p, o, &b);
-  hipStream_t st = c->stream;
   const size_t MN = p.MN();
-  const int P = p.M + p.d2, Q = p.N + p.d2;
   std::vector<double> tau((size_t)o.levels * 7);
   tau_table(o.levels, o.tauMode, (double)sigma, tau.data());
   HtWork hw;
@@ -284,6 +303,14 @@ int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const
   return GBM3D_OK;
 }
 
+int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const gbm3d_opts& o, float* dU,
+                 float* dU1) {
+  Plan p;
+  GBM3D_TRY(make_plan(o, m, n, &p));
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, pipeline_bytes(p, o)));
+  return run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena);
+}
+
 int set_dev(gbm3d_ctx* c) {
   if (!c) return fail(GBM3D_ERR_ARG, "NULL context");
   GBM3D_HIP(hipSetDevice(c->device));
@@ -337,6 +364,16 @@ int gbm3d_create(int device, gbm3d_ctx** out) {
     return fail(GBM3D_ERR_HIP, "hipStreamCreate failed");
   }
   c->stream = c->own;
+  c->bstreams.assign(8, nullptr);
+  c->bjoin.assign(8, nullptr);
+  bool ok = hipEventCreateWithFlags(&c->bfork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < 8 && ok; ++i)
+    ok = hipStreamCreateWithFlags(&c->bstreams[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&c->bjoin[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    delete c;
+    return fail(GBM3D_ERR_HIP, "batch stream/event creation failed");
+  }
   // Aggregation masks for every supported block size (computed in double,
   // stored as float): step 1 sin(pi*linspace(.1,.9,d)) outer product,
   // step 2 sin(pi*linspace(.2,.8,d)) outer product.
@@ -369,6 +406,12 @@ int gbm3d_destroy(gbm3d_ctx* c) {
   if (c->io) (void)hipFree(c->io);
   if (c->masks) (void)hipFree(c->masks);
   for (auto e : c->pool) (void)hipEventDestroy(e);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  for (auto sb : c->bstreams)
+    if (sb) (void)hipStreamDestroy(sb);
+  for (auto e : c->bjoin)
+    if (e) (void)hipEventDestroy(e);
+  if (c->bfork) (void)hipEventDestroy(c->bfork);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
   return GBM3D_OK;
@@ -656,6 +699,192 @@ int gbm3d_aggregate_wie(gbm3d_ctx* c, const float* V2f, const float* gain2, cons
   GBM3D_TRY(download(c, U, dU, mn));
   GBM3D_TRY(download(c, W, dW, mn));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+namespace {
+
+struct BatchJob {
+  int count;
+  const float* const* dI;
+  const float* const* dUo;
+  const int *m, *n;
+  const float* sigma;
+  float* const* dU;
+  float* const* dU1;
+  std::vector<Plan> plans;
+};
+
+// Largest images first onto the least-loaded stream; per-stream workspace.
+void assign_streams(const BatchJob& j, const gbm3d_opts& o, int S, std::vector<int>* assign, std::vector<size_t>* wsz) {
+  std::vector<int> order(j.count);
+  for (int i = 0; i < j.count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (int64_t)j.plans[a].M * j.plans[a].N > (int64_t)j.plans[b].M * j.plans[b].N;
+  });
+  std::vector<int64_t> load(S, 0);
+  assign->assign(j.count, 0);
+  wsz->assign(S, 0);
+  for (int i : order) {
+    int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    (*assign)[i] = s;
+    load[s] += (int64_t)j.plans[i].M * j.plans[i].N;
+    (*wsz)[s] = std::max((*wsz)[s], pipeline_bytes(j.plans[i], o));
+  }
+}
+
+int issue_batch(gbm3d_ctx* c, const BatchJob& j, const gbm3d_opts& o, int S, int mw) {
+  std::vector<int> assign;
+  std::vector<size_t> wsz;
+  assign_streams(j, o, S, &assign, &wsz);
+  std::vector<size_t> off(S, 0);
+  size_t tot = 0;
+  for (int s = 0; s < S; ++s) {
+    off[s] = tot;
+    tot += (wsz[s] + 4095) & ~(size_t)4095;
+  }
+  if (tot > c->arena_bytes) return fail(GBM3D_ERR_NOMEM, "batch workspace not reserved");
+  c->in_batch = true;
+  c->match_waves = mw;
+  int st = GBM3D_OK;
+  if (hipEventRecord(c->bfork, c->stream) != hipSuccess) st = fail(GBM3D_ERR_HIP, "batch fork");
+  for (int s = 0; s < S && st == GBM3D_OK; ++s)
+    if (hipStreamWaitEvent(c->bstreams[s], c->bfork, 0) != hipSuccess) st = fail(GBM3D_ERR_HIP, "batch fork wait");
+  for (int i = 0; i < j.count && st == GBM3D_OK; ++i) {
+    const int s = assign[i];
+    st = run_pipeline_on(c, j.dI[i], j.plans[i], j.sigma[i], o, j.dU[i], j.dU1 ? j.dU1[i] : nullptr, c->bstreams[s],
+                         c->arena + off[s]);
+  }
+  for (int s = 0; s < S; ++s) {
+    // join every branch even after an error so the capture stays well formed
+    (void)hipEventRecord(c->bjoin[s], c->bstreams[s]);
+    (void)hipStreamWaitEvent(c->stream, c->bjoin[s], 0);
+  }
+  c->in_batch = false;
+  c->match_waves = 1;
+  return st;
+}
+
+size_t batch_bytes(const BatchJob& j, const gbm3d_opts& o, int S) {
+  std::vector<int> assign;
+  std::vector<size_t> wsz;
+  assign_streams(j, o, S, &assign, &wsz);
+  size_t tot = 0;
+  for (size_t w : wsz) tot += (w + 4095) & ~(size_t)4095;
+  return tot;
+}
+
+std::vector<int64_t> batch_key(const BatchJob& j, const gbm3d_opts& o, const gbm3d_ext& e, int S, int mw) {
+  std::vector<int64_t> k = {j.count, S, mw, e.match_bf16, e.rescore};
+  for (int i = 0; i < j.count; ++i) {
+    uint32_t sb;
+    std::memcpy(&sb, &j.sigma[i], 4);
+    k.push_back(j.m[i]);
+    k.push_back(j.n[i]);
+    k.push_back(sb);
+    k.push_back((int64_t)(intptr_t)j.dI[i]);
+    k.push_back((int64_t)(intptr_t)j.dU[i]);
+    k.push_back(j.dU1 ? (int64_t)(intptr_t)j.dU1[i] : 0);
+  }
+  const int32_t* ow = reinterpret_cast<const int32_t*>(&o);
+  for (size_t i = 0; i < sizeof(o) / 4; ++i) k.push_back(ow[i]);
+  return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gbm3d_denoise_batch_device(gbm3d_ctx* c, int count, const float* const* dI, const int* m, const int* n,
+                               const float* sigma, const gbm3d_opts* opts, float* const* dU, float* const* dU1,
+                               int flags) {
+  GBM3D_TRY(set_dev(c));
+  if (count < 1 || !dI || !m || !n || !sigma || !dU) return fail(GBM3D_ERR_ARG, "batch: NULL or empty");
+  gbm3d_opts o;
+  GBM3D_TRY(resolve(opts, &o));
+  BatchJob j{count, dI, nullptr, m, n, sigma, dU, dU1, {}};
+  j.plans.resize(count);
+  for (int i = 0; i < count; ++i) {
+    if (!dI[i] || !dU[i]) return fail(GBM3D_ERR_ARG, "batch: NULL image " + std::to_string(i));
+    GBM3D_TRY(make_plan(o, m[i], n[i], &j.plans[i]));
+  }
+  int S = std::min(count, 4), mw = 4;
+  if (flags & GBM3D_BATCH_TUNE) {
+    std::vector<int64_t> tk = batch_key(j, o, c->ext, 0, 0);
+    if (tk != c->tkey) {
+      hipEvent_t a, b;
+      GBM3D_HIP(hipEventCreate(&a));
+      GBM3D_HIP(hipEventCreate(&b));
+      double best = 1e300;
+      int bs = S, bm = mw;
+      for (int cs : {1, 2, 4, 8}) {
+        if (cs > count) continue;
+        GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, batch_bytes(j, o, cs)));
+        for (int cm : {1, 2, 4}) {
+          GBM3D_TRY(issue_batch(c, j, o, cs, cm));  // warm
+          GBM3D_HIP(hipEventRecord(a, c->stream));
+          GBM3D_TRY(issue_batch(c, j, o, cs, cm));
+          GBM3D_HIP(hipEventRecord(b, c->stream));
+          GBM3D_HIP(hipEventSynchronize(b));
+          float ms = 0.f;
+          GBM3D_HIP(hipEventElapsedTime(&ms, a, b));
+          if (ms < best) {
+            best = ms;
+            bs = cs;
+            bm = cm;
+          }
+        }
+      }
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      c->tkey = tk;
+      c->tuned_streams = bs;
+      c->tuned_match_waves = bm;
+      c->tuned_ms = best;
+    }
+  }
+  if (c->tuned_streams > 0 && c->tkey == batch_key(j, o, c->ext, 0, 0)) {
+    S = c->tuned_streams;
+    mw = c->tuned_match_waves;
+  }
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, batch_bytes(j, o, S)));
+  if (!(flags & GBM3D_BATCH_GRAPH)) return issue_batch(c, j, o, S, mw);
+  std::vector<int64_t> key = batch_key(j, o, c->ext, S, mw);
+  key.push_back((int64_t)(intptr_t)c->arena);  // a grown arena invalidates the captured addresses
+  if (!c->gexec || key != c->gkey) {
+    if (c->gexec) {
+      (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+    GBM3D_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
+    const int st = issue_batch(c, j, o, S, mw);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (st != GBM3D_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (ec != hipSuccess) return fail(GBM3D_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      c->gexec = nullptr;
+      return fail(GBM3D_ERR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(ei));
+    }
+    c->gkey = key;
+  }
+  GBM3D_HIP(hipGraphLaunch(c->gexec, c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_batch_info(gbm3d_ctx* c, int* streams, int* match_waves, double* tuned_ms) {
+  GBM3D_TRY(set_dev(c));
+  if (streams) *streams = c->tuned_streams;
+  if (match_waves) *match_waves = c->tuned_match_waves;
+  if (tuned_ms) *tuned_ms = c->tuned_ms;
   return GBM3D_OK;
 }
 

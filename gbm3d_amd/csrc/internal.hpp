@@ -73,6 +73,9 @@ int launch_sum_map(const float* pad, int P, int Q, int d, float* sm, hipStream_t
 // (>= K) candidates, same S / volume outputs as launch_match.
 int launch_match_bf16(const float* pad, const float* bn, const float* smap, int M, int N, int d, int d2, int K,
                       int R, uint32_t* S, float* vol, hipStream_t st);
+// Same with nw (1, 2, 4) reference tiles per workgroup.
+int launch_match_w(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
+                   int nw, hipStream_t st);
 // Gather V (d x d x K x G) from a padded image through S.
 int launch_gather_groups(const float* pad, const uint32_t* S, int M, int N, int d, int d2, int K,
                          float* V, hipStream_t st);

@@ -195,17 +195,17 @@ struct MatchCfg {
 
 // One wave per reference tile, 4 waves per workgroup.  The (D2+D)^2 search
 // window is staged in LDS; window index a + D2*b <-> Ipad(i*D + a, j*D + b).
-template <int D, int D2>
-__global__ __launch_bounds__(256) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
-                                                    int M, int N, int K, uint32_t* __restrict__ S,
-                                                    float* __restrict__ vol) {
+template <int D, int D2, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
+                                                        int M, int N, int K, uint32_t* __restrict__ S,
+                                                        float* __restrict__ vol) {
   using C = MatchCfg<D, D2>;
   constexpr int S2 = D2 / 2;
-  __shared__ float win[4][C::WR * C::LDW];
-  __shared__ uint64_t scr[4][64];
+  __shared__ float win[NW][C::WR * C::LDW];
+  __shared__ uint64_t scr[NW][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int Mt = M / D, Nt = N / D;
-  const int g = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+  const int g = xcd_remap(blockIdx.x, gridDim.x) * NW + wave;
   if (g >= Mt * Nt) return;
   const int ti = g % Mt, tj = g / Mt;
   const int P = M + D2, Pm = P - D + 1;
@@ -269,19 +269,27 @@ __global__ __launch_bounds__(256) void match_kernel(const float* __restrict__ pa
 }
 
 template <int D, int D2>
-static void launch_match_t(const float* pad, const float* bn, int M, int N, int K, uint32_t* S, float* vol,
+static void launch_match_t(const float* pad, const float* bn, int M, int N, int K, uint32_t* S, float* vol, int nw,
                            hipStream_t st) {
   const int G = (M / D) * (N / D);
-  hipLaunchKernelGGL((match_kernel<D, D2>), dim3((G + 3) / 4), dim3(256), 0, st, pad, bn, M, N, K, S, vol);
+  if (nw == 1)
+    hipLaunchKernelGGL((match_kernel<D, D2, 1>), dim3(G), dim3(64), 0, st, pad, bn, M, N, K, S, vol);
+  else if (nw == 2)
+    hipLaunchKernelGGL((match_kernel<D, D2, 2>), dim3((G + 1) / 2), dim3(128), 0, st, pad, bn, M, N, K, S, vol);
+  else
+    hipLaunchKernelGGL((match_kernel<D, D2, 4>), dim3((G + 3) / 4), dim3(256), 0, st, pad, bn, M, N, K, S, vol);
 }
 
-int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
-                 hipStream_t st) {
+// nw: reference tiles (waves) per workgroup, 1, 2 or 4 (a launch-shape choice
+// of the batch autotuner; results do not depend on it).
+int launch_match_w(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
+                   int nw, hipStream_t st) {
   if (M % d || N % d || M < d2 || N < d2) return fail(GBM3D_ERR_ARG, "match: image size");
   if (K < 1 || K > 64 || K > d2 * d2) return fail(GBM3D_ERR_UNSUPPORTED, "match: numMax must be in [1, 64]");
+  if (nw != 1 && nw != 2 && nw != 4) return fail(GBM3D_ERR_ARG, "match: waves per workgroup must be 1, 2 or 4");
 #define GBM3D_MATCH_CASE(DD, DD2)                                     \
   if (d == DD && d2 == DD2) {                                         \
-    launch_match_t<DD, DD2>(pad, bn, M, N, K, S, vol, st);            \
+    launch_match_t<DD, DD2>(pad, bn, M, N, K, S, vol, nw, st);        \
     return check_launch("match");                                     \
   }
   GBM3D_MATCH_CASE(4, 32)
@@ -292,6 +300,11 @@ int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2,
   GBM3D_MATCH_CASE(16, 48)
 #undef GBM3D_MATCH_CASE
   return fail(GBM3D_ERR_UNSUPPORTED, "match: (blockSize, matchSize) must be (4|8|16|32, 32) or (8|16, 48)");
+}
+
+int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
+                 hipStream_t st) {
+  return launch_match_w(pad, bn, M, N, d, d2, K, S, vol, 4, st);
 }
 
 // ------------------------------------------------ BF16-MFMA matching

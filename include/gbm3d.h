@@ -131,6 +131,24 @@ int gbm3d_denoise_device(gbm3d_ctx* ctx, const float* dI, int m, int n,
                          float sigma, const gbm3d_opts* opts, float* dU,
                          float* dU1);
 
+/* ------------------------------------------------------------- batches
+ * [U_i, out_i] = GBM3D(I_i, sigma_i, opts) for `count` independent images
+ * (device buffers, sizes m[i] x n[i]) sharing one opts.  Images run on
+ * parallel streams (largest first onto the least-loaded stream), each stream
+ * with its own workspace.  GBM3D_BATCH_GRAPH captures the whole batch into
+ * one hipGraph on the first call and replays it while the batch (sizes,
+ * sigmas, buffers, opts, launch shape) is unchanged.  GBM3D_BATCH_TUNE times
+ * the launch-shape candidates (parallel streams 1/2/4/8 x reference tiles
+ * per matching workgroup 1/2/4) once per batch and keeps the fastest.
+ * Asynchronous on the context stream.  No reference counterpart: the nearest
+ * reference form is the parfor over tiles of source/GBM3D_distributed.m:54. */
+enum { GBM3D_BATCH_GRAPH = 1, GBM3D_BATCH_TUNE = 2 };
+int gbm3d_denoise_batch_device(gbm3d_ctx* ctx, int count, const float* const* dI, const int* m, const int* n,
+                               const float* sigma, const gbm3d_opts* opts, float* const* dU,
+                               float* const* dU1, int flags);
+/* Launch shape in use (after tuning) and the tuned batch time in ms. */
+int gbm3d_batch_info(gbm3d_ctx* ctx, int* streams, int* match_waves, double* tuned_ms);
+
 /* ------------------------------------------------- stage entry points
  * Each replaces one reference stage for stage-isolated use; m, n are the
  * (already block-multiple) image sizes the stage sees. */

@@ -246,3 +246,51 @@ def test_bf16_variant_end_to_end():
         assert abs(synth.psnr(clean, Ubf) - synth.psnr(clean, U32)) < 0.01
     c32.close()
     cbf.close()
+
+
+def _batch_inputs():
+    sizes = [(96, 128), (200, 150), (64, 64), (130, 260), (256, 192), (100, 100)]
+    sigmas = [10.0, 25.0, 40.0, 5.0, 50.0, 20.0]
+    imgs = [synth.make_pair(300 + i, m, n, s)[1] for i, ((m, n), s) in enumerate(zip(sizes, sigmas))]
+    return imgs, sigmas
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_equals_single_image_runs(ctx, graph):
+    """A batch over parallel streams (and its hipGraph) gives, bit for bit,
+    the per-image results; graph replays interleaved with eager work too."""
+    imgs, sigmas = _batch_inputs()
+    ref = [ctx.denoise(I, s, None)[0] for I, s in zip(imgs, sigmas)]
+    import gbm3d_amd
+
+    bctx = gbm3d_amd.Context(0)
+    out = bctx.denoise_batch(imgs, sigmas, None, graph=graph)
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+    if graph:
+        from gbm3d_amd import device
+
+        dI = [device.DeviceArray.from_host(I) for I in imgs]
+        dU = [device.DeviceArray(I.shape) for I in imgs]
+        for rep in range(3):
+            bctx.denoise_batch_device([d.ptr for d in dI], [I.shape[0] for I in imgs], [I.shape[1] for I in imgs],
+                                      sigmas, None, [d.ptr for d in dU], graph=True)
+            bctx.synchronize()
+            for d, r in zip(dU, ref):
+                np.testing.assert_array_equal(d.to_host(), r)
+            bctx.denoise(imgs[rep], sigmas[rep], None)  # eager work between replays
+    bctx.close()
+
+
+def test_batch_autotune_picks_a_valid_shape_and_keeps_results(ctx):
+    imgs, sigmas = _batch_inputs()
+    ref = [ctx.denoise(I, s, None)[0] for I, s in zip(imgs, sigmas)]
+    import gbm3d_amd
+
+    bctx = gbm3d_amd.Context(0)
+    out = bctx.denoise_batch(imgs, sigmas, None, graph=True, tune=True)
+    info = bctx.batch_info()
+    assert info["streams"] in (1, 2, 4, 8) and info["match_waves"] in (1, 2, 4) and info["tuned_ms"] > 0
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+    bctx.close()
