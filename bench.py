@@ -60,8 +60,9 @@ def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2):
         out["ht_syn0"] = dict(bound="hbm", unit="GB/s", work=2.0 * vol_bytes, launches=stage_n["ht_syn0"],
                               ms=stage_ms["ht_syn0"] / stage_n["ht_syn0"], peak=PEAK_HBM_GBS)
     if stage_n["agg1_prep"]:
-        # read both cycle-spin volumes, write Vavg
-        out["agg1_prep"] = dict(bound="hbm", unit="GB/s", work=3.0 * vol_bytes, launches=stage_n["agg1_prep"],
+        # read the group-contiguous Vavg once, write one weight per group
+        G1 = MN // (d1 * d1)
+        out["agg1_prep"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G1 * 4), launches=stage_n["agg1_prep"],
                                 ms=stage_ms["agg1_prep"] / stage_n["agg1_prep"], peak=PEAK_HBM_GBS)
     if stage_n["agg1"]:
         G1 = MN // (d1 * d1)
@@ -203,6 +204,45 @@ def batched_report(args):
         "bitwise_equal_to_plain": all(np.array_equal(a, b) for a, b in zip(U_b, U_plain))}))
 
 
+# kernel symbol prefix -> roofline key; the largest dispatch of a kernel is the
+# one measured (the finest level for the wavelet synthesis)
+TRAFFIC_KERNELS = {"gbm3d::ht_syn_fast_kernel": "ht_syn0", "void gbm3d::match_kernel": "match2",
+                   "void gbm3d::wiener_kernel": "wiener", "void gbm3d::aggregate_batched_kernel<1": "agg1",
+                   "void gbm3d::aggregate_batched_kernel<2": "agg2", "void gbm3d::agg1_prep_wave_kernel": "agg1_prep"}
+
+
+def traffic_from(fetch_dir, write_dir, copy_mib, out_path):
+    """HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+    (KiB per dispatch), each scaled so the copy kernel of copy_mib MiB reads
+    and writes exactly copy_mib MiB."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            rows += [r for r in csv.DictReader(open(f)) if r["Counter_Name"] == counter]
+        return rows
+
+    res = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = load(d, counter)
+        copy = [float(r["Counter_Value"]) for r in rows if "stream_copy_kernel" in r["Kernel_Name"]]
+        scale = copy_mib * 1024.0 / copy[0]
+        res[counter + "_scale"] = scale
+        for pre, key in TRAFFIC_KERNELS.items():
+            sel = [r for r in rows if r["Kernel_Name"].startswith(pre)]
+            if not sel:
+                continue
+            gmax = max(int(r["Grid_Size"]) for r in sel)
+            vals = [float(r["Counter_Value"]) for r in sel if int(r["Grid_Size"]) == gmax]
+            res.setdefault(key, {})[counter] = float(np.mean(vals)) * 1024.0 * scale
+    out = {k: v.get("FETCH_SIZE", 0.0) + v.get("WRITE_SIZE", 0.0) for k, v in res.items() if isinstance(v, dict)}
+    out["_detail"] = res
+    json.dump(out, open(out_path, "w"), indent=1)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,7 +254,12 @@ def main():
                     help="with --profile-only: also run one device copy of MIB MiB (byte-counter calibration)")
     ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched"])
     ap.add_argument("--size", type=int, default=0, help="image side for --workload precision")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="derive profiles/traffic.json from two rocprofv3 --pmc runs made with --calibrate-copy 256")
     args = ap.parse_args()
+    if args.traffic_from:
+        traffic_from(args.traffic_from[0], args.traffic_from[1], 256, os.path.join(HERE, "profiles", "traffic.json"))
+        return
     if args.workload == "precision":
         precision_report(args)
         return

@@ -251,10 +251,10 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
       GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, b.pad1, st));
     }
     GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bs, p.d2, p.K, b.S1, b.vol, st, GBM3D_STAGE_MATCH1));
-    GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, st, ht_hook, c));
+    GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, p.bs, st, ht_hook, c));
     {
       StageTimer t(c, GBM3D_STAGE_AGG1_PREP);
-      GBM3D_TRY(launch_agg1_prep(nullptr, 0, b.vavg, p.M, p.N, p.K, p.bs, nullptr, b.w1, st));
+      GBM3D_TRY(launch_agg1_prep(b.vavg, p.M, p.N, p.K, p.bs, b.w1, st));
     }
     {
       StageTimer t(c, GBM3D_STAGE_AGG1);
@@ -565,7 +565,7 @@ int gbm3d_ht_filter(gbm3d_ctx* c, const float* Volume, int m, int n, int K, int 
   HtWork hw;
   ht_work_bind(htw, m, n, K, levels, cycleSpin, &hw);
   GBM3D_TRY(upload(c, in, Volume, vol));
-  GBM3D_TRY(run_ht(in, m, n, K, levels, cycleSpin, tau, hw, out, c->stream));
+  GBM3D_TRY(run_ht(in, m, n, K, levels, cycleSpin, tau, hw, out, 0, c->stream));
   GBM3D_TRY(download(c, Vavg, out, vol));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
   return GBM3D_OK;
@@ -578,23 +578,25 @@ int gbm3d_aggregate_ht(gbm3d_ctx* c, const float* Vavg, const uint32_t* S, int m
   GBM3D_TRY(stage_sizes(m, n, d, d2, K));
   const size_t mn = (size_t)m * n, G = mn / ((size_t)d * d);
   Carve cv{nullptr};
-  auto lay = [&](Carve& k, float** va, uint32_t** dS, float** w, float** dU, float** dW) {
+  auto lay = [&](Carve& k, float** va, float** vg, uint32_t** dS, float** w, float** dU, float** dW) {
     *va = k.take<float>(mn * K);
+    *vg = k.take<float>(mn * K);
     *dS = k.take<uint32_t>(G * K);
     *w = k.take<float>(G);
     *dU = k.take<float>(mn);
     *dW = k.take<float>(mn);
   };
-  float *va, *w, *dU, *dW;
+  float *va, *vg, *w, *dU, *dW;
   uint32_t* dS;
-  lay(cv, &va, &dS, &w, &dU, &dW);
+  lay(cv, &va, &vg, &dS, &w, &dU, &dW);
   GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
   Carve k2{c->arena};
-  lay(k2, &va, &dS, &w, &dU, &dW);
+  lay(k2, &va, &vg, &dS, &w, &dU, &dW);
   GBM3D_TRY(upload(c, va, Vavg, mn * K));
   GBM3D_TRY(upload(c, dS, S, G * K));
-  GBM3D_TRY(launch_agg1_prep(nullptr, 0, va, m, n, K, d, nullptr, w, c->stream));
-  GBM3D_TRY(launch_aggregate_m(1, va, w, dS, c->masks + kMaskStride * mask_slot(d), m, n, d, d2, K, 0, dU, dW,
+  GBM3D_TRY(launch_mosaic_to_groups(va, m, n, K, d, vg, c->stream));
+  GBM3D_TRY(launch_agg1_prep(vg, m, n, K, d, w, c->stream));
+  GBM3D_TRY(launch_aggregate_m(1, vg, w, dS, c->masks + kMaskStride * mask_slot(d), m, n, d, d2, K, 0, dU, dW,
                                c->stream));
   GBM3D_TRY(download(c, U, dU, mn));
   GBM3D_TRY(download(c, W, dW, mn));

@@ -91,17 +91,19 @@ void ht_work_bind(float* base, int M, int N, int K, int levels, int cycleSpin, H
 // hook(user, stage, 1) before and hook(user, stage, 0) after each kernel group
 // (stages GBM3D_STAGE_HT_LO / _HT_SYN / _HT_SYN0); may be null.
 typedef void (*StageHook)(void* user, int stage, int begin);
-// Writes Vavg = max(mean over cycle spins of the rebuilt volume, 0).
+// Writes Vavg = max(mean over cycle spins of the rebuilt volume, 0), in the
+// mosaic layout (gd == 0) or group-contiguous with gd x gd blocks
+// (Vavg[u + gd*(v + gd*(k + K*g))], the aggregation's layout).
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau,
-           const HtWork& w, float* vavg, hipStream_t st, StageHook hook = nullptr, void* user = nullptr);
+           const HtWork& w, float* vavg, int gd, hipStream_t st, StageHook hook = nullptr, void* user = nullptr);
 
-// Step-1 aggregation prep: Vavg = max(mean over spins of vrec, 0) (or a given
-// Vavg when nspin == 0) and per-group weights 1/(TV(Vavg)+eps).
-int launch_agg1_prep(const float* vrec, int nspin, const float* vavg_in, int M, int N, int K, int d,
-                     float* vavg, float* wgt, hipStream_t st);
+// Step-1 group weights 1/(TV(Vavg)+eps) from the group-contiguous Vavg.
+int launch_agg1_prep(const float* vavg_groups, int M, int N, int K, int d, float* wgt, hipStream_t st);
+// Mosaic (M x N x K) -> group-contiguous (d x d x K x G) copy.
+int launch_mosaic_to_groups(const float* src, int M, int N, int K, int d, float* dst, hipStream_t st);
 // Vavg = max(mean over spins of vrec, 0), elementwise.
-int launch_combine_spins(const float* vrec, int nspin, size_t vol, float* out, hipStream_t st);
-// Gather aggregation. step 1: blocks from the mosaic Vavg, weight wgt[g] * sine mask (.1,.9);
+
+// Gather aggregation. step 1: blocks from the group-contiguous Vavg, weight wgt[g] * sine mask (.1,.9);
 // step 2: blocks from V2f (d x d x K x G), weight mask(.2,.8)/gain2[g].
 // masks: device array [2][d*d] (step-1 mask, then step-2 mask).
 // Writes U, W for one match spin in the unshifted frame (shift 'spin').

@@ -27,80 +27,17 @@ __device__ __forceinline__ float third_diff(float x0, float x1, float x2, float 
   return f - e;
 }
 
-__global__ __launch_bounds__(256) void agg1_prep_kernel(const float* __restrict__ vrec, int nspin,
-                                                        const float* __restrict__ vavg_in, int M, int N, int K,
-                                                        int d, float* __restrict__ vavg, float* __restrict__ wgt) {
-  extern __shared__ float sl[];  // two d x d slices, [v][u]
-  __shared__ float red[3][256];
-  const int Mt = M / d;
-  const int g = blockIdx.x;
-  const int ti = g % Mt, tj = g / Mt;
-  const size_t MN = (size_t)M * N, vol = MN * K;
-  const int dd = d * d;
-  float s2 = 0.f, s1 = 0.f, s3 = 0.f;
-  for (int k = 0; k < K; ++k) {
-    float* cur = sl + (k & 1) * dd;
-    const float* prev = sl + ((k + 1) & 1) * dd;
-    for (int e = threadIdx.x; e < dd; e += 256) {
-      const int u = e % d, v = e / d;
-      const size_t o = (size_t)(ti * d + u) + (size_t)M * (tj * d + v) + MN * k;
-      float x;
-      if (nspin > 0) {
-        float acc = 0.f;
-        for (int s = 0; s < nspin; ++s) acc += vrec[vol * s + o];
-        x = acc / (float)nspin;
-        x = x > 0.f ? x : 0.f;  // max(mean(Vrec,4), 0)
-        vavg[o] = x;
-      } else {
-        x = vavg_in[o];
-      }
-      if (k > 0) s3 += fabsf(x - prev[e]);  // |diff(U,1,3)|, same thread wrote prev[e]
-      cur[e] = x;
-    }
-    __syncthreads();
-    // third differences along dim 2 (v) and dim 1 (u) of slice k
-    for (int e = threadIdx.x; e < d * (d - 3); e += 256) {
-      const int u = e % d, v = e / d;
-      const float* p = cur + u + d * v;
-      s2 += fabsf(third_diff(p[0], p[d], p[2 * d], p[3 * d]));
-      const int uu = e % (d - 3), vv = e / (d - 3);
-      const float* q = cur + uu + d * vv;
-      s1 += fabsf(third_diff(q[0], q[1], q[2], q[3]));
-    }
-    __syncthreads();
-  }
-  red[0][threadIdx.x] = s2;
-  red[1][threadIdx.x] = s1;
-  red[2][threadIdx.x] = s3;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    float D = red[0][0] * 0.25f;
-    D = D + red[1][0] * 0.25f;
-    D = D + red[2][0];
-    wgt[g] = 1.0f / (D + 2.220446049250313e-16f);  // 1/(D(blocks)+eps)
-  }
-}
-
-// One wave per group: lane e < d*d owns pixel (u, v) = (e % d, e / d) of every
-// slice; slices go through a 2-slice LDS buffer for the in-plane differences.
+// One wave per group; lane e < d*d owns pixel (u, v) = (e % d, e / d) of
+// every slice, slices staged one at a time in LDS for the in-plane differences.
 template <int D>
-__global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __restrict__ vrec, int nspin,
-                                                             const float* __restrict__ vavg_in, int M, int N,
-                                                             int K, float* __restrict__ vavg,
+__global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __restrict__ vg, int G, int K,
                                                              float* __restrict__ wgt) {
   constexpr int DD = D * D, PER = (DD + 63) / 64;
   __shared__ float sl[4][DD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int Mt = M / D, G = Mt * (N / D);
-  const int g = blockIdx.x * 4 + wave;
+  const int g = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
   if (g >= G) return;
-  const int ti = g % Mt, tj = g / Mt;
-  const size_t MN = (size_t)M * N, vol = MN * K;
+  const float* blk = vg + (size_t)DD * K * g;
   float* cur = sl[wave];
   float prev[PER];
   float s1 = 0.f, s2 = 0.f, s3 = 0.f;
@@ -109,18 +46,7 @@ __global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __rest
     for (int i = 0; i < PER; ++i) {
       const int e = lane + 64 * i;
       if (e < DD) {
-        const int u = e % D, v = e / D;
-        const size_t o = (size_t)(ti * D + u) + (size_t)M * (tj * D + v) + MN * k;
-        float x;
-        if (nspin > 0) {
-          float acc = 0.f;
-          for (int s = 0; s < nspin; ++s) acc += vrec[vol * s + o];
-          x = acc / (float)nspin;
-          x = x > 0.f ? x : 0.f;  // max(mean(Vrec,4), 0)
-          vavg[o] = x;
-        } else {
-          x = vavg_in[o];
-        }
+        const float x = blk[(size_t)DD * k + e];
         if (k > 0) s3 += fabsf(x - prev[i]);  // |diff(U,1,3)|
         prev[i] = x;
         cur[e] = x;
@@ -149,130 +75,48 @@ __global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __rest
   }
 }
 
-int launch_agg1_prep(const float* vrec, int nspin, const float* vavg_in, int M, int N, int K, int d, float* vavg,
-                     float* wgt, hipStream_t st) {
-  if (d < 4 || M % d || N % d) return fail(GBM3D_ERR_ARG, "agg1 prep: block size");
+int launch_agg1_prep(const float* vg, int M, int N, int K, int d, float* wgt, hipStream_t st) {
+  if (M % d || N % d) return fail(GBM3D_ERR_ARG, "agg1 prep: block size");
   const int G = (M / d) * (N / d);
   if (G == 0) return GBM3D_OK;
   const dim3 grid((G + 3) / 4);
   switch (d) {
-    case 4: hipLaunchKernelGGL(agg1_prep_wave_kernel<4>, grid, dim3(256), 0, st, vrec, nspin, vavg_in, M, N, K, vavg, wgt); break;
-    case 8: hipLaunchKernelGGL(agg1_prep_wave_kernel<8>, grid, dim3(256), 0, st, vrec, nspin, vavg_in, M, N, K, vavg, wgt); break;
-    case 16: hipLaunchKernelGGL(agg1_prep_wave_kernel<16>, grid, dim3(256), 0, st, vrec, nspin, vavg_in, M, N, K, vavg, wgt); break;
-    case 32: hipLaunchKernelGGL(agg1_prep_wave_kernel<32>, grid, dim3(256), 0, st, vrec, nspin, vavg_in, M, N, K, vavg, wgt); break;
-    default: {
-      const size_t lds = sizeof(float) * 2 * (size_t)d * d;
-      hipLaunchKernelGGL(agg1_prep_kernel, dim3(G), dim3(256), lds, st, vrec, nspin, vavg_in, M, N, K, d, vavg, wgt);
-    }
+    case 4: hipLaunchKernelGGL(agg1_prep_wave_kernel<4>, grid, dim3(256), 0, st, vg, G, K, wgt); break;
+    case 8: hipLaunchKernelGGL(agg1_prep_wave_kernel<8>, grid, dim3(256), 0, st, vg, G, K, wgt); break;
+    case 16: hipLaunchKernelGGL(agg1_prep_wave_kernel<16>, grid, dim3(256), 0, st, vg, G, K, wgt); break;
+    case 32: hipLaunchKernelGGL(agg1_prep_wave_kernel<32>, grid, dim3(256), 0, st, vg, G, K, wgt); break;
+    default: return fail(GBM3D_ERR_UNSUPPORTED, "block size must be 4, 8, 16 or 32");
   }
   return check_launch("agg1_prep");
 }
 
-// Vavg = max(mean over spins, 0) elementwise (denoiseAndAggregateGPU.m:45).
-__global__ void combine_spins_kernel(const float* __restrict__ vrec, int nspin, size_t vol, float* __restrict__ out) {
+// dst[u + d*(v + d*(k + K*g))] = src(i*d + u, j*d + v, k), g = i + Mt*j.
+__global__ void mosaic_to_groups_kernel(const float* __restrict__ src, int M, int N, int K, int d,
+                                        float* __restrict__ dst) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= vol) return;
-  float acc = 0.f;
-  for (int s = 0; s < nspin; ++s) acc += vrec[vol * s + e];
-  const float x = acc / (float)nspin;
-  out[e] = x > 0.f ? x : 0.f;
+  const size_t total = (size_t)M * N * K;
+  if (e >= total) return;
+  const int dd = d * d, Mt = M / d;
+  const int u = (int)(e % d), v = (int)((e / d) % d);
+  const size_t gk = e / dd;
+  const int k = (int)(gk % K);
+  const size_t g = gk / K;
+  const int ti = (int)(g % Mt), tj = (int)(g / Mt);
+  dst[e] = src[(size_t)(ti * d + u) + (size_t)M * ((size_t)(tj * d + v) + (size_t)N * k)];
 }
 
-int launch_combine_spins(const float* vrec, int nspin, size_t vol, float* out, hipStream_t st) {
-  if (vol == 0) return GBM3D_OK;
-  hipLaunchKernelGGL(combine_spins_kernel, dim3((unsigned)((vol + 255) / 256)), dim3(256), 0, st, vrec, nspin, vol, out);
-  return check_launch("combine_spins");
+int launch_mosaic_to_groups(const float* src, int M, int N, int K, int d, float* dst, hipStream_t st) {
+  if (M % d || N % d) return fail(GBM3D_ERR_ARG, "mosaic_to_groups: size");
+  const size_t total = (size_t)M * N * K;
+  if (total == 0) return GBM3D_OK;
+  hipLaunchKernelGGL(mosaic_to_groups_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, src, M, N, K,
+                     d, dst);
+  return check_launch("mosaic_to_groups");
 }
+
 
 // ---------------------------------------------------------- aggregation
 constexpr int AT = 16;  // output tile edge per wave
-
-// masks: [0, d*d) step-1 sine mask (.1,.9), [d*d, 2*d*d) step-2 mask (.2,.8)
-template <int STEP>
-__global__ __launch_bounds__(256) void aggregate_kernel(const float* __restrict__ blocks,
-                                                        const float* __restrict__ wgt,
-                                                        const uint32_t* __restrict__ S,
-                                                        const float* __restrict__ masks, int M, int N, int d,
-                                                        int d2, int K, int spin, float* __restrict__ U,
-                                                        float* __restrict__ W) {
-  __shared__ float accU[4][AT * AT], accW[4][AT * AT];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int tilesR = (M + AT - 1) / AT, tilesC = (N + AT - 1) / AT;
-  const int tile = blockIdx.x * 4 + wave;
-  if (tile >= tilesR * tilesC) return;
-  const int R0 = (tile % tilesR) * AT, C0 = (tile / tilesR) * AT;
-  float* aU = accU[wave];
-  float* aW = accW[wave];
-  for (int e = lane; e < AT * AT; e += 64) {
-    aU[e] = 0.f;
-    aW[e] = 0.f;
-  }
-  wave_sync();
-  const int s = d2 / 2, Mt = M / d, Nt = N / d;
-  const int dd = d * d;
-  // tile rows y whose blocks can reach rows [R0, R0+AT): y*d in [R0+s-d2-d+2, R0+AT-1+s]
-  auto first = [&](int r) {
-    const int x = r + s - d2 - d + 2;
-    return x > 0 ? (x + d - 1) / d : 0;
-  };
-  const int ylo = first(R0), yhi = min(Mt - 1, (R0 + AT - 1 + s) / d);
-  const int xlo = first(C0), xhi = min(Nt - 1, (C0 + AT - 1 + s) / d);
-  const int ny = yhi - ylo + 1, nx = xhi - xlo + 1;
-  const int ncand = (ny > 0 && nx > 0) ? ny * nx * K : 0;
-  const size_t MN = (size_t)M * N;
-  for (int base = 0; base < ncand; base += 64) {
-    // candidate t = ((y - ylo) * nx + (x - xlo)) * K + k: lexicographic (y, x, k)
-    const int t = base + lane;
-    int br = 0, bc = 0, g = 0, k = 0;
-    bool hit = false;
-    if (t < ncand) {
-      k = t % K;
-      const int yx = t / K;
-      const int y = ylo + yx / nx, x = xlo + yx % nx;
-      g = y + Mt * x;
-      const uint32_t idx = S[(size_t)K * g + k] - 1u;
-      br = y * d + (int)(idx % (uint32_t)d2) - s;
-      bc = x * d + (int)(idx / (uint32_t)d2) - s;
-      hit = br < R0 + AT && br + d > R0 && bc < C0 + AT && bc + d > C0;
-    }
-    uint64_t mask = __ballot(hit);
-    while (mask) {
-      const int src = __builtin_ctzll(mask);
-      mask &= mask - 1;
-      const int gb = __shfl(g, src, 64), kb = __shfl(k, src, 64);
-      const int rr = __shfl(br, src, 64), cc = __shfl(bc, src, 64);
-      const float wg = wgt[gb];
-      for (int e = lane; e < dd; e += 64) {
-        const int u = e % d, v = e / d;
-        const int pr = rr + u - R0, pc = cc + v - C0;
-        if (pr < 0 || pr >= AT || pc < 0 || pc >= AT) continue;
-        float b;
-        if (STEP == 1) {
-          const int ti = gb % Mt, tj = gb / Mt;
-          b = blocks[(size_t)(ti * d + u) + (size_t)M * (tj * d + v) + MN * kb];
-          const float mk = masks[e];
-          const float c = (wg * b) * mk;  // weight*blocks(:,:,i).*mask in single
-          aU[pr + AT * pc] = aU[pr + AT * pc] + c;
-          aW[pr + AT * pc] = aW[pr + AT * pc] + wg * mk;
-        } else {
-          b = blocks[(size_t)e + (size_t)dd * (kb + (size_t)K * gb)];
-          const float wt = masks[dd + e] / wg;  // w / norm(filt)^2
-          aU[pr + AT * pc] = aU[pr + AT * pc] + wt * b;
-          aW[pr + AT * pc] = aW[pr + AT * pc] + wt;
-        }
-      }
-      wave_sync();
-    }
-  }
-  // write the tile to the unshifted frame (circshift back by -spin)
-  for (int e = lane; e < AT * AT; e += 64) {
-    const int pr = R0 + (e % AT), pc = C0 + (e / AT);
-    if (pr >= M || pc >= N) continue;
-    const size_t o = (size_t)pmod_i(pr - spin, M) + (size_t)M * pmod_i(pc - spin, N);
-    U[o] = aU[e];
-    W[o] = aW[e];
-  }
-}
 
 // Batched variant.  Lane e of the wave owns pixel (e % D, e / D) (+64i) of
 // every block; overlapping blocks come from the ballot in (y, x, k) order,
@@ -314,7 +158,6 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   const int xlo = first(C0), xhi = min(Nt - 1, (C0 + AT - 1 + s) / D);
   const int ny = yhi - ylo + 1, nx = xhi - xlo + 1;
   const int ncand = (ny > 0 && nx > 0) ? ny * nx * K : 0;
-  const size_t MN = (size_t)M * N;
   for (int base = 0; base < ncand; base += 64) {
     const int t = base + lane;
     int br = 0, bc = 0, g = 0, k = 0;
@@ -345,8 +188,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
         if (src[j] < 0) continue;
         const int gb = __builtin_amdgcn_readlane(g, src[j]), kb = __builtin_amdgcn_readlane(k, src[j]);
         const int rr = __builtin_amdgcn_readlane(br, src[j]), cc = __builtin_amdgcn_readlane(bc, src[j]);
-        const float* bp = STEP == 1 ? blocks + (size_t)((gb % Mt) * D) + (size_t)M * ((gb / Mt) * D) + MN * kb
-                                    : blocks + (size_t)DD * (kb + (size_t)K * gb);
+        const float* bp = blocks + (size_t)DD * (kb + (size_t)K * gb);
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
           const int e = lane + 64 * i;
@@ -354,7 +196,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
           const int pr = rr + u, pc = cc + v;
           val[j][i] = 0.f;
           if (e < DD && pr >= 0 && pr < AT && pc >= 0 && pc < AT)
-            val[j][i] = STEP == 1 ? bp[u + (size_t)M * v] : bp[e];
+            val[j][i] = bp[e];
         }
       }
 #pragma unroll
@@ -413,11 +255,7 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
   GBM3D_AGG_CASE(16)
   GBM3D_AGG_CASE(32)
 #undef GBM3D_AGG_CASE
-  if (step == 1)
-    hipLaunchKernelGGL(aggregate_kernel<1>, grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d, d2, K, spin, U, W);
-  else
-    hipLaunchKernelGGL(aggregate_kernel<2>, grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d, d2, K, spin, U, W);
-  return check_launch("aggregate");
+  return fail(GBM3D_ERR_UNSUPPORTED, "aggregate: block size must be 4, 8, 16 or 32");
 }
 
 // --------------------------------------------------------- normalisation

@@ -99,24 +99,36 @@ constexpr int SPP = SP + 1;
 // read from acc (acc == nullptr reads 0).  Coarser levels always kStore.
 enum { kStore = 0, kAccum = 1, kFinal = 2 };
 
-__device__ __forceinline__ void put(float* Y, const float* acc, size_t o, float v, int mode, float cs) {
+// Offset of voxel (pr, pc, z) of a p x q x r volume: mosaic (column-major) for
+// gd == 0, else group-contiguous with gd x gd blocks ([group][z][col][row]),
+// the layout the aggregation reads.
+__device__ __forceinline__ size_t vol_off(int pr, int pc, int z, int p, int q, int r, int gd) {
+  if (gd == 0) return (size_t)pr + (size_t)p * ((size_t)pc + (size_t)q * z);
+  const int g = pr / gd + (p / gd) * (pc / gd);
+  return (size_t)(pr % gd) + (size_t)gd * ((size_t)(pc % gd) + (size_t)gd * ((size_t)z + (size_t)r * g));
+}
+
+// Store of one rebuilt voxel; the accumulator is always in mosaic layout.
+__device__ __forceinline__ void put(float* Y, const float* acc, int pr, int pc, int z, float v, int p, int q, int r,
+                                   int mode, float cs, int gd) {
+  const size_t om = vol_off(pr, pc, z, p, q, r, 0);
   if (mode == kStore) {
-    Y[o] = v;
-  } else {
-    const float a = acc ? acc[o] : 0.f;
-    const float t = a + v;
-    if (mode == kAccum) {
-      Y[o] = t;
-    } else {
-      const float mean = t / cs;
-      Y[o] = mean > 0.f ? mean : 0.f;  // max(mean(Vrec,4), 0)
-    }
+    Y[om] = v;
+    return;
   }
+  const float a = acc ? acc[om] : 0.f;
+  const float t = a + v;
+  if (mode == kAccum) {
+    Y[om] = t;
+    return;
+  }
+  const float mean = t / cs;
+  Y[vol_off(pr, pc, z, p, q, r, gd)] = mean > 0.f ? mean : 0.f;  // max(mean(Vrec,4), 0)
 }
 
 __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
                                                      int p, int q, int r, int sh, Tau7 tau,
-                                                     float* Y, const float* acc, int mode, float cs) {
+                                                     float* Y, const float* acc, int mode, float cs, int gd) {
   __shared__ float Z[2][SR * SRP];   // z-lo / z-hi planes, [col][row]
   __shared__ float A[2][CP * SRP];   // dim-2 lo / hi, [c][row]
   __shared__ float B[4][CP * CPP];   // bands b1 | b2<<1, [c][t]
@@ -199,16 +211,15 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
     __syncthreads();
   }
   // z synthesis: plane 2t3 <- g[0], plane 2t3-1 <- g[1]; lo g = [c c], hi g = [-c c]
-  float* Ye = Y + pq * pmod_i(2 * t3 - sh, r);
-  float* Yo = Y + pq * pmod_i(2 * t3 - 1 - sh, r);
+  const int ze = pmod_i(2 * t3 - sh, r), zo = pmod_i(2 * t3 - 1 - sh, r);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int e = threadIdx.x + 256 * i;
     const int x = e % SP, y = e / SP;
     if (x0 + x >= p || y0 + y >= q) continue;
-    const size_t o = (size_t)pmod_i(x0 + x - sh, p) + (size_t)p * pmod_i(y0 + y - sh, q);
-    put(Ye, acc ? acc + (Ye - Y) : nullptr, o, fCc * out[0][i] - fCc * out[1][i], mode, cs);
-    put(Yo, acc ? acc + (Yo - Y) : nullptr, o, fCc * out[0][i] + fCc * out[1][i], mode, cs);
+    const int pr = pmod_i(x0 + x - sh, p), pc = pmod_i(y0 + y - sh, q);
+    put(Y, acc, pr, pc, ze, fCc * out[0][i] - fCc * out[1][i], p, q, r, mode, cs, gd);
+    put(Y, acc, pr, pc, zo, fCc * out[0][i] + fCc * out[1][i], p, q, r, mode, cs, gd);
   }
 }
 
@@ -273,7 +284,7 @@ static_assert(4 * C * ES <= REG1, "E fits in A");
 __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
                                                           int p, int q, int r, int sh, Tau7 tau,
                                                           float* Y, const float* acc, int mode,
-                                                          float cs) {
+                                                          float cs, int gd) {
   using namespace fs;
   __shared__ float lds[REG0 + REG1];
   __shared__ int ridx[R], cidx[R];
@@ -391,16 +402,15 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
       }
       syn_lo_hi<4>(lo, hi, o[zb]);
     }
-    float* Ye = Y + pq * pmod_i(2 * t3 - sh, r);
-    float* Yo = Y + pq * pmod_i(2 * t3 - 1 - sh, r);
+    const int ze = pmod_i(2 * t3 - sh, r), zo = pmod_i(2 * t3 - 1 - sh, r);
     if (x0 + x < p) {
-      const size_t row = (size_t)ridx[x + 9];
+      const int pr = ridx[x + 9];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         if (y0 + yl + i >= q) continue;
-        const size_t off = row + (size_t)p * cidx[yl + i + 9];
-        put(Ye, acc ? acc + (Ye - Y) : nullptr, off, fCc * o[0][i] - fCc * o[1][i], mode, cs);
-        put(Yo, acc ? acc + (Yo - Y) : nullptr, off, fCc * o[0][i] + fCc * o[1][i], mode, cs);
+        const int pc = cidx[yl + i + 9];
+        put(Y, acc, pr, pc, ze, fCc * o[0][i] - fCc * o[1][i], p, q, r, mode, cs, gd);
+        put(Y, acc, pr, pc, zo, fCc * o[0][i] + fCc * o[1][i], p, q, r, mode, cs, gd);
       }
     }
   }
@@ -511,7 +521,8 @@ void ht_work_bind(float* base, int M, int N, int K, int levels, int cycleSpin, H
 }
 
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau, const HtWork& w,
-           float* vavg, hipStream_t st, StageHook hook, void* user) {
+           float* vavg, int gd, hipStream_t st, StageHook hook, void* user) {
+  if (gd != 0 && (M % gd || N % gd)) return fail(GBM3D_ERR_ARG, "ht: group layout block size");
   auto mark = [&](int stage, int begin) {
     if (hook) hook(user, stage, begin);
   };
@@ -550,15 +561,16 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
       const float* acc = (l == 0 && s > 0) ? w.vrec : nullptr;
       const int mode = l > 0 ? kStore : (last ? kFinal : (s == 0 ? kStore : kAccum));
       const float cs = (float)cycleSpin;
+      const int lgd = (l == 0 && last) ? gd : 0;
       dim3 grid((p + SP - 1) / SP, (q + SP - 1) / SP, r / 2);
       const int stage = l == 0 ? GBM3D_STAGE_HT_SYN0 : GBM3D_STAGE_HT_SYN;
       mark(stage, 1);
       if (p >= 10 && q >= 10 && !g_ht_generic)
         hipLaunchKernelGGL(ht_syn_fast_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out,
-                           acc, mode, cs);
+                           acc, mode, cs, lgd);
       else
         hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out, acc,
-                           mode, cs);
+                           mode, cs, lgd);
       GBM3D_TRY(check_launch("ht_syn"));
       mark(stage, 0);
     }
