@@ -26,7 +26,10 @@ GBM3D_ERR_MISSING_FIELD = 5
 FILT = {"ht": 0, "median": 1}
 FILT_NAME = {v: k for k, v in FILT.items()}
 
-STAGES = ("pad", "match1", "ht_lo", "ht_syn", "ht_syn0", "agg1_prep", "agg1", "match2", "wiener", "agg2", "norm")
+STAGES = ("pad", "match1", "ht_lo", "ht_syn", "ht_syn0", "agg1_prep", "agg1", "match2", "wiener", "agg2", "norm",
+          "shrink")
+
+GBM3D_SHRINK_PARAMS = 49600
 
 LIB_NAME = "libgbm3d.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -57,7 +60,8 @@ class Opts(C.Structure):
 class Ext(C.Structure):
     """struct gbm3d_ext (include/gbm3d.h)."""
 
-    _fields_ = [("match_bf16", C.c_int32), ("rescore", C.c_int32), ("reserved", C.c_int32 * 6)]
+    _fields_ = [("match_bf16", C.c_int32), ("rescore", C.c_int32), ("learned", C.c_int32),
+                ("learned_bound", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 _INT_FIELDS = ("numMax", "numMin", "levels", "cycleSpin", "tauMode", "blockSize",
@@ -152,6 +156,14 @@ SIGNATURES = {
     "gbm3d_set_timing": (_I, [_P, _I]),
     "gbm3d_device_copy": (_I, [_P, _P, _P, C.c_size_t]),
     "gbm3d_get_stage_times": (_I, [_P, _P, _P]),
+    "gbm3d_shrink_set_weights": (_I, [_P, _P, _I]),
+    "gbm3d_shrink_stats": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I)]),
+    "gbm3d_shrink_groups": (_I, [_P, _P, _I, _F, _P, _P]),
+    "gbm3d_shrink_features": (_I, [_P, _P, _P, _I, _I, _F, _P, _I, _P, _P, _P]),
+    "gbm3d_shrink_forward": (_I, [_P, _P, _P, _P, _I, _P]),
+    "gbm3d_shrink_grad": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, C.POINTER(_D)]),
+    "gbm3d_shrink_adam": (_I, [_P, _P, _P, _P, _P, _I, _F, _F, _F, _F]),
+    "gbm3d_synth_pair": (_I, [_P, C.c_uint64, _I, _I, _F, _P, _P]),
 }
 
 _lib = None

@@ -95,13 +95,76 @@ class Context:
     def synchronize(self):
         check(self.lib.gbm3d_synchronize(self.handle))
 
-    def set_variant(self, match_bf16: bool = False, rescore: int = 0):
-        """Precision variant: BF16-MFMA distances + exact FP32 rescoring of the
-        best `rescore` (>= 2*numMax) candidates per tile.  False = reference."""
+    def set_variant(self, match_bf16: bool = False, rescore: int = 0, learned: bool = False,
+                    learned_bound: float = 1.0, weights=None):
+        """Forms beside the reference path (all False = reference).
+
+        match_bf16: BF16-MFMA distances + exact FP32 rescoring of the best
+        `rescore` (>= 2*numMax) candidates per tile.
+        learned: learned shrinkage replaces the step-1 hard threshold; an image
+        whose estimated learned residual exceeds learned_bound x the
+        hard-threshold estimate is recomputed on the reference path.  Loads
+        `weights` (parameter vector) or the shipped trained weights."""
+        if learned:
+            from . import shrink
+
+            self.load_shrink_weights(shrink.load_weights() if weights is None else weights)
         e = _abi.Ext()
         e.match_bf16 = int(bool(match_bf16))
         e.rescore = int(rescore)
+        e.learned = int(bool(learned))
+        e.learned_bound = int(round(learned_bound * 1000)) if learned else 0
         check(self.lib.gbm3d_set_ext(self.handle, C.byref(e)))
+
+    # ------------------------------------------------------ learned shrinkage
+    def load_shrink_weights(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        check(self.lib.gbm3d_shrink_set_weights(self.handle, ptr(p), p.size))
+
+    def shrink_stats(self, image: int = 0):
+        """(learned residual estimate, hard-threshold estimate, fell back) of the last call."""
+        a, b, f = C.c_double(), C.c_double(), C.c_int()
+        check(self.lib.gbm3d_shrink_stats(self.handle, image, C.byref(a), C.byref(b), C.byref(f)))
+        return a.value, b.value, bool(f.value)
+
+    def shrink_groups(self, V, sigma: float, want_gains: bool = False):
+        """Fused learned shrinkage of groups V (8, 8, 16, ...) -> (Vout, gains (G, 64, 16) or None)."""
+        V = np.asfortranarray(V, dtype=np.float32)
+        G = V.size // (8 * 8 * 16)
+        out = np.zeros_like(V)
+        g = np.zeros((8, 8, 16, G), dtype=np.float32, order="F") if want_gains else None
+        check(self.lib.gbm3d_shrink_groups(self.handle, ptr(V), G, C.c_float(sigma), ptr(out), ptr(g)))
+        gains = None if g is None else g.reshape(64, 16, G, order="F").transpose(2, 0, 1)
+        return out, gains
+
+    def synth_pair(self, seed: int, m: int, n: int, sigma: float, dClean: int | None, dNoisy: int | None):
+        check(self.lib.gbm3d_synth_pair(self.handle, C.c_uint64(seed), m, n, C.c_float(sigma),
+                                        C.c_void_p(dClean) if dClean else None,
+                                        C.c_void_p(dNoisy) if dNoisy else None))
+
+    def shrink_features(self, dClean: int, dNoisy: int, m: int, n: int, sigma: float, dSel: int, nsel: int,
+                        dX: int, dT: int, dWt: int):
+        check(self.lib.gbm3d_shrink_features(self.handle, C.c_void_p(dClean), C.c_void_p(dNoisy), m, n,
+                                             C.c_float(sigma), C.c_void_p(dSel), nsel, C.c_void_p(dX),
+                                             C.c_void_p(dT), C.c_void_p(dWt)))
+
+    def shrink_forward(self, dParams: int, dX: int, dSig: int, ngroups: int, dGains: int):
+        check(self.lib.gbm3d_shrink_forward(self.handle, C.c_void_p(dParams), C.c_void_p(dX), C.c_void_p(dSig),
+                                            ngroups, C.c_void_p(dGains)))
+
+    def shrink_grad(self, dParams: int, dX: int, dT: int, dWt: int, dSig: int, ngroups: int, dGrad: int,
+                    want_loss: bool = True):
+        loss = C.c_double()
+        check(self.lib.gbm3d_shrink_grad(self.handle, C.c_void_p(dParams), C.c_void_p(dX), C.c_void_p(dT),
+                                         C.c_void_p(dWt), C.c_void_p(dSig), ngroups, C.c_void_p(dGrad),
+                                         C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def shrink_adam(self, dParams: int, dM: int, dV: int, dGrad: int, step: int, lr: float, beta1: float = 0.9,
+                    beta2: float = 0.999, eps: float = 1e-8):
+        check(self.lib.gbm3d_shrink_adam(self.handle, C.c_void_p(dParams), C.c_void_p(dM), C.c_void_p(dV),
+                                         C.c_void_p(dGrad), step, C.c_float(lr), C.c_float(beta1),
+                                         C.c_float(beta2), C.c_float(eps)))
 
     def set_timing(self, on: bool):
         check(self.lib.gbm3d_set_timing(self.handle, int(bool(on))))

@@ -33,6 +33,13 @@ struct gbm3d_ctx {
   double tuned_ms = 0.0;
   std::vector<int64_t> tkey;
   int match_waves = 1;  // reference tiles per matching workgroup (single-image path)
+  // learned shrinkage
+  uint16_t* shrink_wb = nullptr;  // BF16 weights [4][64][192]
+  float* shrink_fb = nullptr;     // FP32 biases and sigma-channel weights
+  float* sstats = nullptr;        // per image and spin: [kShrinkGrid][2] residual partials
+  size_t sstats_floats = 0;
+  std::vector<double> s_learned, s_ht;  // last call, per image
+  std::vector<int> s_fell;
   struct Mark {
     int stage;
     hipEvent_t a, b;
@@ -227,7 +234,7 @@ size_t pipeline_bytes(const Plan& p, const gbm3d_opts& o) {
 
 // One image through both steps on stream st, workspace at ws (pipeline_bytes).
 int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, const gbm3d_opts& o, float* dU,
-                    float* dU1, hipStream_t st, char* ws) {
+                    float* dU1, hipStream_t st, char* ws, float* sstats) {
   const int m = p.m, n = p.n;
   Carve cv{ws};
   Bufs b;
@@ -250,8 +257,18 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
       StageTimer t(c, GBM3D_STAGE_PAD);
       GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, b.pad1, st));
     }
-    GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bs, p.d2, p.K, b.S1, b.vol, st, GBM3D_STAGE_MATCH1));
-    GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, p.bs, st, ht_hook, c));
+    if (c->ext.learned) {
+      // variant: learned gains on the DCT-Haar coefficients of each group,
+      // gathered straight from the search image (no mosaic volume)
+      GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bs, p.d2, p.K, b.S1, nullptr, st, GBM3D_STAGE_MATCH1));
+      StageTimer t(c, GBM3D_STAGE_SHRINK);
+      const long G1 = (long)(p.M / p.bs) * (p.N / p.bs);
+      GBM3D_TRY(launch_shrink(b.pad1, b.S1, p.M, p.d2, G1, sigma, c->shrink_wb, c->shrink_fb, b.vavg, nullptr,
+                              sstats + (size_t)2 * kShrinkGrid * ii, nullptr, st));
+    } else {
+      GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bs, p.d2, p.K, b.S1, b.vol, st, GBM3D_STAGE_MATCH1));
+      GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, p.bs, st, ht_hook, c));
+    }
     {
       StageTimer t(c, GBM3D_STAGE_AGG1_PREP);
       GBM3D_TRY(launch_agg1_prep(b.vavg, p.M, p.N, p.K, p.bs, b.w1, st));
@@ -303,12 +320,84 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
   return GBM3D_OK;
 }
 
+// The learned form needs loaded weights, 8 x 8 x 16 step-1 groups and a
+// residual buffer of [images][spins][kShrinkGrid][2].
+int learned_setup(gbm3d_ctx* c, const gbm3d_opts& o, int images) {
+  if (!c->ext.learned) return GBM3D_OK;
+  if (!c->shrink_wb) return fail(GBM3D_ERR_ARG, "learned shrinkage: no weights loaded (gbm3d_shrink_set_weights)");
+  if (o.blockSize != 8 || o.numMax != 16)
+    return fail(GBM3D_ERR_UNSUPPORTED, "learned shrinkage needs blockSize 8 and numMax 16");
+  const size_t need = (size_t)images * GBM3D_MAX_SPINS * 2 * kShrinkGrid;
+  if (need > c->sstats_floats) {
+    if (c->sstats) GBM3D_HIP(hipFree(c->sstats));
+    c->sstats = nullptr;
+    c->sstats_floats = 0;
+    GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->sstats), need * sizeof(float)));
+    c->sstats_floats = need;
+  }
+  return GBM3D_OK;
+}
+
+float* image_stats(gbm3d_ctx* c, int i) {
+  return c->ext.learned ? c->sstats + (size_t)i * GBM3D_MAX_SPINS * 2 * kShrinkGrid : nullptr;
+}
+
+int shrink_grid_of(const Plan& p) {
+  const long G = (long)(p.M / p.bs) * (p.N / p.bs);
+  const long passes = (G + 15) / 16;
+  return (int)std::min<long>(passes, kShrinkGrid);
+}
+
+// After a learned-form call has finished: residual estimates per image; the
+// images whose learned estimate exceeds bound x the hard-threshold estimate
+// are recomputed on the reference path (eagerly, on the context stream).
+int learned_judge(gbm3d_ctx* c, const gbm3d_opts& o, int count, const std::vector<Plan>& plans,
+                  const float* const* dI, const float* sigma, float* const* dU, float* const* dU1) {
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  const size_t per = (size_t)GBM3D_MAX_SPINS * 2 * kShrinkGrid;
+  std::vector<float> h((size_t)count * per);
+  GBM3D_HIP(hipMemcpy(h.data(), c->sstats, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  const double bound = (c->ext.learned_bound > 0 ? c->ext.learned_bound : 1000) / 1000.0;
+  c->s_learned.assign(count, 0.0);
+  c->s_ht.assign(count, 0.0);
+  c->s_fell.assign(count, 0);
+  std::vector<int> redo;
+  for (int i = 0; i < count; ++i) {
+    const int grid = shrink_grid_of(plans[i]);
+    for (int sp = 0; sp < plans[i].n1; ++sp)
+      for (int b = 0; b < grid; ++b) {
+        c->s_learned[i] += h[i * per + (size_t)2 * kShrinkGrid * sp + 2 * b];
+        c->s_ht[i] += h[i * per + (size_t)2 * kShrinkGrid * sp + 2 * b + 1];
+      }
+    if (!(c->s_learned[i] <= bound * c->s_ht[i])) redo.push_back(i);
+  }
+  if (redo.empty()) return GBM3D_OK;
+  const gbm3d_ext keep = c->ext;
+  c->ext.learned = 0;
+  int st = GBM3D_OK;
+  for (int i : redo) {
+    c->s_fell[i] = 1;
+    st = ensure(&c->arena, &c->arena_bytes, pipeline_bytes(plans[i], o));
+    if (st == GBM3D_OK)
+      st = run_pipeline_on(c, dI[i], plans[i], sigma[i], o, dU[i], dU1 ? dU1[i] : nullptr, c->stream, c->arena,
+                           nullptr);
+    if (st != GBM3D_OK) break;
+  }
+  c->ext = keep;
+  return st;
+}
+
 int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const gbm3d_opts& o, float* dU,
                  float* dU1) {
   Plan p;
   GBM3D_TRY(make_plan(o, m, n, &p));
+  GBM3D_TRY(learned_setup(c, o, 1));
   GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, pipeline_bytes(p, o)));
-  return run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena);
+  GBM3D_TRY(run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena, image_stats(c, 0)));
+  if (!c->ext.learned) return GBM3D_OK;
+  std::vector<Plan> plans{p};
+  float* u1[1] = {dU1};
+  return learned_judge(c, o, 1, plans, &dI, &sigma, &dU, dU1 ? u1 : nullptr);
 }
 
 int set_dev(gbm3d_ctx* c) {
@@ -405,6 +494,9 @@ int gbm3d_destroy(gbm3d_ctx* c) {
   if (c->arena) (void)hipFree(c->arena);
   if (c->io) (void)hipFree(c->io);
   if (c->masks) (void)hipFree(c->masks);
+  if (c->shrink_wb) (void)hipFree(c->shrink_wb);
+  if (c->shrink_fb) (void)hipFree(c->shrink_fb);
+  if (c->sstats) (void)hipFree(c->sstats);
   for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (auto sb : c->bstreams)
@@ -758,7 +850,7 @@ int issue_batch(gbm3d_ctx* c, const BatchJob& j, const gbm3d_opts& o, int S, int
   for (int i = 0; i < j.count && st == GBM3D_OK; ++i) {
     const int s = assign[i];
     st = run_pipeline_on(c, j.dI[i], j.plans[i], j.sigma[i], o, j.dU[i], j.dU1 ? j.dU1[i] : nullptr, c->bstreams[s],
-                         c->arena + off[s]);
+                         c->arena + off[s], image_stats(c, i));
   }
   for (int s = 0; s < S; ++s) {
     // join every branch even after an error so the capture stays well formed
@@ -780,7 +872,7 @@ size_t batch_bytes(const BatchJob& j, const gbm3d_opts& o, int S) {
 }
 
 std::vector<int64_t> batch_key(const BatchJob& j, const gbm3d_opts& o, const gbm3d_ext& e, int S, int mw) {
-  std::vector<int64_t> k = {j.count, S, mw, e.match_bf16, e.rescore};
+  std::vector<int64_t> k = {j.count, S, mw, e.match_bf16, e.rescore, e.learned, e.learned_bound};
   for (int i = 0; i < j.count; ++i) {
     uint32_t sb;
     std::memcpy(&sb, &j.sigma[i], 4);
@@ -795,6 +887,8 @@ std::vector<int64_t> batch_key(const BatchJob& j, const gbm3d_opts& o, const gbm
   for (size_t i = 0; i < sizeof(o) / 4; ++i) k.push_back(ow[i]);
   return k;
 }
+
+int batch_run(gbm3d_ctx* c, BatchJob& j, const gbm3d_opts& o, int flags);
 
 }  // namespace
 
@@ -813,6 +907,29 @@ int gbm3d_denoise_batch_device(gbm3d_ctx* c, int count, const float* const* dI, 
     if (!dI[i] || !dU[i]) return fail(GBM3D_ERR_ARG, "batch: NULL image " + std::to_string(i));
     GBM3D_TRY(make_plan(o, m[i], n[i], &j.plans[i]));
   }
+  GBM3D_TRY(learned_setup(c, o, count));
+  if (c->ext.learned) {
+    // learned form: run, then judge every image against the bound
+    GBM3D_TRY(batch_run(c, j, o, flags));
+    return learned_judge(c, o, count, j.plans, dI, sigma, dU, dU1);
+  }
+  return batch_run(c, j, o, flags);
+}
+
+int gbm3d_batch_info(gbm3d_ctx* c, int* streams, int* match_waves, double* tuned_ms) {
+  GBM3D_TRY(set_dev(c));
+  if (streams) *streams = c->tuned_streams;
+  if (match_waves) *match_waves = c->tuned_match_waves;
+  if (tuned_ms) *tuned_ms = c->tuned_ms;
+  return GBM3D_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int batch_run(gbm3d_ctx* c, BatchJob& j, const gbm3d_opts& o, int flags) {
+  const int count = j.count;
   int S = std::min(count, 4), mw = 4;
   if (flags & GBM3D_BATCH_TUNE) {
     std::vector<int64_t> tk = batch_key(j, o, c->ext, 0, 0);
@@ -856,6 +973,7 @@ int gbm3d_denoise_batch_device(gbm3d_ctx* c, int count, const float* const* dI, 
   if (!(flags & GBM3D_BATCH_GRAPH)) return issue_batch(c, j, o, S, mw);
   std::vector<int64_t> key = batch_key(j, o, c->ext, S, mw);
   key.push_back((int64_t)(intptr_t)c->arena);  // a grown arena invalidates the captured addresses
+  key.push_back((int64_t)(intptr_t)c->sstats);
   if (!c->gexec || key != c->gkey) {
     if (c->gexec) {
       (void)hipGraphExecDestroy(c->gexec);
@@ -882,12 +1000,137 @@ int gbm3d_denoise_batch_device(gbm3d_ctx* c, int count, const float* const* dI, 
   return GBM3D_OK;
 }
 
-int gbm3d_batch_info(gbm3d_ctx* c, int* streams, int* match_waves, double* tuned_ms) {
+}  // namespace
+
+// ---------------------------------------------------------- learned shrinkage
+extern "C" {
+
+int gbm3d_shrink_set_weights(gbm3d_ctx* c, const float* params, int count) {
   GBM3D_TRY(set_dev(c));
-  if (streams) *streams = c->tuned_streams;
-  if (match_waves) *match_waves = c->tuned_match_waves;
-  if (tuned_ms) *tuned_ms = c->tuned_ms;
+  if (!params || count != GBM3D_SHRINK_PARAMS)
+    return fail(GBM3D_ERR_ARG, "shrink weights: expected " + std::to_string(GBM3D_SHRINK_PARAMS) + " values");
+  for (int i = 0; i < count; ++i)
+    if (!std::isfinite(params[i])) return fail(GBM3D_ERR_ARG, "shrink weights: non-finite value");
+  std::vector<uint16_t> wb((size_t)shrink::NL * shrink::CH * shrink::KW);
+  std::vector<float> fb(shrink::FB);
+  shrink_pack_weights(params, wb.data(), fb.data());
+  if (!c->shrink_wb) GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->shrink_wb), wb.size() * sizeof(uint16_t)));
+  if (!c->shrink_fb) GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->shrink_fb), fb.size() * sizeof(float)));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  GBM3D_HIP(hipMemcpy(c->shrink_wb, wb.data(), wb.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  GBM3D_HIP(hipMemcpy(c->shrink_fb, fb.data(), fb.size() * sizeof(float), hipMemcpyHostToDevice));
   return GBM3D_OK;
+}
+
+int gbm3d_shrink_stats(gbm3d_ctx* c, int image, double* est_learned, double* est_ht, int* fell_back) {
+  GBM3D_TRY(set_dev(c));
+  if (image < 0 || image >= (int)c->s_learned.size()) return fail(GBM3D_ERR_ARG, "shrink stats: no such image");
+  if (est_learned) *est_learned = c->s_learned[image];
+  if (est_ht) *est_ht = c->s_ht[image];
+  if (fell_back) *fell_back = c->s_fell[image];
+  return GBM3D_OK;
+}
+
+int gbm3d_shrink_groups(gbm3d_ctx* c, const float* V, int G, float sigma, float* Vout, float* gains) {
+  GBM3D_TRY(set_dev(c));
+  if (!V || !Vout || G < 0) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (!c->shrink_wb) return fail(GBM3D_ERR_ARG, "learned shrinkage: no weights loaded (gbm3d_shrink_set_weights)");
+  if (G == 0) return GBM3D_OK;
+  const size_t n = (size_t)64 * 16 * G;
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** a, float** o, float** g) {
+    *a = k.take<float>(n);
+    *o = k.take<float>(n);
+    *g = k.take<float>(gains ? n : 0);
+  };
+  float *a, *o, *g;
+  lay(cv, &a, &o, &g);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &a, &o, &g);
+  GBM3D_TRY(upload(c, a, V, n));
+  GBM3D_TRY(launch_shrink(a, nullptr, 0, 0, G, sigma, c->shrink_wb, c->shrink_fb, o, gains ? g : nullptr, nullptr,
+                          nullptr, c->stream));
+  GBM3D_TRY(download(c, Vout, o, n));
+  if (gains) GBM3D_TRY(download(c, gains, g, n));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_shrink_features(gbm3d_ctx* c, const float* dClean, const float* dNoisy, int m, int n, float sigma,
+                          const uint32_t* dSel, int nsel, float* dX, float* dT, float* dWt) {
+  GBM3D_TRY(set_dev(c));
+  if (!dClean || !dNoisy || !dSel || !dX || !dT || !dWt || nsel < 0) return fail(GBM3D_ERR_ARG, "NULL argument");
+  constexpr int d = 8, d2 = 32, K = 16;
+  if (m % d || n % d || m < d2 || n < d2) return fail(GBM3D_ERR_ARG, "shrink features: m, n multiples of 8, >= 32");
+  if (!(sigma > 0.f)) return fail(GBM3D_ERR_ARG, "shrink features: sigma must be > 0");
+  const size_t P = (size_t)m + d2, Q = (size_t)n + d2, G = (size_t)m * n / (d * d);
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** pc, float** pn, float** bn, uint32_t** S) {
+    *pc = k.take<float>(P * Q);
+    *pn = k.take<float>(P * Q);
+    *bn = k.take<float>((P - d + 1) * (Q - d + 1));
+    *S = k.take<uint32_t>(G * K);
+  };
+  float *pc, *pn, *bn;
+  uint32_t* S;
+  lay(cv, &pc, &pn, &bn, &S);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &pc, &pn, &bn, &S);
+  GBM3D_TRY(launch_search_pad(dClean, m, n, 0, d2, pc, c->stream));
+  GBM3D_TRY(launch_search_pad(dNoisy, m, n, 0, d2, pn, c->stream));
+  GBM3D_TRY(launch_norm_map(pn, (int)P, (int)Q, d, bn, c->stream));
+  GBM3D_TRY(launch_match_w(pn, bn, m, n, d, d2, K, S, nullptr, c->match_waves, c->stream));
+  return launch_shrink_features(pc, pn, S, m, d2, dSel, nsel, sigma, dX, dT, dWt, c->stream);
+}
+
+int gbm3d_shrink_forward(gbm3d_ctx* c, const float* dParams, const float* dX, const float* dSig, int ngroups,
+                         float* dGains) {
+  GBM3D_TRY(set_dev(c));
+  if (!dParams || !dX || !dSig || !dGains || ngroups < 0) return fail(GBM3D_ERR_ARG, "NULL argument");
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, shrink_train_floats(ngroups) * sizeof(float)));
+  return shrink_forward(dParams, dX, dSig, ngroups, reinterpret_cast<float*>(c->arena), dGains, c->stream);
+}
+
+int gbm3d_shrink_grad(gbm3d_ctx* c, const float* dParams, const float* dX, const float* dT, const float* dWt,
+                      const float* dSig, int ngroups, float* dGrad, double* loss) {
+  GBM3D_TRY(set_dev(c));
+  if (!dParams || !dX || !dT || !dWt || !dSig || !dGrad) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (ngroups < 1) return fail(GBM3D_ERR_ARG, "shrink grad: empty batch");
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, shrink_train_floats(ngroups) * sizeof(float)));
+  float* ws = reinterpret_cast<float*>(c->arena);
+  int lb = 0;
+  GBM3D_TRY(shrink_grad(dParams, dX, dT, dWt, dSig, ngroups, ws, dGrad, nullptr, &lb, c->stream));
+  if (loss) {
+    // the loss partials follow the gradient workspace (see shrink_train_floats)
+    const size_t off = shrink_train_floats(ngroups) - 1024;
+    std::vector<float> h(lb);
+    GBM3D_HIP(hipMemcpyAsync(h.data(), ws + off, lb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    GBM3D_HIP(hipStreamSynchronize(c->stream));
+    double acc = 0.0;
+    for (float v : h) acc += v;
+    *loss = acc / ngroups;
+  }
+  return GBM3D_OK;
+}
+
+int gbm3d_shrink_adam(gbm3d_ctx* c, float* dParams, float* dM, float* dV, const float* dGrad, int step, float lr,
+                      float beta1, float beta2, float eps) {
+  GBM3D_TRY(set_dev(c));
+  if (!dParams || !dM || !dV || !dGrad) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (step < 1 || !(beta1 >= 0.f && beta1 < 1.f) || !(beta2 >= 0.f && beta2 < 1.f) || !(eps > 0.f))
+    return fail(GBM3D_ERR_ARG, "adam: step >= 1, 0 <= beta < 1, eps > 0");
+  const double lr_t = lr * std::sqrt(1.0 - std::pow((double)beta2, step)) / (1.0 - std::pow((double)beta1, step));
+  return launch_shrink_adam(dParams, dM, dV, dGrad, GBM3D_SHRINK_PARAMS, (float)lr_t, beta1, beta2, eps, c->stream);
+}
+
+int gbm3d_synth_pair(gbm3d_ctx* c, uint64_t seed, int m, int n, float sigma, float* dClean, float* dNoisy) {
+  GBM3D_TRY(set_dev(c));
+  if (m < 1 || n < 1) return fail(GBM3D_ERR_ARG, "synth: empty image");
+  SynthParams prm;
+  synth_params(seed, m, n, &prm);
+  return launch_synth_pair(prm, seed, m, n, sigma, dClean, dNoisy, c->stream);
 }
 
 }  // extern "C"

@@ -101,7 +101,6 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
 int launch_agg1_prep(const float* vavg_groups, int M, int N, int K, int d, float* wgt, hipStream_t st);
 // Mosaic (M x N x K) -> group-contiguous (d x d x K x G) copy.
 int launch_mosaic_to_groups(const float* src, int M, int N, int K, int d, float* dst, hipStream_t st);
-// Vavg = max(mean over spins of vrec, 0), elementwise.
 
 // Gather aggregation. step 1: blocks from the group-contiguous Vavg, weight wgt[g] * sine mask (.1,.9);
 // step 2: blocks from V2f (d x d x K x G), weight mask(.2,.8)/gain2[g].
@@ -117,5 +116,52 @@ int launch_wiener(const float* pad_pilot, const float* pad_noisy, const uint32_t
 // out = sum_s U_s / (sum_s W_s + sum_t Wextra_t), cropped to m x n, double math.
 int launch_normalize(const float* U, int nU, const float* W, int nW, const float* Wextra, int nWextra,
                      int M, int N, int m, int n, float* out, hipStream_t st);
+
+// ------------------------------------------------------- learned shrinkage
+// Network over the DCT-Haar coefficients of 8 x 8 x 16 groups (4 conv layers
+// of width 3 along the 16 positions, 64 channels, sigma input channel).
+namespace shrink {
+constexpr int CH = 64, POS = 16, TAPS = 3, KW = TAPS * CH, NL = 4;
+constexpr int OFF_WSIG = KW * CH, OFF_B0 = OFF_WSIG + TAPS * CH;
+__host__ __device__ constexpr int off_w(int l) { return l == 0 ? 0 : OFF_B0 + CH + (l - 1) * (KW * CH + CH); }
+__host__ __device__ constexpr int off_b(int l) { return l == 0 ? OFF_B0 : off_w(l) + KW * CH; }
+constexpr int NPARAM = off_b(NL - 1) + CH;
+static_assert(NPARAM == GBM3D_SHRINK_PARAMS, "parameter count");
+constexpr int FB = NL * CH + TAPS * CH;  // biases then sigma-channel weights (FP32)
+}  // namespace shrink
+constexpr int kShrinkGrid = 256;    // persistent blocks of the fused kernel
+constexpr int kShrinkChunk = 1024;  // data rows per weight-gradient block
+// params -> BF16 weights wb [4][64 cout][192 (tap, cin)] and FP32 fb (host).
+void shrink_pack_weights(const float* params, uint16_t* wb, float* fb);
+// Fused shrinkage of G groups: gathered from the padded image through S
+// (S != nullptr) or read from src as d x d x K x G; writes filtered groups
+// (clipped at 0), optionally the gains, and per-block residual estimates
+// stats[2 * b] (learned) / stats[2 * b + 1] (hard-threshold proxy).
+int launch_shrink(const float* src, const uint32_t* S, int M, int d2, long G, float sigma, const uint16_t* wb,
+                  const float* fb, float* vout, float* gout, float* stats, int* grid_used, hipStream_t st);
+// Training rows of the selected groups (noisy features, oracle Wiener gains, loss weights).
+int launch_shrink_features(const float* padC, const float* padN, const uint32_t* S, int M, int d2,
+                           const uint32_t* sel, int nsel, float sigma, float* X, float* Tg, float* Wt,
+                           hipStream_t st);
+size_t shrink_train_floats(int ngroups);
+int shrink_forward(const float* par, const float* X, const float* sig, int ngroups, float* ws, float* gains,
+                   hipStream_t st);
+int shrink_grad(const float* par, const float* X, const float* Tg, const float* Wt, const float* sig, int ngroups,
+                float* ws, float* grad, float* loss_part, int* loss_blocks, hipStream_t st);
+int launch_shrink_adam(float* p, float* m, float* v, const float* g, int n, float lr_t, float b1, float b2,
+                       float eps, hipStream_t st);
+
+// ------------------------------------------------------ synthetic test images
+// Shape parameters of one clean image (the construction of gbm3d_synth_pair).
+struct SynthParams {
+  int rect[12][4];     // r0, r1, c0, c1 (half-open)
+  double rect_val[12];
+  double disk[8][4];   // centre row, centre col, radius^2, value
+  int tex[4];          // r0, r1, c0, c1
+  double tex_theta, tex_period, tex_phase;
+};
+void synth_params(uint64_t seed, int m, int n, SynthParams* p);
+int launch_synth_pair(const SynthParams& p, uint64_t seed, int m, int n, float sigma, float* clean, float* noisy,
+                      hipStream_t st);
 
 }  // namespace gbm3d

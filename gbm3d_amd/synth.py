@@ -46,33 +46,55 @@ class _Stream:
         return lo + (hi - lo) * v
 
 
-def clean_image(seed: int, m: int, n: int) -> np.ndarray:
-    """The clean synthetic image (float64, shape (m, n))."""
-    st = _Stream(seed * 2 + 1)
-    img = np.zeros((m, n), dtype=np.float64)
-    rr, cc = np.meshgrid(np.arange(m, dtype=np.float64), np.arange(n, dtype=np.float64), indexing="ij")
+# Uniform draws per image, in stream order: 12 rectangles x (h, w, r0, c0,
+# value), 8 disks x (radius, row, col, value), texture (h, w, r0, c0, theta,
+# period, phase).
+N_DRAWS = 12 * 5 + 8 * 4 + 7
+
+
+def shape_params(seed: int, m: int, n: int) -> dict:
+    """Shape parameters of the clean image (the first N_DRAWS stream values)."""
+    u = uniform(seed * 2 + 1, np.arange(N_DRAWS, dtype=np.uint64))
+    it = iter(u.tolist())
+
+    def draw(lo, hi):
+        return lo + (hi - lo) * next(it)
+
+    rects, disks = [], []
     for _ in range(12):
-        h = int(st.u(m / 16, m / 2))
-        w = int(st.u(n / 16, n / 2))
-        r0 = int(st.u(0, max(m - h, 1)))
-        c0 = int(st.u(0, max(n - w, 1)))
-        val = st.u(0, 255)
-        img[r0:r0 + h, c0:c0 + w] = val
+        h = int(draw(m / 16, m / 2))
+        w = int(draw(n / 16, n / 2))
+        r0 = int(draw(0, max(m - h, 1)))
+        c0 = int(draw(0, max(n - w, 1)))
+        rects.append((r0, r0 + h, c0, c0 + w, draw(0, 255)))
     s = min(m, n)
     for _ in range(8):
-        rad = st.u(s / 32, s / 6)
-        cr = st.u(0, m)
-        ccn = st.u(0, n)
-        val = st.u(0, 255)
-        img[(rr - cr) ** 2 + (cc - ccn) ** 2 <= rad * rad] = val
-    h = int(st.u(m / 8, m / 3))
-    w = int(st.u(n / 8, n / 3))
-    r0 = int(st.u(0, max(m - h, 1)))
-    c0 = int(st.u(0, max(n - w, 1)))
-    theta = st.u(0, np.pi)
-    period = st.u(6, 16)
-    phase = st.u(0, 2 * np.pi)
-    box = (slice(r0, r0 + h), slice(c0, c0 + w))
+        rad = draw(s / 32, s / 6)
+        cr = draw(0, m)
+        ccn = draw(0, n)
+        disks.append((cr, ccn, rad * rad, draw(0, 255)))
+    h = int(draw(m / 8, m / 3))
+    w = int(draw(n / 8, n / 3))
+    r0 = int(draw(0, max(m - h, 1)))
+    c0 = int(draw(0, max(n - w, 1)))
+    theta = draw(0, np.pi)
+    period = draw(6, 16)
+    phase = draw(0, 2 * np.pi)
+    tex = (r0, r0 + h, c0, c0 + w, theta, period, phase)
+    return {"rects": rects, "disks": disks, "texture": tex}
+
+
+def clean_image(seed: int, m: int, n: int) -> np.ndarray:
+    """The clean synthetic image (float64, shape (m, n))."""
+    prm = shape_params(seed, m, n)
+    img = np.zeros((m, n), dtype=np.float64)
+    rr, cc = np.meshgrid(np.arange(m, dtype=np.float64), np.arange(n, dtype=np.float64), indexing="ij")
+    for r0, r1, c0, c1, val in prm["rects"]:
+        img[r0:r1, c0:c1] = val
+    for cr, ccn, rad2, val in prm["disks"]:
+        img[(rr - cr) ** 2 + (cc - ccn) ** 2 <= rad2] = val
+    r0, r1, c0, c1, theta, period, phase = prm["texture"]
+    box = (slice(r0, r1), slice(c0, c1))
     img[box] += 20.0 * np.sin(2 * np.pi * (rr[box] * np.cos(theta) + cc[box] * np.sin(theta)) / period + phase)
     return np.clip(img, 0.0, 255.0)
 

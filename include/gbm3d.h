@@ -111,7 +111,14 @@ typedef struct gbm3d_ext {
   int32_t match_bf16; /* 1: block distances on BF16 MFMA (FP32 accumulation),
                          then exact FP32 rescoring of the best candidates    */
   int32_t rescore;    /* candidates rescored per tile, >= 2*numMax (0: 2*numMax) */
-  int32_t reserved[6];
+  int32_t learned;    /* 1: learned shrinkage replaces the step-1 wavelet hard
+                         threshold (blockSize 8, numMax 16; weights from
+                         gbm3d_shrink_set_weights)                            */
+  int32_t learned_bound; /* fallback bound in 1/1000: an image whose estimated
+                         learned residual exceeds bound x the hard-threshold
+                         estimate is recomputed on the reference path
+                         (0: 1000)                                           */
+  int32_t reserved[4];
 } gbm3d_ext;
 int gbm3d_set_ext(gbm3d_ctx* ctx, const gbm3d_ext* ext);
 int gbm3d_get_ext(gbm3d_ctx* ctx, gbm3d_ext* ext);
@@ -204,7 +211,8 @@ enum gbm3d_stage {
   GBM3D_STAGE_WIENER = 8,     /* step-2 Wiener filter (one kernel)                    */
   GBM3D_STAGE_AGG2 = 9,       /* step-2 gather aggregation (one kernel)               */
   GBM3D_STAGE_NORM = 10,      /* spin sums / normalisation / crop                     */
-  GBM3D_NUM_STAGES = 11
+  GBM3D_STAGE_SHRINK = 11,    /* step-1 learned shrinkage (one kernel, variant)       */
+  GBM3D_NUM_STAGES = 12
 };
 int gbm3d_set_timing(gbm3d_ctx* ctx, int enable);
 /* Copy nbytes (a multiple of 16) between device buffers with a plain
@@ -213,6 +221,61 @@ int gbm3d_set_timing(gbm3d_ctx* ctx, int enable);
 int gbm3d_device_copy(gbm3d_ctx* ctx, void* dst, const void* src, size_t nbytes);
 /* ms[GBM3D_NUM_STAGES], launches[GBM3D_NUM_STAGES] (either may be NULL). */
 int gbm3d_get_stage_times(gbm3d_ctx* ctx, double* ms, int* launches);
+
+/* ---------------------------------------------------- learned shrinkage
+ * A variant beside the reference path (no reference counterpart): a 4-layer
+ * 1-D convolutional network (width 3, 64 channels, BF16 weights, FP32
+ * accumulation) over the DCT-II x Haar-K coefficients of each 8 x 8 x 16
+ * step-1 group, with sigma/50 as an input channel, predicts per-coefficient
+ * gains that replace the step-1 hard threshold.  Targets are the Wiener gains
+ * |c_clean|^2/(|c_clean|^2 + sigma^2) of the clean group (the gain formula of
+ * source/denoiseAndAggregateWieFast.m:49).
+ * Parameter vector (FP32): W0[3][64][64] | wsig[3][64] | b0[64] |
+ * (W_l[3][64][64] | b_l[64]) for l = 1..3; W[tap][cin][cout], tap 0 reads
+ * position k-1. */
+#define GBM3D_SHRINK_PARAMS 49600
+int gbm3d_shrink_set_weights(gbm3d_ctx* ctx, const float* params, int count);
+/* Residual estimates of image `image` of the last denoise call with the
+ * learned form (sum over coefficients of the expected squared error of the
+ * learned gains and of a 2.7 sigma hard threshold) and whether it fell back. */
+int gbm3d_shrink_stats(gbm3d_ctx* ctx, int image, double* est_learned,
+                       double* est_ht, int* fell_back);
+/* Learned shrinkage of G groups (host buffers, 8 x 8 x 16 x G): filtered
+ * groups (clipped at 0) and, when gains != NULL, the applied gains in
+ * coefficient order (t = u + 8v, k). */
+int gbm3d_shrink_groups(gbm3d_ctx* ctx, const float* V, int G, float sigma,
+                        float* Vout, float* gains);
+/* Training on device buffers (context stream).  Rows r = 16 i + k of group
+ * i, 64 columns (coefficient t).
+ * features: step-1 matching (spin 0, 8x8 blocks, matchSize 32, K 16) on the
+ * noisy image, then for the nsel groups listed in dSel: X = c_noisy/sigma,
+ * T = Wiener gains of the clean group, Wt = loss weights
+ * min(1 + c_clean^2/sigma^2, 100) (0 on the fixed DC approximation).
+ * m, n: multiples of 8, >= 32. */
+int gbm3d_shrink_features(gbm3d_ctx* ctx, const float* dClean,
+                          const float* dNoisy, int m, int n, float sigma,
+                          const uint32_t* dSel, int nsel, float* dX,
+                          float* dT, float* dWt);
+/* Gains of ngroups groups (sigma per group in dSig). */
+int gbm3d_shrink_forward(gbm3d_ctx* ctx, const float* dParams, const float* dX,
+                         const float* dSig, int ngroups, float* dGains);
+/* Gradient of sum(Wt (g - T)^2)/ngroups; *loss (nullable) synchronises. */
+int gbm3d_shrink_grad(gbm3d_ctx* ctx, const float* dParams, const float* dX,
+                      const float* dT, const float* dWt, const float* dSig,
+                      int ngroups, float* dGrad, double* loss);
+/* One Adam step (bias-corrected, step >= 1) on the parameter vector. */
+int gbm3d_shrink_adam(gbm3d_ctx* ctx, float* dParams, float* dM, float* dV,
+                      const float* dGrad, int step, float lr, float beta1,
+                      float beta2, float eps);
+
+/* ------------------------------------------------------ synthetic inputs
+ * Seeded clean / noisy image pair on device buffers (m x n, column-major):
+ * background 0, 12 rectangles then 8 disks of intensity U[0,255], a
+ * sinusoid of amplitude 20 and period U[6,16] px on a rectangle, clip to
+ * [0,255]; noisy = clean + sigma N(0,1), not clipped.  Either output may be
+ * NULL.  No reference counterpart (test and training data). */
+int gbm3d_synth_pair(gbm3d_ctx* ctx, uint64_t seed, int m, int n, float sigma,
+                     float* dClean, float* dNoisy);
 
 #ifdef __cplusplus
 }
