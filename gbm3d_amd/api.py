@@ -102,8 +102,8 @@ class Context:
         match_bf16: BF16-MFMA distances + exact FP32 rescoring of the best
         `rescore` (>= 2*numMax) candidates per tile.
         learned: learned shrinkage replaces the step-1 hard threshold; an image
-        whose estimated learned residual exceeds learned_bound x the
-        hard-threshold estimate is recomputed on the reference path.  Loads
+        from which the learned gains remove more than learned_bound x its
+        noise energy is recomputed on the reference path.  Loads
         `weights` (parameter vector) or the shipped trained weights."""
         if learned:
             from . import shrink
@@ -122,7 +122,8 @@ class Context:
         check(self.lib.gbm3d_shrink_set_weights(self.handle, ptr(p), p.size))
 
     def shrink_stats(self, image: int = 0):
-        """(learned residual estimate, hard-threshold estimate, fell back) of the last call."""
+        """(energy removed by the learned gains, by a 2.7 sigma hard threshold,
+        both / noise energy; fell back) of image `image` of the last call."""
         a, b, f = C.c_double(), C.c_double(), C.c_int()
         check(self.lib.gbm3d_shrink_stats(self.handle, image, C.byref(a), C.byref(b), C.byref(f)))
         return a.value, b.value, bool(f.value)

@@ -342,15 +342,14 @@ float* image_stats(gbm3d_ctx* c, int i) {
   return c->ext.learned ? c->sstats + (size_t)i * GBM3D_MAX_SPINS * 2 * kShrinkGrid : nullptr;
 }
 
-int shrink_grid_of(const Plan& p) {
-  const long G = (long)(p.M / p.bs) * (p.N / p.bs);
-  const long passes = (G + 15) / 16;
-  return (int)std::min<long>(passes, kShrinkGrid);
-}
+int shrink_grid_of(const Plan& p) { return shrink_grid((long)(p.M / p.bs) * (p.N / p.bs)); }
 
-// After a learned-form call has finished: residual estimates per image; the
-// images whose learned estimate exceeds bound x the hard-threshold estimate
-// are recomputed on the reference path (eagerly, on the context stream).
+// After a learned-form call has finished: per image, the energy the learned
+// gains removed from the noisy coefficients relative to the noise energy
+// (coefficients x sigma^2, all spins).  Removing more than `bound` times the
+// noise energy means the shrinkage also removed signal (the discrepancy
+// principle); such images are recomputed on the reference path (eagerly, on
+// the context stream).
 int learned_judge(gbm3d_ctx* c, const gbm3d_opts& o, int count, const std::vector<Plan>& plans,
                   const float* const* dI, const float* sigma, float* const* dU, float* const* dU1) {
   GBM3D_HIP(hipStreamSynchronize(c->stream));
@@ -369,7 +368,10 @@ int learned_judge(gbm3d_ctx* c, const gbm3d_opts& o, int count, const std::vecto
         c->s_learned[i] += h[i * per + (size_t)2 * kShrinkGrid * sp + 2 * b];
         c->s_ht[i] += h[i * per + (size_t)2 * kShrinkGrid * sp + 2 * b + 1];
       }
-    if (!(c->s_learned[i] <= bound * c->s_ht[i])) redo.push_back(i);
+    const double noise = (double)plans[i].n1 * plans[i].M * plans[i].N * plans[i].K * (double)sigma[i] * sigma[i];
+    c->s_learned[i] /= noise;
+    c->s_ht[i] /= noise;
+    if (!(c->s_learned[i] <= bound)) redo.push_back(i);
   }
   if (redo.empty()) return GBM3D_OK;
   const gbm3d_ext keep = c->ext;

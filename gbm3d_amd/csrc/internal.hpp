@@ -131,12 +131,15 @@ constexpr int FB = NL * CH + TAPS * CH;  // biases then sigma-channel weights (F
 }  // namespace shrink
 constexpr int kShrinkGrid = 256;    // persistent blocks of the fused kernel
 constexpr int kShrinkChunk = 1024;  // data rows per weight-gradient block
+// Blocks of the fused shrinkage launch for G groups (its stats entries).
+int shrink_grid(long G);
 // params -> BF16 weights wb [4][64 cout][192 (tap, cin)] and FP32 fb (host).
 void shrink_pack_weights(const float* params, uint16_t* wb, float* fb);
 // Fused shrinkage of G groups: gathered from the padded image through S
-// (S != nullptr) or read from src as d x d x K x G; writes filtered groups
-// (clipped at 0), optionally the gains, and per-block residual estimates
-// stats[2 * b] (learned) / stats[2 * b + 1] (hard-threshold proxy).
+// (S != nullptr) or read from src as d x d x K x G; coefficients scaled by
+// the gains; writes filtered groups (clipped at 0), optionally the gains, and
+// per-block sums of the energy removed from the noisy coefficients,
+// stats[2 * b] (learned gains) / stats[2 * b + 1] (hard threshold at 2.7 sigma).
 int launch_shrink(const float* src, const uint32_t* S, int M, int d2, long G, float sigma, const uint16_t* wb,
                   const float* fb, float* vout, float* gout, float* stats, int* grid_used, hipStream_t st);
 // Training rows of the selected groups (noisy features, oracle Wiener gains, loss weights).

@@ -13,8 +13,8 @@
 //   g  = sigmoid(conv3(h3; W3) + b3)
 //
 // (conv3: width 3, zero padding; the constant sigma / 50 input channel is the
-// wsig term).  g(t, k) multiplies c(t, k); g(0, 14..15) is fixed to 1 as in
-// the Wiener filter (denoiseAndAggregateWieFast.m:50).  The filtered group is
+// wsig term).  g(0, 14..15) is fixed to 1 as in the Wiener filter
+// (denoiseAndAggregateWieFast.m:50); c(t, k) is multiplied by g(t, k).  The filtered group is
 // the inverse transform clipped at 0, written group-contiguous for the step-1
 // aggregation.  Matrix products are BF16 MFMA (16x16x32) with FP32
 // accumulation; operands are rounded to BF16 to nearest even.
@@ -72,8 +72,86 @@ __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __ex
 constexpr int KS = KW + 8;   // LDS row stride of a weight row (bf16)
 constexpr int AS = CH + 8;   // LDS row stride of an activation row (bf16)
 constexpr int AR = POS + 2;  // activation rows: positions -1 .. 16
-constexpr int GPW = 4;       // groups per wave and pass
-constexpr int WAVES = 4;
+constexpr int GPW = 1;       // groups per wave and pass
+constexpr int WAVES = 12;    // 768 threads share one LDS copy of the weights
+// Per-wave LDS region: the transform scratch (16 x 64 floats) of the wave's
+// group, later its activation rows (they alias: the activations are written
+// once the transform is done, and the gains overwrite them once the last layer
+// has read them).
+constexpr int REGION = POS * 64;  // floats
+static_assert(2 * REGION >= GPW * AR * AS, "region holds the activation rows");
+
+// Coefficients of group g (clamped to G - 1) for lane t = u + 8v: the
+// group's 16 blocks, 2-D DCT of every slice, Haar along the slices.
+template <int MODE>
+__device__ __forceinline__ void load_group(const float* __restrict__ src, const uint32_t* __restrict__ S, int M,
+                                           int d2, long G, long g, int lane, float* X, const float* T,
+                                           float (&x)[POS]) {
+  if (g >= G) g = G - 1;
+  const int u = lane & 7, v = lane >> 3;
+  if constexpr (MODE == 0) {
+    const int Mt = M / 8;
+    const size_t P = (size_t)M + d2;
+    const int ti = (int)(g % Mt), tj = (int)(g / Mt);
+#pragma unroll
+    for (int k = 0; k < POS; ++k) {
+      const uint32_t idx = S[(size_t)POS * g + k] - 1u;
+      const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
+      x[k] = src[(size_t)(ti * 8 + a + u) + P * (size_t)(tj * 8 + b + v)];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < POS; ++k) x[k] = src[(size_t)lane + 64 * ((size_t)k + POS * (size_t)g)];
+  }
+  page_transform<8, POS, false>(x, X, T, u, v);
+  haar_dec<POS, 3>(x);
+}
+
+// Network input rows of group j: c / sigma rounded to BF16, positions 0..15
+// at rows 1..16.
+__device__ __forceinline__ void put_act(uint16_t* A, int j, int lane, const float (&x)[POS], float inv_s) {
+#pragma unroll
+  for (int k = 0; k < POS; ++k) A[(j * AR + k + 1) * AS + lane] = bf16_rne(x[k] * inv_s);
+}
+
+// Gains from the last layer's accumulators, shrinkage, removed-energy sums,
+// inverse transform, clip at 0 and the store of group g (when g < G).
+__device__ __forceinline__ void finish_group(const f32x4 (&acc)[4], const float (&c)[POS], long g, long G,
+                                             int lane, float* X, const float* T, const float* b3, float tau_h, float s2, float* __restrict__ vout,
+                                             float* __restrict__ gout, float* e_l, float* e_h) {
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int pos = 4 * (lane >> 4) + i, ch = ct * 16 + (lane & 15);
+      X[pos * 64 + ch] = sigmoidf_(acc[ct][i] + b3[ch]);
+    }
+  wave_sync();
+  const bool live = g < G;
+  float* go = (gout && live) ? gout + (size_t)lane + 64 * POS * (size_t)g : nullptr;
+  float y[POS];
+#pragma unroll
+  for (int k = 0; k < POS; ++k) {
+    const float a = (lane == 0 && k >= POS - 2) ? 1.f : X[k * 64 + lane];
+    const float cc = c[k], c2 = cc * cc;
+    if (live) {
+      // energy removed from the noisy coefficients by the learned gains and
+      // by a hard threshold at tau_h
+      *e_l += (1.f - a) * (1.f - a) * c2;
+      *e_h += fabsf(cc) > tau_h ? 0.f : c2;
+    }
+    if (go) go[64 * k] = a;
+    y[k] = cc * a;
+  }
+  wave_sync();
+  haar_rec<POS, 3>(y);
+  page_transform<8, POS, true>(y, X, T, lane & 7, lane >> 3);
+  if (live) {
+    float* vo = vout + (size_t)lane + 64 * POS * (size_t)g;
+#pragma unroll
+    for (int k = 0; k < POS; ++k) vo[64 * k] = y[k] > 0.f ? y[k] : 0.f;
+  }
+}
 
 }  // namespace
 
@@ -81,14 +159,13 @@ constexpr int WAVES = 4;
 // MODE 1: groups read from V (d x d x K x G, stage entry point).
 // Persistent grid: every block loads the BF16 weights into LDS once.
 template <int MODE>
-__global__ __launch_bounds__(256) void shrink_kernel(const float* __restrict__ src, const uint32_t* __restrict__ S,
+__global__ __launch_bounds__(768) void shrink_kernel(const float* __restrict__ src, const uint32_t* __restrict__ S,
                                                      int M, int d2, long G, float sigma, float tau_h,
                                                      const uint16_t* __restrict__ wb, const float* __restrict__ fb,
                                                      DctTable<8> tab, float* __restrict__ vout,
                                                      float* __restrict__ gout, float* __restrict__ stats) {
   __shared__ __align__(16) uint16_t wl[NL * CH * KS];
-  __shared__ __align__(16) uint16_t act[WAVES][GPW * AR * AS];
-  __shared__ __align__(16) float xs[WAVES][POS * 64];
+  __shared__ __align__(16) float region[WAVES][REGION];
   __shared__ float bias[NL * CH];
   __shared__ float sb0[3 * CH];
   __shared__ float T[64];
@@ -107,50 +184,25 @@ __global__ __launch_bounds__(256) void shrink_kernel(const float* __restrict__ s
     sb0[e] = fb[ch] + s * sig_taps(fb + NL * CH, pos, ch);
   }
   if (threadIdx.x < 64) T[threadIdx.x] = tab.t[threadIdx.x];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint16_t* A = act[wave];
-  for (int j = 0; j < GPW; ++j) {  // rows -1 and 16 stay zero (the conv padding)
-    A[(j * AR) * AS + lane] = 0;
-    A[(j * AR + AR - 1) * AS + lane] = 0;
-  }
   __syncthreads();
 
-  const int u = lane & 7, v = lane >> 3;
-  const int Mt = M / 8;
-  const size_t P = (size_t)M + d2;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float* X = region[wave];
+  uint16_t* A = reinterpret_cast<uint16_t*>(region[wave]);
   const float inv_s = 1.0f / sigma, s2 = sigma * sigma;
-  float* X = xs[wave];
   float e_l = 0.f, e_h = 0.f;
   const long stride = (long)gridDim.x * WAVES * GPW;
-  for (long base = ((long)blockIdx.x * WAVES + wave) * GPW; base < G; base += stride) {
-    float c[GPW][POS];
-#pragma unroll
-    for (int j = 0; j < GPW; ++j) {
-      const long g = base + j < G ? base + j : G - 1;
-      float x[POS];
-      if constexpr (MODE == 0) {
-        const int ti = (int)(g % Mt), tj = (int)(g / Mt);
-#pragma unroll
-        for (int k = 0; k < POS; ++k) {
-          const uint32_t idx = S[(size_t)POS * g + k] - 1u;
-          const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
-          x[k] = src[(size_t)(ti * 8 + a + u) + P * (size_t)(tj * 8 + b + v)];
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < POS; ++k) x[k] = src[(size_t)lane + 64 * ((size_t)k + POS * (size_t)g)];
-      }
-      page_transform<8, POS, false>(x, X, T, u, v);
-      haar_dec<POS, 3>(x);
-#pragma unroll
-      for (int k = 0; k < POS; ++k) {
-        c[j][k] = x[k];
-        A[(j * AR + k + 1) * AS + lane] = bf16_rne(x[k] * inv_s);
-      }
-    }
+  static_assert(GPW == 1, "one group per wave and pass");
+  for (long base = (long)blockIdx.x * WAVES + wave; base < G; base += stride) {
+    float c0[POS];  // the group's coefficients, in registers
+    load_group<MODE>(src, S, M, d2, G, base, lane, X, T, c0);
+    // the transform scratch is free now (page_transform ends with a wave sync)
+    A[lane] = 0;  // rows -1 and 16: the conv zero padding
+    A[(AR - 1) * AS + lane] = 0;
+    put_act(A, 0, lane, c0, inv_s);
     wave_sync();
     f32x4 acc[GPW][4];
-#pragma unroll
+#pragma unroll 1
     for (int L = 0; L < NL; ++L) {
 #pragma unroll
       for (int j = 0; j < GPW; ++j)
@@ -172,6 +224,7 @@ __global__ __launch_bounds__(256) void shrink_kernel(const float* __restrict__ s
       }
       if (L < NL - 1) {
         wave_sync();  // every lane has read this layer's input
+        const float* add = L == 0 ? sb0 : bias + L * CH;
 #pragma unroll
         for (int j = 0; j < GPW; ++j)
 #pragma unroll
@@ -179,50 +232,16 @@ __global__ __launch_bounds__(256) void shrink_kernel(const float* __restrict__ s
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
               const int pos = 4 * (lane >> 4) + i, ch = ct * 16 + (lane & 15);
-              const float add = L == 0 ? sb0[(pos == 0 ? 0 : (pos == POS - 1 ? 2 : 1)) * CH + ch] : bias[L * CH + ch];
-              const float z = acc[j][ct][i] + add;
+              const int row = L == 0 ? (pos == 0 ? 0 : (pos == POS - 1 ? 2 : 1)) : 0;
+              const float z = acc[j][ct][i] + add[row * CH + ch];
               A[(j * AR + pos + 1) * AS + ch] = bf16_rne(z > 0.f ? z : 0.f);
             }
         wave_sync();
       }
     }
-    // gains, shrinkage, inverse transform
-#pragma unroll
-    for (int j = 0; j < GPW; ++j) {
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int pos = 4 * (lane >> 4) + i, ch = ct * 16 + (lane & 15);
-          X[pos * 64 + ch] = sigmoidf_(acc[j][ct][i] + bias[(NL - 1) * CH + ch]);
-        }
-      wave_sync();
-      const bool live = base + j < G;
-      const long g = live ? base + j : G - 1;
-      float y[POS], gk[POS];
-#pragma unroll
-      for (int k = 0; k < POS; ++k) {
-        gk[k] = (lane == 0 && k >= POS - 2) ? 1.f : X[k * 64 + lane];
-        const float cc = c[j][k];
-        const float ex = fmaxf(cc * cc - s2, 0.f);
-        if (live) {
-          e_l += gk[k] * gk[k] * s2 + (1.f - gk[k]) * (1.f - gk[k]) * ex;
-          e_h += fabsf(cc) > tau_h ? s2 : ex;
-        }
-        y[k] = cc * gk[k];
-      }
-      wave_sync();
-      haar_rec<POS, 3>(y);
-      page_transform<8, POS, true>(y, X, T, u, v);
-      if (live) {
-#pragma unroll
-        for (int k = 0; k < POS; ++k) {
-          const size_t o = (size_t)lane + 64 * ((size_t)k + POS * (size_t)g);
-          vout[o] = y[k] > 0.f ? y[k] : 0.f;
-          if (gout) gout[o] = gk[k];
-        }
-      }
-    }
+    wave_sync();  // the last layer has read the activations: X may overwrite them
+    const float* b3 = bias + (NL - 1) * CH;
+    finish_group(acc[0], c0, base, G, lane, X, T, b3, tau_h, s2, vout, gout, &e_l, &e_h);
   }
   e_l = wave_sum_f32(e_l);
   e_h = wave_sum_f32(e_h);
@@ -242,6 +261,11 @@ __global__ __launch_bounds__(256) void shrink_kernel(const float* __restrict__ s
   }
 }
 
+int shrink_grid(long G) {
+  const long passes = (G + WAVES * GPW - 1) / (WAVES * GPW);
+  return (int)(passes < kShrinkGrid ? passes : kShrinkGrid);
+}
+
 void shrink_pack_weights(const float* p, uint16_t* wb, float* fb) {
   for (int L = 0; L < NL; ++L)
     for (int co = 0; co < CH; ++co)
@@ -258,16 +282,15 @@ int launch_shrink(const float* src, const uint32_t* S, int M, int d2, long G, fl
     return GBM3D_OK;
   }
   if (S && (M % 8 || d2 <= 0)) return fail(GBM3D_ERR_ARG, "shrink: geometry");
-  const long passes = (G + WAVES * GPW - 1) / (WAVES * GPW);
-  const int grid = (int)(passes < kShrinkGrid ? passes : kShrinkGrid);
+  const int grid = shrink_grid(G);
   if (grid_used) *grid_used = grid;
   const DctTable<8> tab = make_dct_table<8>();
   const float tau_h = 2.7f * sigma;
   if (S)
-    hipLaunchKernelGGL(shrink_kernel<0>, dim3(grid), dim3(256), 0, st, src, S, M, d2, G, sigma, tau_h, wb, fb, tab,
+    hipLaunchKernelGGL(shrink_kernel<0>, dim3(grid), dim3(64 * WAVES), 0, st, src, S, M, d2, G, sigma, tau_h, wb, fb, tab,
                        vout, gout, stats);
   else
-    hipLaunchKernelGGL(shrink_kernel<1>, dim3(grid), dim3(256), 0, st, src, S, M, d2, G, sigma, tau_h, wb, fb, tab,
+    hipLaunchKernelGGL(shrink_kernel<1>, dim3(grid), dim3(64 * WAVES), 0, st, src, S, M, d2, G, sigma, tau_h, wb, fb, tab,
                        vout, gout, stats);
   return check_launch("shrink");
 }

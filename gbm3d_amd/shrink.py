@@ -117,6 +117,18 @@ class Dataset:
         return self.X.ptr + o, self.T.ptr + o, self.Wt.ptr + o, self.sig.ptr + 4 * g0
 
 
+def baseline_losses(val: Dataset) -> dict:
+    """Validation loss of fixed shrinkage rules on the same rows: hard
+    threshold at 2.7 sigma and the empirical Wiener gain max(x^2 - 1, 0)/x^2."""
+    X, T, Wt = val.X.to_host(), val.T.to_host(), val.Wt.to_host()
+    x2 = X.astype(np.float64) ** 2
+    out = {}
+    for name, g in (("hard_threshold_2.7", (x2 > 2.7 ** 2).astype(np.float64)),
+                    ("empirical_wiener", np.maximum(x2 - 1.0, 0.0) / np.maximum(x2, 1e-30))):
+        out[name] = float(np.sum(Wt * (g - T) ** 2) / val.ngroups)
+    return out
+
+
 def train(ctx, data: Dataset, val: Dataset, epochs: int = 6, batch: int = 2048, lr: float = 1e-3,
           seed: int = 0, log=print):
     """Adam with cosine-decayed step size over contiguous batches in shuffled order."""
@@ -150,6 +162,8 @@ def train(ctx, data: Dataset, val: Dataset, epochs: int = 6, batch: int = 2048, 
             X, T, W, S = data.slice(int(b) * batch, batch)
             run += ctx.shrink_grad(p.ptr, X, T, W, S, batch, grad.ptr)
             ctx.shrink_adam(p.ptr, m.ptr, v.ptr, grad.ptr, step, float(lr_t))
+            if step % 200 == 0:
+                log(json.dumps({"step": step, "of": total, "s": round(time.time() - t0, 1)}))
         vl = val_loss()
         hist.append({"epoch": ep + 1, "train_loss": run / nb, "val_loss": vl, "s": round(time.time() - t0, 1)})
         log(json.dumps(hist[-1]))
@@ -177,11 +191,13 @@ def main(argv=None):
     val = Dataset(ctx, a.val_images, a.groups, a.size, seed=2)
     t_data = time.time() - t0
     print(json.dumps({"data_s": round(t_data, 1), "groups": data.ngroups, "val_groups": val.ngroups}), flush=True)
+    base = baseline_losses(val)
+    print(json.dumps({"baseline_val_loss": base}), flush=True)
     params, hist = train(ctx, data, val, a.epochs, a.batch, a.lr, a.seed,
                          log=lambda s: print(s, flush=True))
     meta = {"images": a.images, "groups_per_image": a.groups, "size": a.size, "sigma": [5, 50],
             "val_images": a.val_images, "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "seed": a.seed,
-            "data_seconds": round(t_data, 1), "history": hist}
+            "data_seconds": round(t_data, 1), "baseline_val_loss": base, "history": hist}
     save_weights(a.out, params, meta)
     print(json.dumps({"saved": a.out, "val_loss": hist[-1]["val_loss"]}), flush=True)
     return 0

@@ -114,10 +114,10 @@ typedef struct gbm3d_ext {
   int32_t learned;    /* 1: learned shrinkage replaces the step-1 wavelet hard
                          threshold (blockSize 8, numMax 16; weights from
                          gbm3d_shrink_set_weights)                            */
-  int32_t learned_bound; /* fallback bound in 1/1000: an image whose estimated
-                         learned residual exceeds bound x the hard-threshold
-                         estimate is recomputed on the reference path
-                         (0: 1000)                                           */
+  int32_t learned_bound; /* fallback bound in 1/1000: an image from which the
+                         learned gains remove more than bound x its noise
+                         energy (coefficients x sigma^2) is recomputed on the
+                         reference path (0: 1000)                             */
   int32_t reserved[4];
 } gbm3d_ext;
 int gbm3d_set_ext(gbm3d_ctx* ctx, const gbm3d_ext* ext);
@@ -235,14 +235,15 @@ int gbm3d_get_stage_times(gbm3d_ctx* ctx, double* ms, int* launches);
  * position k-1. */
 #define GBM3D_SHRINK_PARAMS 49600
 int gbm3d_shrink_set_weights(gbm3d_ctx* ctx, const float* params, int count);
-/* Residual estimates of image `image` of the last denoise call with the
- * learned form (sum over coefficients of the expected squared error of the
- * learned gains and of a 2.7 sigma hard threshold) and whether it fell back. */
+/* Image `image` of the last denoise call with the learned form: energy the
+ * learned gains removed from the noisy coefficients and energy a 2.7 sigma
+ * hard threshold would remove, both relative to the noise energy
+ * (coefficients x sigma^2, all groups and spins), and whether it fell back. */
 int gbm3d_shrink_stats(gbm3d_ctx* ctx, int image, double* est_learned,
                        double* est_ht, int* fell_back);
 /* Learned shrinkage of G groups (host buffers, 8 x 8 x 16 x G): filtered
- * groups (clipped at 0) and, when gains != NULL, the applied gains in
- * coefficient order (t = u + 8v, k). */
+ * groups (coefficients times the gains, clipped at 0) and, when gains !=
+ * NULL, the network gains in coefficient order (t = u + 8v, k). */
 int gbm3d_shrink_groups(gbm3d_ctx* ctx, const float* V, int G, float sigma,
                         float* Vout, float* gains);
 /* Training on device buffers (context stream).  Rows r = 16 i + k of group

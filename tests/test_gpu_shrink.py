@@ -259,7 +259,7 @@ def test_learned_fallback_reproduces_reference_path(ctx):
         el, eh, fell = ctx.shrink_stats(0)
     finally:
         ctx.set_variant()
-    assert fell and el > 1e-3 * eh
+    assert fell and el > 1e-3
     assert np.array_equal(U, U0) and np.array_equal(out["U1"], out0["U1"])
 
 
