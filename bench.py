@@ -31,6 +31,10 @@ SEED = 2048
 OPTS = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
+# learned shrinkage: 4 conv layers x 2 FLOP x 16 positions x 64 out x 192 in, per group
+SHRINK_FLOP_PER_GROUP = 4 * 2 * 16 * 64 * 192
+DPSNR_BOUND = -0.05   # a form is eligible as the chosen form when PSNR >= plain - 0.05 dB
 
 
 def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2):
@@ -64,6 +68,12 @@ def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2):
         G1 = MN // (d1 * d1)
         out["agg1_prep"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G1 * 4), launches=stage_n["agg1_prep"],
                                 ms=stage_ms["agg1_prep"] / stage_n["agg1_prep"], peak=PEAK_HBM_GBS)
+    if stage_n.get("shrink"):
+        # fused learned shrinkage of one spin: the network's BF16 MFMA work
+        G1 = MN // (d1 * d1)
+        out["shrink"] = dict(bound="mfma", unit="TFLOP/s", work=float(G1 * SHRINK_FLOP_PER_GROUP),
+                             launches=stage_n["shrink"], ms=stage_ms["shrink"] / stage_n["shrink"],
+                             peak=PEAK_BF16_TFLOPS)
     if stage_n["agg1"]:
         G1 = MN // (d1 * d1)
         out["agg1"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G1 * K * 4 + G1 * 4 + 2 * MN * 4),
@@ -204,6 +214,97 @@ def batched_report(args):
         "bitwise_equal_to_plain": all(np.array_equal(a, b) for a, b in zip(U_b, U_plain))}))
 
 
+def learned_report(args):
+    """64 synthetic 512x512 images, sigma ~ U[5,50]: one graph-captured batch
+    launch with the learned shrinkage replacing the step-1 hard threshold, vs
+    the same batch on the plain path (the judge of its PSNR); errors of both
+    against FP64 oracle runs of the same pipelines on image 0."""
+    import gbm3d_amd
+    import oracle
+    from gbm3d_amd import device, shrink, synth
+    from oracle import shrink_oracle as so
+
+    count, size = 64, 512
+    u = synth.uniform(0x5EED5, np.arange(count))
+    sig = [float(5 + 45 * x) for x in u]
+    pairs = [synth.make_pair(5000 + i, size, size, sig[i]) for i in range(count)]
+    cleans, imgs = [p[0] for p in pairs], [p[1] for p in pairs]
+    mp = count * size * size / 1e6
+    ctx = gbm3d_amd.Context(0)
+    stream = device.Stream()
+    ctx.set_stream(stream.handle)
+    dI = [device.DeviceArray.from_host(x) for x in imgs]
+    dU = [device.DeviceArray(x.shape) for x in imgs]
+    dU1 = [device.DeviceArray(x.shape) for x in imgs]
+    e0, e1 = device.Event(), device.Event()
+    res = {}
+    for name, kw in (("plain", {}), ("learned", {"learned": True})):
+        ctx.set_variant(**kw)
+
+        def run():
+            ctx.denoise_batch_device([d.ptr for d in dI], [size] * count, [size] * count, sig, OPTS,
+                                     [d.ptr for d in dU], [d.ptr for d in dU1], graph=True, tune=True)
+
+        t0 = time.perf_counter()
+        run()
+        stream.synchronize()
+        tune_s = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            run()
+        e0.record(stream)
+        for _ in range(args.steps):
+            run()
+        e1.record(stream)
+        stream.synchronize()
+        ms = e0.elapsed_ms(e1) / args.steps
+        U = [d.to_host() for d in dU]
+        U1 = [d.to_host() for d in dU1]
+        r = {"ms": round(ms, 3), "mpix_s": round(mp / (ms * 1e-3), 2), "tune_and_capture_s": round(tune_s, 3),
+             "tuned": ctx.batch_info(), "psnr": [synth.psnr(c, x) for c, x in zip(cleans, U)],
+             "psnr_pilot": [synth.psnr(c, x) for c, x in zip(cleans, U1)], "U0": U[0], "U10": U1[0]}
+        if kw.get("learned"):
+            st = [ctx.shrink_stats(i) for i in range(count)]
+            r["fell_back"] = int(sum(x[2] for x in st))
+            r["removed_energy_vs_noise"] = [round(x[0], 4) for x in st]
+        res[name] = r
+    ctx.set_variant()
+    pp, pl = np.array(res["plain"]["psnr"]), np.array(res["learned"]["psnr"])
+    qp, ql = np.array(res["plain"]["psnr_pilot"]), np.array(res["learned"]["psnr_pilot"])
+    # FP64 oracle runs of the same pipelines on image 0
+    params = shrink.load_weights()
+    nth = min(16, os.cpu_count() or 1)
+    oracle.set_threads(nth)
+    U64, U164 = oracle.gbm3d(imgs[0], sig[0], OPTS, precision=64, nthreads=nth)
+    L164 = so.learned_pilot(imgs[0].astype(np.float64), sig[0], params, OPTS, precision=64, exact=True)
+    L64 = oracle.step2_from_pilot(L164, imgs[0], sig[0], OPTS, precision=64)
+    err = {"plain": {"max_abs_U": float(np.abs(res["plain"]["U0"] - U64).max()),
+                     "max_abs_U1": float(np.abs(res["plain"]["U10"] - U164).max()),
+                     "dpsnr_vs_fp64_db": round(synth.psnr(cleans[0], res["plain"]["U0"]) - synth.psnr(cleans[0], U64),
+                                               4)},
+           "learned": {"max_abs_U": float(np.abs(res["learned"]["U0"] - L64).max()),
+                       "max_abs_U1": float(np.abs(res["learned"]["U10"] - L164).max()),
+                       "mean_abs_U1": float(np.abs(res["learned"]["U10"] - L164).mean()),
+                       "dpsnr_vs_fp64_db": round(synth.psnr(cleans[0], res["learned"]["U0"])
+                                                 - synth.psnr(cleans[0], L64), 4),
+                       "note": "FP64 oracle network without BF16 rounding"}}
+    meta = shrink.weights_meta()
+    print(json.dumps({
+        "workload": "learned shrinkage: 64 synthetic 512x512 images, sigma U[5,50], 8x8 blocks, one graph launch",
+        "images": count, "mpix": round(mp, 3),
+        "plain_batched": {k: res["plain"][k] for k in ("ms", "mpix_s", "tuned")},
+        "learned_batched": {k: res["learned"][k] for k in ("ms", "mpix_s", "tuned", "fell_back")},
+        "speedup": round(res["plain"]["ms"] / res["learned"]["ms"], 3),
+        "psnr_plain_mean_db": round(float(pp.mean()), 4), "psnr_learned_mean_db": round(float(pl.mean()), 4),
+        "dpsnr_mean_db": round(float((pl - pp).mean()), 4), "dpsnr_min_db": round(float((pl - pp).min()), 4),
+        "dpsnr_max_db": round(float((pl - pp).max()), 4),
+        "pilot_psnr_plain_mean_db": round(float(qp.mean()), 4), "pilot_psnr_learned_mean_db": round(float(ql.mean()), 4),
+        "images_within_0.05db": int(np.sum(pl - pp >= DPSNR_BOUND)),
+        "vs_fp64_oracle_image0": err,
+        "training": {k: meta.get(k) for k in ("images", "groups_per_image", "size", "sigma", "epochs", "batch",
+                                              "baseline_val_loss")} | {"val_loss": meta["history"][-1]["val_loss"]},
+    }))
+
+
 # kernel symbol prefix -> roofline key; the largest dispatch of a kernel is the
 # one measured (the finest level for the wavelet synthesis)
 TRAFFIC_KERNELS = {"gbm3d::ht_syn_fast_kernel": "ht_syn0", "void gbm3d::match_kernel": "match2",
@@ -252,7 +353,7 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="warmup + steps only (for rocprofv3 runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="with --profile-only: also run one device copy of MIB MiB (byte-counter calibration)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched", "learned"])
     ap.add_argument("--size", type=int, default=0, help="image side for --workload precision")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="derive profiles/traffic.json from two rocprofv3 --pmc runs made with --calibrate-copy 256")
@@ -265,6 +366,9 @@ def main():
         return
     if args.workload == "batched":
         batched_report(args)
+        return
+    if args.workload == "learned":
+        learned_report(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (one image per launch); --gpus must be 1", file=sys.stderr)
@@ -280,24 +384,26 @@ def main():
     dI = device.DeviceArray.from_host(noisy.astype(np.float32))
     dU = device.DeviceArray((SIZE, SIZE))
     dU1 = device.DeviceArray((SIZE, SIZE))
+    e0, e1 = device.Event(), device.Event()
 
     def step():
         ctx.denoise_device(dI.ptr, SIZE, SIZE, SIGMA, OPTS, dU.ptr, dU1.ptr)
 
-    for _ in range(args.warmup):
-        step()
-    stream.synchronize()
-    e0, e1 = device.Event(), device.Event()
-    device.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    stream.synchronize()
-    device.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_ms(e1) / args.steps
+    def timed(steps):
+        for _ in range(args.warmup):
+            step()
+        stream.synchronize()
+        device.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            step()
+        e1.record(stream)
+        stream.synchronize()
+        device.synchronize()
+        return e0.elapsed_ms(e1) / steps, time.perf_counter() - t0
+
+    ms, wall = timed(args.steps)
     if args.profile_only:
         if args.calibrate_copy:
             nb = args.calibrate_copy << 20
@@ -306,73 +412,88 @@ def main():
             stream.synchronize()
         print(json.dumps({"ms_per_step": ms}))
         return
-
-    U = dU.to_host()
-    U1 = dU1.to_host()
-    psnr = synth.psnr(clean, U)
-
-    # one more step with the stage timers on (same work, separate from the timed region)
-    ctx.set_timing(True)
-    step()
-    st = ctx.stage_times()
-    ctx.set_timing(False)
-    stage_ms = {k: v[0] for k, v in st.items()}
-    stage_n = {k: v[1] for k, v in st.items()}
     ro = gbm3d_amd.resolve_opts(OPTS)
-    rl = kernel_rooflines(stage_ms, stage_n, SIZE, SIZE, ro["numMax"], ro["blockSize"], ro["matchSize"],
-                          len(ro["matchSpins"]), len(ro["matchSpinsWie"]))
-    dom = max(rl, key=lambda k: rl[k]["ms"] * rl[k]["launches"])
-    d = rl[dom]
-    prof = os.path.join(HERE, "profiles", "traffic.json")
-    traffic = None
-    if os.path.exists(prof):
-        traffic = json.load(open(prof)).get(dom)
-    roofline = {"bound": d["bound"], "achieved": round(d["achieved"], 3), "peak": d["peak"], "unit": d["unit"],
-                "frac": round(d["frac"], 4), "traffic": traffic, "kernel": dom,
-                "avg_launch_ms": round(d["ms"], 5)}
 
-    # precision variant: BF16-MFMA distances + exact FP32 rescoring (same timed loop)
+    def staged():
+        """One more step with the stage timers on (same work, outside the timed region)."""
+        ctx.set_timing(True)
+        step()
+        st = ctx.stage_times()
+        ctx.set_timing(False)
+        stage_ms = {k: v[0] for k, v in st.items()}
+        stage_n = {k: v[1] for k, v in st.items()}
+        rl = kernel_rooflines(stage_ms, stage_n, SIZE, SIZE, ro["numMax"], ro["blockSize"], ro["matchSize"],
+                              len(ro["matchSpins"]), len(ro["matchSpinsWie"]))
+        return stage_ms, rl
+
+    U, U1 = dU.to_host(), dU1.to_host()
+    psnr = synth.psnr(clean, U)
+    stage_ms, rl = staged()
+    forms = {"plain": {"ms": ms, "wall": wall, "psnr": psnr, "pilot": synth.psnr(clean, U1), "stage_ms": stage_ms,
+                       "rl": rl, "U": U}}
+
+    # forms beside the plain path, timed the same way on the same resident input
     variants = {}
-    ctx.set_variant(match_bf16=True)
-    for _ in range(args.warmup):
-        step()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    stream.synchronize()
-    ms_bf = e0.elapsed_ms(e1) / args.steps
-    Ubf = dU.to_host()
-    ctx.set_timing(True)
-    step()
-    st_bf = ctx.stage_times()
-    ctx.set_timing(False)
-    ctx.set_variant(match_bf16=False)
-    variants["bf16_match"] = {
-        "mpix_s": round(SIZE * SIZE / (ms_bf * 1e-3) / 1e6, 3), "ms_per_step": round(ms_bf, 4),
-        "psnr_db": round(synth.psnr(clean, Ubf), 4), "dpsnr_db": round(synth.psnr(clean, Ubf) - psnr, 4),
-        "match_ms": round(st_bf["match1"][0] + st_bf["match2"][0], 4),
-        "max_abs_vs_plain": float(np.abs(Ubf - U).max())}
+    for name, kw in (("bf16_match", {"match_bf16": True}), ("learned", {"learned": True}),
+                     ("learned_bf16_match", {"learned": True, "match_bf16": True})):
+        ctx.set_variant(**kw)
+        msv, wallv = timed(args.steps)
+        Uv, U1v = dU.to_host(), dU1.to_host()
+        smv, rlv = staged()
+        pv = synth.psnr(clean, Uv)
+        forms[name] = {"ms": msv, "wall": wallv, "psnr": pv, "pilot": synth.psnr(clean, U1v), "stage_ms": smv,
+                       "rl": rlv, "U": Uv}
+        variants[name] = {"mpix_s": round(SIZE * SIZE / (msv * 1e-3) / 1e6, 3), "ms_per_step": round(msv, 4),
+                          "psnr_db": round(pv, 4), "dpsnr_db": round(pv - psnr, 4),
+                          "psnr_pilot_db": round(forms[name]["pilot"], 4),
+                          "max_abs_vs_plain": float(np.abs(Uv - U).max()),
+                          "stages_ms": {k: round(v, 4) for k, v in smv.items() if v > 0}}
+        if kw.get("learned"):
+            el, eh, fell = ctx.shrink_stats(0)
+            variants[name].update({"removed_energy_vs_noise": round(el, 4),
+                                   "hard_threshold_removed_energy_vs_noise": round(eh, 4), "fell_back": fell})
+            if "shrink" in rlv:
+                k = rlv["shrink"]
+                variants[name]["shrink_kernel"] = {"achieved_tflops": round(k["achieved"], 2), "peak": k["peak"],
+                                                   "frac": round(k["frac"], 4), "avg_launch_ms": round(k["ms"], 4)}
+    ctx.set_variant()
+
+    # chosen form: the fastest whose PSNR stays within DPSNR_BOUND of the plain path
+    eligible = [n for n, f in forms.items() if f["psnr"] - psnr >= DPSNR_BOUND]
+    chosen = min(eligible, key=lambda n: forms[n]["ms"])
+    cf = forms[chosen]
+    rlc = cf["rl"]
+    dom = max(rlc, key=lambda k: rlc[k]["ms"] * rlc[k]["launches"])
+    d = rlc[dom]
+    prof = os.path.join(HERE, "profiles", "traffic.json")
+    traffic = json.load(open(prof)).get(dom) if os.path.exists(prof) else None
+    roofline = {"bound": d["bound"], "achieved": round(d["achieved"], 3), "peak": d["peak"], "unit": d["unit"],
+                "frac": round(d["frac"], 4), "traffic": traffic, "kernel": dom, "avg_launch_ms": round(d["ms"], 5)}
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(min(16, os.cpu_count() or 1))
 
-    mpix = SIZE * SIZE / (ms * 1e-3) / 1e6
+    mpix = SIZE * SIZE / (cf["ms"] * 1e-3) / 1e6
+    plain_mpix = SIZE * SIZE / (ms * 1e-3) / 1e6
     line = {
         "metric": METRIC, "value": round(mpix, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": round(cf["ms"], 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "plain GBM3D two-step denoise, one 2048x2048 FP32 synthetic image (seed 2048), "
-                               "sigma=25, 8x8 blocks both steps, K=16, matchSize 32, spins [0 4]/[0 4], cycleSpin 2, "
-                               "3 levels", "form": "plain FP32 GPU path", "size": SIZE, "sigma": SIGMA},
-        "psnr_db": round(psnr, 4), "psnr_noisy_db": round(synth.psnr(clean, noisy), 4),
-        "psnr_pilot_db": round(synth.psnr(clean, U1), 4), "dpsnr_db": None,
-        "wall_s_timed": round(wall, 4),
-        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "config": {"workload": "GBM3D two-step denoise, one 2048x2048 FP32 synthetic image (seed 2048), sigma=25, "
+                               "8x8 blocks both steps, K=16, matchSize 32, spins [0 4]/[0 4], cycleSpin 2, 3 levels",
+                   "form": chosen, "chosen_by": f"fastest form with PSNR >= plain {DPSNR_BOUND} dB",
+                   "size": SIZE, "sigma": SIGMA},
+        "plain_mpix_s": round(plain_mpix, 3), "plain_ms_per_step": round(ms, 4),
+        "speedup_vs_plain": round(ms / cf["ms"], 4),
+        "psnr_db": round(cf["psnr"], 4), "psnr_plain_db": round(psnr, 4),
+        "dpsnr_db": round(cf["psnr"] - psnr, 4),
+        "psnr_noisy_db": round(synth.psnr(clean, noisy), 4), "psnr_pilot_db": round(cf["pilot"], 4),
+        "wall_s_timed": round(cf["wall"], 4),
+        "stages_ms": {k: round(v, 4) for k, v in cf["stage_ms"].items() if v > 0},
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                    for k, v in rl.items()},
+                    for k, v in rlc.items()},
         "variants": variants,
         "cpu_baseline": cpu,
     }

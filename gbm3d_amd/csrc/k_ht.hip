@@ -272,6 +272,8 @@ __device__ __forceinline__ void syn_lo_hi(const float (&lo)[R2 / 2 + 5], const f
   }
 }
 
+constexpr int kSynPairsPerBlock = 4;  // z-pairs per block of the register-blocked synthesis
+
 namespace fs {
 constexpr int P = 32, C = 21, R = 50;    // output tile, coefficients, input rows per edge
 constexpr int ZS = R + 1, AS = R + 1, BS = C + 1, ES = P + 16;  // ES: the two half-waves of phase 5 hit disjoint banks
@@ -281,10 +283,14 @@ static_assert(8 * C * BS <= REG0, "B fits in Z");
 static_assert(4 * C * ES <= REG1, "E fits in A");
 }  // namespace fs
 
+// One block per 32 x 32 output tile and a run of `zpb` z-pairs: the loads of
+// the next pair (input planes and the coarser reconstruction) are issued into
+// registers before the current pair's filter phases run, so their latency
+// overlaps the LDS work.
 __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
                                                           int p, int q, int r, int sh, Tau7 tau,
                                                           float* Y, const float* acc, int mode,
-                                                          float cs, int gd) {
+                                                          float cs, int gd, int zpb) {
   using namespace fs;
   __shared__ float lds[REG0 + REG1];
   __shared__ int ridx[R], cidx[R];
@@ -295,19 +301,23 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
   const int tid = threadIdx.x;
   const int nbx = gridDim.x, nby = gridDim.y;
   const int lb = xcd_remap(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
-  const int x0 = (lb % nbx) * P, y0 = ((lb / nbx) % nby) * P, t3 = lb / (nbx * nby);
+  const int x0 = (lb % nbx) * P, y0 = ((lb / nbx) % nby) * P, zc = lb / (nbx * nby);
+  const int tz0 = zc * zpb, tz1 = min(r / 2, tz0 + zpb);
   if (tid < R)
     ridx[tid] = pmod_i(x0 - 9 - sh + tid, p);
   else if (tid < 2 * R)
     cidx[tid - R] = pmod_i(y0 - 9 - sh + tid - R, q);
-  const size_t pq = (size_t)p * q;
-  const float* X0 = X + pq * pmod_i(2 * t3 - sh, r);
-  const float* X1 = X + pq * pmod_i(2 * t3 - 1 - sh, r);
   __syncthreads();
-  // z analysis of the pair (2t3-1, 2t3); all loads of a thread issued together
-  {
-    constexpr int NIT = (R * R + 255) / 256;
-    float a[NIT], b[NIT];
+  const size_t pq = (size_t)p * q;
+  const int ph = p >> 1, qh = q >> 1;
+  constexpr int NIT = (R * R + 255) / 256;
+  // phase-3 roles (fixed per thread): coefficient column, run, dim-2 band, z band
+  const int c3 = tid % C, run3 = (tid / C) & 1, b23 = (tid / (2 * C)) & 1, zb3 = tid / (4 * C);
+  const bool coarse = tid < 2 * C;  // (zb, b2) = (0, 0): the approximation comes from Ylo
+  float a[NIT], b[NIT], yl[11];
+  auto fetch = [&](int t3) {
+    const float* X0 = X + pq * pmod_i(2 * t3 - sh, r);
+    const float* X1 = X + pq * pmod_i(2 * t3 - 1 - sh, r);
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int e = tid + 256 * it;
@@ -318,6 +328,15 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
         b[it] = X1[o];
       }
     }
+    if (coarse) {
+      const size_t col = (size_t)ph * ((size_t)pmod_i(y0 / 2 + c3, qh) + (size_t)qh * t3);
+#pragma unroll
+      for (int i = 0; i < 11; ++i) yl[i] = Ylo[(size_t)pmod_i(x0 / 2 + run3 * 10 + i, ph) + col];
+    }
+  };
+  if (tz0 < tz1) fetch(tz0);
+  for (int t3 = tz0; t3 < tz1; ++t3) {
+    // z analysis of the pair (2t3-1, 2t3) from the prefetched registers
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int e = tid + 256 * it;
@@ -327,92 +346,94 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
         Z[R * ZS + cc * ZS + rr] = fCc * b[it] - fCc * a[it];
       }
     }
-  }
-  __syncthreads();
-  // dim-2 analysis: two runs of 11 coefficients, [0,11) and [10,21), per (z band, row)
-  if (tid < 2 * 2 * R) {
-    const int rr = tid % R, run = (tid / R) & 1, zb = tid / (2 * R);
-    const int c0 = run * 10;
-    float v[30];
+    float ycur[11];
 #pragma unroll
-    for (int j = 0; j < 30; ++j) v[j] = Z[zb * R * ZS + (2 * c0 + j) * ZS + rr];
-    float lo[11], hi[11];
-    ana_lo_hi<11>(v, lo, hi);
+    for (int i = 0; i < 11; ++i) ycur[i] = yl[i];
+    if (t3 + 1 < tz1) fetch(t3 + 1);  // in flight during the phases below
+    __syncthreads();
+    // dim-2 analysis: two runs of 11 coefficients, [0,11) and [10,21), per (z band, row)
+    if (tid < 2 * 2 * R) {
+      const int rr = tid % R, run = (tid / R) & 1, zb = tid / (2 * R);
+      const int c0 = run * 10;
+      float v[30];
 #pragma unroll
-    for (int i = 0; i < 11; ++i) {
-      A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
-      A[((zb * 2 + 1) * C + c0 + i) * AS + rr] = hi[i];
-    }
-  }
-  __syncthreads();
-  // dim-1 analysis + hard threshold (the approximation is the rebuilt coarser level)
-  const int ph = p >> 1, qh = q >> 1;
-  if (tid < 2 * 2 * 2 * C) {
-    const int c = tid % C, run = (tid / C) & 1, b2 = (tid / (2 * C)) & 1, zb = tid / (4 * C);
-    const int t0 = run * 10;
-    float v[30];
+      for (int j = 0; j < 30; ++j) v[j] = Z[zb * R * ZS + (2 * c0 + j) * ZS + rr];
+      float lo[11], hi[11];
+      ana_lo_hi<11>(v, lo, hi);
 #pragma unroll
-    for (int j = 0; j < 30; ++j) v[j] = A[((zb * 2 + b2) * C + c) * AS + 2 * t0 + j];
-    float lo[11], hi[11];
-    ana_lo_hi<11>(v, lo, hi);
-    const int bl = (b2 << 1) | (zb << 2), bh = bl | 1;
-    const float tl = tau.t[band_j(bl)], th = tau.t[band_j(bh)];
-    float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + t0;
-    float* Bh = Bl + C * BS;
-    if (bl == 0) {
-      const size_t col = (size_t)ph * ((size_t)pmod_i(y0 / 2 + c, qh) + (size_t)qh * t3);
-#pragma unroll
-      for (int i = 0; i < 11; ++i) Bl[i] = Ylo[(size_t)pmod_i(x0 / 2 + t0 + i, ph) + col];
-    } else {
-#pragma unroll
-      for (int i = 0; i < 11; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
-    }
-#pragma unroll
-    for (int i = 0; i < 11; ++i) Bh[i] = hi[i] * (fabsf(hi[i]) > th ? 1.f : 0.f);
-  }
-  __syncthreads();
-  // dim-1 synthesis: runs of 16 outputs per (z band, dim-2 band, coefficient column)
-  if (tid < 2 * 2 * 2 * C) {
-    const int c = tid % C, xr = (tid / C) & 1, b2 = (tid / (2 * C)) & 1, zb = tid / (4 * C);
-    const int xl = xr * 16;
-    const float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + xl / 2;
-    const float* Bh = Bl + C * BS;
-    float lo[13], hi[13], out[16];
-#pragma unroll
-    for (int j = 0; j < 13; ++j) {
-      lo[j] = Bl[j];
-      hi[j] = Bh[j];
-    }
-    syn_lo_hi<16>(lo, hi, out);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) E[((zb * 2 + b2) * C + c) * ES + xl + i] = out[i];
-  }
-  __syncthreads();
-  // dim-2 synthesis (runs of 4), then z synthesis and the store
-  {
-    const int x = tid % P, yl = (tid / P) * 4;
-    float o[2][4];
-#pragma unroll
-    for (int zb = 0; zb < 2; ++zb) {
-      float lo[7], hi[7];
-#pragma unroll
-      for (int j = 0; j < 7; ++j) {
-        lo[j] = E[((zb * 2 + 0) * C + yl / 2 + j) * ES + x];
-        hi[j] = E[((zb * 2 + 1) * C + yl / 2 + j) * ES + x];
-      }
-      syn_lo_hi<4>(lo, hi, o[zb]);
-    }
-    const int ze = pmod_i(2 * t3 - sh, r), zo = pmod_i(2 * t3 - 1 - sh, r);
-    if (x0 + x < p) {
-      const int pr = ridx[x + 9];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (y0 + yl + i >= q) continue;
-        const int pc = cidx[yl + i + 9];
-        put(Y, acc, pr, pc, ze, fCc * o[0][i] - fCc * o[1][i], p, q, r, mode, cs, gd);
-        put(Y, acc, pr, pc, zo, fCc * o[0][i] + fCc * o[1][i], p, q, r, mode, cs, gd);
+      for (int i = 0; i < 11; ++i) {
+        A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
+        A[((zb * 2 + 1) * C + c0 + i) * AS + rr] = hi[i];
       }
     }
+    __syncthreads();
+    // dim-1 analysis + hard threshold (the approximation is the rebuilt coarser level)
+    if (tid < 2 * 2 * 2 * C) {
+      const int t0 = run3 * 10;
+      float v[30];
+#pragma unroll
+      for (int j = 0; j < 30; ++j) v[j] = A[((zb3 * 2 + b23) * C + c3) * AS + 2 * t0 + j];
+      float lo[11], hi[11];
+      ana_lo_hi<11>(v, lo, hi);
+      const int bl = (b23 << 1) | (zb3 << 2), bh = bl | 1;
+      const float tl = tau.t[band_j(bl)], th = tau.t[band_j(bh)];
+      float* Bl = B + ((zb3 * 4 + (b23 << 1)) * C + c3) * BS + t0;
+      float* Bh = Bl + C * BS;
+      if (coarse) {
+#pragma unroll
+        for (int i = 0; i < 11; ++i) Bl[i] = ycur[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 11; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
+      }
+#pragma unroll
+      for (int i = 0; i < 11; ++i) Bh[i] = hi[i] * (fabsf(hi[i]) > th ? 1.f : 0.f);
+    }
+    __syncthreads();
+    // dim-1 synthesis: runs of 16 outputs per (z band, dim-2 band, coefficient column)
+    if (tid < 2 * 2 * 2 * C) {
+      const int c = tid % C, xr = (tid / C) & 1, b2 = (tid / (2 * C)) & 1, zb = tid / (4 * C);
+      const int xl = xr * 16;
+      const float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + xl / 2;
+      const float* Bh = Bl + C * BS;
+      float lo[13], hi[13], out[16];
+#pragma unroll
+      for (int j = 0; j < 13; ++j) {
+        lo[j] = Bl[j];
+        hi[j] = Bh[j];
+      }
+      syn_lo_hi<16>(lo, hi, out);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) E[((zb * 2 + b2) * C + c) * ES + xl + i] = out[i];
+    }
+    __syncthreads();
+    // dim-2 synthesis (runs of 4), then z synthesis and the store
+    {
+      const int x = tid % P, yq = (tid / P) * 4;
+      float o[2][4];
+#pragma unroll
+      for (int zb = 0; zb < 2; ++zb) {
+        float lo[7], hi[7];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+          lo[j] = E[((zb * 2 + 0) * C + yq / 2 + j) * ES + x];
+          hi[j] = E[((zb * 2 + 1) * C + yq / 2 + j) * ES + x];
+        }
+        syn_lo_hi<4>(lo, hi, o[zb]);
+      }
+      const int ze = pmod_i(2 * t3 - sh, r), zo = pmod_i(2 * t3 - 1 - sh, r);
+      if (x0 + x < p) {
+        const int pr = ridx[x + 9];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (y0 + yq + i >= q) continue;
+          const int pc = cidx[yq + i + 9];
+          put(Y, acc, pr, pc, ze, fCc * o[0][i] - fCc * o[1][i], p, q, r, mode, cs, gd);
+          put(Y, acc, pr, pc, zo, fCc * o[0][i] + fCc * o[1][i], p, q, r, mode, cs, gd);
+        }
+      }
+    }
+    __syncthreads();  // E (phase 5) and B (phase 4) are overwritten by the next pair
   }
 }
 
@@ -565,10 +586,12 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
       dim3 grid((p + SP - 1) / SP, (q + SP - 1) / SP, r / 2);
       const int stage = l == 0 ? GBM3D_STAGE_HT_SYN0 : GBM3D_STAGE_HT_SYN;
       mark(stage, 1);
-      if (p >= 10 && q >= 10 && !g_ht_generic)
-        hipLaunchKernelGGL(ht_syn_fast_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out,
-                           acc, mode, cs, lgd);
-      else
+      if (p >= 10 && q >= 10 && !g_ht_generic) {
+        const int zpb = kSynPairsPerBlock;
+        dim3 gf((p + fs::P - 1) / fs::P, (q + fs::P - 1) / fs::P, (r / 2 + zpb - 1) / zpb);
+        hipLaunchKernelGGL(ht_syn_fast_kernel, gf, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out,
+                           acc, mode, cs, lgd, zpb);
+      } else
         hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out, acc,
                            mode, cs, lgd);
       GBM3D_TRY(check_launch("ht_syn"));

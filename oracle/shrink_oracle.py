@@ -208,13 +208,18 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 
 # ------------------------------------------------------------ whole stage
-def shrink_groups(V, sigma, params):
-    """Filtered groups (8, 8, 16, G) and gains (G, 64, 16), as the fused kernel."""
+def shrink_groups(V, sigma, params, exact=False):
+    """Filtered groups (8, 8, 16, G) and gains (G, 64, 16), as the fused kernel
+    (exact=True: FP64 network without BF16 rounding)."""
     c = coefficients(V)
     G = c.shape[0]
-    c32 = c.astype(np.float32)
-    X = (c32 * np.float32(1.0 / np.float32(sigma))).transpose(0, 2, 1)   # (G, 16, 64)
-    _, Hs = forward(params, X, np.full(G, sigma, np.float32))
+    if exact:
+        X = (c / sigma).transpose(0, 2, 1)
+        _, Hs = forward(np.asarray(params, np.float64), X, np.full(G, float(sigma)), exact=True)
+    else:
+        c32 = c.astype(np.float32)
+        X = (c32 * np.float32(1.0 / np.float32(sigma))).transpose(0, 2, 1)   # (G, 16, 64)
+        _, Hs = forward(params, X, np.full(G, sigma, np.float32))
     g = Hs[-1].transpose(0, 2, 1).astype(np.float64)                     # (G, 64, 16)
     g[:, 0, 14:] = 1.0
     out = inverse(c * g)
@@ -233,3 +238,41 @@ def features(Vc, Vn, sigma):
     Wt[:, 0, 14:] = 0
     tr = (0, 2, 1)
     return X.transpose(tr), T.transpose(tr), Wt.transpose(tr)
+
+
+def gather_groups(pad, S, d=8, d2=32):
+    """Groups (d, d, K, G) of the padded search image through S (1-based window
+    indices a + d2*b + 1, source/matchBlocksCPU.m:103-111)."""
+    K, Mt, Nt = S.shape
+    V = np.zeros((d, d, K, Mt, Nt))
+    for tj in range(Nt):
+        for ti in range(Mt):
+            for k in range(K):
+                idx = int(S[k, ti, tj]) - 1
+                a, b = idx % d2, idx // d2
+                V[:, :, k, ti, tj] = pad[ti * d + a:ti * d + a + d, tj * d + b:tj * d + b + d]
+    return V.reshape(d, d, K, Mt * Nt, order="F")
+
+
+def learned_pilot(I, sigma, params, opts, precision=32, exact=False):
+    """Step 1 with the learned shrinkage in place of the hard threshold, composed
+    from the oracle's stages (source/GBM3D.m:50-57 with the filter replaced):
+    per match spin circshift, matchBlocksCPU, learned shrinkage of every group,
+    aggregation with the step-1 weights, shift back; U1 = sum U / sum W.
+    I: M x N (block multiple)."""
+    from . import aggregate_ht, match, resolve_opts, search_pad
+
+    ro = resolve_opts(opts)
+    M, N = I.shape
+    Us, Ws = np.zeros((M, N)), np.zeros((M, N))
+    for sp in ro["matchSpins"]:
+        Is = np.roll(I, (sp, sp), (0, 1))
+        S, _, _ = match(Is, 8, 32, 16, precision=precision)
+        Vn = gather_groups(search_pad(Is, 32), S)
+        Vf, _ = shrink_groups(Vn, sigma, params, exact=exact)
+        Mt, Nt = M // 8, N // 8
+        mosaic = Vf.reshape(8, 8, 16, Mt, Nt, order="F").transpose(3, 0, 4, 1, 2).reshape(M, N, 16)
+        U, W = aggregate_ht(mosaic, S, 8, 32, precision=precision)
+        Us += np.roll(U.astype(np.float64), (-sp, -sp), (0, 1))
+        Ws += np.roll(W.astype(np.float64), (-sp, -sp), (0, 1))
+    return Us / Ws

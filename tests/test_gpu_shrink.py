@@ -36,16 +36,7 @@ def test_synth_pair_matches_host_construction(ctx, m, n, seed, sigma):
     assert np.mean(gn == noisy) > 0.999
 
 
-def _groups(pad, S, d=8, d2=32):
-    K, Mt, Nt = S.shape
-    V = np.zeros((d, d, K, Mt, Nt))
-    for tj in range(Nt):
-        for ti in range(Mt):
-            for k in range(K):
-                idx = int(S[k, ti, tj]) - 1
-                a, b = idx % d2, idx // d2
-                V[:, :, k, ti, tj] = pad[ti * d + a:ti * d + a + d, tj * d + b:tj * d + b + d]
-    return V.reshape(d, d, K, Mt * Nt, order="F")
+_groups = so.gather_groups
 
 
 def test_features_vs_restatement(ctx):
@@ -203,21 +194,7 @@ OPTS8 = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
 
 
 def _learned_pilot_oracle(I, sigma, params, opts):
-    """Step 1 with learned shrinkage composed from oracle stages (FP32 matching)."""
-    ro = oracle.resolve_opts(opts)
-    M, N = I.shape
-    Us, Ws = np.zeros((M, N)), np.zeros((M, N))
-    for sp in ro["matchSpins"]:
-        Is = np.roll(I, (sp, sp), (0, 1))
-        S, _, _ = oracle.match(Is, 8, 32, 16, precision=32)
-        Vn = _groups(oracle.search_pad(Is, 32), S)
-        Vf, _ = so.shrink_groups(Vn, sigma, params)
-        Mt, Nt = M // 8, N // 8
-        mosaic = Vf.reshape(8, 8, 16, Mt, Nt, order="F").transpose(3, 0, 4, 1, 2).reshape(M, N, 16)
-        U, W = oracle.aggregate_ht(mosaic, S, 8, 32, precision=32)
-        Us += np.roll(U.astype(np.float64), (-sp, -sp), (0, 1))
-        Ws += np.roll(W.astype(np.float64), (-sp, -sp), (0, 1))
-    return Us / Ws
+    return so.learned_pilot(I, sigma, params, opts, precision=32)
 
 
 def _weights():
