@@ -164,34 +164,55 @@ struct Plan {
   size_t MN() const { return (size_t)M * N; }
 };
 
-// All device buffers of one denoise call.
-struct Bufs {
-  float *Ip, *pad1, *pad2, *bn, *smap, *vol, *vavg, *w1, *U1s, *W1s, *pilot, *V2f, *g2, *U2s, *W2s, *htw;
+constexpr int kMaxLanes = 8;  // match spins in flight at once (one stream each)
+
+// Buffers of one match spin in flight.
+struct SpinBufs {
+  float *pad1, *pad2, *bn, *smap, *vol, *vavg, *w1, *V2f, *g2, *htw;
   uint32_t *S1, *S2;
 };
 
-void carve(Carve& cv, const Plan& p, const gbm3d_opts& o, Bufs* b) {
+// All device buffers of one denoise call: shared images and spin sums, and
+// one SpinBufs per lane (spins run lane = spin % lanes).
+struct Bufs {
+  float *Ip, *U1s, *W1s, *pilot, *U2s, *W2s;
+  SpinBufs sb[kMaxLanes];
+};
+
+void carve(Carve& cv, const Plan& p, const gbm3d_opts& o, int lanes, Bufs* b) {
   const size_t MN = p.MN(), P = (size_t)p.M + p.d2, Q = (size_t)p.N + p.d2;
   const int dmin = std::min(p.bs, o.Wiener ? p.bw : p.bs);
   const size_t G1 = MN / ((size_t)p.bs * p.bs), G2 = o.Wiener ? MN / ((size_t)p.bw * p.bw) : 0;
   b->Ip = cv.take<float>(MN);
-  b->pad1 = cv.take<float>(P * Q);
-  b->pad2 = cv.take<float>(P * Q);
-  b->bn = cv.take<float>((P - dmin + 1) * (Q - dmin + 1));
-  b->smap = cv.take<float>((P - dmin + 1) * (Q - dmin + 1));
-  b->S1 = cv.take<uint32_t>((size_t)p.K * G1);
-  b->vol = cv.take<float>(MN * p.K);
-  b->htw = cv.take<float>(ht_work_floats(p.M, p.N, p.K, o.levels, o.cycleSpin));
-  b->vavg = cv.take<float>(MN * p.K);
-  b->w1 = cv.take<float>(G1);
   b->U1s = cv.take<float>(MN * p.n1);
   b->W1s = cv.take<float>(MN * p.n1);
   b->pilot = cv.take<float>(MN);
-  b->S2 = cv.take<uint32_t>((size_t)p.K * G2);
-  b->V2f = cv.take<float>(G2 ? MN * p.K : 0);
-  b->g2 = cv.take<float>(G2);
   b->U2s = cv.take<float>(MN * p.n2);
   b->W2s = cv.take<float>(MN * p.n2);
+  for (int l = 0; l < lanes; ++l) {
+    SpinBufs& sb = b->sb[l];
+    sb.pad1 = cv.take<float>(P * Q);
+    sb.pad2 = cv.take<float>(P * Q);
+    sb.bn = cv.take<float>((P - dmin + 1) * (Q - dmin + 1));
+    sb.smap = cv.take<float>((P - dmin + 1) * (Q - dmin + 1));
+    sb.S1 = cv.take<uint32_t>((size_t)p.K * G1);
+    sb.vol = cv.take<float>(MN * p.K);
+    sb.htw = cv.take<float>(ht_work_floats(p.M, p.N, p.K, o.levels, o.cycleSpin));
+    sb.vavg = cv.take<float>(MN * p.K);
+    sb.w1 = cv.take<float>(G1);
+    sb.S2 = cv.take<uint32_t>((size_t)p.K * G2);
+    sb.V2f = cv.take<float>(G2 ? MN * p.K : 0);
+    sb.g2 = cv.take<float>(G2);
+  }
+}
+
+// Lanes of a single-image call: its match spins are independent until the
+// spin sums are normalised (source/GBM3D.m:50-57, 71-82), so they run on
+// parallel streams; serial under stage timing (the timers record on the
+// context stream).
+int spin_lanes(const gbm3d_ctx* c, const Plan& p) {
+  if (c->timing) return 1;
+  return std::max(1, std::min(kMaxLanes, std::max(p.n1, p.n2)));
 }
 
 // Block matching on a padded image with the context's distance variant.
@@ -225,60 +246,89 @@ int make_plan(const gbm3d_opts& o, int m, int n, Plan* p) {
 }
 
 // Device workspace bytes of one image's pipeline.
-size_t pipeline_bytes(const Plan& p, const gbm3d_opts& o) {
+size_t pipeline_bytes(const Plan& p, const gbm3d_opts& o, int lanes = 1) {
   Carve measure{nullptr};
   Bufs b;
-  carve(measure, p, o, &b);
+  carve(measure, p, o, lanes, &b);
   return measure.off + 256;
 }
 
 // One image through both steps on stream st, workspace at ws (pipeline_bytes).
+// Fork `lanes` streams from st (lanes > 1), or run everything on st.
+int lanes_fork(gbm3d_ctx* c, int lanes, hipStream_t st) {
+  if (lanes <= 1) return GBM3D_OK;
+  GBM3D_HIP(hipEventRecord(c->bfork, st));
+  for (int l = 0; l < lanes; ++l) GBM3D_HIP(hipStreamWaitEvent(c->bstreams[l], c->bfork, 0));
+  return GBM3D_OK;
+}
+
+int lanes_join(gbm3d_ctx* c, int lanes, hipStream_t st) {
+  if (lanes <= 1) return GBM3D_OK;
+  for (int l = 0; l < lanes; ++l) {
+    GBM3D_HIP(hipEventRecord(c->bjoin[l], c->bstreams[l]));
+    GBM3D_HIP(hipStreamWaitEvent(st, c->bjoin[l], 0));
+  }
+  return GBM3D_OK;
+}
+
+// One image through both steps on stream st, workspace at ws
+// (pipeline_bytes(p, o, lanes)); with lanes > 1 the match spins of each step
+// run on the context's batch streams (not while a batch is being issued).
 int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, const gbm3d_opts& o, float* dU,
-                    float* dU1, hipStream_t st, char* ws, float* sstats) {
+                    float* dU1, hipStream_t st, char* ws, float* sstats, int lanes = 1) {
   const int m = p.m, n = p.n;
   Carve cv{ws};
   Bufs b;
-  carve(cv, p, o, &b);
+  carve(cv, p, o, lanes, &b);
   const size_t MN = p.MN();
   std::vector<double> tau((size_t)o.levels * 7);
   tau_table(o.levels, o.tauMode, (double)sigma, tau.data());
-  HtWork hw;
-  ht_work_bind(b.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
   const float* masks1 = c->masks + kMaskStride * mask_slot(p.bs);
+  // spin ii of a step with `nl` lanes runs on lane ii % nl
+  auto lane_stream = [&](int ii, int nl) { return nl > 1 ? c->bstreams[ii % nl] : st; };
 
   {
     StageTimer t(c, GBM3D_STAGE_PAD);
     GBM3D_TRY(launch_mirror_pad(dI, m, n, p.M, p.N, b.Ip, st));  // source/GBM3D.m:22-25
   }
   // ---- step 1: source/GBM3D.m:50-57
+  const int l1 = std::max(1, std::min(lanes, p.n1));
+  GBM3D_TRY(lanes_fork(c, l1, st));
   for (int ii = 0; ii < p.n1; ++ii) {
     const int sp = o.matchSpins[ii];
+    const hipStream_t ls = lane_stream(ii, l1);
+    SpinBufs& sb = b.sb[ii % l1];
     {
       StageTimer t(c, GBM3D_STAGE_PAD);
-      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, b.pad1, st));
+      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad1, ls));
     }
     if (c->ext.learned) {
       // variant: learned gains on the DCT-Haar coefficients of each group,
       // gathered straight from the search image (no mosaic volume)
-      GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bs, p.d2, p.K, b.S1, nullptr, st, GBM3D_STAGE_MATCH1));
+      GBM3D_TRY(do_match(c, sb.pad1, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, nullptr, ls,
+                         GBM3D_STAGE_MATCH1));
       StageTimer t(c, GBM3D_STAGE_SHRINK);
       const long G1 = (long)(p.M / p.bs) * (p.N / p.bs);
-      GBM3D_TRY(launch_shrink(b.pad1, b.S1, p.M, p.d2, G1, sigma, c->shrink_wb, c->shrink_fb, b.vavg, nullptr,
-                              sstats + (size_t)2 * kShrinkGrid * ii, nullptr, st));
+      GBM3D_TRY(launch_shrink(sb.pad1, sb.S1, p.M, p.d2, G1, sigma, c->shrink_wb, c->shrink_fb, sb.vavg, nullptr,
+                              sstats + (size_t)2 * kShrinkGrid * ii, nullptr, ls));
     } else {
-      GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bs, p.d2, p.K, b.S1, b.vol, st, GBM3D_STAGE_MATCH1));
-      GBM3D_TRY(run_ht(b.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, p.bs, st, ht_hook, c));
+      HtWork hw;
+      ht_work_bind(sb.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
+      GBM3D_TRY(do_match(c, sb.pad1, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, sb.vol, ls,
+                         GBM3D_STAGE_MATCH1));
+      GBM3D_TRY(run_ht(sb.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, sb.vavg, p.bs, ls, ht_hook, c));
     }
     {
       StageTimer t(c, GBM3D_STAGE_AGG1_PREP);
-      GBM3D_TRY(launch_agg1_prep(b.vavg, p.M, p.N, p.K, p.bs, b.w1, st));
+      GBM3D_TRY(launch_agg1_prep(sb.vavg, p.M, p.N, p.K, p.bs, sb.w1, ls));
     }
     {
       StageTimer t(c, GBM3D_STAGE_AGG1);
-      GBM3D_TRY(launch_aggregate_m(1, b.vavg, b.w1, b.S1, masks1, p.M, p.N, p.bs, p.d2, p.K, sp, b.U1s + MN * ii,
-                                   b.W1s + MN * ii, st));
+      GBM3D_TRY(launch_aggregate_m(1, sb.vavg, sb.w1, sb.S1, masks1, p.M, p.N, p.bs, p.d2, p.K, sp,
+                                   b.U1s + MN * ii, b.W1s + MN * ii, ls));
     }
   }
+  GBM3D_TRY(lanes_join(c, l1, st));
   {
     StageTimer t(c, GBM3D_STAGE_NORM);
     GBM3D_TRY(launch_normalize(b.U1s, p.n1, b.W1s, p.n1, nullptr, 0, p.M, p.N, p.M, p.N, b.pilot, st));
@@ -291,25 +341,31 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
   // ---- step 2: source/GBM3D.m:70-82
   const float* masks2 = c->masks + kMaskStride * mask_slot(p.bw);
   const long G2 = (long)(p.M / p.bw) * (p.N / p.bw);
+  const int l2 = std::max(1, std::min(lanes, p.n2));
+  GBM3D_TRY(lanes_fork(c, l2, st));
   for (int ii = 0; ii < p.n2; ++ii) {
     const int sp = o.matchSpinsWie[ii];
+    const hipStream_t ls = lane_stream(ii, l2);
+    SpinBufs& sb = b.sb[ii % l2];
     {
       StageTimer t(c, GBM3D_STAGE_PAD);
-      GBM3D_TRY(launch_search_pad(b.pilot, p.M, p.N, sp, p.d2, b.pad1, st));
-      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, b.pad2, st));
+      GBM3D_TRY(launch_search_pad(b.pilot, p.M, p.N, sp, p.d2, sb.pad1, ls));
+      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad2, ls));
     }
-    GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, p.M, p.N, p.bw, p.d2, p.K, b.S2, nullptr, st, GBM3D_STAGE_MATCH2));
+    GBM3D_TRY(do_match(c, sb.pad1, sb.bn, sb.smap, p.M, p.N, p.bw, p.d2, p.K, sb.S2, nullptr, ls,
+                       GBM3D_STAGE_MATCH2));
     {
       StageTimer t(c, GBM3D_STAGE_WIENER);
-      GBM3D_TRY(launch_wiener(b.pad1, b.pad2, b.S2, nullptr, nullptr, p.M, p.N, p.bw, p.d2, p.K, G2, sigma, b.V2f,
-                              b.g2, st));
+      GBM3D_TRY(launch_wiener(sb.pad1, sb.pad2, sb.S2, nullptr, nullptr, p.M, p.N, p.bw, p.d2, p.K, G2, sigma,
+                              sb.V2f, sb.g2, ls));
     }
     {
       StageTimer t(c, GBM3D_STAGE_AGG2);
-      GBM3D_TRY(launch_aggregate_m(2, b.V2f, b.g2, b.S2, masks2, p.M, p.N, p.bw, p.d2, p.K, sp, b.U2s + MN * ii,
-                                   b.W2s + MN * ii, st));
+      GBM3D_TRY(launch_aggregate_m(2, sb.V2f, sb.g2, sb.S2, masks2, p.M, p.N, p.bw, p.d2, p.K, sp,
+                                   b.U2s + MN * ii, b.W2s + MN * ii, ls));
     }
   }
+  GBM3D_TRY(lanes_join(c, l2, st));
   {
     // U = sum(U,3)./sum(W,3): slices past numel(matchSpinsWie) still hold the
     // step-1 weights (U, W are sized by numel(matchSpins), :26-27).
@@ -394,8 +450,9 @@ int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const
   Plan p;
   GBM3D_TRY(make_plan(o, m, n, &p));
   GBM3D_TRY(learned_setup(c, o, 1));
-  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, pipeline_bytes(p, o)));
-  GBM3D_TRY(run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena, image_stats(c, 0)));
+  const int lanes = spin_lanes(c, p);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, pipeline_bytes(p, o, lanes)));
+  GBM3D_TRY(run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena, image_stats(c, 0), lanes));
   if (!c->ext.learned) return GBM3D_OK;
   std::vector<Plan> plans{p};
   float* u1[1] = {dU1};

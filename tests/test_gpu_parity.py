@@ -294,3 +294,17 @@ def test_batch_autotune_picks_a_valid_shape_and_keeps_results(ctx):
     for a, b in zip(out, ref):
         np.testing.assert_array_equal(a, b)
     bctx.close()
+
+
+def test_parallel_spin_lanes_match_serial_run(ctx):
+    """Match spins of one image run on parallel streams; under stage timing they
+    run serially on the context stream.  Both give the same bits."""
+    _, noisy = synth.make_pair(301, 192, 160, 25.0)
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
+    U_par, o_par = ctx.denoise(noisy, 25.0, opts)
+    ctx.set_timing(True)
+    try:
+        U_ser, o_ser = ctx.denoise(noisy, 25.0, opts)
+    finally:
+        ctx.set_timing(False)
+    assert np.array_equal(U_par, U_ser) and np.array_equal(o_par["U1"], o_ser["U1"])
