@@ -246,6 +246,14 @@ class Context:
         check(self.lib.gbm3d_ht_filter(self.handle, ptr(V), m, n, K, levels, cycle_spin, ptr(tau), ptr(out)))
         return out
 
+    def median_filter(self, V):
+        """Vavg = medfilt3(Volume, [3 3 3]) (the filtType 'median' branch)."""
+        V = np.asfortranarray(V, dtype=np.float32)
+        m, n, K = V.shape
+        out = np.zeros_like(V)
+        check(self.lib.gbm3d_median_filter(self.handle, ptr(V), m, n, K, ptr(out)))
+        return out
+
     def aggregate_ht(self, Vavg, S, d: int, d2: int):
         Vavg = np.asfortranarray(Vavg, dtype=np.float32)
         S = np.asfortranarray(S, dtype=np.uint32)
@@ -311,10 +319,15 @@ def matchBlocksCPU(I, sigma: float, opts: dict):
 
 
 def denoiseAndAggregateGPU(Volume, S, tau, opts: dict):
-    """[U, W] = denoiseAndAggregateGPU(Volume, S, tau, Psi, Psi2, opts) ('ht')."""
+    """[U, W] = denoiseAndAggregateGPU(Volume, S, tau, Psi, Psi2, opts) ('ht' or 'median')."""
     o = checkBM3Dopts(opts)
     ctx = default_context()
-    Vavg = ctx.ht_filter(Volume, o["levels"], o["cycleSpin"], tau)
+    if o.get("filtType", "ht") == "median":
+        Vavg = ctx.median_filter(Volume)   # denoiseAndAggregateGPU.m:49-52
+    elif o.get("filtType", "ht") == "ht":
+        Vavg = ctx.ht_filter(Volume, o["levels"], o["cycleSpin"], tau)
+    else:
+        raise ValueError("unrecognized filtering")   # denoiseAndAggregateGPU.m:54
     return ctx.aggregate_ht(Vavg, S, o["blockSize"], o["matchSize"])
 
 

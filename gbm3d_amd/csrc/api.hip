@@ -131,10 +131,8 @@ struct StageTimer {
 
 int validate(const gbm3d_opts& o, int m, int n, int* M, int* N) {
   if (m < 1 || n < 1) return fail(GBM3D_ERR_ARG, "image must be non-empty");
-  if (o.filtType != GBM3D_FILT_HT) {
-    if (o.filtType == GBM3D_FILT_MEDIAN) return fail(GBM3D_ERR_UNSUPPORTED, "filtType 'median' is not built");
-    return fail(GBM3D_ERR_ARG, "unrecognized filtering");
-  }
+  if (o.filtType != GBM3D_FILT_HT && o.filtType != GBM3D_FILT_MEDIAN)
+    return fail(GBM3D_ERR_ARG, "unrecognized filtering");  // denoiseAndAggregateGPU.m:54
   if (o.numMax != o.numMin) return fail(GBM3D_ERR_UNSUPPORTED, "numMax ~= numMin (tolerance matching) is not built");
   if (o.nMatchSpins < 1 || o.nMatchSpins > GBM3D_MAX_SPINS) return fail(GBM3D_ERR_ARG, "matchSpins: 1..16 entries");
   if (o.Wiener) {
@@ -316,7 +314,13 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
       ht_work_bind(sb.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
       GBM3D_TRY(do_match(c, sb.pad1, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, sb.vol, ls,
                          GBM3D_STAGE_MATCH1));
-      GBM3D_TRY(run_ht(sb.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, sb.vavg, p.bs, ls, ht_hook, c));
+      if (o.filtType == GBM3D_FILT_MEDIAN) {
+        StageTimer t(c, GBM3D_STAGE_HT_SYN0);
+        GBM3D_TRY(launch_median3(sb.vol, p.M, p.N, p.K, p.bs, sb.vavg, ls));  // denoiseAndAggregateGPU.m:49-52
+      } else {
+        GBM3D_TRY(run_ht(sb.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau.data(), hw, sb.vavg, p.bs, ls, ht_hook,
+                         c));
+      }
     }
     {
       StageTimer t(c, GBM3D_STAGE_AGG1_PREP);
@@ -717,6 +721,28 @@ int gbm3d_ht_filter(gbm3d_ctx* c, const float* Volume, int m, int n, int K, int 
   ht_work_bind(htw, m, n, K, levels, cycleSpin, &hw);
   GBM3D_TRY(upload(c, in, Volume, vol));
   GBM3D_TRY(run_ht(in, m, n, K, levels, cycleSpin, tau, hw, out, 0, c->stream));
+  GBM3D_TRY(download(c, Vavg, out, vol));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_median_filter(gbm3d_ctx* c, const float* Volume, int m, int n, int K, float* Vavg) {
+  GBM3D_TRY(set_dev(c));
+  if (!Volume || !Vavg) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (m < 1 || n < 1 || K < 1) return fail(GBM3D_ERR_ARG, "median_filter: size");
+  const size_t vol = (size_t)m * n * K;
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** in, float** out) {
+    *in = k.take<float>(vol);
+    *out = k.take<float>(vol);
+  };
+  float *in, *out;
+  lay(cv, &in, &out);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &in, &out);
+  GBM3D_TRY(upload(c, in, Volume, vol));
+  GBM3D_TRY(launch_median3(in, m, n, K, 0, out, c->stream));
   GBM3D_TRY(download(c, Vavg, out, vol));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
   return GBM3D_OK;

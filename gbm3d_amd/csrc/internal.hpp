@@ -97,6 +97,10 @@ typedef void (*StageHook)(void* user, int stage, int begin);
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau,
            const HtWork& w, float* vavg, int gd, hipStream_t st, StageHook hook = nullptr, void* user = nullptr);
 
+// The 'median' step-1 filter: Vavg = medfilt3(vol, [3 3 3]) (edge values
+// repeated), mosaic (gd == 0) or group-contiguous output.
+int launch_median3(const float* vol, int M, int N, int K, int gd, float* out, hipStream_t st);
+
 // Step-1 group weights 1/(TV(Vavg)+eps) from the group-contiguous Vavg.
 int launch_agg1_prep(const float* vavg_groups, int M, int N, int K, int d, float* wgt, hipStream_t st);
 // Mosaic (M x N x K) -> group-contiguous (d x d x K x G) copy.

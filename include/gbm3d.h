@@ -173,6 +173,12 @@ int gbm3d_match(gbm3d_ctx* ctx, const float* I, int m, int n, int d, int d2,
 int gbm3d_ht_filter(gbm3d_ctx* ctx, const float* Volume, int m, int n, int K,
                     int levels, int cycleSpin, const double* tau, float* Vavg);
 
+/* Vavg = medfilt3(Volume, [3 3 3]): the filtType 'median' branch of
+ * denoiseAndAggregateGPU -- source/denoiseAndAggregateGPU.m:49-52.  Volume,
+ * Vavg: m x n x K (host); outside the volume the edge value repeats. */
+int gbm3d_median_filter(gbm3d_ctx* ctx, const float* Volume, int m, int n,
+                        int K, float* Vavg);
+
 /* [U, W] = aggregation of denoiseAndAggregateGPU   -- :59-92.
  * U, W: m x n (crop of the padded canvas). */
 int gbm3d_aggregate_ht(gbm3d_ctx* ctx, const float* Vavg, const uint32_t* S,

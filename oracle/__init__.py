@@ -140,6 +140,16 @@ def ht_filter(V, levels, cycle_spin, tau, precision=64, want_margin=False):
     return (out, mm.value) if want_margin else out
 
 
+def median3(V, precision=64):
+    """medfilt3(Volume, [3 3 3]) of the step-1 'median' branch."""
+    dt, suf = _dt(precision)
+    V = _f(V, dt)
+    m, n, K = V.shape
+    out = np.zeros_like(V)
+    _check(getattr(lib(), "orc_median3_" + suf)(_p(V), m, n, K, _p(out)))
+    return out
+
+
 def agg_weights(Vavg, d, precision=64):
     dt, suf = _dt(precision)
     Vavg = _f(Vavg, dt)

@@ -830,9 +830,34 @@ void aggregate_wie(const T* V2f, const T* gain2, const uint32_t* S, int m, int n
 // --------------------------------------------------------------- the driver
 // [U, out] = GBM3D(I, sigma, opts)  (source/GBM3D.m:21-83) on resolved opts.
 // Returns 0, or a gbm3d_status for unsupported options.
+// Vavg = medfilt3(Volume, [3 3 3]) (source/denoiseAndAggregateGPU.m:49-52):
+// the median of the 27 values of the 3x3x3 neighbourhood, the volume extended
+// by MATLAB's default 'symmetric' padding, which for a window of 3 repeats the
+// edge value.  No clip at 0 on this branch.
+template <class T>
+void median3(const T* V, int m, int n, int K, T* out) {
+  auto at = [&](int r, int c, int k) {
+    r = std::min(std::max(r, 0), m - 1);
+    c = std::min(std::max(c, 0), n - 1);
+    k = std::min(std::max(k, 0), K - 1);
+    return V[r + (size_t)m * (c + (size_t)n * k)];
+  };
+  std::vector<T> w(27);
+  for (int k = 0; k < K; ++k)
+    for (int c = 0; c < n; ++c)
+      for (int r = 0; r < m; ++r) {
+        int e = 0;
+        for (int dk = -1; dk <= 1; ++dk)
+          for (int dc = -1; dc <= 1; ++dc)
+            for (int dr = -1; dr <= 1; ++dr) w[e++] = at(r + dr, c + dc, k + dk);
+        std::nth_element(w.begin(), w.begin() + 13, w.end());
+        out[r + (size_t)m * (c + (size_t)n * k)] = w[13];
+      }
+}
+
 template <class T>
 int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout, T* U1out, bool gather) {
-  if (o.filtType != GBM3D_FILT_HT) return GBM3D_ERR_UNSUPPORTED;
+  if (o.filtType != GBM3D_FILT_HT && o.filtType != GBM3D_FILT_MEDIAN) return GBM3D_ERR_UNSUPPORTED;
   if (o.numMax != o.numMin) return GBM3D_ERR_UNSUPPORTED;
   if (o.Wiener && o.nMatchSpinsWie == GBM3D_UNSET) return GBM3D_ERR_MISSING_FIELD;
   const int bs = o.blockSize, d2 = o.matchSize, K = o.numMax;
@@ -852,7 +877,10 @@ int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout,
       int sp = o.matchSpins[ii];
       std::vector<T> Is = circshift2<T>(I.data(), M, N, sp);
       match_step1<T>(Is.data(), M, N, bs, d2, K, S.data(), Vol.data(), nullptr);
-      ht_filter<T>(Vol.data(), M, N, K, o.levels, o.cycleSpin, tau.data(), Vavg.data(), nullptr);
+      if (o.filtType == GBM3D_FILT_MEDIAN)
+        median3<T>(Vol.data(), M, N, K, Vavg.data());
+      else
+        ht_filter<T>(Vol.data(), M, N, K, o.levels, o.cycleSpin, tau.data(), Vavg.data(), nullptr);
       aggregate_ht<T>(Vavg.data(), S.data(), M, N, bs, d2, K, U.data(), W.data(), gather);
       for (int c = 0; c < N; ++c)
         for (int r = 0; r < M; ++r) {

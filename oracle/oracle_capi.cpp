@@ -104,6 +104,17 @@ MATCH1(f32, float)
 HT(f64, double)
 HT(f32, float)
 
+#define MED(SUF, T)                                                                                  \
+  int orc_median3_##SUF(const T* V, int m, int n, int K, T* Vavg) {                                 \
+    if (m < 1 || n < 1 || K < 1) return GBM3D_ERR_ARG;                                              \
+    return guarded([&] {                                                                             \
+      median3<T>(V, m, n, K, Vavg);                                                                  \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }
+MED(f64, double)
+MED(f32, float)
+
 #define AGG(SUF, T)                                                                                  \
   int orc_agg_ht_##SUF(const T* Vavg, const uint32_t* S, int m, int n, int d, int d2, int K, T* U,  \
                        T* W, int gather) {                                                           \

@@ -199,8 +199,8 @@ def test_full_size_2048_properties(ctx, sigma):
 
 def test_error_paths(ctx):
     clean, noisy = synth.make_pair(1, 64, 64, 25.0)
-    with pytest.raises(GBM3DError, match="median"):
-        ctx.denoise(noisy, 25.0, {"filtType": "median", "matchSpinsWie": [0]})
+    with pytest.raises(GBM3DError, match="unrecognized filtering"):
+        ctx.denoise(noisy, 25.0, {"filtType": 7, "matchSpinsWie": [0]})
     with pytest.raises(GBM3DError, match="matchSpinsWie"):
         ctx.denoise(noisy, 25.0, {"blockSize": 8})
     with pytest.raises(GBM3DError, match="numMax"):
@@ -308,3 +308,29 @@ def test_parallel_spin_lanes_match_serial_run(ctx):
     finally:
         ctx.set_timing(False)
     assert np.array_equal(U_par, U_ser) and np.array_equal(o_par["U1"], o_ser["U1"])
+
+
+@pytest.mark.parametrize("shape,seed", [((40, 24, 16), 5), ((64, 64, 8), 6)])
+def test_median_filter_bit_exact(ctx, shape, seed):
+    """medfilt3(Volume, [3 3 3]) is a selection: bit-exact vs the oracle,
+    including heavy ties (integer values) and the volume edges."""
+    rng = np.random.default_rng(seed)
+    V = rng.normal(100, 40, shape).astype(np.float32)
+    V[: shape[0] // 2] = np.round(V[: shape[0] // 2] / 20) * 20
+    out = ctx.median_filter(V)
+    ref = O.median3(V, precision=32)
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("m,n,sigma,seed", [(128, 96, 25.0, 7), (96, 136, 15.0, 8)])
+def test_median_branch_end_to_end_vs_oracle(ctx, m, n, sigma, seed):
+    """filtType 'median' (denoiseAndAggregateGPU.m:49-52) through the whole path."""
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4], "filtType": "median"}
+    clean, noisy = synth.make_pair(seed, m, n, sigma)
+    U, out = ctx.denoise(noisy, sigma, opts)
+    U32, U132 = O.gbm3d(noisy, sigma, opts, precision=32)
+    assert np.max(np.abs(out["U1"] - U132)) <= 1e-3
+    U2 = O.step2_from_pilot(out["U1"], noisy, sigma, opts)
+    assert np.max(np.abs(U - U2)) <= 1e-3
+    U64, _ = O.gbm3d(noisy, sigma, opts, precision=64)
+    assert abs(synth.psnr(clean, U) - synth.psnr(clean, U64)) <= 0.01

@@ -228,3 +228,13 @@ def test_options_restatement():
     assert c["matchSpins"] == [0, 4] and c["matchSize"] == 32 and "matchSpinsWie" not in c
     c = O.check_opts({"blockSize": 64, "blockSizeWie": 16})
     assert c["matchSize"] == 64 and c["matchSpins"] == [0, 8]
+
+
+def test_median3_matches_scipy_median_filter():
+    """medfilt3 with edge values repeated == scipy's median filter, mode 'nearest'."""
+    from scipy import ndimage
+
+    rng = np.random.default_rng(11)
+    V = rng.normal(0, 1, (13, 9, 8))
+    V[::3] = np.round(V[::3])
+    assert np.array_equal(O.median3(V), ndimage.median_filter(V, size=3, mode="nearest"))
