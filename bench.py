@@ -305,6 +305,46 @@ def learned_report(args):
     }))
 
 
+def distributed_report(args):
+    """GBM3D_distributed (source/GBM3D_distributed.m) on the 2048^2 sigma 25
+    image: tiles of 256 + blockSize on parallel streams, ramp stitching; vs the
+    single-image path, and vs the FP64 restatement on a 600x520 crop."""
+    import gbm3d_amd
+    import oracle
+    from gbm3d_amd import device, synth
+
+    clean, noisy = synth.make_pair(SEED, SIZE, SIZE, SIGMA)
+    ctx = gbm3d_amd.Context(0)
+    stream = device.Stream()
+    ctx.set_stream(stream.handle)
+    dI = device.DeviceArray.from_host(noisy)
+    dU = device.DeviceArray((SIZE, SIZE))
+    e0, e1 = device.Event(), device.Event()
+    res = {}
+    for name, fn in (("single_image", ctx.denoise_device), ("distributed", ctx.denoise_distributed_device)):
+        def run():
+            fn(dI.ptr, SIZE, SIZE, SIGMA, OPTS, dU.ptr)
+        for _ in range(args.warmup):
+            run()
+        e0.record(stream)
+        for _ in range(args.steps):
+            run()
+        e1.record(stream)
+        stream.synchronize()
+        ms = e0.elapsed_ms(e1) / args.steps
+        res[name] = {"ms": round(ms, 3), "mpix_s": round(SIZE * SIZE / (ms * 1e-3) / 1e6, 2),
+                     "psnr_db": round(synth.psnr(clean, dU.to_host()), 4)}
+    c2, n2 = synth.make_pair(600, 600, 520, SIGMA)
+    U, _ = ctx.denoise_distributed(n2, SIGMA, OPTS)
+    U64, _ = oracle.gbm3d_distributed(n2, SIGMA, OPTS, precision=64)
+    print(json.dumps({
+        "workload": "GBM3D_distributed, 2048x2048 synthetic image, sigma 25, 8x8 blocks, tiles of 264",
+        "cases_per_step": len(oracle.distribution(SIZE, SIZE, [0, 4], 8)),
+        **res,
+        "crop_600x520_vs_fp64_oracle": {"max_abs": float(np.abs(U - U64).max()),
+                                        "dpsnr_db": round(synth.psnr(c2, U) - synth.psnr(c2, U64), 4)}}))
+
+
 # kernel symbol prefix -> roofline key; the largest dispatch of a kernel is the
 # one measured (the finest level for the wavelet synthesis)
 TRAFFIC_KERNELS = {"gbm3d::ht_syn_fast_kernel": "ht_syn0", "void gbm3d::match_kernel": "match2",
@@ -353,7 +393,8 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="warmup + steps only (for rocprofv3 runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="with --profile-only: also run one device copy of MIB MiB (byte-counter calibration)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched", "learned"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched", "learned",
+                                                          "distributed"])
     ap.add_argument("--size", type=int, default=0, help="image side for --workload precision")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="derive profiles/traffic.json from two rocprofv3 --pmc runs made with --calibrate-copy 256")
@@ -369,6 +410,9 @@ def main():
         return
     if args.workload == "learned":
         learned_report(args)
+        return
+    if args.workload == "distributed":
+        distributed_report(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (one image per launch); --gpus must be 1", file=sys.stderr)

@@ -145,6 +145,8 @@ SIGNATURES = {
     "gbm3d_synchronize": (_I, [_P]),
     "gbm3d_denoise": (_I, [_P, _P, _I, _I, _F, C.POINTER(Opts), _P, _P]),
     "gbm3d_denoise_device": (_I, [_P, _P, _I, _I, _F, C.POINTER(Opts), _P, _P]),
+    "gbm3d_denoise_distributed": (_I, [_P, _P, _I, _I, _F, C.POINTER(Opts), _P, _P]),
+    "gbm3d_denoise_distributed_device": (_I, [_P, _P, _I, _I, _F, C.POINTER(Opts), _P, _P]),
     "gbm3d_denoise_batch_device": (_I, [_P, _I, _P, _P, _P, _P, C.POINTER(Opts), _P, _P, _I]),
     "gbm3d_batch_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "gbm3d_match": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),

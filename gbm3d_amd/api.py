@@ -192,6 +192,26 @@ class Context:
                                      C.byref(o) if o is not None else None, ptr(U), ptr(U1)))
         return U, {"U1": U1}
 
+    def denoise_distributed(self, I, sigma: float, opts: dict | None = None):
+        """[U, out] = GBM3D_distributed(I, sigma, opts): out["U2"] is the first estimate."""
+        I = np.asfortranarray(I, dtype=np.float32)
+        if I.ndim != 2:
+            raise ValueError("GBM3D_distributed expects a 2-D grayscale image")
+        m, n = I.shape
+        U = np.zeros((m, n), dtype=np.float32, order="F")
+        U2 = np.zeros_like(U)
+        o = opts_to_struct(opts)
+        check(self.lib.gbm3d_denoise_distributed(self.handle, ptr(I), m, n, C.c_float(sigma),
+                                                 C.byref(o) if o is not None else None, ptr(U), ptr(U2)))
+        return U, {"U2": U2}
+
+    def denoise_distributed_device(self, dI: int, m: int, n: int, sigma: float, opts, dU: int,
+                                   dU2: int | None = None):
+        o = opts_to_struct(opts)
+        check(self.lib.gbm3d_denoise_distributed_device(self.handle, C.c_void_p(dI), m, n, C.c_float(sigma),
+                                                        C.byref(o) if o is not None else None, C.c_void_p(dU),
+                                                        C.c_void_p(dU2) if dU2 else None))
+
     def denoise_device(self, dI: int, m: int, n: int, sigma: float, opts, dU: int, dU1: int | None = None):
         """Device pointers (ints, column-major float32); asynchronous."""
         o = opts_to_struct(opts)
@@ -310,6 +330,13 @@ def default_context() -> Context:
 def GBM3D(I, sigma: float, opts: dict | None = None):
     """[U, out] = GBM3D(I, sigma, opts)  (source/GBM3D.m:1-84); out['U1'] is the pilot."""
     return default_context().denoise(I, sigma, opts)
+
+
+def GBM3D_distributed(I, sigma: float, opts: dict | None = None):
+    """[U, out] = GBM3D_distributed(I, sigma, opts) (source/GBM3D_distributed.m)."""
+    if opts is None:
+        opts = {"profile": "default"}
+    return default_context().denoise_distributed(I, sigma, opts)
 
 
 def matchBlocksCPU(I, sigma: float, opts: dict):

@@ -7,6 +7,7 @@
 //     [U, out] = gbm3d_mex(I, sigma, opts)      % source/GBM3D.m:1
 //     opts     = gbm3d_mex('setBM3Dopts', profile)
 //     opts     = gbm3d_mex('checkBM3Dopts', opts)
+//     [U, out] = gbm3d_mex('GBM3D_distributed', I, sigma, opts)   % source/GBM3D_distributed.m:1
 // I may be double or single (m x n, real); U and out.U1 come back as double,
 // as GBM3D.m returns them.  Library errors become MATLAB errors with id
 // gbm3d:<status>.  One library context (device 0, its own stream) lives for
@@ -121,11 +122,71 @@ mxArray* from_opts(const gbm3d_opts& o) {
 
 }  // namespace
 
+// Image argument -> float column-major copy (double or single input).
+std::vector<float> to_image(const mxArray* I, int* m, int* n) {
+  if (mxIsComplex(I) || mxGetNumberOfDimensions(I) != 2 || !(mxIsDouble(I) || mxIsSingle(I)))
+    mexErrMsgIdAndTxt("gbm3d:arg", "I must be a real 2-D double or single image");
+  *m = (int)mxGetM(I);
+  *n = (int)mxGetN(I);
+  std::vector<float> in((size_t)*m * *n);
+  if (mxIsDouble(I)) {
+    const double* p = mxGetPr(I);
+    for (size_t i = 0; i < in.size(); ++i) in[i] = (float)p[i];
+  } else {
+    std::memcpy(in.data(), mxGetData(I), in.size() * sizeof(float));
+  }
+  return in;
+}
+
+mxArray* to_double(const std::vector<float>& v, int m, int n) {
+  mxArray* a = mxCreateDoubleMatrix(m, n, mxREAL);
+  double* p = mxGetPr(a);
+  for (size_t i = 0; i < v.size(); ++i) p[i] = v[i];
+  return a;
+}
+
+void ensure_ctx() {
+  if (!g_ctx) {
+    int st = gbm3d_create(0, &g_ctx);
+    if (st) fail(st);
+    mexAtExit(release);
+  }
+}
+
+void distributed(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
+  if (nrhs < 3) mexErrMsgIdAndTxt("gbm3d:arg", "usage: gbm3d_mex('GBM3D_distributed', I, sigma, opts)");
+  int m, n;
+  std::vector<float> in = to_image(prhs[1], &m, &n);
+  if (mxGetNumberOfElements(prhs[2]) != 1)
+    mexErrMsgIdAndTxt("gbm3d:unsupported", "sigma must be a scalar (coloured-noise PSD is not built)");
+  const float sigma = (float)mxGetScalar(prhs[2]);
+  gbm3d_opts o;
+  const gbm3d_opts* op = nullptr;
+  if (nrhs >= 4) {
+    to_opts(prhs[3], &o);
+    op = &o;
+  }
+  ensure_ctx();
+  std::vector<float> U(in.size()), U2(in.size());
+  int st = gbm3d_denoise_distributed(g_ctx, in.data(), m, n, sigma, op, U.data(), U2.data());
+  if (st) fail(st);
+  plhs[0] = to_double(U, m, n);
+  if (nlhs > 1) {
+    const char* f[] = {"U2"};
+    plhs[1] = mxCreateStructMatrix(1, 1, 1, f);
+    mxSetField(plhs[1], 0, "U2", to_double(U2, m, n));
+  }
+}
+
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   if (nrhs >= 1 && mxIsChar(prhs[0])) {
     char cmd[32] = {0};
     mxGetString(prhs[0], cmd, sizeof(cmd));
     gbm3d_opts o;
+    if (!std::strcmp(cmd, "GBM3D_distributed")) {
+      distributed(nlhs, plhs, nrhs, prhs);
+      return;
+    }
     if (!std::strcmp(cmd, "setBM3Dopts")) {
       char prof[32] = "default";
       if (nrhs >= 2 && mxIsChar(prhs[1])) mxGetString(prhs[1], prof, sizeof(prof));

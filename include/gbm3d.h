@@ -138,6 +138,22 @@ int gbm3d_denoise_device(gbm3d_ctx* ctx, const float* dI, int m, int n,
                          float sigma, const gbm3d_opts* opts, float* dU,
                          float* dU1);
 
+/*
+ * [U, out] = GBM3D_distributed(I, sigma, opts) -- source/GBM3D_distributed.m:1-122
+ * Tiles of 256 + blockSize (utilities/getBM3D_distribution.m), each tile and
+ * match spin denoised on its own, stitched with ramp windows
+ * (utilities/makeInterpWindow.m).  U2 (nullable): out.U2, the first estimate
+ * (out.U1 of GBM3D for images below 256 + blockSize, which run GBM3D).
+ * The learned form is not supported here.
+ */
+int gbm3d_denoise_distributed(gbm3d_ctx* ctx, const float* I, int m, int n,
+                              float sigma, const gbm3d_opts* opts, float* U,
+                              float* U2);
+int gbm3d_denoise_distributed_device(gbm3d_ctx* ctx, const float* dI, int m,
+                                     int n, float sigma,
+                                     const gbm3d_opts* opts, float* dU,
+                                     float* dU2);
+
 /* ------------------------------------------------------------- batches
  * [U_i, out_i] = GBM3D(I_i, sigma_i, opts) for `count` independent images
  * (device buffers, sizes m[i] x n[i]) sharing one opts.  Images run on

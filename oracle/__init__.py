@@ -268,3 +268,37 @@ def step2_from_pilot(pilot, noisy, sigma, opts=None, precision=32):
         Us += np.roll(U.astype(np.float64), (-sp, -sp), (0, 1))
         Ws += np.roll(W.astype(np.float64), (-sp, -sp), (0, 1))
     return Us / Ws
+
+
+def distribution(m, n, spins, k):
+    """getBM3D_distribution -> (cases, 5) int rows (y1, y2, x1, x2, spin), 1-based inclusive."""
+    sp = np.ascontiguousarray(np.atleast_1d(spins), dtype=np.int32)
+    cap = 4096
+    rows = np.zeros((cap, 5), dtype=np.int32)
+    cnt = lib().orc_distribution(m, n, _p(sp), sp.size, k, _p(rows), cap)
+    if cnt < 0:
+        raise RuntimeError("too many cases")
+    return rows[:cnt].copy()
+
+
+def interp_window(a, b, k):
+    w = np.zeros((a, b), dtype=np.float64, order="F")
+    _check(lib().orc_interp_window(a, b, k, _p(w)))
+    return w
+
+
+def gbm3d_distributed(I, sigma, opts=None, precision=64, pilot=None):
+    """[U, out.U2] = GBM3D_distributed(I, sigma, opts); pilot: skip step 1."""
+    dt, suf = _dt(precision)
+    I = _f(I, dt)
+    m, n = I.shape
+    o = Opts()
+    src = opts_to_struct(opts)
+    lib().orc_resolve_opts(C.byref(src) if src is not None else None, C.byref(o))
+    U = np.zeros((m, n), dtype=dt, order="F")
+    U2 = np.zeros_like(U)
+    P = None if pilot is None else _f(pilot, dt)
+    fn = getattr(lib(), "orc_gbm3d_distributed_" + suf)
+    sig = C.c_double(sigma) if precision == 64 else C.c_float(sigma)
+    _check(fn(_p(I), m, n, sig, C.byref(o), _p(U), _p(U2), _p(P)))
+    return U, U2

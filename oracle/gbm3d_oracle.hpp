@@ -943,4 +943,140 @@ int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout,
   return 0;
 }
 
+
+// ------------------------------------------------------ distributed driver
+// getBM3D_distribution (utilities/getBM3D_distribution.m:10-48): one case per
+// (tile row, tile column, spin), tiles of d0 = 256 + k with stride d0 - k; a
+// tile that reaches the last row/column is moved back so its size is a
+// multiple of k (at least 2k).  Rows {y1, y2, x1, x2, spin}, 1-based inclusive.
+struct TileCase {
+  int y1, y2, x1, x2, spin;
+};
+
+inline std::vector<TileCase> distribution(int m, int n, const int32_t* spins, int nspins, int k) {
+  const int d0 = 256 + k;
+  const int Xcnt = (n - k + (d0 - k) - 1) / (d0 - k), Ycnt = (m - k + (d0 - k) - 1) / (d0 - k);
+  std::vector<TileCase> R;
+  for (int i = 1; i <= Ycnt; ++i) {
+    const int indY = (i - 1) * (d0 - k);
+    for (int j = 1; j <= Xcnt; ++j) {
+      const int indX = (j - 1) * (d0 - k);
+      for (int s = 0; s < nspins; ++s) {
+        TileCase t{indY + 1, indY + d0, indX + 1, indX + d0, spins[s]};
+        if (t.x2 >= n) t.x1 = std::min(n - (n - t.x1 + 1 + k - 1) / k * k + 1, n - 2 * k + 1);
+        if (t.y2 >= m) t.y1 = std::min(m - (m - t.y1 + 1 + k - 1) / k * k + 1, m - 2 * k + 1);
+        R.push_back(t);
+      }
+    }
+  }
+  for (auto& t : R) {
+    t.y2 = std::min(t.y2, m);
+    t.x2 = std::min(t.x2, n);
+  }
+  return R;
+}
+
+// makeInterpWindow (utilities/makeInterpWindow.m:8-13): a x b ones with
+// linspace(.01, 1, k) ramps applied to the top rows, left columns, bottom rows
+// (flipped) and right columns (flipped), in that order; column-major.
+inline std::vector<double> interp_window(int a, int b, int k) {
+  std::vector<double> w((size_t)a * b, 1.0);
+  const std::vector<double> ramp = linspace(0.01, 1.0, k);
+  for (int c = 0; c < b; ++c)
+    for (int r = 0; r < k && r < a; ++r) w[r + (size_t)a * c] *= ramp[r];
+  for (int c = 0; c < k && c < b; ++c)
+    for (int r = 0; r < a; ++r) w[r + (size_t)a * c] *= ramp[c];
+  for (int c = 0; c < b; ++c)
+    for (int r = std::max(0, a - k); r < a; ++r) w[r + (size_t)a * c] *= ramp[a - 1 - r];
+  for (int c = std::max(0, b - k); c < b; ++c)
+    for (int r = 0; r < a; ++r) w[r + (size_t)a * c] *= ramp[b - 1 - c];
+  return w;
+}
+
+// [U, out] = GBM3D_distributed(I, sigma, opts) (source/GBM3D_distributed.m:1-122):
+// each case's tile (circshifted by its spin) runs matchBlocksCPU +
+// denoiseAndAggregateGPU, the unnormalised U and W are shifted back and
+// stitched with the tile's ramp window, U2 = sum(U w) / sum(w W); step 2 the
+// same with matchBlocksWie on the U2 tile and denoiseAndAggregateWieFast.
+// pilot != nullptr: skip step 1 and use it as U2.  Accumulation in double.
+template <class T>
+int gbm3d_distributed(const T* I, int m, int n, double sigma, const gbm3d_opts& o, T* Uout, T* U2out,
+                      const T* pilot) {
+  const int k = o.blockSize;
+  if (m < 256 + k || n < 256 + k) {  // :22-25
+    if (pilot) return GBM3D_ERR_ARG;
+    return gbm3d<T>(I, m, n, sigma, o, Uout, U2out, true);
+  }
+  if (o.filtType != GBM3D_FILT_HT && o.filtType != GBM3D_FILT_MEDIAN) return GBM3D_ERR_UNSUPPORTED;
+  if (o.numMax != o.numMin) return GBM3D_ERR_UNSUPPORTED;
+  if (o.Wiener && o.nMatchSpinsWie == GBM3D_UNSET) return GBM3D_ERR_MISSING_FIELD;
+  const int d2 = o.matchSize, K = o.numMax, bw = o.blockSizeWie;
+  std::vector<double> tau((size_t)o.levels * 7);
+  get_tau(o.levels, o.tauMode, sigma, tau.data());
+  const size_t mn = (size_t)m * n;
+  std::vector<double> num(mn), den(mn);
+  std::vector<T> U2(mn);
+  auto tile = [&](const T* src, const TileCase& t, int th, int tw, std::vector<T>& dst) {
+    dst.resize((size_t)th * tw);
+    for (int c = 0; c < tw; ++c)
+      for (int r = 0; r < th; ++r)
+        dst[r + (size_t)th * c] = src[(size_t)(t.y1 - 1 + pmod(r - t.spin, th)) + (size_t)m * (t.x1 - 1 + pmod(c - t.spin, tw))];
+  };
+  auto stitch = [&](const TileCase& t, int th, int tw, const std::vector<T>& U, const std::vector<T>& W) {
+    const std::vector<double> w = interp_window(th, tw, k);
+    for (int c = 0; c < tw; ++c)
+      for (int r = 0; r < th; ++r) {
+        const size_t src = pmod(r + t.spin, th) + (size_t)th * pmod(c + t.spin, tw);
+        const size_t dst = (size_t)(t.y1 - 1 + r) + (size_t)m * (t.x1 - 1 + c);
+        num[dst] += (double)U[src] * w[r + (size_t)th * c];
+        den[dst] += w[r + (size_t)th * c] * (double)W[src];
+      }
+  };
+  if (pilot) {
+    std::copy(pilot, pilot + mn, U2.begin());
+  } else {
+    for (const TileCase& t : distribution(m, n, o.matchSpins, o.nMatchSpins, k)) {  // :53-61
+      const int th = t.y2 - t.y1 + 1, tw = t.x2 - t.x1 + 1;
+      if (th % (1 << o.levels) || tw % (1 << o.levels)) return GBM3D_ERR_UNSUPPORTED;
+      std::vector<T> Il;
+      tile(I, t, th, tw, Il);
+      const int Mt = th / k, Nt = tw / k;
+      std::vector<uint32_t> S((size_t)K * Mt * Nt);
+      std::vector<T> Vol((size_t)th * tw * K), Vavg(Vol.size()), U((size_t)th * tw), W(U.size());
+      match_step1<T>(Il.data(), th, tw, k, d2, K, S.data(), Vol.data(), nullptr);
+      if (o.filtType == GBM3D_FILT_MEDIAN)
+        median3<T>(Vol.data(), th, tw, K, Vavg.data());
+      else
+        ht_filter<T>(Vol.data(), th, tw, K, o.levels, o.cycleSpin, tau.data(), Vavg.data(), nullptr);
+      aggregate_ht<T>(Vavg.data(), S.data(), th, tw, k, d2, K, U.data(), W.data(), true);
+      stitch(t, th, tw, U, W);  // :65-79
+    }
+    for (size_t e = 0; e < mn; ++e) U2[e] = (T)(num[e] / den[e]);
+  }
+  if (U2out) std::copy(U2.begin(), U2.end(), U2out);
+  if (!o.Wiener) {  // :81-84
+    std::copy(U2.begin(), U2.end(), Uout);
+    return 0;
+  }
+  std::fill(num.begin(), num.end(), 0.0);
+  std::fill(den.begin(), den.end(), 0.0);
+  for (const TileCase& t : distribution(m, n, o.matchSpinsWie, o.nMatchSpinsWie, k)) {  // :94-104
+    const int th = t.y2 - t.y1 + 1, tw = t.x2 - t.x1 + 1;
+    if (th % bw || tw % bw) return GBM3D_ERR_UNSUPPORTED;
+    std::vector<T> Il, Pl;
+    tile(I, t, th, tw, Il);
+    tile(U2.data(), t, th, tw, Pl);
+    const int Mt = th / bw, Nt = tw / bw;
+    const long G = (long)Mt * Nt;
+    std::vector<uint32_t> S((size_t)K * G);
+    std::vector<T> V1((size_t)bw * bw * K * G), V2(V1.size()), V2f(V1.size()), g2(G), U((size_t)th * tw), W(U.size());
+    match_step2<T>(Pl.data(), Il.data(), th, tw, bw, d2, K, S.data(), V1.data(), V2.data(), nullptr);
+    wiener<T>(V1.data(), V2.data(), bw, K, G, sigma, V2f.data(), g2.data());
+    aggregate_wie<T>(V2f.data(), g2.data(), S.data(), th, tw, bw, d2, K, U.data(), W.data(), true);
+    stitch(t, th, tw, U, W);  // :107-120
+  }
+  for (size_t e = 0; e < mn; ++e) Uout[e] = (T)(num[e] / den[e]);
+  return 0;
+}
+
 }  // namespace gbm3d_oracle

@@ -194,4 +194,36 @@ int orc_gbm3d_f32(const float* I, int m, int n, float sigma, const gbm3d_opts* r
   return guarded([&] { return gbm3d<float>(I, m, n, (double)sigma, *resolved, U, U1, true); });
 }
 
+
+int orc_distribution(int m, int n, const int32_t* spins, int nspins, int k, int32_t* rows, int max_rows) {
+  std::vector<TileCase> R = distribution(m, n, spins, nspins, k);
+  if ((int)R.size() > max_rows) return -(int)R.size();
+  for (size_t i = 0; i < R.size(); ++i) {
+    rows[5 * i + 0] = R[i].y1;
+    rows[5 * i + 1] = R[i].y2;
+    rows[5 * i + 2] = R[i].x1;
+    rows[5 * i + 3] = R[i].x2;
+    rows[5 * i + 4] = R[i].spin;
+  }
+  return (int)R.size();
+}
+
+int orc_interp_window(int a, int b, int k, double* w) {
+  if (a < 1 || b < 1 || k < 2) return GBM3D_ERR_ARG;
+  std::vector<double> v = interp_window(a, b, k);
+  std::copy(v.begin(), v.end(), w);
+  return 0;
+}
+
+int orc_gbm3d_distributed_f64(const double* I, int m, int n, double sigma, const gbm3d_opts* resolved, double* U,
+                              double* U2, const double* pilot) {
+  if (!I || !U || !resolved || m < 1 || n < 1) return GBM3D_ERR_ARG;
+  return guarded([&] { return gbm3d_distributed<double>(I, m, n, sigma, *resolved, U, U2, pilot); });
+}
+int orc_gbm3d_distributed_f32(const float* I, int m, int n, float sigma, const gbm3d_opts* resolved, float* U,
+                              float* U2, const float* pilot) {
+  if (!I || !U || !resolved || m < 1 || n < 1) return GBM3D_ERR_ARG;
+  return guarded([&] { return gbm3d_distributed<float>(I, m, n, sigma, *resolved, U, U2, pilot); });
+}
+
 }  // extern "C"

@@ -334,3 +334,51 @@ def test_median_branch_end_to_end_vs_oracle(ctx, m, n, sigma, seed):
     assert np.max(np.abs(U - U2)) <= 1e-3
     U64, _ = O.gbm3d(noisy, sigma, opts, precision=64)
     assert abs(synth.psnr(clean, U) - synth.psnr(clean, U64)) <= 0.01
+
+
+@pytest.mark.parametrize("m,n,opts", [
+    (600, 520, {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}),
+    (560, 300, None),
+])
+def test_distributed_vs_oracle(ctx, m, n, opts):
+    """GBM3D_distributed (source/GBM3D_distributed.m): tiles of 256 + blockSize,
+    per-case step 1 / step 2, ramp-window stitching.  Chained like the
+    single-image test: U2 vs the FP32 restatement, U vs the restatement's step 2
+    on the GPU's own U2, PSNR vs the FP64 restatement."""
+    clean, noisy = synth.make_pair(400 + m, m, n, 25.0)
+    U, out = ctx.denoise_distributed(noisy, 25.0, opts)
+    U32, U232 = O.gbm3d_distributed(noisy, 25.0, opts, precision=32)
+    assert np.max(np.abs(out["U2"] - U232)) <= 1e-3
+    Uc, _ = O.gbm3d_distributed(noisy, 25.0, opts, precision=32, pilot=out["U2"])
+    assert np.max(np.abs(U - Uc)) <= 1e-3
+    U64, _ = O.gbm3d_distributed(noisy, 25.0, opts, precision=64)
+    assert abs(synth.psnr(clean, U) - synth.psnr(clean, U64)) <= 0.01
+
+
+def test_distributed_single_tile_and_small_images_are_gbm3d(ctx):
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
+    _, noisy = synth.make_pair(90, 264, 264, 25.0)      # exactly one tile
+    U, out = ctx.denoise_distributed(noisy, 25.0, opts)
+    V, o1 = ctx.denoise(noisy, 25.0, opts)
+    assert np.max(np.abs(U - V)) <= 1e-4 and np.max(np.abs(out["U2"] - o1["U1"])) <= 1e-4
+    _, small = synth.make_pair(91, 200, 300, 25.0)      # below 256 + blockSize: GBM3D itself
+    U, out = ctx.denoise_distributed(small, 25.0, opts)
+    V, o1 = ctx.denoise(small, 25.0, opts)
+    assert np.array_equal(U, V) and np.array_equal(out["U2"], o1["U1"])
+
+
+def test_distributed_graph_replays_match_eager_run(ctx):
+    """The distributed driver's cases are captured into a graph and replayed;
+    replays interleaved with other work all equal the eager (stage-timed, serial) run."""
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
+    _, noisy = synth.make_pair(92, 544, 288, 20.0)
+    _, other = synth.make_pair(93, 96, 96, 20.0)
+    ctx.set_timing(True)
+    try:
+        U_eager, o_eager = ctx.denoise_distributed(noisy, 20.0, opts)
+    finally:
+        ctx.set_timing(False)
+    for _ in range(3):
+        U, out = ctx.denoise_distributed(noisy, 20.0, opts)
+        assert np.array_equal(U, U_eager) and np.array_equal(out["U2"], o_eager["U2"])
+        ctx.denoise(other, 20.0, opts)

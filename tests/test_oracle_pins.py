@@ -238,3 +238,43 @@ def test_median3_matches_scipy_median_filter():
     V = rng.normal(0, 1, (13, 9, 8))
     V[::3] = np.round(V[::3])
     assert np.array_equal(O.median3(V), ndimage.median_filter(V, size=3, mode="nearest"))
+
+
+def test_distribution_matches_hand_worked_cases():
+    """getBM3D_distribution (utilities/getBM3D_distribution.m:10-48), worked by hand."""
+    R = O.distribution(600, 520, [0], 8)
+    assert R.tolist() == [[1, 264, 1, 264, 0], [1, 264, 257, 520, 0], [257, 520, 1, 264, 0],
+                          [257, 520, 257, 520, 0], [513, 600, 1, 264, 0], [513, 600, 257, 520, 0]]
+    R = O.distribution(1000, 1000, [0, 4], 16)
+    assert R[-1].tolist() == [761, 1000, 761, 1000, 4]     # last tile moved back to 15 blocks
+    assert len(R) == 4 * 4 * 2
+    for m, n, k in [(1000, 777, 16), (300, 900, 8), (2048, 2048, 8)]:
+        R = O.distribution(m, n, [0], k)
+        cover = np.zeros((m, n), bool)
+        for y1, y2, x1, x2, _ in R:
+            assert (y2 - y1 + 1) % k == 0 and (x2 - x1 + 1) % k == 0 and y2 <= m and x2 <= n
+            cover[y1 - 1:y2, x1 - 1:x2] = True
+        assert cover.all()
+
+
+def test_interp_window_closed_form():
+    """makeInterpWindow (utilities/makeInterpWindow.m:8-13) on a 6 x 5 tile, k = 3."""
+    w = O.interp_window(6, 5, 3)
+    ramp = np.array([0.01, 0.505, 1.0])
+    rows = np.ones(6)
+    rows[:3] *= ramp
+    rows[3:] *= ramp[::-1]
+    cols = np.ones(5)
+    cols[:3] *= ramp
+    cols[2:] *= ramp[::-1]
+    assert np.allclose(w, np.outer(rows, cols), rtol=1e-15)
+
+
+def test_distributed_single_tile_is_gbm3d():
+    """An image of exactly 256 + blockSize is one tile: the window cancels in
+    sum(U w)/sum(W w), so GBM3D_distributed = GBM3D (no padding needed)."""
+    clean, noisy = synth.make_pair(77, 264, 264, 25.0)
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
+    U, U2 = O.gbm3d_distributed(noisy, 25.0, opts, precision=64)
+    V, V1 = O.gbm3d(noisy, 25.0, opts, precision=64)
+    assert np.max(np.abs(U2 - V1)) < 1e-9 and np.max(np.abs(U - V)) < 1e-9
