@@ -192,6 +192,20 @@ class Context:
                                      C.byref(o) if o is not None else None, ptr(U), ptr(U1)))
         return U, {"U1": U1}
 
+    def denoise_colored(self, I, sigma_psd, sigma_wie, opts: dict | None = None):
+        """[U, out] = GBM3D(I, sigmaPSD, opts) for coloured noise; sigma_wie is
+        getColoredWie(sigmaPSD, blockSizeWie) (blockSizeWie x blockSizeWie)."""
+        I = np.asfortranarray(I, dtype=np.float32)
+        m, n = I.shape
+        psd = np.asfortranarray(sigma_psd, dtype=np.float64)
+        sw = None if sigma_wie is None else np.asfortranarray(sigma_wie, dtype=np.float64)
+        U = np.zeros((m, n), dtype=np.float32, order="F")
+        U1 = np.zeros_like(U)
+        o = opts_to_struct(opts)
+        check(self.lib.gbm3d_denoise_colored(self.handle, ptr(I), m, n, ptr(psd), psd.shape[0], psd.shape[1],
+                                             ptr(sw), C.byref(o) if o is not None else None, ptr(U), ptr(U1)))
+        return U, {"U1": U1}
+
     def denoise_distributed(self, I, sigma: float, opts: dict | None = None):
         """[U, out] = GBM3D_distributed(I, sigma, opts): out["U2"] is the first estimate."""
         I = np.asfortranarray(I, dtype=np.float32)
@@ -327,9 +341,15 @@ def default_context() -> Context:
     return _default
 
 
-def GBM3D(I, sigma: float, opts: dict | None = None):
-    """[U, out] = GBM3D(I, sigma, opts)  (source/GBM3D.m:1-84); out['U1'] is the pilot."""
-    return default_context().denoise(I, sigma, opts)
+def GBM3D(I, sigma, opts: dict | None = None, sigma_wie=None):
+    """[U, out] = GBM3D(I, sigma, opts)  (source/GBM3D.m:1-84); out['U1'] is the pilot.
+
+    sigma may be a matrix sigmaPSD (coloured noise, :29-36); then sigma_wie =
+    getColoredWie(sigmaPSD, blockSizeWie) must be given (it needs makeGausPSF,
+    which the reference does not contain)."""
+    if np.ndim(sigma) == 2 and np.size(sigma) > 1:
+        return default_context().denoise_colored(I, sigma, sigma_wie, opts)
+    return default_context().denoise(I, float(np.asarray(sigma).ravel()[0]), opts)
 
 
 def GBM3D_distributed(I, sigma: float, opts: dict | None = None):

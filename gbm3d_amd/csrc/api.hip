@@ -36,6 +36,10 @@ struct gbm3d_ctx {
   double tuned_ms = 0.0;
   std::vector<int64_t> tkey;
   int match_waves = 1;  // reference tiles per matching workgroup (single-image path)
+  // coloured noise of the current call (gbm3d_denoise_colored)
+  bool colored = false;
+  std::vector<double> ctau;    // levels x 7
+  std::vector<float> wie_s2;   // blockSizeWie^2: sigmaWie.^2
   // learned shrinkage
   uint16_t* shrink_wb = nullptr;  // BF16 weights [4][64][192]
   float* shrink_fb = nullptr;     // FP32 biases and sigma-channel weights
@@ -290,7 +294,10 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
   carve(cv, p, o, lanes, &b);
   const size_t MN = p.MN();
   std::vector<double> tau((size_t)o.levels * 7);
-  tau_table(o.levels, o.tauMode, (double)sigma, tau.data());
+  if (c->colored)
+    tau = c->ctau;  // getColoredTau (source/GBM3D.m:33-35)
+  else
+    tau_table(o.levels, o.tauMode, (double)sigma, tau.data());
   const float* masks1 = c->masks + kMaskStride * mask_slot(p.bs);
   // spin ii of a step with `nl` lanes runs on lane ii % nl
   auto lane_stream = [&](int ii, int nl) { return nl > 1 ? c->bstreams[ii % nl] : st; };
@@ -371,7 +378,7 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
     {
       StageTimer t(c, GBM3D_STAGE_WIENER);
       GBM3D_TRY(launch_wiener(sb.pad1, sb.pad2, sb.S2, nullptr, nullptr, p.M, p.N, p.bw, p.d2, p.K, G2, sigma,
-                              sb.V2f, sb.g2, ls));
+                              sb.V2f, sb.g2, ls, c->colored ? c->wie_s2.data() : nullptr));
     }
     {
       StageTimer t(c, GBM3D_STAGE_AGG2);
@@ -1546,6 +1553,38 @@ int gbm3d_denoise_distributed(gbm3d_ctx* c, const float* I, int m, int n, float 
   if (U2) GBM3D_TRY(download(c, U2, dU2, mn));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
   return GBM3D_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------- coloured noise
+extern "C" {
+
+int gbm3d_denoise_colored(gbm3d_ctx* c, const float* I, int m, int n, const double* sigmaPSD, int d1, int d2,
+                          const double* sigmaWie, const gbm3d_opts* opts, float* U, float* U1) {
+  GBM3D_TRY(set_dev(c));
+  if (!I || !U || !sigmaPSD || m < 1 || n < 1 || d1 < 1 || d2 < 1) return fail(GBM3D_ERR_ARG, "NULL or empty input");
+  if (c->ext.learned) return fail(GBM3D_ERR_UNSUPPORTED, "coloured noise: the learned form is not supported");
+  gbm3d_opts o;
+  GBM3D_TRY(resolve_checked(opts, &o));
+  if (o.Wiener && !sigmaWie)
+    return fail(GBM3D_ERR_UNSUPPORTED, "coloured noise: sigmaWie (getColoredWie) must be given");
+  std::vector<double> tau((size_t)o.levels * 7);
+  colored_tau(o.levels, sigmaPSD, d1, d2, tau.data());
+  std::vector<float> s2;
+  if (o.Wiener) {
+    const int bw = o.blockSizeWie;
+    s2.resize((size_t)bw * bw);
+    for (int e = 0; e < bw * bw; ++e) s2[e] = (float)(sigmaWie[e] * sigmaWie[e]);
+  }
+  c->colored = true;
+  c->ctau = tau;
+  c->wie_s2 = s2;
+  // the scalar sigma is not read on this path (matching uses it only for the
+  // inactive tolerance branch, matchBlocksCPU.m:20-22)
+  const int st = gbm3d_denoise(c, I, m, n, 1.0f, &o, U, U1);
+  c->colored = false;
+  return st;
 }
 
 }  // extern "C"

@@ -119,6 +119,51 @@ void tau_table(int levels, int mode, double sigma, double* tau) {
     }
 }
 
+// getColoredTau (utilities/getColoredTau.m:9-57): for level i and band j,
+// tau = 6 sqrt(sum(PSD .* |F|^2) / sum(|F|^2)), F the d1 x d2 DFT of the band's
+// 2-D analysis filter (outer product of the 10-tap low / 2-tap high filters,
+// nearest-neighbour upsampled by 2^(i-1)).  The filter is separable, so
+// |F|^2 = |R(k1)|^2 |C(k2)|^2 and both sums factor per axis.
+void colored_tau(int levels, const double* psd, int d1, int d2, double* tau) {
+  const double Ca = kCa, Cb = kCb, Cc = kCc;
+  const double lo[10] = {Ca, -Ca, -Cb, Cb, Cc, Cc, Cb, -Cb, -Ca, Ca};
+  const double hi[10] = {0, 0, 0, 0, -Cc, Cc, 0, 0, 0, 0};
+  // band j -> (rows filter, columns filter): Psi(:,:,j+1) = R'*C, 1 = low, 0 = high
+  const int rlo[7] = {0, 1, 1, 0, 0, 1, 0}, clo[7] = {1, 0, 1, 0, 1, 0, 0};
+  auto power = [](const double* f, int s, int n) {  // |DFT_n(kron(f, ones(s)))|^2 (fft2 pads or truncates)
+    std::vector<double> p(n);
+    const int L = std::min(10 * s, n);
+    for (int k = 0; k < n; ++k) {
+      double re = 0.0, im = 0.0;
+      for (int t = 0; t < L; ++t) {
+        const double a = -2.0 * M_PI * (double)(((long long)k * t) % n) / n;
+        re += f[t / s] * std::cos(a);
+        im += f[t / s] * std::sin(a);
+      }
+      p[k] = re * re + im * im;
+    }
+    return p;
+  };
+  for (int i = 0; i < levels; ++i) {
+    const int s = 1 << i;
+    const std::vector<double> Rl = power(lo, s, d1), Rh = power(hi, s, d1), Cl = power(lo, s, d2),
+                              Ch = power(hi, s, d2);
+    for (int j = 0; j < 7; ++j) {
+      const std::vector<double>& R = rlo[j] ? Rl : Rh;
+      const std::vector<double>& C = clo[j] ? Cl : Ch;
+      double num = 0.0, sr = 0.0, sc = 0.0;
+      for (int k2 = 0; k2 < d2; ++k2) {
+        double col = 0.0;
+        for (int k1 = 0; k1 < d1; ++k1) col += psd[k1 + (size_t)d1 * k2] * R[k1];
+        num += col * C[k2];
+      }
+      for (double v : R) sr += v;
+      for (double v : C) sc += v;
+      tau[i + levels * j] = 6.0 * std::sqrt(num / (sr * sc));
+    }
+  }
+}
+
 }  // namespace gbm3d
 
 extern "C" {
@@ -146,6 +191,13 @@ int gbm3d_resolve_opts(const gbm3d_opts* in, gbm3d_opts* out) { return gbm3d::re
 int gbm3d_get_tau(int levels, int mode, double sigma, double* tau) {
   if (levels < 1 || !tau) return gbm3d::fail(GBM3D_ERR_ARG, "get_tau: levels < 1 or NULL");
   gbm3d::tau_table(levels, mode, sigma, tau);
+  return GBM3D_OK;
+}
+
+int gbm3d_get_colored_tau(int levels, const double* sigmaPSD, int d1, int d2, double* tau) {
+  if (levels < 1 || !sigmaPSD || !tau || d1 < 1 || d2 < 1)
+    return gbm3d::fail(GBM3D_ERR_ARG, "get_colored_tau: bad argument");
+  gbm3d::colored_tau(levels, sigmaPSD, d1, d2, tau);
   return GBM3D_OK;
 }
 

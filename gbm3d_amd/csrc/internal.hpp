@@ -44,6 +44,8 @@ int check_launch(const char* what);
 // ------------------------------------------------------------------ options
 int resolve(const gbm3d_opts* in, gbm3d_opts* out);
 void tau_table(int levels, int mode, double sigma, double* tau);
+// Coloured-noise thresholds (levels x 7, column-major) from sigma^2 * PSD (d1 x d2).
+void colored_tau(int levels, const double* psd, int d1, int d2, double* tau);
 
 // ------------------------------------------------------------ the filters
 // bior1.5 (x, y) x Haar (z) of waveTransforms3D/getWaveFilters3Dbior.m:16-27,49-50.
@@ -114,9 +116,10 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
                        int M, int N, int d, int d2, int K, int spin, float* U, float* W, hipStream_t st);
 // Step-2 Wiener filter on groups gathered from the padded pilot / noisy images
 // (pads != nullptr) or read from V1/V2 arrays.
+// sig2tab (host, d x d, nullable): per-coefficient noise variance (coloured noise).
 int launch_wiener(const float* pad_pilot, const float* pad_noisy, const uint32_t* S, const float* V1,
                   const float* V2, int M, int N, int d, int d2, int K, long G, float sigma, float* V2f,
-                  float* gain2, hipStream_t st);
+                  float* gain2, hipStream_t st, const float* sig2tab = nullptr);
 // out = sum_s U_s / (sum_s W_s + sum_t Wextra_t), cropped to m x n, double math.
 int launch_normalize(const float* U, int nU, const float* W, int nW, const float* Wextra, int nWextra,
                      int M, int N, int m, int n, float* out, hipStream_t st);

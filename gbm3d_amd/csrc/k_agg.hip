@@ -222,10 +222,13 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
             }
           }
         }
-        wave_sync();
+        // no barrier between blocks: LDS operations of one wave execute in
+        // program order, and the compiler keeps these possibly-aliasing
+        // read-modify-writes in order, so each pixel sums in (y, x, k) order
       }
     }
   }
+  wave_sync();
   for (int e = lane; e < AT * AT; e += 64) {
     const int pr = R0 + (e % AT), pc = C0 + (e / AT);
     if (pr >= M || pc >= N) continue;

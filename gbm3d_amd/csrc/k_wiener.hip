@@ -18,11 +18,19 @@
 
 namespace gbm3d {
 
+// Noise variance per DCT coefficient (u, v): sigma^2 everywhere for white
+// noise, sigmaWie(u, v)^2 for coloured noise (the implicit expansion of the
+// d x d sigma in denoiseAndAggregateWieFast.m:49).
+template <int D>
+struct Sig2Tab {
+  float s[D * D];
+};
+
 template <int D, int K, bool GATHER>
 __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ padP, const float* __restrict__ padN,
                                                      const uint32_t* __restrict__ S, const float* __restrict__ V1,
                                                      const float* __restrict__ V2, int M, int N, int d2, long G,
-                                                     float sigma2, DctTable<D> tab, float* __restrict__ V2f,
+                                                     Sig2Tab<D> sig2, DctTable<D> tab, float* __restrict__ V2f,
                                                      float* __restrict__ gain2) {
   constexpr int TPG = D * D, GPB = 256 / TPG;
   constexpr int L = K < 16 ? 2 : 3;
@@ -63,6 +71,7 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
   page_transform<D, K, false>(c2, sm, T, u, v);
   haar_dec<K, L>(c2);
   float n2 = 0.f;
+  const float sigma2 = sig2.s[t];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const float a2 = c1[k] * c1[k];
@@ -102,11 +111,13 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
 
 template <int D, int K>
 static int launch_wiener_t(const float* padP, const float* padN, const uint32_t* S, const float* V1, const float* V2,
-                           int M, int N, int d2, long G, float sigma, float* V2f, float* gain2, hipStream_t st) {
+                           int M, int N, int d2, long G, float sigma, const float* sig2tab, float* V2f, float* gain2,
+                           hipStream_t st) {
   const DctTable<D> tab = make_dct_table<D>();
   constexpr int GPB = 256 / (D * D);
   dim3 grid((unsigned)((G + GPB - 1) / GPB));
-  const float s2 = sigma * sigma;
+  Sig2Tab<D> s2;
+  for (int e = 0; e < D * D; ++e) s2.s[e] = sig2tab ? sig2tab[e] : sigma * sigma;
   if (padP)
     hipLaunchKernelGGL((wiener_kernel<D, K, true>), grid, dim3(256), 0, st, padP, padN, S, V1, V2, M, N, d2, G, s2,
                        tab, V2f, gain2);
@@ -117,11 +128,12 @@ static int launch_wiener_t(const float* padP, const float* padN, const uint32_t*
 }
 
 int launch_wiener(const float* pad_pilot, const float* pad_noisy, const uint32_t* S, const float* V1, const float* V2,
-                  int M, int N, int d, int d2, int K, long G, float sigma, float* V2f, float* gain2, hipStream_t st) {
+                  int M, int N, int d, int d2, int K, long G, float sigma, float* V2f, float* gain2, hipStream_t st,
+                  const float* sig2tab) {
   if (G <= 0) return GBM3D_OK;
-#define GBM3D_WIE_CASE(DD, KK)                                                                          \
-  if (d == DD && K == KK)                                                                               \
-    return launch_wiener_t<DD, KK>(pad_pilot, pad_noisy, S, V1, V2, M, N, d2, G, sigma, V2f, gain2, st);
+#define GBM3D_WIE_CASE(DD, KK)                                                                                  \
+  if (d == DD && K == KK)                                                                                       \
+    return launch_wiener_t<DD, KK>(pad_pilot, pad_noisy, S, V1, V2, M, N, d2, G, sigma, sig2tab, V2f, gain2, st);
   GBM3D_WIE_CASE(8, 16)
   GBM3D_WIE_CASE(8, 8)
   GBM3D_WIE_CASE(8, 32)

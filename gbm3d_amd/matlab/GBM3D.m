@@ -2,10 +2,20 @@ function [U,out] = GBM3D(I,sigma,opts)
 % [U,out] = GBM3D(I,sigma,opts) -- drop-in for source/GBM3D.m of the
 % reference, running on an AMD MI355X through libgbm3d (gbm3d_mex).
 % I: m x n real image (double or single), sigma: scalar noise level on the
-% image's scale, opts: optional struct (see setBM3Dopts / checkBM3Dopts).
+% image's scale, or a matrix sigmaPSD = sigma^2*PSD for coloured noise,
+% opts: optional struct (see setBM3Dopts / checkBM3Dopts).
 % U: denoised image (double), out.U1: the first-step estimate.
 if nargin < 3
-    [U,out] = gbm3d_mex(I,sigma);
+    opts.profile = 'default';
+end
+if size(sigma,1) > 1
+    % coloured noise (source/GBM3D.m:33-35): the per-coefficient Wiener sigma
+    % comes from the reference's getColoredWie (it needs makeGausPSF)
+    o = opts;
+    if isfield(o,'profile'), o = setBM3Dopts(o.profile); end
+    o = checkBM3Dopts(o);
+    sigmaWie = getColoredWie(sigma,o.blockSizeWie);
+    [U,out] = gbm3d_mex('GBM3D_colored',I,sigma,sigmaWie,opts);
 else
     [U,out] = gbm3d_mex(I,sigma,opts);
 end

@@ -8,6 +8,7 @@
 //     opts     = gbm3d_mex('setBM3Dopts', profile)
 //     opts     = gbm3d_mex('checkBM3Dopts', opts)
 //     [U, out] = gbm3d_mex('GBM3D_distributed', I, sigma, opts)   % source/GBM3D_distributed.m:1
+//     [U, out] = gbm3d_mex('GBM3D_colored', I, sigmaPSD, sigmaWie, opts)   % source/GBM3D.m:33-35
 // I may be double or single (m x n, real); U and out.U1 come back as double,
 // as GBM3D.m returns them.  Library errors become MATLAB errors with id
 // gbm3d:<status>.  One library context (device 0, its own stream) lives for
@@ -178,11 +179,40 @@ void distributed(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   }
 }
 
+void colored(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
+  if (nrhs < 4) mexErrMsgIdAndTxt("gbm3d:arg", "usage: gbm3d_mex('GBM3D_colored', I, sigmaPSD, sigmaWie, opts)");
+  int m, n;
+  std::vector<float> in = to_image(prhs[1], &m, &n);
+  if (!mxIsDouble(prhs[2]) || !mxIsDouble(prhs[3])) mexErrMsgIdAndTxt("gbm3d:arg", "sigmaPSD, sigmaWie: double");
+  const int d1 = (int)mxGetM(prhs[2]), d2 = (int)mxGetN(prhs[2]);
+  gbm3d_opts o;
+  const gbm3d_opts* op = nullptr;
+  if (nrhs >= 5) {
+    to_opts(prhs[4], &o);
+    op = &o;
+  }
+  ensure_ctx();
+  std::vector<float> U(in.size()), U1(in.size());
+  int st = gbm3d_denoise_colored(g_ctx, in.data(), m, n, mxGetPr(prhs[2]), d1, d2, mxGetPr(prhs[3]), op, U.data(),
+                                 U1.data());
+  if (st) fail(st);
+  plhs[0] = to_double(U, m, n);
+  if (nlhs > 1) {
+    const char* f[] = {"U1"};
+    plhs[1] = mxCreateStructMatrix(1, 1, 1, f);
+    mxSetField(plhs[1], 0, "U1", to_double(U1, m, n));
+  }
+}
+
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   if (nrhs >= 1 && mxIsChar(prhs[0])) {
     char cmd[32] = {0};
     mxGetString(prhs[0], cmd, sizeof(cmd));
     gbm3d_opts o;
+    if (!std::strcmp(cmd, "GBM3D_colored")) {
+      colored(nlhs, plhs, nrhs, prhs);
+      return;
+    }
     if (!std::strcmp(cmd, "GBM3D_distributed")) {
       distributed(nlhs, plhs, nrhs, prhs);
       return;

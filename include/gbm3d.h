@@ -154,6 +154,23 @@ int gbm3d_denoise_distributed_device(gbm3d_ctx* ctx, const float* dI, int m,
                                      const gbm3d_opts* opts, float* dU,
                                      float* dU2);
 
+/*
+ * Coloured noise: [U, out] = GBM3D(I, sigmaPSD, opts) with a matrix sigma --
+ * source/GBM3D.m:29-36.  sigmaPSD: d1 x d2 (sigma^2 x the noise power
+ * spectrum, column-major); thresholds from getColoredTau
+ * (utilities/getColoredTau.m); sigmaWie: blockSizeWie^2 per-coefficient
+ * Wiener sigma of getColoredWie (utilities/getColoredWie.m), required when
+ * opts.Wiener (it needs makeGausPSF, which the reference does not contain,
+ * so the caller computes it).  Host buffers.
+ */
+int gbm3d_denoise_colored(gbm3d_ctx* ctx, const float* I, int m, int n,
+                          const double* sigmaPSD, int d1, int d2,
+                          const double* sigmaWie, const gbm3d_opts* opts,
+                          float* U, float* U1);
+/* tau = getColoredTau(levels, sigmaPSD) (levels x 7, column-major). */
+int gbm3d_get_colored_tau(int levels, const double* sigmaPSD, int d1, int d2,
+                          double* tau);
+
 /* ------------------------------------------------------------- batches
  * [U_i, out_i] = GBM3D(I_i, sigma_i, opts) for `count` independent images
  * (device buffers, sizes m[i] x n[i]) sharing one opts.  Images run on

@@ -170,7 +170,8 @@ def aggregate_ht(Vavg, S, d, d2, precision=64, gather=True):
     return U, W
 
 
-def wiener(V1, V2, sigma, precision=64):
+def wiener(V1, V2, sigma, precision=64, sigma_wie=None):
+    """sigma_wie (d x d): per-coefficient sigma (coloured noise) instead of sigma."""
     dt, suf = _dt(precision)
     V1 = _f(V1, dt)
     V2 = _f(V2, dt)
@@ -178,7 +179,12 @@ def wiener(V1, V2, sigma, precision=64):
     G = V1.size // (d * d * K)
     V2f = np.zeros_like(V2)
     g2 = np.zeros(V1.shape[3:], dtype=dt, order="F")
-    _check(getattr(lib(), "orc_wiener_" + suf)(_p(V1), _p(V2), d, K, C.c_long(G), C.c_double(sigma), _p(V2f), _p(g2)))
+    if sigma_wie is None:
+        _check(getattr(lib(), "orc_wiener_" + suf)(_p(V1), _p(V2), d, K, C.c_long(G), C.c_double(sigma), _p(V2f),
+                                                   _p(g2)))
+    else:
+        sw = _f(sigma_wie, np.float64)
+        _check(getattr(lib(), "orc_wiener_tab_" + suf)(_p(V1), _p(V2), d, K, C.c_long(G), _p(sw), _p(V2f), _p(g2)))
     return V2f, g2
 
 
@@ -240,7 +246,7 @@ def gbm3d(I, sigma, opts=None, precision=64, nthreads=0):
     return U, U1
 
 
-def step2_from_pilot(pilot, noisy, sigma, opts=None, precision=32):
+def step2_from_pilot(pilot, noisy, sigma, opts=None, precision=32, sigma_wie=None):
     """Step 2 of GBM3D (source/GBM3D.m:70-83) run on a given pilot U1.
 
     pilot, noisy: M x N (already block-multiple sizes).  Composes the oracle's
@@ -263,7 +269,7 @@ def step2_from_pilot(pilot, noisy, sigma, opts=None, precision=32):
         P = np.roll(pilot, (sp, sp), (0, 1))
         I = np.roll(noisy, (sp, sp), (0, 1))
         S, V1, V2, _ = match_wie(P, I, d, d2, K, precision=precision)
-        V2f, g2 = wiener(V1, V2, sigma, precision=precision)
+        V2f, g2 = wiener(V1, V2, sigma, precision=precision, sigma_wie=sigma_wie)
         U, W = aggregate_wie(V2f, g2, S, M, N, d2, precision=precision)
         Us += np.roll(U.astype(np.float64), (-sp, -sp), (0, 1))
         Ws += np.roll(W.astype(np.float64), (-sp, -sp), (0, 1))
@@ -302,3 +308,28 @@ def gbm3d_distributed(I, sigma, opts=None, precision=64, pilot=None):
     sig = C.c_double(sigma) if precision == 64 else C.c_float(sigma)
     _check(fn(_p(I), m, n, sig, C.byref(o), _p(U), _p(U2), _p(P)))
     return U, U2
+
+
+def colored_tau(levels, psd):
+    psd = _f(psd, np.float64)
+    d1, d2 = psd.shape
+    tau = np.zeros((levels, 7), dtype=np.float64, order="F")
+    _check(lib().orc_colored_tau(levels, _p(psd), d1, d2, _p(tau)))
+    return tau
+
+
+def gbm3d_colored(I, psd, sigma_wie, opts=None, precision=64):
+    """[U, out.U1] = GBM3D(I, sigmaPSD, opts) with the caller's getColoredWie output."""
+    dt, suf = _dt(precision)
+    I = _f(I, dt)
+    m, n = I.shape
+    psd = _f(psd, np.float64)
+    sw = None if sigma_wie is None else _f(sigma_wie, np.float64)
+    o = Opts()
+    src = opts_to_struct(opts)
+    lib().orc_resolve_opts(C.byref(src) if src is not None else None, C.byref(o))
+    U = np.zeros((m, n), dtype=dt, order="F")
+    U1 = np.zeros_like(U)
+    fn = getattr(lib(), "orc_gbm3d_colored_" + suf)
+    _check(fn(_p(I), m, n, _p(psd), psd.shape[0], psd.shape[1], _p(sw), C.byref(o), _p(U), _p(U1)))
+    return U, U1

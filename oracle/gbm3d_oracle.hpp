@@ -711,11 +711,15 @@ void haar_rec(T* x, int R, int levels) {
 }
 
 // denoiseAndAggregateWieFast.m:19-66 on G groups (V: d x d x K x G).
+// sigWie (d x d, nullable): per-coefficient sigma (coloured noise; the d x d
+// sigma of :49 expands over the slices and groups).
 template <class T>
-void wiener(const T* V1, const T* V2, int d, int K, long G, double sigma, T* V2f, T* gain2) {
+void wiener(const T* V1, const T* V2, int d, int K, long G, double sigma, T* V2f, T* gain2,
+            const double* sigWie = nullptr) {
   const int levels = K < 16 ? 2 : 3;  // :23-24
   std::vector<T> Tm = dct_matrix<T>(d);
-  const T s2 = (T)(sigma * sigma);
+  std::vector<T> s2tab((size_t)d * d);
+  for (int e = 0; e < d * d; ++e) s2tab[e] = sigWie ? (T)(sigWie[e] * sigWie[e]) : (T)(sigma * sigma);
   const size_t gs = (size_t)d * d * K;
 #pragma omp parallel
   {
@@ -760,7 +764,7 @@ void wiener(const T* V1, const T* V2, int d, int K, long G, double sigma, T* V2f
       T n2 = 0;
       for (size_t e = 0; e < gs; ++e) {
         T a2 = A[e] * A[e];
-        T f = a2 / (a2 + s2);  // :49
+        T f = a2 / (a2 + s2tab[e % (d * d)]);  // :49
         int k = (int)(e / (d * d));
         if (e % (d * d) == 0 && k >= 14) f = 1;  // filt(1,1,15:end) = 1 (:50)
         B[e] *= f;
@@ -855,8 +859,43 @@ void median3(const T* V, int m, int n, int K, T* out) {
       }
 }
 
+// Coloured-noise thresholds (utilities/getColoredTau.m:9-57): 6 sqrt(sum(PSD
+// |F|^2)/sum(|F|^2)) with F the 2-D DFT (d1 x d2) of the band's upsampled
+// 2-D filter, evaluated directly as a 2-D DFT of the filter (no separation).
+inline void colored_tau(int levels, const double* psd, int d1, int d2, double* tau) {
+  const double lo[10] = {0.016572815184060, -0.016572815184060, -0.121533978016438, 0.121533978016438,
+                         0.70710678118654752440, 0.70710678118654752440, 0.121533978016438, -0.121533978016438,
+                         -0.016572815184060, 0.016572815184060};
+  const double hi[10] = {0, 0, 0, 0, -0.70710678118654752440, 0.70710678118654752440, 0, 0, 0, 0};
+  const double* rows[7] = {hi, lo, lo, hi, hi, lo, hi};  // Psi(:,:,j+1) = rows' * cols
+  const double* cols[7] = {lo, hi, lo, hi, lo, hi, hi};
+  for (int i = 0; i < levels; ++i) {
+    const int s = 1 << i, L1 = std::min(10 * s, d1), L2 = std::min(10 * s, d2);
+    for (int j = 0; j < 7; ++j) {
+      double num = 0.0, den = 0.0;
+      for (int k2 = 0; k2 < d2; ++k2)
+        for (int k1 = 0; k1 < d1; ++k1) {
+          double re = 0.0, im = 0.0;
+          for (int t2 = 0; t2 < L2; ++t2)
+            for (int t1 = 0; t1 < L1; ++t1) {
+              const double v = rows[j][t1 / s] * cols[j][t2 / s];
+              if (v == 0.0) continue;
+              const double a = -2.0 * M_PI * ((double)k1 * t1 / d1 + (double)k2 * t2 / d2);
+              re += v * std::cos(a);
+              im += v * std::sin(a);
+            }
+          const double p = re * re + im * im;
+          num += psd[k1 + (size_t)d1 * k2] * p;
+          den += p;
+        }
+      tau[i + levels * j] = 6.0 * std::sqrt(num / den);
+    }
+  }
+}
+
 template <class T>
-int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout, T* U1out, bool gather) {
+int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout, T* U1out, bool gather,
+          const double* tau_in = nullptr, const double* sigWie = nullptr) {
   if (o.filtType != GBM3D_FILT_HT && o.filtType != GBM3D_FILT_MEDIAN) return GBM3D_ERR_UNSUPPORTED;
   if (o.numMax != o.numMin) return GBM3D_ERR_UNSUPPORTED;
   if (o.Wiener && o.nMatchSpinsWie == GBM3D_UNSET) return GBM3D_ERR_MISSING_FIELD;
@@ -864,7 +903,10 @@ int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout,
   const int M = (m + bs - 1) / bs * bs, N = (n + bs - 1) / bs * bs;  // :23-24
   std::vector<T> I = mirror_pad<T>(I0, m, n, M, N);
   std::vector<double> tau((size_t)o.levels * 7);
-  get_tau(o.levels, o.tauMode, sigma, tau.data());
+  if (tau_in)
+    std::copy(tau_in, tau_in + tau.size(), tau.begin());
+  else
+    get_tau(o.levels, o.tauMode, sigma, tau.data());
   const int n1 = o.nMatchSpins, n2 = o.Wiener ? o.nMatchSpinsWie : 0;
   const int nsl = std::max(n1, n2);
   const size_t MN = (size_t)M * N;
@@ -918,7 +960,7 @@ int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout,
       std::vector<T> Ps = circshift2<T>(U1.data(), M, N, sp);
       std::vector<T> Is = circshift2<T>(I.data(), M, N, sp);
       match_step2<T>(Ps.data(), Is.data(), M, N, d, d2, K, S.data(), V1.data(), V2.data(), nullptr);
-      wiener<T>(V1.data(), V2.data(), d, K, G, sigma, V2f.data(), g2.data());
+      wiener<T>(V1.data(), V2.data(), d, K, G, sigma, V2f.data(), g2.data(), sigWie);
       aggregate_wie<T>(V2f.data(), g2.data(), S.data(), M, N, d, d2, K, U.data(), W.data(), gather);
       for (int c = 0; c < N; ++c)
         for (int r = 0; r < M; ++r) {

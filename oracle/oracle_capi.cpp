@@ -145,6 +145,14 @@ MED(f32, float)
       wiener<T>(V1, V2, d, K, G, sigma, V2f, gain2);                                                 \
       return 0;                                                                                      \
     });                                                                                              \
+  }                                                                                                  \
+  int orc_wiener_tab_##SUF(const T* V1, const T* V2, int d, int K, long G, const double* sigWie,     \
+                           T* V2f, T* gain2) {                                                       \
+    if (d < 1 || K < 4 || (K % 4) || !sigWie) return GBM3D_ERR_ARG;                                  \
+    return guarded([&] {                                                                             \
+      wiener<T>(V1, V2, d, K, G, 1.0, V2f, gain2, sigWie);                                           \
+      return 0;                                                                                      \
+    });                                                                                              \
   }
 AGG(f64, double)
 AGG(f32, float)
@@ -194,6 +202,31 @@ int orc_gbm3d_f32(const float* I, int m, int n, float sigma, const gbm3d_opts* r
   return guarded([&] { return gbm3d<float>(I, m, n, (double)sigma, *resolved, U, U1, true); });
 }
 
+
+int orc_colored_tau(int levels, const double* psd, int d1, int d2, double* tau) {
+  if (levels < 1 || !psd || !tau || d1 < 1 || d2 < 1) return GBM3D_ERR_ARG;
+  colored_tau(levels, psd, d1, d2, tau);
+  return 0;
+}
+
+int orc_gbm3d_colored_f64(const double* I, int m, int n, const double* psd, int d1, int d2, const double* sigWie,
+                          const gbm3d_opts* resolved, double* U, double* U1) {
+  if (!I || !U || !resolved || !psd) return GBM3D_ERR_ARG;
+  return guarded([&] {
+    std::vector<double> tau((size_t)resolved->levels * 7);
+    colored_tau(resolved->levels, psd, d1, d2, tau.data());
+    return gbm3d<double>(I, m, n, 1.0, *resolved, U, U1, true, tau.data(), sigWie);
+  });
+}
+int orc_gbm3d_colored_f32(const float* I, int m, int n, const double* psd, int d1, int d2, const double* sigWie,
+                          const gbm3d_opts* resolved, float* U, float* U1) {
+  if (!I || !U || !resolved || !psd) return GBM3D_ERR_ARG;
+  return guarded([&] {
+    std::vector<double> tau((size_t)resolved->levels * 7);
+    colored_tau(resolved->levels, psd, d1, d2, tau.data());
+    return gbm3d<float>(I, m, n, 1.0, *resolved, U, U1, true, tau.data(), sigWie);
+  });
+}
 
 int orc_distribution(int m, int n, const int32_t* spins, int nspins, int k, int32_t* rows, int max_rows) {
   std::vector<TileCase> R = distribution(m, n, spins, nspins, k);

@@ -382,3 +382,27 @@ def test_distributed_graph_replays_match_eager_run(ctx):
         U, out = ctx.denoise_distributed(noisy, 20.0, opts)
         assert np.array_equal(U, U_eager) and np.array_equal(out["U2"], o_eager["U2"])
         ctx.denoise(other, 20.0, opts)
+
+
+def test_colored_noise_vs_oracle(ctx):
+    """Matrix sigma (source/GBM3D.m:29-36): getColoredTau thresholds and a
+    per-coefficient Wiener sigma (the caller's getColoredWie output)."""
+    m, n = 96, 80
+    clean, _ = synth.make_pair(501, m, n, 0.0)
+    rng = np.random.default_rng(502)
+    fy, fx = np.fft.fftfreq(m)[:, None], np.fft.fftfreq(n)[None, :]
+    ghat = np.exp(-(fy ** 2 + fx ** 2) / (2 * 0.15 ** 2))            # low-pass noise colouring
+    white = rng.normal(0, 30.0, (m, n))
+    noisy = (clean + np.real(np.fft.ifft2(np.fft.fft2(white) * ghat))).astype(np.float32)
+    psd = 30.0 ** 2 * np.abs(ghat) ** 2
+    sigma_wie = np.asfortranarray(np.linspace(4, 12, 64).reshape(8, 8))
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
+    U, out = ctx.denoise_colored(noisy, psd, sigma_wie, opts)
+    U32, U132 = O.gbm3d_colored(noisy, psd, sigma_wie, opts, precision=32)
+    assert np.max(np.abs(out["U1"] - U132)) <= 1e-3
+    U64, _ = O.gbm3d_colored(noisy, psd, sigma_wie, opts, precision=64)
+    assert abs(synth.psnr(clean, U) - synth.psnr(clean, U64)) <= 0.01
+    U2 = O.step2_from_pilot(out["U1"], noisy, 1.0, opts, sigma_wie=sigma_wie)   # chained step 2
+    assert np.max(np.abs(U - U2)) <= 1e-3
+    with pytest.raises(GBM3DError, match="sigmaWie"):
+        ctx.denoise_colored(noisy, psd, None, opts)

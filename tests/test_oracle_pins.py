@@ -278,3 +278,43 @@ def test_distributed_single_tile_is_gbm3d():
     U, U2 = O.gbm3d_distributed(noisy, 25.0, opts, precision=64)
     V, V1 = O.gbm3d(noisy, 25.0, opts, precision=64)
     assert np.max(np.abs(U2 - V1)) < 1e-9 and np.max(np.abs(U - V)) < 1e-9
+
+
+def _colored_tau_literal(levels, psd):
+    """getColoredTau as written (utilities/getColoredTau.m:43-57): fft2 of the
+    nearest-neighbour upsampled 2-D filter, zero padded to the PSD size."""
+    Ca, Cb, Cc = 0.016572815184060, 0.121533978016438, np.sqrt(2) / 2
+    lo = np.array([Ca, -Ca, -Cb, Cb, Cc, Cc, Cb, -Cb, -Ca, Ca])
+    hi = np.zeros(10)
+    hi[4], hi[5] = -Cc, Cc
+    psi = [np.outer(hi, lo), np.outer(lo, hi), np.outer(lo, lo), np.outer(hi, hi), np.outer(hi, lo),
+           np.outer(lo, hi), np.outer(hi, hi)]
+    d1, d2 = psd.shape
+    tau = np.zeros((levels, 7))
+    for i in range(levels):
+        s = 2 ** i
+        for j in range(7):
+            P = np.kron(psi[j], np.ones((s, s)))        # imresize(.., 2^(i-1), 'nearest')
+            P = np.abs(np.fft.fft2(P, s=(d1, d2))) ** 2
+            tau[i, j] = np.sqrt(np.sum(psd * P) / np.sum(P))
+    return 6 * tau
+
+
+def test_colored_tau_matches_literal_fft_and_white_closed_form():
+    rng = np.random.default_rng(12)
+    psd = rng.uniform(10, 900, (48, 40))
+    assert np.allclose(O.colored_tau(3, psd), _colored_tau_literal(3, psd), rtol=1e-10)
+    assert np.allclose(O.colored_tau(3, np.full((32, 24), 625.0)), 150.0, rtol=1e-12)   # white: 6 sigma
+
+
+def test_library_colored_tau_matches_oracle():
+    """The library's host-side (separable) getColoredTau against the oracle's 2-D DFT form."""
+    import ctypes as C
+    from gbm3d_amd._abi import load_library
+
+    lib = load_library()
+    rng = np.random.default_rng(13)
+    psd = np.asfortranarray(rng.uniform(1, 400, (64, 56)))
+    tau = np.zeros((3, 7), order="F")
+    assert lib.gbm3d_get_colored_tau(3, psd.ctypes.data_as(C.c_void_p), 64, 56, tau.ctypes.data_as(C.c_void_p)) == 0
+    assert np.allclose(tau, O.colored_tau(3, psd), rtol=1e-10)
