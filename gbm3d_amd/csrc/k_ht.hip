@@ -287,7 +287,7 @@ static_assert(4 * C * ES <= REG1, "E fits in A");
 // the next pair (input planes and the coarser reconstruction) are issued into
 // registers before the current pair's filter phases run, so their latency
 // overlaps the LDS work.
-__global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_syn_fast_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
                                                           int p, int q, int r, int sh, Tau7 tau,
                                                           float* Y, const float* acc, int mode,
                                                           float cs, int gd, int zpb) {
@@ -437,6 +437,247 @@ __global__ __launch_bounds__(256) void ht_syn_fast_kernel(const float* __restric
   }
 }
 
+// ------------------------------------- finest level, both cycle spins fused
+// Cycle spin s (of 2) in original coordinates: circshift(V,[s s s]), the
+// decimated transform and the inverse shift amount to the same analysis and
+// synthesis with the coefficient anchors at parity s in x, y and z:
+//   C(a) = sum_k h[k] V(a - k),  a = s (mod 2);  Vrec_s(o) = sum_k g[k] C(o + k).
+// So one 51 x 51 input window per plane serves both spins: spin 0 reads its
+// 50 x 50 sub-window at (0, 0), spin 1 at (1, 1), and spin 1's synthesis runs
+// start one output earlier (odd output parity).  The block owns a 32 x 32
+// output column over all r planes and walks the z anchors az = 0..r-1 (spin
+// az & 1, planes az-1 and az), so each output plane is complete after two
+// consecutive anchors; Vavg = max((Vrec_0 + Vrec_1) / 2, 0) is stored once.
+// The per-anchor arithmetic is that of ht_syn_fast_kernel with sh = s.
+
+// out[i] = sum_{k: i+k odd} s_lo(k) lo[(i+k-1)/2] + s_hi(k) hi[(i+k-1)/2]: a run of
+// R2 outputs starting at an odd output 2m-1, coefficient window starting at m.
+// Per output the taps are taken in the same order as in syn_lo_hi.
+template <int R2>
+__device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[R2 / 2 + 4], const float (&hi)[R2 / 2 + 4],
+                                              float (&out)[R2]) {
+#pragma unroll
+  for (int i = 0; i < R2; ++i) {
+    float a = 0.f;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      if (((i + k) & 1) == 1) {
+        const int j = (i + k - 1) >> 1;
+        a = fmaf(s_hi(k), hi[j], a);
+        if (k == 4 || k == 5) a = fmaf(fCc, lo[j], a);
+      }
+    }
+    out[i] = a;
+  }
+}
+
+namespace f2 {
+constexpr int P = 32, C = 21, R = 50, W = R + 1;  // output tile, coefficients, sub-window, window
+constexpr int ZS = W + 1, AS = R + 1, BS = C + 1, ES = P + 16;
+constexpr int REG0 = 2 * W * ZS;  // Z (later B)
+constexpr int REG1 = 4 * C * AS;  // A (later E)
+constexpr int NIT = (W * W + 255) / 256;
+static_assert(8 * C * BS <= REG0, "B fits in Z");
+static_assert(4 * C * ES <= REG1, "E fits in A");
+}  // namespace f2
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_syn0_cs2_kernel(const float* __restrict__ X, const float* __restrict__ Ylo0,
+                                                          const float* __restrict__ Ylo1, int p, int q, int r,
+                                                          Tau7 tau, float* __restrict__ Y, int gd) {
+  using namespace f2;
+  __shared__ float lds[REG0 + REG1];
+  __shared__ int ridx[W], cidx[W];
+  __shared__ float ycs[2 * C * 11];  // the anchor's coarse values, staged per coarse-role thread
+  float* Z = lds;
+  float* B = lds;
+  float* A = lds + REG0;
+  float* E = lds + REG0;
+  const int tid = threadIdx.x;
+  const int nbx = gridDim.x, nby = gridDim.y;
+  const int lb = xcd_remap(blockIdx.x + nbx * blockIdx.y, nbx * nby);
+  const int x0 = (lb % nbx) * P, y0 = (lb / nbx) * P;  // output tile origin (original coordinates)
+  if (tid < W)
+    ridx[tid] = pmod_i(x0 - 9 + tid, p);
+  else if (tid < 2 * W)
+    cidx[tid - W] = pmod_i(y0 - 9 + tid - W, q);
+  __syncthreads();
+  const size_t pq = (size_t)p * q;
+  const int ph = p >> 1, qh = q >> 1, rh = r >> 1;
+  const int c3 = tid % C, run3 = (tid / C) & 1, b23 = (tid / (2 * C)) & 1, zb3 = tid / (4 * C);
+  const bool coarse = tid < 2 * C;
+  float cur[NIT], prev[NIT], yl[11];
+  auto fetch_plane = [&](float (&dst)[NIT], int z) {
+    const float* Xz = X + pq * z;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = tid + 256 * it;
+      dst[it] = 0.f;
+      if (e < W * W) dst[it] = Xz[(size_t)ridx[e % W] + (size_t)p * cidx[e / W]];
+    }
+  };
+  // coarse (rebuilt level-1) approximation of anchor az: spin s = az & 1, level-1 plane (az + s) / 2
+  auto fetch_coarse = [&](int az) {
+    if (!coarse) return;
+    const int s = az & 1;
+    const float* Yl = s ? Ylo1 : Ylo0;
+    const int t3 = ((az + s) >> 1) % rh;
+    const size_t col = (size_t)ph * ((size_t)pmod_i((y0 >> 1) + s + c3, qh) + (size_t)qh * t3);
+#pragma unroll
+    for (int i = 0; i < 11; ++i) yl[i] = Yl[(size_t)pmod_i((x0 >> 1) + s + run3 * 10 + i, ph) + col];
+  };
+  fetch_plane(prev, r - 1);
+  fetch_plane(cur, 0);
+  fetch_coarse(0);
+  // this thread's outputs: row x, columns yq..yq+3 of every plane
+  const int xo = tid % P, yq = (tid / P) * 4;
+  float first[4], pend[4];
+  for (int az = 0; az < r; ++az) {
+    const int s = az & 1;
+    // z analysis of the pair (az-1, az): a = plane az, b = plane az-1
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = tid + 256 * it;
+      if (e < W * W) {
+        const int rr = e % W, cc = e / W;
+        Z[cc * ZS + rr] = fCc * cur[it] + fCc * prev[it];
+        Z[W * ZS + cc * ZS + rr] = fCc * prev[it] - fCc * cur[it];
+      }
+    }
+    if (coarse) {
+#pragma unroll
+      for (int i = 0; i < 11; ++i) ycs[tid * 11 + i] = yl[i];
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) prev[it] = cur[it];
+    if (az + 1 < r) {  // in flight during the phases below
+      fetch_plane(cur, az + 1);
+      fetch_coarse(az + 1);
+    }
+    __syncthreads();
+    // dim-2 analysis on the spin's sub-window (offset s in rows and columns)
+    if (tid < 2 * 2 * R) {
+      const int rr = tid % R, run = (tid / R) & 1, zb = tid / (2 * R);
+      const int c0 = run * 10;
+      const float* Zc = Z + zb * W * ZS + (2 * c0 + s) * ZS + rr + s;
+      float v[30];
+#pragma unroll
+      for (int j = 0; j < 30; ++j) v[j] = Zc[j * ZS];
+      float lo[11], hi[11];
+      ana_lo_hi<11>(v, lo, hi);
+#pragma unroll
+      for (int i = 0; i < 11; ++i) {
+        A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
+        A[((zb * 2 + 1) * C + c0 + i) * AS + rr] = hi[i];
+      }
+    }
+    __syncthreads();
+    // dim-1 analysis + hard threshold (the approximation is the rebuilt coarser level)
+    if (tid < 2 * 2 * 2 * C) {
+      const int t0 = run3 * 10;
+      float v[30];
+#pragma unroll
+      for (int j = 0; j < 30; ++j) v[j] = A[((zb3 * 2 + b23) * C + c3) * AS + 2 * t0 + j];
+      float lo[11], hi[11];
+      ana_lo_hi<11>(v, lo, hi);
+      const int bl = (b23 << 1) | (zb3 << 2), bh = bl | 1;
+      const float tl = tau.t[band_j(bl)], th = tau.t[band_j(bh)];
+      float* Bl = B + ((zb3 * 4 + (b23 << 1)) * C + c3) * BS + t0;
+      float* Bh = Bl + C * BS;
+      if (coarse) {
+#pragma unroll
+        for (int i = 0; i < 11; ++i) Bl[i] = ycs[tid * 11 + i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 11; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
+      }
+#pragma unroll
+      for (int i = 0; i < 11; ++i) Bh[i] = hi[i] * (fabsf(hi[i]) > th ? 1.f : 0.f);
+    }
+    __syncthreads();
+    // dim-1 synthesis: 16 outputs per (z band, dim-2 band, coefficient column),
+    // E indexed by the output row of the tile (sub-window row + s)
+    if (tid < 2 * 2 * 2 * C) {
+      const int c = tid % C, xr = (tid / C) & 1, b2 = (tid / (2 * C)) & 1, zb = tid / (4 * C);
+      const int xl = xr * 16;
+      const float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + xl / 2;
+      const float* Bh = Bl + C * BS;
+      float* Eo = E + ((zb * 2 + b2) * C + c) * ES + xl;
+      float out[16];
+      if (s == 0) {
+        float lo[13], hi[13];
+#pragma unroll
+        for (int j = 0; j < 13; ++j) {
+          lo[j] = Bl[j];
+          hi[j] = Bh[j];
+        }
+        syn_lo_hi<16>(lo, hi, out);
+      } else {  // sub-window outputs xl-1 .. xl+14: coefficients xl/2 .. xl/2+11
+        float lo[12], hi[12];
+#pragma unroll
+        for (int j = 0; j < 12; ++j) {
+          lo[j] = Bl[j];
+          hi[j] = Bh[j];
+        }
+        syn_lo_hi_odd<16>(lo, hi, out);
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) Eo[i] = out[i];
+    }
+    __syncthreads();
+    // dim-2 synthesis (runs of 4 tile columns), z synthesis, spin combine, store
+    {
+      float o[2][4];
+#pragma unroll
+      for (int zb = 0; zb < 2; ++zb) {
+        const float* El = E + ((zb * 2 + 0) * C + yq / 2) * ES + xo;
+        const float* Eh = E + ((zb * 2 + 1) * C + yq / 2) * ES + xo;
+        if (s == 0) {
+          float lo[7], hi[7];
+#pragma unroll
+          for (int j = 0; j < 7; ++j) {
+            lo[j] = El[j * ES];
+            hi[j] = Eh[j * ES];
+          }
+          syn_lo_hi<4>(lo, hi, o[zb]);
+        } else {
+          float lo[6], hi[6];
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            lo[j] = El[j * ES];
+            hi[j] = Eh[j * ES];
+          }
+          syn_lo_hi_odd<4>(lo, hi, o[zb]);
+        }
+      }
+      float ve[4], vo[4];  // plane az (even tap g[0]) and plane az-1 (odd tap g[1])
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        ve[i] = fCc * o[0][i] - fCc * o[1][i];
+        vo[i] = fCc * o[0][i] + fCc * o[1][i];
+      }
+      auto store_plane = [&](int z, const float (&u)[4], const float (&v)[4]) {
+        if (x0 + xo >= p) return;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (y0 + yq + i >= q) continue;
+          const float mean = (u[i] + v[i]) / 2.f;  // mean(Vrec,4), then max(.,0) (denoiseAndAggregateGPU.m:45)
+          Y[vol_off(x0 + xo, y0 + yq + i, z, p, q, r, gd)] = mean > 0.f ? mean : 0.f;
+        }
+      };
+      if (az == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) first[i] = vo[i];  // plane r-1, completed by anchor r-1
+      } else {
+        store_plane(az - 1, pend, vo);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pend[i] = ve[i];
+      if (az == r - 1) store_plane(r - 1, first, pend);
+    }
+    __syncthreads();  // E (phase 5) and B (phase 4) are overwritten by the next anchor
+  }
+}
+
 namespace fl {
 constexpr int P = 32, R = 2 * P + 8, ZS = R + 1, OS = P + 1;
 }
@@ -556,6 +797,11 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
     tl[l].t[0] = 0.f;
     for (int j = 1; j <= 7; ++j) tl[l].t[j] = (float)tau[l + levels * (j - 1)];
   }
+  // Two cycle spins (every profile): the finest level of both spins in one
+  // pass over the volume.  GBM3D_HT_UNFUSED=1 keeps the per-spin kernels, for
+  // cross-checking (the two give bitwise equal results).
+  const char* unf = getenv("GBM3D_HT_UNFUSED");
+  const bool fused = cycleSpin == 2 && M >= 10 && N >= 10 && !g_ht_generic && !(unf && unf[0] == '1');
   for (int s = 0; s < cycleSpin; ++s) {
     // analysis chain (approximations only)
     mark(GBM3D_STAGE_HT_LO, 1);
@@ -571,7 +817,7 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
     }
     mark(GBM3D_STAGE_HT_LO, 0);
     // synthesis chain, coarse to fine
-    for (int l = levels - 1; l >= 0; --l) {
+    for (int l = levels - 1; l >= (fused ? 1 : 0); --l) {
       const int p = M >> l, q = N >> l, r = K >> l;
       const float* in = l == 0 ? vol : w.lo[l - 1][s];
       const float* ylo = (l == levels - 1) ? w.lo[l][s] : w.rec[l][s];
@@ -597,6 +843,15 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
       GBM3D_TRY(check_launch("ht_syn"));
       mark(stage, 0);
     }
+  }
+  if (fused) {
+    const float* y0 = levels == 1 ? w.lo[0][0] : w.rec[0][0];
+    const float* y1 = levels == 1 ? w.lo[0][1] : w.rec[0][1];
+    dim3 grid((M + f2::P - 1) / f2::P, (N + f2::P - 1) / f2::P, 1);
+    mark(GBM3D_STAGE_HT_SYN0, 1);
+    hipLaunchKernelGGL(ht_syn0_cs2_kernel, grid, dim3(256), 0, st, vol, y0, y1, M, N, K, tl[0], vavg, gd);
+    GBM3D_TRY(check_launch("ht_syn0_cs2"));
+    mark(GBM3D_STAGE_HT_SYN0, 0);
   }
   return GBM3D_OK;
 }

@@ -98,6 +98,28 @@ def test_ht_filter_vs_fp64_oracle(ctx, m, n, cs, seed):
     assert np.abs(Vg - Vo).max() <= 1e-3
 
 
+@pytest.mark.parametrize("m,n,K,levels,seed", [(64, 64, 16, 3, 51), (96, 160, 16, 3, 52), (48, 80, 8, 2, 53),
+                                                (16, 16, 2, 1, 54), (256, 200, 16, 3, 55)])
+def test_ht_fused_cycle_spins_equal_per_spin_kernels(ctx, monkeypatch, m, n, K, levels, seed):
+    """The finest level of both cycle spins in one pass (anchors of parity s in
+    original coordinates) is the per-spin computation with its operations in
+    the same order: bitwise equal, in the mosaic layout and, end to end, in the
+    group-contiguous layout the aggregation reads."""
+    rng = np.random.default_rng(seed)
+    V = (rng.standard_normal((m, n, K)) * 40 + 100).astype(np.float32)
+    tau = O.get_tau(levels, 1, 25.0)
+    a = ctx.ht_filter(V, levels, 2, tau)
+    monkeypatch.setenv("GBM3D_HT_UNFUSED", "1")
+    b = ctx.ht_filter(V, levels, 2, tau)
+    np.testing.assert_array_equal(a, b)
+    clean, noisy = synth.make_pair(seed, 128, 96, 25.0)
+    Ub, ob = ctx.denoise(noisy, 25.0, {"blockSize": 8, "matchSpinsWie": [0, 4]})
+    monkeypatch.delenv("GBM3D_HT_UNFUSED")
+    Ua, oa = ctx.denoise(noisy, 25.0, {"blockSize": 8, "matchSpinsWie": [0, 4]})
+    np.testing.assert_array_equal(oa["U1"], ob["U1"])
+    np.testing.assert_array_equal(Ua, Ub)
+
+
 def test_ht_filter_perfect_reconstruction(ctx):
     """tau = 0 keeps every coefficient: the filter is max(V, 0) (closed form)."""
     rng = np.random.default_rng(3)
