@@ -473,7 +473,7 @@ __device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[R2 / 2 + 4], con
 
 namespace f2 {
 constexpr int P = 32, C = 21, R = 50, W = R + 1;  // output tile, coefficients, sub-window, window
-constexpr int ZS = W + 1, AS = R + 1, BS = C + 1, ES = P + 16;
+constexpr int ZS = W, AS = R + 1, BS = C + 1, ES = P + 17;  // odd strides: conflict-free column walks
 constexpr int REG0 = 2 * W * ZS;  // Z (later B)
 constexpr int REG1 = 4 * C * AS;  // A (later E)
 constexpr int NIT = (W * W + 255) / 256;
@@ -481,12 +481,13 @@ static_assert(8 * C * BS <= REG0, "B fits in Z");
 static_assert(4 * C * ES <= REG1, "E fits in A");
 }  // namespace f2
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_syn0_cs2_kernel(const float* __restrict__ X, const float* __restrict__ Ylo0,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ht_syn0_cs2_kernel(const float* __restrict__ X, const float* __restrict__ Ylo0,
                                                           const float* __restrict__ Ylo1, int p, int q, int r,
                                                           Tau7 tau, float* __restrict__ Y, int gd) {
   using namespace f2;
   __shared__ float lds[REG0 + REG1];
-  __shared__ int ridx[W], cidx[W];
+  __shared__ int ridx[W], cidxp[W];  // window rows, p x window columns (circular)
+  __shared__ int yrow[2][C], ycol[2][C];  // per spin: coarse rows, ph x coarse columns (circular)
   __shared__ float ycs[2 * C * 11];  // the anchor's coarse values, staged per coarse-role thread
   float* Z = lds;
   float* B = lds;
@@ -496,51 +497,78 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   const int nbx = gridDim.x, nby = gridDim.y;
   const int lb = xcd_remap(blockIdx.x + nbx * blockIdx.y, nbx * nby);
   const int x0 = (lb % nbx) * P, y0 = (lb / nbx) * P;  // output tile origin (original coordinates)
-  if (tid < W)
+  const int ph = p >> 1, qh = q >> 1, rh = r >> 1;
+  if (tid < W) {
     ridx[tid] = pmod_i(x0 - 9 + tid, p);
-  else if (tid < 2 * W)
-    cidx[tid - W] = pmod_i(y0 - 9 + tid - W, q);
+  } else if (tid < 2 * W) {
+    cidxp[tid - W] = p * pmod_i(y0 - 9 + tid - W, q);
+  } else if (tid < 2 * W + 2 * C) {
+    const int k = tid - 2 * W, sp = k / C;
+    yrow[sp][k % C] = pmod_i((x0 >> 1) + sp + k % C, ph);
+  } else if (tid < 2 * W + 4 * C) {
+    const int k = tid - 2 * W - 2 * C, sp = k / C;
+    ycol[sp][k % C] = ph * pmod_i((y0 >> 1) + sp + k % C, qh);
+  }
   __syncthreads();
   const size_t pq = (size_t)p * q;
-  const int ph = p >> 1, qh = q >> 1, rh = r >> 1;
   const int c3 = tid % C, run3 = (tid / C) & 1, b23 = (tid / (2 * C)) & 1, zb3 = tid / (4 * C);
   const bool coarse = tid < 2 * C;
   float cur[NIT], prev[NIT], yl[11];
+  int off[NIT];  // in-plane offsets of this thread's window elements e = tid + 256 it (plane < 2^31 elements)
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int e = tid + 256 * it;
+    off[it] = e < W * W ? ridx[e % W] + cidxp[e / W] : 0;
+  }
   auto fetch_plane = [&](float (&dst)[NIT], int z) {
     const float* Xz = X + pq * z;
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      const int e = tid + 256 * it;
       dst[it] = 0.f;
-      if (e < W * W) dst[it] = Xz[(size_t)ridx[e % W] + (size_t)p * cidx[e / W]];
+      if (tid + 256 * it < W * W) dst[it] = Xz[off[it]];
     }
   };
   // coarse (rebuilt level-1) approximation of anchor az: spin s = az & 1, level-1 plane (az + s) / 2
   auto fetch_coarse = [&](int az) {
     if (!coarse) return;
     const int s = az & 1;
-    const float* Yl = s ? Ylo1 : Ylo0;
     const int t3 = ((az + s) >> 1) % rh;
-    const size_t col = (size_t)ph * ((size_t)pmod_i((y0 >> 1) + s + c3, qh) + (size_t)qh * t3);
+    const float* Yl = (s ? Ylo1 : Ylo0) + (size_t)ph * qh * t3 + ycol[s][c3];
 #pragma unroll
-    for (int i = 0; i < 11; ++i) yl[i] = Yl[(size_t)pmod_i((x0 >> 1) + s + run3 * 10 + i, ph) + col];
+    for (int i = 0; i < 11; ++i) yl[i] = Yl[yrow[s][run3 * 10 + i]];
   };
   fetch_plane(prev, r - 1);
   fetch_plane(cur, 0);
   fetch_coarse(0);
-  // this thread's outputs: row x, columns yq..yq+3 of every plane
+  // this thread's outputs: row x, columns yq..yq+3 of every plane; offsets of
+  // plane 0 (-1: outside the image) and the plane stride, mosaic or grouped
+  // (volume < 2^31 elements)
   const int xo = tid % P, yq = (tid / P) * 4;
+  int obase[4];
+  const int zstride = gd ? gd * gd : p * q;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int pr = x0 + xo, pc = y0 + yq + i;
+    obase[i] = -1;
+    if (pr < p && pc < q) {
+      if (gd == 0) {
+        obase[i] = pr + p * pc;
+      } else {
+        const int g = pr / gd + (p / gd) * (pc / gd);
+        obase[i] = pr % gd + gd * (pc % gd + gd * r * g);
+      }
+    }
+  }
   float first[4], pend[4];
   for (int az = 0; az < r; ++az) {
     const int s = az & 1;
     // z analysis of the pair (az-1, az): a = plane az, b = plane az-1
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      const int e = tid + 256 * it;
+      const int e = tid + 256 * it;  // = column * W + row, ZS == W
       if (e < W * W) {
-        const int rr = e % W, cc = e / W;
-        Z[cc * ZS + rr] = fCc * cur[it] + fCc * prev[it];
-        Z[W * ZS + cc * ZS + rr] = fCc * prev[it] - fCc * cur[it];
+        Z[e] = fCc * cur[it] + fCc * prev[it];
+        Z[W * ZS + e] = fCc * prev[it] - fCc * cur[it];
       }
     }
     if (coarse) {
@@ -656,12 +684,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
         vo[i] = fCc * o[0][i] + fCc * o[1][i];
       }
       auto store_plane = [&](int z, const float (&u)[4], const float (&v)[4]) {
-        if (x0 + xo >= p) return;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          if (y0 + yq + i >= q) continue;
+          if (obase[i] < 0) continue;
           const float mean = (u[i] + v[i]) / 2.f;  // mean(Vrec,4), then max(.,0) (denoiseAndAggregateGPU.m:45)
-          Y[vol_off(x0 + xo, y0 + yq + i, z, p, q, r, gd)] = mean > 0.f ? mean : 0.f;
+          Y[obase[i] + z * zstride] = mean > 0.f ? mean : 0.f;
         }
       };
       if (az == 0) {
@@ -801,7 +828,8 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
   // pass over the volume.  GBM3D_HT_UNFUSED=1 keeps the per-spin kernels, for
   // cross-checking (the two give bitwise equal results).
   const char* unf = getenv("GBM3D_HT_UNFUSED");
-  const bool fused = cycleSpin == 2 && M >= 10 && N >= 10 && !g_ht_generic && !(unf && unf[0] == '1');
+  const bool fused = cycleSpin == 2 && M >= 10 && N >= 10 && (size_t)M * N * K < ((size_t)1 << 31) &&
+                     !g_ht_generic && !(unf && unf[0] == '1');
   for (int s = 0; s < cycleSpin; ++s) {
     // analysis chain (approximations only)
     mark(GBM3D_STAGE_HT_LO, 1);
