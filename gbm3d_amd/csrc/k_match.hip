@@ -309,18 +309,31 @@ int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2,
 
 // ------------------------------------------------ BF16-MFMA matching
 // Precision variant.  Cross terms on BF16 MFMA with FP32 accumulation, then
-// the candidates whose approximate score is within the smallest R (R >= 2K)
-// are rescored with exactly the FP32 chain of match_kernel, so their keys are
-// bit-identical to the FP32 path's; the final top-K is taken over those keys.
+// every candidate whose approximate score is at most the R-th smallest lane
+// minimum (R >= K; at least R candidates, at most 256 before a slower exact
+// fallback) is rescored with exactly the FP32 chain of match_kernel, so their
+// keys are bit-identical to the FP32 path's; the final top-K is taken over
+// those keys.
 //
-// corr(a,b) = sum_v sum_r X_v(a,r) Y(r, b+v), X_v(a,r) = x(r-a, v) (a band
-// Toeplitz matrix of block column v), Y the search window: per v two distinct
-// 16x16 A tiles (r-a in [0,16) and [16,32)); B tiles are four consecutive
-// window rows of one column, read as one 8-byte LDS load.  Both operands are
-// centred on the reference block's mean c:
-//   <x,y> = <x-c, y-c> + c*(sum y + sum x) - D^2 c^2   (sums in FP32).
-typedef short bf4_t __attribute__((ext_vector_type(4)));
+// corr(a,b) = sum_v sum_r X_v(a,r) Y(r, b+v), X_v(a,r) = x(r-a, v): per block
+// column v a band Toeplitz A operand (one fragment serves every 16-row M tile,
+// since it depends on r - a only), and per (M tile, N tile) one
+// v_mfma_f32_16x16x32_bf16 whose K = 32 window rows cover the band (r - a in
+// [0, D), D <= 16).  B fragments are 8 consecutive window rows of one window
+// column: one 16-byte LDS load.
+//
+// A workgroup holds NWT reference tiles adjacent along the tile columns; they
+// share one staged search window (FP32 for the rescoring and the volume, BF16
+// for the MFMA) and the norm / sum maps of their candidates.  The BF16 window
+// is centred on the mean c of the workgroup's reference blocks, which keeps
+// BF16's relative precision on the differences that rank the candidates:
+//   <x,y> = <x-c, y-c> + c*(sum y) + (terms common to the tile),
+// so the approximate score is |y|^2 - 2(<x-c,y-c> + c sum y).
+// A reference block of zero norm (every exact distance Inf or NaN) takes the
+// exact FP32 search over all candidates.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short to_bf16(float x) {  // round to nearest even
   uint32_t b = __float_as_uint(x);
@@ -336,149 +349,193 @@ __device__ __forceinline__ uint32_t ord_f32(float d) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+constexpr int kRescoreMax = 256;  // rescored candidates per tile before the round-by-round fallback
+
 template <int D, int D2>
 struct Bf16Cfg {
-  static constexpr int WR = D2 + D;
-  static constexpr int KS = (WR + 15) / 16;
-  static constexpr int WRP = KS * 16;       // bf16 window rows, zero-padded
-  static constexpr int LDB = WRP + 4;       // bf16 column stride (multiple of 4: 8-byte aligned B loads)
-  static constexpr int LDW = (WR % 2 == 0) ? WR + 1 : WR;
+  static constexpr int NWT = (D == 16 && D2 == 48) ? 2 : 4;  // reference tiles (waves) per workgroup
+  static constexpr int WR = D2 + D;                           // window rows
+  static constexpr int WC = D2 + D + (NWT - 1) * D;           // union window columns
+  static constexpr int LDW = (WR % 2 == 0) ? WR + 1 : WR;     // odd: conflict-free columns
   static constexpr int MT = D2 / 16, NT = D2 / 16;
-  static constexpr int NC = MT * NT * 4;    // candidates per lane
-  static_assert(D2 % 16 == 0, "match size must be a multiple of 16");
+  static constexpr int RB = D2 + 16;  // BF16 rows read by the B fragments (zero beyond WR)
+  static constexpr int LB = RB + 8;   // BF16 column stride: 16-byte aligned, conflict-free 16-byte reads
+  static constexpr int CC = D2 + (NWT - 1) * D;  // candidate columns of the workgroup
+  static constexpr int LN = D2 + 1;
+  static constexpr int NC = MT * NT * 4;  // candidates per lane
+  static constexpr int XP = 48;           // zero-padded BF16 reference column, entry s <-> row s - 16
+  static_assert(D2 % 16 == 0 && D <= 16, "match size a multiple of 16, block size <= 16");
+  static_assert(RB >= WR, "B rows");
 };
 
 template <int D, int D2>
-__global__ __launch_bounds__(256) void match_bf16_kernel(const float* __restrict__ pad,
-                                                         const float* __restrict__ bn,
-                                                         const float* __restrict__ smap, int M, int N, int K,
-                                                         int R, uint32_t* __restrict__ S, float* __restrict__ vol) {
+__global__ __launch_bounds__(256) void match_bf16_kernel(
+    const float* __restrict__ pad, const float* __restrict__ bn, const float* __restrict__ smap, int M, int N, int K,
+    int R, uint32_t* __restrict__ S, float* __restrict__ vol) {
   using C = Bf16Cfg<D, D2>;
-  constexpr int S2 = D2 / 2;
-  __shared__ float win[4][C::WR * C::LDW];
-  __shared__ __attribute__((aligned(16))) unsigned short wb[4][C::WRP * C::LDB];
-  __shared__ uint64_t scr[4][64];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int S2 = D2 / 2, NWT = C::NWT, NTH = 64 * NWT;
+  __shared__ float Wf[C::WC * C::LDW];
+  __shared__ __attribute__((aligned(16))) unsigned short Wb[C::WC * C::LB];
+  __shared__ float Bn[C::CC * C::LN], Sm[C::CC * C::LN];
+  __shared__ unsigned short xpad[NWT][D * C::XP];
+  __shared__ unsigned short clist[NWT][kRescoreMax];  // window indices of the rescored candidates
+  __shared__ uint64_t scr[NWT][64];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int Mt = M / D, Nt = N / D;
-  const int g = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
-  if (g >= Mt * Nt) return;
-  const int ti = g % Mt, tj = g / Mt;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti = blk % Mt, tj0 = (blk / Mt) * NWT;
+  const int nval = min(NWT, Nt - tj0);
   const int P = M + D2, Pm = P - D + 1;
-  float* W = win[wave];
-  unsigned short* Wb = wb[wave];
-  const float* src = pad + (size_t)(ti * D) + (size_t)P * (tj * D);
-  for (int e = lane; e < C::WR * C::WR; e += 64) {
+  // centring constant: mean of the workgroup's reference blocks
+  float csum = 0.f;
+  for (int w = 0; w < nval; ++w) csum += smap[(ti * D + S2) + (size_t)Pm * ((tj0 + w) * D + S2)];
+  const float cm = csum / (float)(nval * D * D);
+  const float* src = pad + (size_t)(ti * D) + (size_t)P * (tj0 * D);
+  const int wcv = D2 + D + (nval - 1) * D;  // window columns inside the padded image
+  for (int e = tid; e < C::WR * C::WC; e += NTH) {
     const int rr = e % C::WR, cc = e / C::WR;
-    W[rr + C::LDW * cc] = src[rr + (size_t)P * cc];
+    Wf[rr + C::LDW * cc] = cc < wcv ? src[rr + (size_t)P * cc] : 0.f;
   }
-  const size_t cpos = (ti * D + S2) + (size_t)Pm * (tj * D + S2);
-  const float nx = bn[cpos];
-  const float sx = smap[cpos];
-  const float cm = sx / (float)(D * D);
-  wave_sync();
-  for (int e = lane; e < C::WRP * C::WR; e += 64) {
-    const int rr = e % C::WRP, cc = e / C::WRP;
-    Wb[rr + C::LDB * cc] = rr < C::WR ? to_bf16(W[rr + C::LDW * cc] - cm) : (unsigned short)0;
+  const int ccv = D2 + (nval - 1) * D;
+  for (int e = tid; e < D2 * C::CC; e += NTH) {
+    const int a = e % D2, cc = e / D2;
+    const size_t o = (size_t)(ti * D + a) + (size_t)Pm * (tj0 * D + cc);
+    Bn[a + C::LN * cc] = cc < ccv ? bn[o] : 0.f;
+    Sm[a + C::LN * cc] = cc < ccv ? smap[o] : 0.f;
   }
-  // A tiles: X_v(a, r) for r - a = 16*delta + 4*(lane/16) + i - lane%16
-  bf4_t A[D][2];
+  __syncthreads();
+  for (int e = tid; e < C::RB * C::WC; e += NTH) {
+    const int rr = e % C::RB, cc = e / C::RB;
+    Wb[rr + C::LB * cc] = rr < C::WR ? to_bf16(Wf[rr + C::LDW * cc] - cm) : (unsigned short)0;
+  }
+  unsigned short* xp = xpad[wave];
+  for (int e = lane; e < D * C::XP; e += 64) {
+    const int v = e / C::XP, t = e % C::XP - 16;
+    xp[e] = (t >= 0 && t < D) ? to_bf16(Wf[(S2 + t) + C::LDW * (S2 + v + D * wave)] - cm) : (unsigned short)0;
+  }
+  __syncthreads();
+  if (wave >= nval) return;  // no barrier below
+  const int g = ti + Mt * (tj0 + wave);
+  const int q = lane >> 4, l = lane & 15;
+  const float* Wt = Wf + C::LDW * (D * wave);  // this tile's window
+  const float nx = Bn[S2 + C::LN * (S2 + D * wave)];
+  // exact key of window candidate (a, b): the FMA chain of match_kernel
+  auto exact_key = [&](int a, int b) {
+    float cr = 0.f;
+#pragma unroll 1
+    for (int v = 0; v < D; ++v)
 #pragma unroll
-  for (int v = 0; v < D; ++v)
+      for (int u = 0; u < D; ++u) cr = fmaf(Wt[(S2 + u) + C::LDW * (S2 + v)], Wt[(a + u) + C::LDW * (b + v)], cr);
+    const float bnp = Bn[a + C::LN * (b + D * wave)];
+    const float num = (nx + bnp) - 2.0f * cr;
+    return cand_key(num / nx, a + D2 * b);
+  };
+  auto cand_ab = [&](int j, int& a, int& b) {
+    const int mt = j / (4 * C::NT), nt = (j / 4) % C::NT, i = j % 4;
+    a = 16 * mt + 4 * q + i;
+    b = 16 * nt + l;
+  };
+  uint64_t best;
+  if (nx == 0.f) {
+    uint64_t ekey[C::NC];
+#pragma unroll 1
+    for (int j = 0; j < C::NC; ++j) {
+      int a, b;
+      cand_ab(j, a, b);
+      ekey[j] = exact_key(a, b);
+    }
+    best = wave_topk<C::NC>(ekey, K, scr[wave], lane);
+  } else {
+    bf16x8_t A[D];
 #pragma unroll
-    for (int dl = 0; dl < 2; ++dl)
+    for (int v = 0; v < D; ++v) {
+      unsigned short t[8];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int t = 16 * dl + 4 * (lane >> 4) + i - (lane & 15);
-        A[v][dl][i] = (t >= 0 && t < D) ? (short)to_bf16(W[(S2 + t) + C::LDW * (S2 + v)] - cm) : (short)0;
-      }
-  wave_sync();
-  f4_t acc[C::MT][C::NT];
+      for (int j = 0; j < 8; ++j) t[j] = xp[v * C::XP + 8 * q - l + 16 + j];
+      A[v] = __builtin_bit_cast(bf16x8_t, t);
+    }
+    f4_t acc[C::MT][C::NT];
 #pragma unroll
-  for (int mt = 0; mt < C::MT; ++mt)
+    for (int mt = 0; mt < C::MT; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < C::NT; ++nt) acc[mt][nt] = (f4_t){0.f, 0.f, 0.f, 0.f};
-  for (int v = 0; v < D; ++v) {
+      for (int nt = 0; nt < C::NT; ++nt) acc[mt][nt] = (f4_t){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int nt = 0; nt < C::NT; ++nt) {
+    for (int v = 0; v < D; ++v) {
 #pragma unroll
-      for (int ks = 0; ks < C::KS; ++ks) {
-        const bf4_t B = *reinterpret_cast<const bf4_t*>(&Wb[(16 * ks + 4 * (lane >> 4)) +
-                                                            C::LDB * (16 * nt + (lane & 15) + v)]);
+      for (int nt = 0; nt < C::NT; ++nt) {
+        const unsigned short* col = Wb + C::LB * (16 * nt + l + v + D * wave) + 8 * q;
 #pragma unroll
         for (int mt = 0; mt < C::MT; ++mt) {
-          const int dl = ks - mt;
-          if (dl == 0 || dl == 1)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[v][dl], B, acc[mt][nt], 0, 0, 0);
+          const bf16x8_t B = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(col + 16 * mt));
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[v], B, acc[mt][nt], 0, 0, 0);
         }
       }
     }
-  }
-  // approximate scores num = |y|^2 - 2<x,y> (|x|^2 is common to the tile)
-  const float ccorr = cm * sx - (float)(D * D) * cm * cm;
-  uint32_t key[C::NC];
+    uint32_t key[C::NC];
 #pragma unroll
-  for (int mt = 0; mt < C::MT; ++mt)
+    for (int mt = 0; mt < C::MT; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < C::NT; ++nt)
+      for (int nt = 0; nt < C::NT; ++nt)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int a = 16 * mt + 4 * (lane >> 4) + i, b = 16 * nt + (lane & 15);
-        const size_t o = (ti * D + a) + (size_t)Pm * (tj * D + b);
-        const float corr = acc[mt][nt][i] + cm * smap[o] + ccorr;
-        key[(mt * C::NT + nt) * 4 + i] = ord_f32(bn[o] - 2.0f * corr);
+        for (int i = 0; i < 4; ++i) {
+          const int a = 16 * mt + 4 * q + i, cc = 16 * nt + l + D * wave;
+          const float sc = Bn[a + C::LN * cc] - 2.0f * (acc[mt][nt][i] + cm * Sm[a + C::LN * cc]);
+          key[(mt * C::NT + nt) * 4 + i] = ord_f32(sc);
+        }
+    // rescored set: every candidate whose approximate key is <= theta, theta =
+    // R-th smallest lane minimum (at least R of them), compacted in LDS
+    uint32_t lmin = 0xffffffffu;
+#pragma unroll
+    for (int j = 0; j < C::NC; ++j) lmin = min(lmin, key[j]);
+    const uint32_t theta = (uint32_t)__shfl((int)wave_sort64<uint32_t>(lmin, lane), R - 1, 64);
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < C::NC; ++j) cnt += key[j] <= theta ? 1 : 0;
+    const int off = wave_excl_scan(cnt, lane);
+    const int total = __shfl(off + cnt, 63, 64);
+    unsigned short* cl = clist[wave];
+    if (total <= kRescoreMax) {
+      int pos = off;
+#pragma unroll
+      for (int j = 0; j < C::NC; ++j)
+        if (key[j] <= theta) {
+          int a, b;
+          cand_ab(j, a, b);
+          cl[pos++] = (unsigned short)(a + D2 * b);
+        }
+      wave_sync();
+      uint64_t ek[kRescoreMax / 64];
+#pragma unroll
+      for (int rr = 0; rr < kRescoreMax / 64; ++rr) {
+        const int c = lane + 64 * rr;
+        ek[rr] = ~0ull;
+        if (c < total) {
+          const int w = cl[c];
+          ek[rr] = exact_key(w % D2, w / D2);
+        }
       }
-  // candidates to rescore: every key <= theta, theta = R-th smallest lane minimum
-  uint32_t lmin = 0xffffffffu;
+      best = wave_topk<kRescoreMax / 64>(ek, K, scr[wave], lane);
+    } else {
+      // long runs of equal scores: the R smallest (score, index) pairs, one round each
+      int mine = -1;
+      uint64_t prev = 0;
+      for (int r = 0; r < R; ++r) {
+        uint64_t lm = ~0ull;
 #pragma unroll
-  for (int j = 0; j < C::NC; ++j) lmin = min(lmin, key[j]);
-  const uint32_t theta = (uint32_t)__shfl((int)wave_sort64<uint32_t>(lmin, lane), R - 1, 64);
-  int cnt = 0;
-#pragma unroll
-  for (int j = 0; j < C::NC; ++j) cnt += key[j] <= theta ? 1 : 0;
-  const int off = wave_excl_scan(cnt, lane);
-  const int total = __shfl(off + cnt, 63, 64);
-  auto cand_idx = [&](int j) {
-    const int mt = j / (4 * C::NT), nt = (j / 4) % C::NT, i = j % 4;
-    return (16 * mt + 4 * (lane >> 4) + i) + D2 * (16 * nt + (lane & 15));
-  };
-  int mine = -1;  // window index this lane rescores
-  if (total <= 64) {
-    int pos = off;
-#pragma unroll
-    for (int j = 0; j < C::NC; ++j)
-      if (key[j] <= theta) scr[wave][pos++] = (uint64_t)cand_idx(j);
-    wave_sync();
-    if (lane < total) mine = (int)scr[wave][lane];
-    wave_sync();
-  } else {
-    // many equal scores: the R smallest (score, index) pairs, one round each
-    uint64_t prev = 0;
-    for (int r = 0; r < R; ++r) {
-      uint64_t lm = ~0ull;
-#pragma unroll
-      for (int j = 0; j < C::NC; ++j) {
-        const uint64_t kk = ((uint64_t)key[j] << 32) | (uint32_t)cand_idx(j);
-        if ((r == 0 || kk > prev) && kk < lm) lm = kk;
+        for (int j = 0; j < C::NC; ++j) {
+          int a, b;
+          cand_ab(j, a, b);
+          const uint64_t kk = ((uint64_t)key[j] << 32) | (uint32_t)(a + D2 * b);
+          if ((r == 0 || kk > prev) && kk < lm) lm = kk;
+        }
+        lm = wave_min_u64(lm);
+        if (lane == r) mine = (int)(uint32_t)lm;
+        prev = lm;
       }
-      lm = wave_min_u64(lm);
-      if (lane == r) mine = (int)(uint32_t)lm;
-      prev = lm;
+      const uint64_t ek = mine >= 0 ? exact_key(mine % D2, mine / D2) : ~0ull;
+      best = wave_sort64<uint64_t>(ek, lane);
     }
   }
-  // exact FP32 rescoring (the FMA chain of match_kernel)
-  uint64_t ek = ~0ull;
-  if (mine >= 0) {
-    const int a = mine % D2, b = mine / D2;
-    float cr = 0.f;
-    for (int v = 0; v < D; ++v)
-#pragma unroll
-      for (int u = 0; u < D; ++u)
-        cr = fmaf(W[(S2 + u) + C::LDW * (S2 + v)], W[(a + u) + C::LDW * (b + v)], cr);
-    const float bnp = bn[(ti * D + a) + (size_t)Pm * (tj * D + b)];
-    const float num = (nx + bnp) - 2.0f * cr;
-    ek = cand_key(num / nx, mine);
-  }
-  const uint64_t best = wave_sort64<uint64_t>(ek, lane);
   int idx = (int)(uint32_t)best;
   if (lane == 0) idx = S2 + D2 * S2;  // slot 1 := window centre (matchBlocksCPU.m:103)
   if (lane < K) S[lane + (size_t)K * g] = (uint32_t)idx + 1u;
@@ -487,10 +544,10 @@ __global__ __launch_bounds__(256) void match_bf16_kernel(const float* __restrict
   for (int k = 0; k < K; ++k) {
     const int ik = __builtin_amdgcn_readlane(idx, k);
     const int a = ik % D2, b = ik / D2;
-    float* dst = vol + MN * k + (size_t)(ti * D) + (size_t)M * (tj * D);
+    float* dst = vol + MN * k + (size_t)(ti * D) + (size_t)M * ((tj0 + wave) * D);
     for (int e = lane; e < D * D; e += 64) {
       const int u = e % D, v = e / D;
-      dst[u + (size_t)M * v] = W[(a + u) + C::LDW * (b + v)];
+      dst[u + (size_t)M * v] = Wt[(a + u) + C::LDW * (b + v)];
     }
   }
 }
@@ -501,12 +558,14 @@ int launch_match_bf16(const float* pad, const float* bn, const float* smap, int 
   if (K < 1 || K > 64) return fail(GBM3D_ERR_UNSUPPORTED, "match: numMax must be in [1, 64]");
   if (R < K) R = K;
   if (R > 64) R = 64;
-  const int G = (M / d) * (N / d);
-  const dim3 grid((G + 3) / 4);
-#define GBM3D_BF16_CASE(DD, DD2)                                                                              \
-  if (d == DD && d2 == DD2) {                                                                                 \
-    hipLaunchKernelGGL((match_bf16_kernel<DD, DD2>), grid, dim3(256), 0, st, pad, bn, smap, M, N, K, R, S, vol); \
-    return check_launch("match_bf16");                                                                        \
+  const int Mt = M / d, Nt = N / d;
+#define GBM3D_BF16_CASE(DD, DD2)                                                                        \
+  if (d == DD && d2 == DD2) {                                                                           \
+    constexpr int nwt = Bf16Cfg<DD, DD2>::NWT;                                                          \
+    const dim3 grid(Mt * ((Nt + nwt - 1) / nwt));                                                       \
+    hipLaunchKernelGGL((match_bf16_kernel<DD, DD2>), grid, dim3(64 * nwt), 0, st, pad, bn, smap, M, N, K, R, \
+                       S, vol);                                                                         \
+    return check_launch("match_bf16");                                                                  \
   }
   GBM3D_BF16_CASE(8, 32)
   GBM3D_BF16_CASE(16, 32)
