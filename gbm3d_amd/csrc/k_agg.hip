@@ -118,10 +118,20 @@ int launch_mosaic_to_groups(const float* src, int M, int N, int K, int d, float*
 // ---------------------------------------------------------- aggregation
 constexpr int AT = 16;  // output tile edge per wave
 
-// Batched variant.  Lane e of the wave owns pixel (e % D, e / D) (+64i) of
-// every block; overlapping blocks come from the ballot in (y, x, k) order,
-// eight at a time: their loads are issued together, then each block is added
-// into the LDS tile accumulator in order.
+// Lane e of the wave owns pixel (u, v) = (e % D, e / D) (+64i) of every block.
+// Hits (candidate blocks that meet the tile) come from a ballot over 64
+// candidates in (y, x, k) order, eight at a time: their loads are issued
+// together (whole blocks, no bounds tests), then each block is added into the
+// LDS tile accumulator in order, predicated on the pixel lying in the tile.
+// U and W are interleaved (one 64-bit read and write per pixel and block); the
+// accumulator's column stride AS makes a half-wave's 64-bit accesses to one
+// block hit distinct banks.  The next round's index and weight loads are
+// issued before the current round's hits are processed.
+template <int D>
+struct AggCfg {
+  static constexpr int AS = D <= 8 ? 24 : 16;
+};
+
 template <int STEP, int D>
 __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __restrict__ blocks,
                                                                 const float* __restrict__ wgt,
@@ -129,23 +139,23 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
                                                                 const float* __restrict__ masks, int M, int N,
                                                                 int d2, int K, int spin, float* __restrict__ U,
                                                                 float* __restrict__ W) {
-  constexpr int DD = D * D, PER = (DD + 63) / 64, NB = 8;
-  __shared__ float accU[4][AT * AT], accW[4][AT * AT];
+  constexpr int DD = D * D, PER = (DD + 63) / 64, NB = 8, AS = AggCfg<D>::AS;
+  __shared__ float2 acc[4][AT * AS];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tilesR = (M + AT - 1) / AT, tilesC = (N + AT - 1) / AT;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
   if (tile >= tilesR * tilesC) return;
   const int R0 = (tile % tilesR) * AT, C0 = (tile / tilesR) * AT;
-  float* aU = accU[wave];
-  float* aW = accW[wave];
-  for (int e = lane; e < AT * AT; e += 64) {
-    aU[e] = 0.f;
-    aW[e] = 0.f;
-  }
+  float2* ac = acc[wave];
+  for (int e = lane; e < AT * AS; e += 64) ac[e] = make_float2(0.f, 0.f);
   float mk[PER];
+  int pu[PER], pv[PER], po[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = lane + 64 * i;
+    pu[i] = e % D;
+    pv[i] = e / D;
+    po[i] = pu[i] + AS * pv[i];
     mk[i] = e < DD ? masks[(STEP == 1 ? 0 : DD) + e] : 0.f;
   }
   wave_sync();
@@ -158,21 +168,36 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   const int xlo = first(C0), xhi = min(Nt - 1, (C0 + AT - 1 + s) / D);
   const int ny = yhi - ylo + 1, nx = xhi - xlo + 1;
   const int ncand = (ny > 0 && nx > 0) ? ny * nx * K : 0;
+  // candidate t = ((y - ylo) * nx + (x - xlo)) * K + k: its group, slot index and weight
+  auto fetch = [&](int t, int& g, uint32_t& sidx, float& w) {
+    g = 0;
+    sidx = 0;
+    w = 0.f;
+    if (t < ncand) {
+      const int yx = t / K;
+      g = (ylo + yx / nx) + Mt * (xlo + yx % nx);
+      sidx = S[(size_t)K * g + t % K];
+      w = wgt[g];
+    }
+  };
+  int gn;
+  uint32_t sn;
+  float wn;
+  fetch(lane, gn, sn, wn);
   for (int base = 0; base < ncand; base += 64) {
-    const int t = base + lane;
-    int br = 0, bc = 0, g = 0, k = 0;
-    float wv = 0.f;
+    const int t = base + lane, g = gn;
+    const uint32_t sidx = sn;
+    const float wv = wn;
+    if (base + 64 < ncand) fetch(t + 64, gn, sn, wn);
+    int br = 0, bc = 0, blk = 0;
     bool hit = false;
     if (t < ncand) {
-      k = t % K;
       const int yx = t / K;
-      const int y = ylo + yx / nx, x = xlo + yx % nx;
-      g = y + Mt * x;
-      const uint32_t idx = S[(size_t)K * g + k] - 1u;
-      br = y * D + (int)(idx % (uint32_t)d2) - s - R0;  // block origin relative to the tile
-      bc = x * D + (int)(idx / (uint32_t)d2) - s - C0;
+      const uint32_t idx = sidx - 1u;
+      br = (ylo + yx / nx) * D + (int)(idx % (uint32_t)d2) - s - R0;  // block origin relative to the tile
+      bc = (xlo + yx % nx) * D + (int)(idx / (uint32_t)d2) - s - C0;
       hit = br < AT && br + D > 0 && bc < AT && bc + D > 0;
-      if (hit) wv = wgt[g];
+      blk = t % K + K * g;
     }
     uint64_t mask = __ballot(hit);
     while (mask) {
@@ -186,40 +211,30 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         if (src[j] < 0) continue;
-        const int gb = __builtin_amdgcn_readlane(g, src[j]), kb = __builtin_amdgcn_readlane(k, src[j]);
-        const int rr = __builtin_amdgcn_readlane(br, src[j]), cc = __builtin_amdgcn_readlane(bc, src[j]);
-        const float* bp = blocks + (size_t)DD * (kb + (size_t)K * gb);
+        const float* bp = blocks + (size_t)DD * __builtin_amdgcn_readlane(blk, src[j]);
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-          const int e = lane + 64 * i;
-          const int u = e % D, v = e / D;
-          const int pr = rr + u, pc = cc + v;
-          val[j][i] = 0.f;
-          if (e < DD && pr >= 0 && pr < AT && pc >= 0 && pc < AT)
-            val[j][i] = bp[e];
-        }
+        for (int i = 0; i < PER; ++i) val[j][i] = (lane + 64 * i < DD) ? bp[lane + 64 * i] : 0.f;
       }
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         if (src[j] < 0) continue;
         const int rr = __builtin_amdgcn_readlane(br, src[j]), cc = __builtin_amdgcn_readlane(bc, src[j]);
         const float wg = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), src[j]));
+        const int ob = rr + AS * cc;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-          const int e = lane + 64 * i;
-          const int u = e % D, v = e / D;
-          const int pr = rr + u, pc = cc + v;
-          if (e < DD && pr >= 0 && pr < AT && pc >= 0 && pc < AT) {
-            const int o = pr + AT * pc;
+          if (lane + 64 * i < DD && (unsigned)(rr + pu[i]) < (unsigned)AT && (unsigned)(cc + pv[i]) < (unsigned)AT) {
+            float2 a = ac[ob + po[i]];
             if (STEP == 1) {
               const float c = (wg * val[j][i]) * mk[i];  // weight*blocks(:,:,i).*mask in single
-              aU[o] = aU[o] + c;
-              aW[o] = aW[o] + wg * mk[i];
+              a.x = a.x + c;
+              a.y = a.y + wg * mk[i];
             } else {
               const float wt = mk[i] / wg;  // w / norm(filt)^2
-              aU[o] = aU[o] + wt * val[j][i];
-              aW[o] = aW[o] + wt;
+              a.x = a.x + wt * val[j][i];
+              a.y = a.y + wt;
             }
+            ac[ob + po[i]] = a;
           }
         }
         // no barrier between blocks: LDS operations of one wave execute in
@@ -230,11 +245,13 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   }
   wave_sync();
   for (int e = lane; e < AT * AT; e += 64) {
-    const int pr = R0 + (e % AT), pc = C0 + (e / AT);
+    const int r = e % AT, c = e / AT;
+    const int pr = R0 + r, pc = C0 + c;
     if (pr >= M || pc >= N) continue;
     const size_t o = (size_t)pmod_i(pr - spin, M) + (size_t)M * pmod_i(pc - spin, N);
-    U[o] = aU[e];
-    W[o] = aW[e];
+    const float2 a = ac[r + AS * c];
+    U[o] = a.x;
+    W[o] = a.y;
   }
 }
 
