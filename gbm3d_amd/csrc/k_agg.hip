@@ -29,10 +29,12 @@ __device__ __forceinline__ float third_diff(float x0, float x1, float x2, float 
 
 // One wave per group; lane e < d*d owns pixel (u, v) = (e % d, e / d) of
 // every slice, slices staged one at a time in LDS for the in-plane differences.
+// The slices are fetched in batches of KB whose loads are all issued before
+// the batch is processed, so one global latency covers KB slices.
 template <int D>
 __global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __restrict__ vg, int G, int K,
                                                              float* __restrict__ wgt) {
-  constexpr int DD = D * D, PER = (DD + 63) / 64;
+  constexpr int DD = D * D, PER = (DD + 63) / 64, KB = PER >= 16 ? 2 : 16 / PER;
   __shared__ float sl[4][DD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
@@ -41,28 +43,41 @@ __global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __rest
   float* cur = sl[wave];
   float prev[PER];
   float s1 = 0.f, s2 = 0.f, s3 = 0.f;
-  for (int k = 0; k < K; ++k) {
+  for (int k0 = 0; k0 < K; k0 += KB) {
+    float xb[KB][PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = lane + 64 * i;
-      if (e < DD) {
-        const float x = blk[(size_t)DD * k + e];
-        if (k > 0) s3 += fabsf(x - prev[i]);  // |diff(U,1,3)|
-        prev[i] = x;
-        cur[e] = x;
-      }
-    }
-    wave_sync();
+    for (int j = 0; j < KB; ++j)
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = lane + 64 * i;
-      if (e < DD) {
-        const int u = e % D, v = e / D;
-        if (v + 3 < D) s2 += fabsf(third_diff(cur[e], cur[e + D], cur[e + 2 * D], cur[e + 3 * D]));
-        if (u + 3 < D) s1 += fabsf(third_diff(cur[e], cur[e + 1], cur[e + 2], cur[e + 3]));
+      for (int i = 0; i < PER; ++i) {
+        const int e = lane + 64 * i;
+        xb[j][i] = (e < DD && k0 + j < K) ? blk[(size_t)DD * (k0 + j) + e] : 0.f;
       }
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      const int k = k0 + j;
+      if (k >= K) break;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = lane + 64 * i;
+        if (e < DD) {
+          const float x = xb[j][i];
+          if (k > 0) s3 += fabsf(x - prev[i]);  // |diff(U,1,3)|
+          prev[i] = x;
+          cur[e] = x;
+        }
+      }
+      wave_sync();
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = lane + 64 * i;
+        if (e < DD) {
+          const int u = e % D, v = e / D;
+          if (v + 3 < D) s2 += fabsf(third_diff(cur[e], cur[e + D], cur[e + 2 * D], cur[e + 3 * D]));
+          if (u + 3 < D) s1 += fabsf(third_diff(cur[e], cur[e + 1], cur[e + 2], cur[e + 3]));
+        }
+      }
+      wave_sync();
     }
-    wave_sync();
   }
   s1 = wave_sum_f32(s1);
   s2 = wave_sum_f32(s2);
