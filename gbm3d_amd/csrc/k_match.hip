@@ -32,24 +32,32 @@ int launch_mirror_pad(const float* src, int m, int n, int M, int N, float* dst, 
 
 // Search image of matchBlocksCPU.m:57-61 for the image circshifted by
 // 'spin' (source/GBM3D.m:51): padded row r is image row
-// (r < s ? r+s : (r-s) mod M) of the shifted image, s = d2/2.
-__device__ __forceinline__ int padmap(int r, int m, int s) { return r < s ? r + s : (r - s) % m; }
+// (r < s ? r+s : (r-s) mod L) of the shifted image, s = d2/2, that is row
+// (that - spin) mod L of the image; for r in [0, L + 2s), s <= L and spin in
+// [0, L) one conditional subtraction or addition does each reduction.
+__device__ __forceinline__ int padmap_spin(int r, int L, int s, int spin) {
+  int x = r < s ? r + s : r - s;  // in [0, L + s)
+  if (x >= L) x -= L;
+  x -= spin;
+  return x < 0 ? x + L : x;
+}
 
-__global__ void search_pad_kernel(const float* __restrict__ img, int M, int N, int spin, int d2,
+// spinR = spin mod M, spinC = spin mod N.
+__global__ void search_pad_kernel(const float* __restrict__ img, int M, int N, int spinR, int spinC, int d2,
                                   float* __restrict__ pad) {
   const int P = M + d2, s = d2 / 2;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int c = blockIdx.y;
   if (r >= P) return;
-  int sr = pmod_i(padmap(r, M, s) - spin, M);
-  int sc = pmod_i(padmap(c, N, s) - spin, N);
+  const int sr = padmap_spin(r, M, s, spinR), sc = padmap_spin(c, N, s, spinC);
   pad[r + (size_t)P * c] = img[sr + (size_t)M * sc];
 }
 
 int launch_search_pad(const float* img, int M, int N, int spin, int d2, float* pad, hipStream_t st) {
   if (M < d2 || N < d2 || (d2 & 1)) return fail(GBM3D_ERR_ARG, "search pad: image smaller than matchSize");
+  const int spinR = ((spin % M) + M) % M, spinC = ((spin % N) + N) % N;
   dim3 grid((M + d2 + 255) / 256, N + d2);
-  hipLaunchKernelGGL(search_pad_kernel, grid, dim3(256), 0, st, img, M, N, spin, d2, pad);
+  hipLaunchKernelGGL(search_pad_kernel, grid, dim3(256), 0, st, img, M, N, spinR, spinC, d2, pad);
   return check_launch("search_pad");
 }
 
@@ -60,7 +68,10 @@ template <int D, bool SQUARE = true>
 __global__ __launch_bounds__(256) void norm_map_kernel(const float* __restrict__ pad, int P, int Q,
                                                        float* __restrict__ bn) {
   constexpr int TR = 64, TC = 16, RB = 4;
-  constexpr int LR = TR + D - 1, LC = TC + D - 1, LDT = LR + 1;
+  // LDT odd with LDT % 4 != 0: the four tile columns a wave reads at once
+  // fall in distinct bank residues (lanes of a column are 4 rows apart)
+  constexpr int LR = TR + D - 1, LC = TC + D - 1, LDT = LR + 2;
+  static_assert(LDT % 2 == 1, "tile stride");
   __shared__ float tile[LC * LDT];
   const int Pm = P - D + 1, Qm = Q - D + 1;
   const int r0 = blockIdx.x * TR, c0 = blockIdx.y * TC;
