@@ -391,6 +391,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-only", action="store_true", help="warmup + steps only (for rocprofv3 runs)")
+    ap.add_argument("--stage-timing", action="store_true",
+                    help="with --profile-only: run the steps with the stage timers on (match spins in series), "
+                         "the schedule whose kernel durations the roofline figures use")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="with --profile-only: also run one device copy of MIB MiB (byte-counter calibration)")
     ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched", "learned",
@@ -447,6 +450,8 @@ def main():
         device.synchronize()
         return e0.elapsed_ms(e1) / steps, time.perf_counter() - t0
 
+    if args.profile_only and args.stage_timing:
+        ctx.set_timing(True)  # the stage-timed (serial) schedule the roofline figures come from
     ms, wall = timed(args.steps)
     if args.profile_only:
         if args.calibrate_copy:
