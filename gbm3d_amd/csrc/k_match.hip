@@ -149,6 +149,12 @@ __device__ __forceinline__ uint64_t cand_key(float dist, int idx) {
   return ((uint64_t)o << 32) | (uint32_t)idx;
 }
 
+// The ordered bits of cand_key's high word, and back.
+__device__ __forceinline__ uint32_t ord_bits(float dist) { return (uint32_t)(cand_key(dist, 0) >> 32); }
+__device__ __forceinline__ float ord_to_f32(uint32_t o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
 // Exact K smallest of the wave's 64*NL keys; lane k (< K) returns the k-th.
 // Fast path: theta = K-th smallest per-lane minimum distance bounds the
 // answer from above, so only keys with distance <= theta (at least K of them)
@@ -229,7 +235,7 @@ __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict_
   wave_sync();
   const float nx = bn[(ti * D + S2) + (size_t)Pm * (tj * D + S2)];
 
-  uint64_t key[C::NL];
+  float numv[C::NL];  // |x|^2 + |y|^2 - 2<x,y> of this lane's candidates
 #pragma unroll
   for (int it = 0; it < C::IT; ++it) {
     const int ch = lane + 64 * it;
@@ -256,12 +262,59 @@ __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict_
     for (int i = 0; i < C::RA; ++i) {
       const int a = a0 + i;
       const float bnp = bn[(ti * D + a) + (size_t)Pm * (tj * D + b)];
-      const float num = (nx + bnp) - 2.0f * acc[i];
-      key[it * C::RA + i] = cand_key(num / nx, a + D2 * b);
+      numv[it * C::RA + i] = (nx + bnp) - 2.0f * acc[i];
     }
   }
+  auto cand_idx = [&](int j) {
+    const int ch = lane + 64 * (j / C::RA);
+    return (ch / D2) * C::RA + j % C::RA + D2 * (ch % D2);
+  };
 
-  const uint64_t best = wave_topk<C::NL>(key, K, scr[wave], lane);
+  // The distance is num / |x|^2 with one |x|^2 per tile.  For a finite positive
+  // |x|^2 the rounded quotient is non-decreasing in num, so with theta = the
+  // K-th smallest lane minimum of num (at least K candidates have num <= theta),
+  // every candidate whose quotient can reach the K smallest has
+  // num <= theta + |theta| 2^-23 (the rounding of the quotient) plus a term for
+  // quotients below the normal range; those (usually < 64) are divided, keyed
+  // exactly like the FP32 restatement's candidates and sorted.
+  uint64_t best = 0;
+  bool done = false;
+  if (nx > 0.f && nx <= 3.4028235e38f) {
+    uint32_t lmin = 0xffffffffu;
+#pragma unroll
+    for (int j = 0; j < C::NL; ++j) lmin = min(lmin, ord_bits(numv[j]));
+    const uint32_t th = (uint32_t)__shfl((int)wave_sort64<uint32_t>(lmin, lane), K - 1, 64);
+    if (th != 0xffffffffu) {
+      const float tnum = ord_to_f32(th);
+      const float bound = fmaf(fabsf(tnum), 0x1p-20f, tnum) + nx * 0x1p-126f;
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < C::NL; ++j) cnt += numv[j] <= bound ? 1 : 0;
+      const int off = wave_excl_scan(cnt, lane);
+      const int total = __shfl(off + cnt, 63, 64);
+      if (total <= 64) {
+        int pos = off;
+#pragma unroll
+        for (int j = 0; j < C::NL; ++j)
+          if (numv[j] <= bound) scr[wave][pos++] = ((uint64_t)__float_as_uint(numv[j]) << 32) | (uint32_t)cand_idx(j);
+        wave_sync();
+        uint64_t ek = ~0ull;
+        if (lane < total) {
+          const uint64_t e = scr[wave][lane];
+          ek = cand_key(__uint_as_float((uint32_t)(e >> 32)) / nx, (int)(uint32_t)e);
+        }
+        wave_sync();
+        best = wave_sort64<uint64_t>(ek, lane);
+        done = true;
+      }
+    }
+  }
+  if (!done) {
+    uint64_t key[C::NL];
+#pragma unroll
+    for (int j = 0; j < C::NL; ++j) key[j] = cand_key(numv[j] / nx, cand_idx(j));
+    best = wave_topk<C::NL>(key, K, scr[wave], lane);
+  }
   int idx = (int)(uint32_t)best;
   if (lane == 0) idx = S2 + D2 * S2;  // slot 1 := window centre (matchBlocksCPU.m:103)
   if (lane < K) S[lane + (size_t)K * g] = (uint32_t)idx + 1u;
