@@ -35,7 +35,7 @@ struct gbm3d_ctx {
   int tuned_streams = 0, tuned_match_waves = 0;
   double tuned_ms = 0.0;
   std::vector<int64_t> tkey;
-  int match_waves = 1;  // reference tiles per matching workgroup (single-image path)
+  int match_waves = 4;  // reference tiles per matching workgroup (single-image path)
   // coloured noise of the current call (gbm3d_denoise_colored)
   bool colored = false;
   std::vector<double> ctau;    // levels x 7
@@ -961,7 +961,7 @@ int issue_batch(gbm3d_ctx* c, const BatchJob& j, const gbm3d_opts& o, int S, int
     (void)hipStreamWaitEvent(c->stream, c->bjoin[s], 0);
   }
   c->in_batch = false;
-  c->match_waves = 1;
+  c->match_waves = 4;
   return st;
 }
 

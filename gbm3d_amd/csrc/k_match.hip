@@ -210,29 +210,39 @@ struct MatchCfg {
   static_assert(NCH % 64 == 0, "chunking");
 };
 
-// One wave per reference tile, 4 waves per workgroup.  The (D2+D)^2 search
-// window is staged in LDS; window index a + D2*b <-> Ipad(i*D + a, j*D + b).
+// One wave per reference tile; a workgroup's NW waves take NW tiles adjacent
+// along the tile rows (same tile column), whose (D2+D)^2 search windows are
+// staged in LDS once as their union, (D2+D+(NW-1)D) x (D2+D); wave w's window
+// starts w*D rows down.  Window index a + D2*b <-> Ipad(i*D + a, j*D + b).
 template <int D, int D2, int NW = 4>
 __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
                                                         int M, int N, int K, uint32_t* __restrict__ S,
                                                         float* __restrict__ vol) {
   using C = MatchCfg<D, D2>;
   constexpr int S2 = D2 / 2;
-  __shared__ float win[NW][C::WR * C::LDW];
+  constexpr int RW = C::WR + (NW - 1) * D;          // union rows
+  constexpr int LDW = (RW % 2 == 0) ? RW + 1 : RW;  // odd: conflict-free columns
+  __shared__ float win[C::WR * LDW];
   __shared__ uint64_t scr[NW][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int Mt = M / D, Nt = N / D;
-  const int g = xcd_remap(blockIdx.x, gridDim.x) * NW + wave;
-  if (g >= Mt * Nt) return;
-  const int ti = g % Mt, tj = g / Mt;
+  const int Mt = M / D, Mb = (Mt + NW - 1) / NW;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti0 = (blk % Mb) * NW, tj = blk / Mb;
+  const int nval = min(NW, Mt - ti0);
   const int P = M + D2, Pm = P - D + 1;
-  float* W = win[wave];
-  const float* src = pad + (size_t)(ti * D) + (size_t)P * (tj * D);
-  for (int e = lane; e < C::WR * C::WR; e += 64) {
-    const int rr = e % C::WR, cc = e / C::WR;
-    W[rr + C::LDW * cc] = src[rr + (size_t)P * cc];
+  const int rows = C::WR + (nval - 1) * D;  // union rows inside the padded image
+  const float* src = pad + (size_t)(ti0 * D) + (size_t)P * (tj * D);
+  for (int e = threadIdx.x; e < RW * C::WR; e += 64 * NW) {
+    const int rr = e % RW, cc = e / RW;
+    win[rr + LDW * cc] = rr < rows ? src[rr + (size_t)P * cc] : 0.f;
   }
-  wave_sync();
+  if (NW > 1)
+    __syncthreads();
+  else
+    wave_sync();
+  if (wave >= nval) return;  // no barrier below
+  const int ti = ti0 + wave, g = ti + Mt * tj;
+  const float* W = win + D * wave;
   const float nx = bn[(ti * D + S2) + (size_t)Pm * (tj * D + S2)];
 
   float numv[C::NL];  // |x|^2 + |y|^2 - 2<x,y> of this lane's candidates
@@ -246,8 +256,8 @@ __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict_
     for (int v = 0; v < D; ++v) {
       float xs[D];
 #pragma unroll
-      for (int u = 0; u < D; ++u) xs[u] = W[(S2 + u) + C::LDW * (S2 + v)];
-      const float* col = W + a0 + C::LDW * (b + v);
+      for (int u = 0; u < D; ++u) xs[u] = W[(S2 + u) + LDW * (S2 + v)];
+      const float* col = W + a0 + LDW * (b + v);
 #pragma unroll
       for (int t = 0; t < C::RA + D - 1; ++t) {
         const float y = col[t];
@@ -327,7 +337,7 @@ __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict_
     float* dst = vol + MN * k + (size_t)(ti * D) + (size_t)M * (tj * D);
     for (int e = lane; e < D * D; e += 64) {
       const int u = e % D, v = e / D;
-      dst[u + (size_t)M * v] = W[(a + u) + C::LDW * (b + v)];
+      dst[u + (size_t)M * v] = W[(a + u) + LDW * (b + v)];
     }
   }
 }
@@ -335,13 +345,14 @@ __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict_
 template <int D, int D2>
 static void launch_match_t(const float* pad, const float* bn, int M, int N, int K, uint32_t* S, float* vol, int nw,
                            hipStream_t st) {
-  const int G = (M / D) * (N / D);
+  const int Mt = M / D, Nt = N / D;
+  const unsigned blocks = (unsigned)(((Mt + nw - 1) / nw) * Nt);
   if (nw == 1)
-    hipLaunchKernelGGL((match_kernel<D, D2, 1>), dim3(G), dim3(64), 0, st, pad, bn, M, N, K, S, vol);
+    hipLaunchKernelGGL((match_kernel<D, D2, 1>), dim3(blocks), dim3(64), 0, st, pad, bn, M, N, K, S, vol);
   else if (nw == 2)
-    hipLaunchKernelGGL((match_kernel<D, D2, 2>), dim3((G + 1) / 2), dim3(128), 0, st, pad, bn, M, N, K, S, vol);
+    hipLaunchKernelGGL((match_kernel<D, D2, 2>), dim3(blocks), dim3(128), 0, st, pad, bn, M, N, K, S, vol);
   else
-    hipLaunchKernelGGL((match_kernel<D, D2, 4>), dim3((G + 3) / 4), dim3(256), 0, st, pad, bn, M, N, K, S, vol);
+    hipLaunchKernelGGL((match_kernel<D, D2, 4>), dim3(blocks), dim3(256), 0, st, pad, bn, M, N, K, S, vol);
 }
 
 // nw: reference tiles (waves) per workgroup, 1, 2 or 4 (a launch-shape choice
