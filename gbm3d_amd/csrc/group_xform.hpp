@@ -103,8 +103,9 @@ __device__ __forceinline__ void group_sync() {
 // 2-D transform of all K slices held one value per thread:
 // forward  X <- T X T'   (thread t = u + D v holds X(u, v));
 // inverse  X <- T' X T.
-// Pass 1 combines along u (column of the slice), pass 2 along v.
-template <int D, int K, bool INV>
+// Pass 1 combines along u (column of the slice), pass 2 along v.  FMA: each
+// dot product as one fused multiply-add chain (else multiply, then add).
+template <int D, int K, bool INV, bool FMA = false>
 __device__ __forceinline__ void page_transform(float (&x)[K], float* sm, const float* __restrict__ T, int u, int v) {
   constexpr int TPG = D * D;
   // stage slices as sm[k][v][u]
@@ -120,7 +121,7 @@ __device__ __forceinline__ void page_transform(float (&x)[K], float* sm, const f
   for (int k = 0; k < K; ++k) {
     float acc = 0.f;
 #pragma unroll
-    for (int j = 0; j < D; ++j) acc += Lrow[j] * sm[k * TPG + v * D + j];
+    for (int j = 0; j < D; ++j) acc = FMA ? fmaf(Lrow[j], sm[k * TPG + v * D + j], acc) : acc + Lrow[j] * sm[k * TPG + v * D + j];
     tmp[k] = acc;
   }
   group_sync<TPG>();
@@ -136,7 +137,7 @@ __device__ __forceinline__ void page_transform(float (&x)[K], float* sm, const f
   for (int k = 0; k < K; ++k) {
     float acc = 0.f;
 #pragma unroll
-    for (int j = 0; j < D; ++j) acc += sm[k * TPG + u * D + j] * Rrow[j];
+    for (int j = 0; j < D; ++j) acc = FMA ? fmaf(sm[k * TPG + u * D + j], Rrow[j], acc) : acc + sm[k * TPG + u * D + j] * Rrow[j];
     x[k] = acc;
   }
   group_sync<TPG>();

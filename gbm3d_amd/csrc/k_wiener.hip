@@ -66,9 +66,9 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
       c2[k] = V2[o];
     }
   }
-  page_transform<D, K, false>(c1, sm, T, u, v);
+  page_transform<D, K, false, true>(c1, sm, T, u, v);
   haar_dec<K, L>(c1);
-  page_transform<D, K, false>(c2, sm, T, u, v);
+  page_transform<D, K, false, true>(c2, sm, T, u, v);
   haar_dec<K, L>(c2);
   float n2 = 0.f;
   const float sigma2 = sig2.s[t];
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
     n2 += f * f;
   }
   haar_rec<K, L>(c2);
-  page_transform<D, K, true>(c2, sm, T, u, v);
+  page_transform<D, K, true, true>(c2, sm, T, u, v);
   if (active) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
