@@ -235,6 +235,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
         if (src[j] < 0) continue;
         const int rr = __builtin_amdgcn_readlane(br, src[j]), cc = __builtin_amdgcn_readlane(bc, src[j]);
         const float wg = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), src[j]));
+        const float rwg = STEP == 1 ? 0.f : 1.0f / wg;  // step 2: one division per block
         const int ob = rr + AS * cc;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
               a.x = a.x + c;
               a.y = a.y + wg * mk[i];
             } else {
-              const float wt = mk[i] / wg;  // w / norm(filt)^2
+              const float wt = mk[i] * rwg;  // w / norm(filt)^2
               a.x = a.x + wt * val[j][i];
               a.y = a.y + wt;
             }

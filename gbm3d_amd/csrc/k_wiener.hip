@@ -41,7 +41,9 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
   __syncthreads();
   const int lg = threadIdx.x / TPG, t = threadIdx.x % TPG;
   const int u = t % D, v = t / D;
+  // one group per wave (D = 8): g is wave-uniform, so its slot indices are scalar loads
   long g = (long)blockIdx.x * GPB + lg;
+  if constexpr (TPG == 64) g = (long)__builtin_amdgcn_readfirstlane((int)g);
   const bool active = g < G;
   if (!active) g = G - 1;
   float* sm = smem[lg];
@@ -75,7 +77,8 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const float a2 = c1[k] * c1[k];
-    float f = a2 / (a2 + sigma2);              // abs(V1).^2./(abs(V1).^2 + sigma.^2)
+    // abs(V1).^2./(abs(V1).^2 + sigma.^2), with the hardware reciprocal (1 ulp)
+    float f = a2 * __builtin_amdgcn_rcpf(a2 + sigma2);
     if (t == 0 && k >= 14) f = 1.f;            // filt(1,1,15:end) = 1
     c2[k] *= f;
     n2 += f * f;
