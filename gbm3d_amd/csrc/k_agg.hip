@@ -55,28 +55,29 @@ __global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __rest
 #pragma unroll
     for (int j = 0; j < KB; ++j) {
       const int k = k0 + j;
-      if (k >= K) break;
+      if (k < K) {  // wave-uniform
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int e = lane + 64 * i;
-        if (e < DD) {
-          const float x = xb[j][i];
-          if (k > 0) s3 += fabsf(x - prev[i]);  // |diff(U,1,3)|
-          prev[i] = x;
-          cur[e] = x;
+        for (int i = 0; i < PER; ++i) {
+          const int e = lane + 64 * i;
+          if (e < DD) {
+            const float x = xb[j][i];
+            if (k > 0) s3 += fabsf(x - prev[i]);  // |diff(U,1,3)|
+            prev[i] = x;
+            cur[e] = x;
+          }
         }
-      }
-      wave_sync();
+        wave_sync();
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int e = lane + 64 * i;
-        if (e < DD) {
-          const int u = e % D, v = e / D;
-          if (v + 3 < D) s2 += fabsf(third_diff(cur[e], cur[e + D], cur[e + 2 * D], cur[e + 3 * D]));
-          if (u + 3 < D) s1 += fabsf(third_diff(cur[e], cur[e + 1], cur[e + 2], cur[e + 3]));
+        for (int i = 0; i < PER; ++i) {
+          const int e = lane + 64 * i;
+          if (e < DD) {
+            const int u = e % D, v = e / D;
+            if (v + 3 < D) s2 += fabsf(third_diff(cur[e], cur[e + D], cur[e + 2 * D], cur[e + 3 * D]));
+            if (u + 3 < D) s1 += fabsf(third_diff(cur[e], cur[e + 1], cur[e + 2], cur[e + 3]));
+          }
         }
+        wave_sync();
       }
-      wave_sync();
     }
   }
   s1 = wave_sum_f32(s1);
@@ -183,15 +184,37 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   const int xlo = first(C0), xhi = min(Nt - 1, (C0 + AT - 1 + s) / D);
   const int ny = yhi - ylo + 1, nx = xhi - xlo + 1;
   const int ncand = (ny > 0 && nx > 0) ? ny * nx * K : 0;
-  // candidate t = ((y - ylo) * nx + (x - xlo)) * K + k: its group, slot index and weight
+  // candidate t = ((y - ylo) * nx + (x - xlo)) * K + k.  Quotients without
+  // integer division: K a power of two takes shifts; yx / nx (yx < 2^16, small
+  // nx) is floor((yx + 1/2) / nx) from a float reciprocal, exact at these sizes.
+  const bool kpow2 = (K & (K - 1)) == 0;
+  const int lk = 31 - __clz(K);
+  const float rnx = 1.0f / (float)max(nx, 1);
+  auto split = [&](int t, int& yx, int& k) {
+    if (kpow2) {
+      yx = t >> lk;
+      k = t & (K - 1);
+    } else {
+      yx = t / K;
+      k = t - yx * K;
+    }
+  };
+  auto tile_of = [&](int yx, int& y, int& x) {
+    const int qy = (int)(((float)yx + 0.5f) * rnx);
+    y = ylo + qy;
+    x = xlo + (yx - qy * nx);
+  };
+  // its group, slot index and weight
   auto fetch = [&](int t, int& g, uint32_t& sidx, float& w) {
     g = 0;
     sidx = 0;
     w = 0.f;
     if (t < ncand) {
-      const int yx = t / K;
-      g = (ylo + yx / nx) + Mt * (xlo + yx % nx);
-      sidx = S[(size_t)K * g + t % K];
+      int yx, k, y, x;
+      split(t, yx, k);
+      tile_of(yx, y, x);
+      g = y + Mt * x;
+      sidx = S[(size_t)K * g + k];
       w = wgt[g];
     }
   };
@@ -204,15 +227,18 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
     const uint32_t sidx = sn;
     const float wv = wn;
     if (base + 64 < ncand) fetch(t + 64, gn, sn, wn);
-    int br = 0, bc = 0, blk = 0;
+    int br = 0, bc = 0, blk = 0, brc = 0;
     bool hit = false;
     if (t < ncand) {
-      const int yx = t / K;
+      int yx, k, y, x;
+      split(t, yx, k);
+      tile_of(yx, y, x);
       const uint32_t idx = sidx - 1u;
-      br = (ylo + yx / nx) * D + (int)(idx % (uint32_t)d2) - s - R0;  // block origin relative to the tile
-      bc = (xlo + yx % nx) * D + (int)(idx / (uint32_t)d2) - s - C0;
+      br = y * D + (int)(idx % (uint32_t)d2) - s - R0;  // block origin relative to the tile
+      bc = x * D + (int)(idx / (uint32_t)d2) - s - C0;
       hit = br < AT && br + D > 0 && bc < AT && bc + D > 0;
-      blk = t % K + K * g;
+      blk = k + K * g;
+      brc = (br + 64) | ((bc + 64) << 16);  // both in (-64, 64) for a hit
     }
     uint64_t mask = __ballot(hit);
     while (mask) {
@@ -233,7 +259,8 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         if (src[j] < 0) continue;
-        const int rr = __builtin_amdgcn_readlane(br, src[j]), cc = __builtin_amdgcn_readlane(bc, src[j]);
+        const int rc = __builtin_amdgcn_readlane(brc, src[j]);
+        const int rr = (rc & 0xffff) - 64, cc = (rc >> 16) - 64;
         const float wg = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), src[j]));
         const float rwg = STEP == 1 ? 0.f : 1.0f / wg;  // step 2: one division per block
         const int ob = rr + AS * cc;
