@@ -776,6 +776,94 @@ __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict
   }
 }
 
+// Level-0 approximation of both cycle spins from one read of the volume.
+// Spin s's coefficient (t1, t2, t3) combines planes (2t3 - s, 2t3 - 1 - s) and
+// the window rows starting at 2 t1 - 9 - s, so a 73 x 73 window over planes
+// 2t3-2 .. 2t3 serves both: spin 1 uses planes (2t3-1, 2t3-2) and the
+// sub-window at (0, 0), spin 0 planes (2t3, 2t3-1) and the sub-window at
+// (1, 1).  Per spin the arithmetic is that of ht_lo_fast_kernel.  Workgroups
+// take the z pairs of one tile consecutively, so the plane two neighbours share
+// is read from the L2.
+namespace fl2 {
+constexpr int P = fl::P, R = fl::R, U = R + 1, ZS = U, OS = P + 1;
+constexpr int NIT = (U * U + 255) / 256;
+}  // namespace fl2
+
+__global__ __launch_bounds__(256) void ht_lo0_cs2_kernel(const float* __restrict__ X, int p, int q, int r,
+                                                         float* __restrict__ Y0, float* __restrict__ Y1) {
+  using namespace fl2;
+  __shared__ float Z[U * ZS];    // z-lowpass plane [col][row] of the union window
+  __shared__ float T2[P * ZS];   // dim-2 lowpass [c][row]
+  __shared__ float Out[P * OS];  // output tile [c][t]
+  __shared__ int ridx[U], cidxp[U];
+  const int tid = threadIdx.x;
+  const int ph = p >> 1, qh = q >> 1, rh = r >> 1;
+  const int nbx = (ph + P - 1) / P, nby = (qh + P - 1) / P;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int t3 = lb % rh, tile = lb / rh;
+  const int t1b = (tile % nbx) * P, t2b = (tile / nbx) * P;
+  if (tid < U)
+    ridx[tid] = pmod_i(2 * t1b - 10 + tid, p);
+  else if (tid < 2 * U)
+    cidxp[tid - U] = p * pmod_i(2 * t2b - 10 + tid - U, q);
+  __syncthreads();
+  const size_t pq = (size_t)p * q;
+  const float* Xa = X + pq * pmod_i(2 * t3 - 2, r);
+  const float* Xb = X + pq * pmod_i(2 * t3 - 1, r);
+  const float* Xc = X + pq * pmod_i(2 * t3, r);
+  float pa[NIT], pb[NIT], pc[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int e = tid + 256 * it;
+    pa[it] = pb[it] = pc[it] = 0.f;
+    if (e < U * U) {
+      const int o = ridx[e % U] + cidxp[e / U];
+      pa[it] = Xa[o];
+      pb[it] = Xb[o];
+      pc[it] = Xc[o];
+    }
+  }
+  // s = 1: planes (2t3-1, 2t3-2), offset 0; s = 0: planes (2t3, 2t3-1), offset 1
+#pragma unroll 1
+  for (int s = 1; s >= 0; --s) {
+    const int off = 1 - s;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = tid + 256 * it;  // = column * U + row
+      if (e < U * U) Z[e] = s ? fCc * pb[it] + fCc * pa[it] : fCc * pc[it] + fCc * pb[it];
+    }
+    __syncthreads();
+    for (int e = tid; e < 4 * R; e += 256) {
+      const int rr = e % R, c0 = (e / R) * 8;
+      float v[24], lo[8];
+#pragma unroll
+      for (int j = 0; j < 24; ++j) v[j] = Z[(2 * c0 + j + off) * ZS + rr + off];
+      ana_lo<8>(v, lo);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) T2[(c0 + i) * ZS + rr] = lo[i];
+    }
+    __syncthreads();
+    if (tid < 4 * P) {
+      const int c = tid % P, t0 = (tid / P) * 8;
+      float v[24], lo[8];
+#pragma unroll
+      for (int j = 0; j < 24; ++j) v[j] = T2[c * ZS + 2 * t0 + j];
+      ana_lo<8>(v, lo);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Out[c * OS + t0 + i] = lo[i];
+    }
+    __syncthreads();
+    float* Y = s ? Y1 : Y0;
+    for (int e = tid; e < P * P; e += 256) {
+      const int t = e % P, c = e / P;
+      const int t1 = t1b + t, t2 = t2b + c;
+      if (t1 < ph && t2 < qh) Y[t1 + (size_t)ph * (t2 + (size_t)qh * t3)] = Out[c * OS + t];
+    }
+    // the next spin rewrites Z only after every thread has passed the barrier
+    // above, and T2 / Out only after the next two barriers
+  }
+}
+
 // --------------------------------------------------------------- driver
 // Force the generic (per-tap predicated) kernels everywhere; set through
 // GBM3D_HT_GENERIC=1 for cross-checking the two implementations.
@@ -830,10 +918,17 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
   const char* unf = getenv("GBM3D_HT_UNFUSED");
   const bool fused = cycleSpin == 2 && M >= 10 && N >= 10 && (size_t)M * N * K < ((size_t)1 << 31) &&
                      !g_ht_generic && !(unf && unf[0] == '1');
+  if (fused) {
+    mark(GBM3D_STAGE_HT_LO, 1);
+    const int nb = ((M / 2 + fl::P - 1) / fl::P) * ((N / 2 + fl::P - 1) / fl::P) * (K / 2);
+    hipLaunchKernelGGL(ht_lo0_cs2_kernel, dim3(nb), dim3(256), 0, st, vol, M, N, K, w.lo[0][0], w.lo[0][1]);
+    GBM3D_TRY(check_launch("ht_lo0_cs2"));
+    mark(GBM3D_STAGE_HT_LO, 0);
+  }
   for (int s = 0; s < cycleSpin; ++s) {
     // analysis chain (approximations only)
     mark(GBM3D_STAGE_HT_LO, 1);
-    for (int l = 0; l < levels; ++l) {
+    for (int l = fused ? 1 : 0; l < levels; ++l) {
       const int p = M >> l, q = N >> l, r = K >> l;
       const float* in = l == 0 ? vol : w.lo[l - 1][s];
       dim3 grid((p / 2 + LT - 1) / LT, (q / 2 + LT - 1) / LT, r / 2);
