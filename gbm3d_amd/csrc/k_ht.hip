@@ -433,7 +433,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
         }
       }
     }
-    __syncthreads();  // E (phase 5) and B (phase 4) are overwritten by the next pair
+    // No barrier here: the next pair first writes Z (over B, last read in phase 4,
+    // behind the barrier above) and writes A (over E, read in phase 5) only
+    // after its own first barrier, which every thread reaches after phase 5.
   }
 }
 
@@ -701,7 +703,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
       for (int i = 0; i < 4; ++i) pend[i] = ve[i];
       if (az == r - 1) store_plane(r - 1, first, pend);
     }
-    __syncthreads();  // E (phase 5) and B (phase 4) are overwritten by the next anchor
+    // No barrier here: the next anchor first writes Z (over B, last read in
+    // phase 4, behind the barrier above) and the coarse staging (read in phase
+    // 3), and writes A (over E, read in phase 5) only after its own first
+    // barrier, which every thread reaches after phase 5.
   }
 }
 
