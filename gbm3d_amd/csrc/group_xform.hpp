@@ -144,6 +144,84 @@ __device__ __forceinline__ void page_transform(float (&x)[K], float* sm, const f
 }
 
 
+// 8-point DCT-II along one line of 8 values (stride ST in LDS), in place, from
+// the table entries (uniform: scalar registers).  T(k, 7-n) = (-1)^k T(k, n),
+// so with s_n = x_n + x_{7-n}, d_n = x_n - x_{7-n} (n < 4)
+//   X_{2m} = sum_{n<4} T(2m, n) s_n,  X_{2m+1} = sum_{n<4} T(2m+1, n) d_n,
+// and the inverse x_n = E_n + O_n, x_{7-n} = E_n - O_n with
+//   E_n = sum_m T(2m, n) X_{2m},  O_n = sum_m T(2m+1, n) X_{2m+1}:
+// 8 additions and 32 fused multiply-adds instead of 64.
+template <bool INV, int ST>
+__device__ __forceinline__ void dct8_line(float* p, const DctTable<8>& tab) {
+  float x[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x[i] = p[i * ST];
+  if (!INV) {
+    float sv[4], dv[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      sv[n] = x[n] + x[7 - n];
+      dv[n] = x[n] - x[7 - n];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      float e = 0.f, o = 0.f;
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        e = fmaf(tab.t[2 * m + 8 * n], sv[n], e);
+        o = fmaf(tab.t[2 * m + 1 + 8 * n], dv[n], o);
+      }
+      p[(2 * m) * ST] = e;
+      p[(2 * m + 1) * ST] = o;
+    }
+  } else {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      float e = 0.f, o = 0.f;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        e = fmaf(tab.t[2 * m + 8 * n], x[2 * m], e);
+        o = fmaf(tab.t[2 * m + 1 + 8 * n], x[2 * m + 1], o);
+      }
+      p[n * ST] = e + o;
+      p[(7 - n) * ST] = e - o;
+    }
+  }
+}
+
+// page_transform for D = 8 with the lines above: the 64 threads of a group
+// take the 8 lines x K slices of each pass two or more at a time.  Slices are
+// staged as sm[k][v][u] at k*72 + v*9 + u (kDct8Slice floats per slice): the
+// line bases of one instruction, 9v + 8k (pass 1) and 8k + u (pass 2), are
+// distinct modulo the 64 LDS banks.
+constexpr int kDct8Row = 9, kDct8Slice = 72;
+
+template <int K, bool INV>
+__device__ __forceinline__ void page_transform_dct8(float (&x)[K], float* sm, const DctTable<8>& tab, int u, int v) {
+  constexpr int TPG = 64;
+  const int t = u + 8 * v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) sm[k * kDct8Slice + v * kDct8Row + u] = x[k];
+  group_sync<TPG>();
+  // pass 1: along u for each (v, k)
+#pragma unroll
+  for (int r = 0; r < (8 * K + TPG - 1) / TPG; ++r) {
+    const int task = t + TPG * r;
+    if (task < 8 * K) dct8_line<INV, 1>(sm + (task >> 3) * kDct8Slice + (task & 7) * kDct8Row, tab);
+  }
+  group_sync<TPG>();
+  // pass 2: along v for each (u, k)
+#pragma unroll
+  for (int r = 0; r < (8 * K + TPG - 1) / TPG; ++r) {
+    const int task = t + TPG * r;
+    if (task < 8 * K) dct8_line<INV, kDct8Row>(sm + (task >> 3) * kDct8Slice + (task & 7), tab);
+  }
+  group_sync<TPG>();
+#pragma unroll
+  for (int k = 0; k < K; ++k) x[k] = sm[k * kDct8Slice + v * kDct8Row + u];
+  group_sync<TPG>();
+}
+
 // T(k, n) = c_k cos(pi (2n + 1) k / (2D)), c_0 = sqrt(1/D), c_k = sqrt(2/D).
 template <int D>
 inline DctTable<D> make_dct_table() {

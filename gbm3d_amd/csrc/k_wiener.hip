@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
                                                      float* __restrict__ gain2) {
   constexpr int TPG = D * D, GPB = 256 / TPG;
   constexpr int L = K < 16 ? 2 : 3;
-  __shared__ float smem[GPB][K * TPG];
+  __shared__ float smem[GPB][D == 8 ? K * kDct8Slice : K * TPG];
   __shared__ float T[D * D];
   __shared__ float red[GPB][TPG > 64 ? TPG / 64 : 1];
   for (int e = threadIdx.x; e < D * D; e += blockDim.x) T[e] = tab.t[e];
@@ -68,9 +68,15 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
       c2[k] = V2[o];
     }
   }
-  page_transform<D, K, false, true>(c1, sm, T, u, v);
-  haar_dec<K, L>(c1);
-  page_transform<D, K, false, true>(c2, sm, T, u, v);
+  if constexpr (D == 8) {
+    page_transform_dct8<K, false>(c1, sm, tab, u, v);
+    haar_dec<K, L>(c1);
+    page_transform_dct8<K, false>(c2, sm, tab, u, v);
+  } else {
+    page_transform<D, K, false, true>(c1, sm, T, u, v);
+    haar_dec<K, L>(c1);
+    page_transform<D, K, false, true>(c2, sm, T, u, v);
+  }
   haar_dec<K, L>(c2);
   float n2 = 0.f;
   const float sigma2 = sig2.s[t];
@@ -84,7 +90,10 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
     n2 += f * f;
   }
   haar_rec<K, L>(c2);
-  page_transform<D, K, true, true>(c2, sm, T, u, v);
+  if constexpr (D == 8)
+    page_transform_dct8<K, true>(c2, sm, tab, u, v);
+  else
+    page_transform<D, K, true, true>(c2, sm, T, u, v);
   if (active) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
