@@ -287,10 +287,12 @@ static_assert(4 * C * ES <= REG1, "E fits in A");
 // the next pair (input planes and the coarser reconstruction) are issued into
 // registers before the current pair's filter phases run, so their latency
 // overlaps the LDS work.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_syn_fast_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
-                                                          int p, int q, int r, int sh, Tau7 tau,
-                                                          float* Y, const float* acc, int mode,
-                                                          float cs, int gd, int zpb) {
+// nsp = 2: one launch for both cycle spins of a coarse level (shift 0), the
+// second spin's workgroups after the first's, reading X2 / Ylo2, writing Y2.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_syn_fast_kernel(
+    const float* __restrict__ X, const float* __restrict__ Ylo, int p, int q, int r, int sh, Tau7 tau, float* Y,
+    const float* acc, int mode, float cs, int gd, int zpb, int nsp, const float* __restrict__ X2,
+    const float* __restrict__ Ylo2, float* Y2) {
   using namespace fs;
   __shared__ float lds[REG0 + REG1];
   __shared__ int ridx[R], cidx[R];
@@ -301,7 +303,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   const int tid = threadIdx.x;
   const int nbx = gridDim.x, nby = gridDim.y;
   const int lb = xcd_remap(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
-  const int x0 = (lb % nbx) * P, y0 = ((lb / nbx) % nby) * P, zc = lb / (nbx * nby);
+  const int x0 = (lb % nbx) * P, y0 = ((lb / nbx) % nby) * P;
+  const int nzc = gridDim.z / nsp, zc = (lb / (nbx * nby)) % nzc;
+  if (lb / (nbx * nby) >= nzc) {  // second spin
+    X = X2;
+    Ylo = Ylo2;
+    Y = Y2;
+  }
   const int tz0 = zc * zpb, tz1 = min(r / 2, tz0 + zpb);
   if (tid < R)
     ridx[tid] = pmod_i(x0 - 9 - sh + tid, p);
@@ -714,8 +722,10 @@ namespace fl {
 constexpr int P = 32, R = 2 * P + 8, ZS = R + 1, OS = P + 1;
 }
 
+// nsp = 2: both cycle spins of a coarse level in one launch (see ht_syn_fast_kernel).
 __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict__ X, int p, int q, int r, int sh,
-                                                         float* __restrict__ Y) {
+                                                         float* __restrict__ Y, int nsp, const float* __restrict__ X2,
+                                                         float* __restrict__ Y2) {
   using namespace fl;
   __shared__ float Z[R * ZS];   // z-lowpass plane [col][row]; later the output tile [c][t]
   __shared__ float T2[P * ZS];  // dim-2 lowpass [c][row]
@@ -723,7 +733,12 @@ __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict
   const int tid = threadIdx.x;
   const int nbx = gridDim.x, nby = gridDim.y;
   const int lb = xcd_remap(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
-  const int t1b = (lb % nbx) * P, t2b = ((lb / nbx) % nby) * P, t3 = lb / (nbx * nby);
+  const int nz = gridDim.z / nsp;
+  const int t1b = (lb % nbx) * P, t2b = ((lb / nbx) % nby) * P, t3 = (lb / (nbx * nby)) % nz;
+  if (lb / (nbx * nby) >= nz) {  // second spin
+    X = X2;
+    Y = Y2;
+  }
   if (tid < R)
     ridx[tid] = pmod_i(2 * t1b - 9 - sh + tid, p);
   else if (tid < 2 * R)
@@ -803,7 +818,7 @@ __global__ __launch_bounds__(256) void ht_lo0_cs2_kernel(const float* __restrict
   __shared__ int ridx[U], cidxp[U];
   const int tid = threadIdx.x;
   const int ph = p >> 1, qh = q >> 1, rh = r >> 1;
-  const int nbx = (ph + P - 1) / P, nby = (qh + P - 1) / P;
+  const int nbx = (ph + P - 1) / P;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int t3 = lb % rh, tile = lb / rh;
   const int t1b = (tile % nbx) * P, t2b = (tile / nbx) * P;
@@ -924,28 +939,64 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
   const bool fused = cycleSpin == 2 && M >= 10 && N >= 10 && (size_t)M * N * K < ((size_t)1 << 31) &&
                      !g_ht_generic && !(unf && unf[0] == '1');
   if (fused) {
+    // level-0 approximation of both spins from one read; coarser levels of
+    // both spins in one launch each (no shift below level 0)
     mark(GBM3D_STAGE_HT_LO, 1);
     const int nb = ((M / 2 + fl::P - 1) / fl::P) * ((N / 2 + fl::P - 1) / fl::P) * (K / 2);
     hipLaunchKernelGGL(ht_lo0_cs2_kernel, dim3(nb), dim3(256), 0, st, vol, M, N, K, w.lo[0][0], w.lo[0][1]);
     GBM3D_TRY(check_launch("ht_lo0_cs2"));
+    for (int l = 1; l < levels; ++l) {
+      const int p = M >> l, q = N >> l, r = K >> l;
+      dim3 grid((p / 2 + LT - 1) / LT, (q / 2 + LT - 1) / LT, 2 * (r / 2));
+      if (p >= 10 && q >= 10) {
+        hipLaunchKernelGGL(ht_lo_fast_kernel, grid, dim3(256), 0, st, w.lo[l - 1][0], p, q, r, 0, w.lo[l][0], 2,
+                           w.lo[l - 1][1], w.lo[l][1]);
+      } else {
+        for (int s = 0; s < 2; ++s) {
+          dim3 g1(grid.x, grid.y, r / 2);
+          hipLaunchKernelGGL(ht_lo_kernel, g1, dim3(256), 0, st, w.lo[l - 1][s], p, q, r, 0, w.lo[l][s]);
+        }
+      }
+      GBM3D_TRY(check_launch("ht_lo"));
+    }
     mark(GBM3D_STAGE_HT_LO, 0);
+    for (int l = levels - 1; l >= 1; --l) {
+      const int p = M >> l, q = N >> l, r = K >> l;
+      const float* ylo0 = (l == levels - 1) ? w.lo[l][0] : w.rec[l][0];
+      const float* ylo1 = (l == levels - 1) ? w.lo[l][1] : w.rec[l][1];
+      mark(GBM3D_STAGE_HT_SYN, 1);
+      if (p >= 10 && q >= 10) {
+        const int zpb = kSynPairsPerBlock, nzc = (r / 2 + zpb - 1) / zpb;
+        dim3 gf((p + fs::P - 1) / fs::P, (q + fs::P - 1) / fs::P, 2 * nzc);
+        hipLaunchKernelGGL(ht_syn_fast_kernel, gf, dim3(256), 0, st, w.lo[l - 1][0], ylo0, p, q, r, 0, tl[l],
+                           w.rec[l - 1][0], nullptr, kStore, 2.f, 0, zpb, 2, w.lo[l - 1][1], ylo1, w.rec[l - 1][1]);
+      } else {
+        dim3 grid((p + SP - 1) / SP, (q + SP - 1) / SP, r / 2);
+        for (int s = 0; s < 2; ++s)
+          hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, w.lo[l - 1][s], s ? ylo1 : ylo0, p, q, r, 0,
+                             tl[l], w.rec[l - 1][s], nullptr, kStore, 2.f, 0);
+      }
+      GBM3D_TRY(check_launch("ht_syn"));
+      mark(GBM3D_STAGE_HT_SYN, 0);
+    }
   }
-  for (int s = 0; s < cycleSpin; ++s) {
+  for (int s = 0; s < (fused ? 0 : cycleSpin); ++s) {  // per-spin chains (cycleSpin != 2, cross-checks)
     // analysis chain (approximations only)
     mark(GBM3D_STAGE_HT_LO, 1);
-    for (int l = fused ? 1 : 0; l < levels; ++l) {
+    for (int l = 0; l < levels; ++l) {
       const int p = M >> l, q = N >> l, r = K >> l;
       const float* in = l == 0 ? vol : w.lo[l - 1][s];
       dim3 grid((p / 2 + LT - 1) / LT, (q / 2 + LT - 1) / LT, r / 2);
       if (p >= 10 && q >= 10 && !g_ht_generic)
-        hipLaunchKernelGGL(ht_lo_fast_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s]);
+        hipLaunchKernelGGL(ht_lo_fast_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s], 1,
+                           nullptr, nullptr);
       else
         hipLaunchKernelGGL(ht_lo_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s]);
       GBM3D_TRY(check_launch("ht_lo"));
     }
     mark(GBM3D_STAGE_HT_LO, 0);
     // synthesis chain, coarse to fine
-    for (int l = levels - 1; l >= (fused ? 1 : 0); --l) {
+    for (int l = levels - 1; l >= 0; --l) {
       const int p = M >> l, q = N >> l, r = K >> l;
       const float* in = l == 0 ? vol : w.lo[l - 1][s];
       const float* ylo = (l == levels - 1) ? w.lo[l][s] : w.rec[l][s];
@@ -964,7 +1015,7 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
         const int zpb = kSynPairsPerBlock;
         dim3 gf((p + fs::P - 1) / fs::P, (q + fs::P - 1) / fs::P, (r / 2 + zpb - 1) / zpb);
         hipLaunchKernelGGL(ht_syn_fast_kernel, gf, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out,
-                           acc, mode, cs, lgd, zpb);
+                           acc, mode, cs, lgd, zpb, 1, nullptr, nullptr, nullptr);
       } else
         hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out, acc,
                            mode, cs, lgd);
