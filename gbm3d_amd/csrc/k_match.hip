@@ -290,9 +290,12 @@ __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict_
   uint64_t best = 0;
   bool done = false;
   if (nx > 0.f && nx <= 3.4028235e38f) {
-    uint32_t lmin = 0xffffffffu;
+    // ord_bits is monotone (NaN last, -0 as +0), so the lane minimum of the
+    // ordered bits is the ordered bits of the NaN-ignoring float minimum
+    float fmn = numv[0];
 #pragma unroll
-    for (int j = 0; j < C::NL; ++j) lmin = min(lmin, ord_bits(numv[j]));
+    for (int j = 1; j < C::NL; ++j) fmn = fminf(fmn, numv[j]);
+    const uint32_t lmin = ord_bits(fmn);
     const uint32_t th = (uint32_t)__shfl((int)wave_sort64<uint32_t>(lmin, lane), K - 1, 64);
     if (th != 0xffffffffu) {
       const float tnum = ord_to_f32(th);

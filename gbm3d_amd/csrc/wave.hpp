@@ -47,12 +47,12 @@ __device__ __forceinline__ T wave_sort64(T v, int lane) {
   for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
-      T p = shfl_xor_any(v, j);
-      bool asc = (lane & k) == 0;
-      bool low = (lane & j) == 0;
-      T mn = v < p ? v : p;
-      T mx = v < p ? p : v;
-      v = (asc == low) ? mn : mx;
+      const T p = shfl_xor_any(v, j);
+      // this lane keeps the smaller of the pair where its ascending/lower
+      // flags agree, else the larger: one compare and a select (equal values
+      // are interchangeable)
+      const bool want_min = ((lane & k) == 0) == ((lane & j) == 0);
+      v = ((v < p) == want_min) ? v : p;
     }
   }
   return v;
