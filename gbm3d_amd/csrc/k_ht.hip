@@ -44,6 +44,17 @@ struct Tau7 {
 // subband bits b = dim1_hi | dim2_hi<<1 | z_hi<<2 -> column j of C{i,j}
 __device__ __forceinline__ int band_j(int b) { return b == 3 ? 4 : (b == 4 ? 3 : b); }
 
+// tau.t[band_j(b)] selected over constant indices: the argument block stays in
+// scalar registers (an indexed read of it is a memory load, and a wait for it
+// would also wait for every load issued before it).
+__device__ __forceinline__ float tau_of(const Tau7& tau, int b) {
+  const int j = band_j(b);
+  float t = tau.t[0];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) t = j == i ? tau.t[i] : t;
+  return t;
+}
+
 // ---------------------------------------------------------------- lo kernel
 constexpr int LT = 32;           // coefficients per tile edge
 constexpr int LR = 2 * LT + 8;   // input rows/cols per tile
@@ -322,6 +333,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   // phase-3 roles (fixed per thread): coefficient column, run, dim-2 band, z band
   const int c3 = tid % C, run3 = (tid / C) & 1, b23 = (tid / (2 * C)) & 1, zb3 = tid / (4 * C);
   const bool coarse = tid < 2 * C;  // (zb, b2) = (0, 0): the approximation comes from Ylo
+  const float tl = tau_of(tau, (b23 << 1) | (zb3 << 2)), th = tau_of(tau, (b23 << 1) | (zb3 << 2) | 1);
   float a[NIT], b[NIT], yl[11];
   auto fetch = [&](int t3) {
     const float* X0 = X + pq * pmod_i(2 * t3 - sh, r);
@@ -383,8 +395,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       for (int j = 0; j < 30; ++j) v[j] = A[((zb3 * 2 + b23) * C + c3) * AS + 2 * t0 + j];
       float lo[11], hi[11];
       ana_lo_hi<11>(v, lo, hi);
-      const int bl = (b23 << 1) | (zb3 << 2), bh = bl | 1;
-      const float tl = tau.t[band_j(bl)], th = tau.t[band_j(bh)];
       float* Bl = B + ((zb3 * 4 + (b23 << 1)) * C + c3) * BS + t0;
       float* Bh = Bl + C * BS;
       if (coarse) {
@@ -523,6 +533,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
   const size_t pq = (size_t)p * q;
   const int c3 = tid % C, run3 = (tid / C) & 1, b23 = (tid / (2 * C)) & 1, zb3 = tid / (4 * C);
   const bool coarse = tid < 2 * C;
+  const float tl = tau_of(tau, (b23 << 1) | (zb3 << 2)), th = tau_of(tau, (b23 << 1) | (zb3 << 2) | 1);
   float cur[NIT], prev[NIT], yl[11];
   int off[NIT];  // in-plane offsets of this thread's window elements e = tid + 256 it (plane < 2^31 elements)
 #pragma unroll
@@ -617,8 +628,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
       for (int j = 0; j < 30; ++j) v[j] = A[((zb3 * 2 + b23) * C + c3) * AS + 2 * t0 + j];
       float lo[11], hi[11];
       ana_lo_hi<11>(v, lo, hi);
-      const int bl = (b23 << 1) | (zb3 << 2), bh = bl | 1;
-      const float tl = tau.t[band_j(bl)], th = tau.t[band_j(bh)];
       float* Bl = B + ((zb3 * 4 + (b23 << 1)) * C + c3) * BS + t0;
       float* Bh = Bl + C * BS;
       if (coarse) {
