@@ -264,6 +264,13 @@ __device__ __forceinline__ void ana_lo(const float (&v)[2 * R + 8], float (&lo)[
   }
 }
 
+// the high band of ana_lo_hi alone (reads v[4 .. 2R+3])
+template <int R>
+__device__ __forceinline__ void ana_hi(const float (&v)[2 * R + 8], float (&hi)[R]) {
+#pragma unroll
+  for (int i = 0; i < R; ++i) hi[i] = fCc * v[2 * i + 4] - fCc * v[2 * i + 5];
+}
+
 // out[i] = sum_{k: i+k even} s_lo(k) lo[(i+k)/2] + s_hi(k) hi[(i+k)/2], i < R2 (window starts at an even output)
 template <int R2>
 __device__ __forceinline__ void syn_lo_hi(const float (&lo)[R2 / 2 + 5], const float (&hi)[R2 / 2 + 5],
@@ -531,8 +538,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
   }
   __syncthreads();
   const size_t pq = (size_t)p * q;
-  const int c3 = tid % C, run3 = (tid / C) & 1, b23 = (tid / (2 * C)) & 1, zb3 = tid / (4 * C);
-  const bool coarse = tid < 2 * C;
+  // phase-3 roles: the 6C tasks of the three detail (z band, dim-2 band) pairs
+  // on threads 0 .. 6C-1 (waves 0 and 1), the 2C tasks of the approximation
+  // pair, whose low band is the rebuilt coarser level (no low-pass analysis),
+  // on threads 128 .. 128+2C-1 (wave 2): no wave runs both kinds
+  static_assert(6 * C <= 128 && 128 + 2 * C <= 256, "phase-3 roles");
+  const int t3r = tid < 128 ? (tid < 6 * C ? tid + 2 * C : -1) : (tid < 128 + 2 * C ? tid - 128 : -1);
+  const int c3 = t3r % C, run3 = (t3r / C) & 1, b23 = (t3r / (2 * C)) & 1, zb3 = t3r / (4 * C);
+  const bool coarse = t3r >= 0 && t3r < 2 * C;
   const float tl = tau_of(tau, (b23 << 1) | (zb3 << 2)), th = tau_of(tau, (b23 << 1) | (zb3 << 2) | 1);
   float cur[NIT], prev[NIT], yl[11];
   int off[NIT];  // in-plane offsets of this thread's window elements e = tid + 256 it (plane < 2^31 elements)
@@ -594,7 +607,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
     }
     if (coarse) {
 #pragma unroll
-      for (int i = 0; i < 11; ++i) ycs[tid * 11 + i] = yl[i];
+      for (int i = 0; i < 11; ++i) ycs[t3r * 11 + i] = yl[i];
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) prev[it] = cur[it];
@@ -621,19 +634,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
     }
     __syncthreads();
     // dim-1 analysis + hard threshold (the approximation is the rebuilt coarser level)
-    if (tid < 2 * 2 * 2 * C) {
+    if (t3r >= 0) {
       const int t0 = run3 * 10;
-      float v[30];
-#pragma unroll
-      for (int j = 0; j < 30; ++j) v[j] = A[((zb3 * 2 + b23) * C + c3) * AS + 2 * t0 + j];
-      float lo[11], hi[11];
-      ana_lo_hi<11>(v, lo, hi);
+      const float* Ac = A + ((zb3 * 2 + b23) * C + c3) * AS + 2 * t0;
       float* Bl = B + ((zb3 * 4 + (b23 << 1)) * C + c3) * BS + t0;
       float* Bh = Bl + C * BS;
+      float v[30], hi[11];
       if (coarse) {
 #pragma unroll
-        for (int i = 0; i < 11; ++i) Bl[i] = ycs[tid * 11 + i];
+        for (int j = 4; j < 26; ++j) v[j] = Ac[j];
+        ana_hi<11>(v, hi);
+#pragma unroll
+        for (int i = 0; i < 11; ++i) Bl[i] = ycs[t3r * 11 + i];
       } else {
+#pragma unroll
+        for (int j = 0; j < 30; ++j) v[j] = Ac[j];
+        float lo[11];
+        ana_lo_hi<11>(v, lo, hi);
 #pragma unroll
         for (int i = 0; i < 11; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
       }
