@@ -319,6 +319,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   float* A = lds + REG0;
   float* E = lds + REG0;
   const int tid = threadIdx.x;
+  __builtin_assume(tid < 256);  // 256-thread blocks: drops the guards of full element rounds
   const int nbx = gridDim.x, nby = gridDim.y;
   const int lb = xcd_remap(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
   const int x0 = (lb % nbx) * P, y0 = ((lb / nbx) % nby) * P;
@@ -337,9 +338,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   const size_t pq = (size_t)p * q;
   const int ph = p >> 1, qh = q >> 1;
   constexpr int NIT = (R * R + 255) / 256;
-  // phase-3 roles (fixed per thread): coefficient column, run, dim-2 band, z band
-  const int c3 = tid % C, run3 = (tid / C) & 1, b23 = (tid / (2 * C)) & 1, zb3 = tid / (4 * C);
-  const bool coarse = tid < 2 * C;  // (zb, b2) = (0, 0): the approximation comes from Ylo
+  // phase-3 roles (fixed per thread): coefficient column, run, dim-2 band, z
+  // band.  The 6C tasks of the three detail (z band, dim-2 band) pairs run on
+  // threads 0 .. 6C-1 (waves 0 and 1), the 2C tasks of the approximation pair,
+  // whose low band comes from Ylo (no low-pass analysis), on threads
+  // 128 .. 128+2C-1 (wave 2): no wave runs both kinds
+  static_assert(6 * C <= 128 && 128 + 2 * C <= 256, "phase-3 roles");
+  const int t3r = tid < 128 ? (tid < 6 * C ? tid + 2 * C : -1) : (tid < 128 + 2 * C ? tid - 128 : -1);
+  const int c3 = t3r % C, run3 = (t3r / C) & 1, b23 = (t3r / (2 * C)) & 1, zb3 = t3r / (4 * C);
+  const bool coarse = t3r >= 0 && t3r < 2 * C;
   const float tl = tau_of(tau, (b23 << 1) | (zb3 << 2)), th = tau_of(tau, (b23 << 1) | (zb3 << 2) | 1);
   float a[NIT], b[NIT], yl[11];
   auto fetch = [&](int t3) {
@@ -395,19 +402,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
     }
     __syncthreads();
     // dim-1 analysis + hard threshold (the approximation is the rebuilt coarser level)
-    if (tid < 2 * 2 * 2 * C) {
+    if (t3r >= 0) {
       const int t0 = run3 * 10;
-      float v[30];
-#pragma unroll
-      for (int j = 0; j < 30; ++j) v[j] = A[((zb3 * 2 + b23) * C + c3) * AS + 2 * t0 + j];
-      float lo[11], hi[11];
-      ana_lo_hi<11>(v, lo, hi);
+      const float* Ac = A + ((zb3 * 2 + b23) * C + c3) * AS + 2 * t0;
       float* Bl = B + ((zb3 * 4 + (b23 << 1)) * C + c3) * BS + t0;
       float* Bh = Bl + C * BS;
+      float v[30], hi[11];
       if (coarse) {
+#pragma unroll
+        for (int j = 4; j < 26; ++j) v[j] = Ac[j];
+        ana_hi<11>(v, hi);
 #pragma unroll
         for (int i = 0; i < 11; ++i) Bl[i] = ycur[i];
       } else {
+#pragma unroll
+        for (int j = 0; j < 30; ++j) v[j] = Ac[j];
+        float lo[11];
+        ana_lo_hi<11>(v, lo, hi);
 #pragma unroll
         for (int i = 0; i < 11; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
       }
@@ -521,6 +532,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
   float* A = lds + REG0;
   float* E = lds + REG0;
   const int tid = threadIdx.x;
+  __builtin_assume(tid < 256);  // 256-thread blocks: drops the guards of full element rounds
   const int nbx = gridDim.x, nby = gridDim.y;
   const int lb = xcd_remap(blockIdx.x + nbx * blockIdx.y, nbx * nby);
   const int x0 = (lb % nbx) * P, y0 = (lb / nbx) * P;  // output tile origin (original coordinates)
