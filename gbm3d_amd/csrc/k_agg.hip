@@ -165,13 +165,14 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   float2* ac = acc[wave];
   for (int e = lane; e < AT * AS; e += 64) ac[e] = make_float2(0.f, 0.f);
   float mk[PER];
-  int pu[PER], pv[PER], po[PER];
+  int pu[PER], pv[PER];
+  float2* pa[PER];  // this lane's pixel of a block at the tile origin
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = lane + 64 * i;
     pu[i] = e % D;
     pv[i] = e / D;
-    po[i] = pu[i] + AS * pv[i];
+    pa[i] = ac + (pu[i] + AS * pv[i]);
     mk[i] = e < DD ? masks[(STEP == 1 ? 0 : DD) + e] : 0.f;
   }
   wave_sync();
@@ -190,6 +191,10 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   const bool kpow2 = (K & (K - 1)) == 0;
   const int lk = 31 - __clz(K);
   const float rnx = 1.0f / (float)max(nx, 1);
+  // window index idx < d2^2 (< 2^22): idx / d2 = floor((idx + 1/2) / d2) from
+  // the float reciprocal, exact (the quotient is at least 1/(2 d2) from an
+  // integer, far above the rounding error)
+  const float rd2 = 1.0f / (float)d2;
   auto split = [&](int t, int& yx, int& k) {
     if (kpow2) {
       yx = t >> lk;
@@ -204,40 +209,41 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
     y = ylo + qy;
     x = xlo + (yx - qy * nx);
   };
-  // its group, slot index and weight
-  auto fetch = [&](int t, int& g, uint32_t& sidx, float& w) {
-    g = 0;
-    sidx = 0;
-    w = 0.f;
+  // its reference tile (y, x), slot k, slot index and weight
+  struct Cand {
+    int y, x, k;
+    uint32_t sidx;
+    float w;
+  };
+  auto fetch = [&](int t, Cand& c) {
+    c.y = c.x = c.k = 0;
+    c.sidx = 0;
+    c.w = 0.f;
     if (t < ncand) {
-      int yx, k, y, x;
-      split(t, yx, k);
-      tile_of(yx, y, x);
-      g = y + Mt * x;
-      sidx = S[(size_t)K * g + k];
-      w = wgt[g];
+      int yx;
+      split(t, yx, c.k);
+      tile_of(yx, c.y, c.x);
+      const int g = c.y + Mt * c.x;
+      c.sidx = S[(size_t)K * g + c.k];
+      c.w = wgt[g];
     }
   };
-  int gn;
-  uint32_t sn;
-  float wn;
-  fetch(lane, gn, sn, wn);
+  Cand cn;
+  fetch(lane, cn);
   for (int base = 0; base < ncand; base += 64) {
-    const int t = base + lane, g = gn;
-    const uint32_t sidx = sn;
-    const float wv = wn;
-    if (base + 64 < ncand) fetch(t + 64, gn, sn, wn);
-    int br = 0, bc = 0, blk = 0, brc = 0;
+    const int t = base + lane;
+    const Cand cc0 = cn;
+    const float wv = cc0.w;
+    if (base + 64 < ncand) fetch(t + 64, cn);
+    int blk = 0, brc = 0;
     bool hit = false;
     if (t < ncand) {
-      int yx, k, y, x;
-      split(t, yx, k);
-      tile_of(yx, y, x);
-      const uint32_t idx = sidx - 1u;
-      br = y * D + (int)(idx % (uint32_t)d2) - s - R0;  // block origin relative to the tile
-      bc = x * D + (int)(idx / (uint32_t)d2) - s - C0;
+      const uint32_t idx = cc0.sidx - 1u;
+      const int bq = (int)(((float)idx + 0.5f) * rd2);
+      const int br = cc0.y * D + ((int)idx - bq * d2) - s - R0;  // block origin relative to the tile
+      const int bc = cc0.x * D + bq - s - C0;
       hit = br < AT && br + D > 0 && bc < AT && bc + D > 0;
-      blk = k + K * g;
+      blk = cc0.k + K * (cc0.y + Mt * cc0.x);
       brc = (br + 64) | ((bc + 64) << 16);  // both in (-64, 64) for a hit
     }
     uint64_t mask = __ballot(hit);
@@ -267,7 +273,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
           if (lane + 64 * i < DD && (unsigned)(rr + pu[i]) < (unsigned)AT && (unsigned)(cc + pv[i]) < (unsigned)AT) {
-            float2 a = ac[ob + po[i]];
+            float2 a = pa[i][ob];
             if (STEP == 1) {
               const float c = (wg * val[j][i]) * mk[i];  // weight*blocks(:,:,i).*mask in single
               a.x = a.x + c;
@@ -277,7 +283,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
               a.x = a.x + wt * val[j][i];
               a.y = a.y + wt;
             }
-            ac[ob + po[i]] = a;
+            pa[i][ob] = a;
           }
         }
         // no barrier between blocks: LDS operations of one wave execute in
