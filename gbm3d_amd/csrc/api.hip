@@ -352,8 +352,8 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
   GBM3D_TRY(lanes_join(c, l1, st));
   {
     StageTimer t(c, GBM3D_STAGE_NORM);
-    GBM3D_TRY(launch_normalize(b.U1s, p.n1, b.W1s, p.n1, nullptr, 0, p.M, p.N, p.M, p.N, b.pilot, st));
-    if (dU1) GBM3D_TRY(launch_normalize(b.U1s, p.n1, b.W1s, p.n1, nullptr, 0, p.M, p.N, m, n, dU1, st));
+    // the pilot (padded size) and, from the same pass, U1 (cropped)
+    GBM3D_TRY(launch_normalize(b.U1s, p.n1, b.W1s, p.n1, nullptr, 0, p.M, p.N, p.M, p.N, b.pilot, st, dU1, m, n));
     if (!o.Wiener) {  // :59-63
       GBM3D_TRY(launch_normalize(b.U1s, p.n1, b.W1s, p.n1, nullptr, 0, p.M, p.N, m, n, dU, st));
       return GBM3D_OK;

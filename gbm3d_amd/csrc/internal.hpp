@@ -120,9 +120,11 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
 int launch_wiener(const float* pad_pilot, const float* pad_noisy, const uint32_t* S, const float* V1,
                   const float* V2, int M, int N, int d, int d2, int K, long G, float sigma, float* V2f,
                   float* gain2, hipStream_t st, const float* sig2tab = nullptr);
-// out = sum_s U_s / (sum_s W_s + sum_t Wextra_t), cropped to m x n, double math.
+// out = sum_s U_s / (sum_s W_s + sum_t Wextra_t), cropped to m x n, double math;
+// out2 (nullable) receives the same values cropped to m2 x n2 (m2 <= m, n2 <= n).
 int launch_normalize(const float* U, int nU, const float* W, int nW, const float* Wextra, int nWextra,
-                     int M, int N, int m, int n, float* out, hipStream_t st);
+                     int M, int N, int m, int n, float* out, hipStream_t st, float* out2 = nullptr, int m2 = 0,
+                     int n2 = 0);
 
 // ------------------------------------------------------- learned shrinkage
 // Network over the DCT-Haar coefficients of 8 x 8 x 16 groups (4 conv layers

@@ -332,7 +332,7 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
 // arithmetic (sum(U,3)./sum(W,3) of source/GBM3D.m:57,82), cropped to m x n.
 __global__ void normalize_kernel(const float* __restrict__ U, int nU, const float* __restrict__ W, int nW,
                                  const float* __restrict__ Wx, int nWx, int M, int N, int m, int n,
-                                 float* __restrict__ out) {
+                                 float* __restrict__ out, float* __restrict__ out2, int m2, int n2) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
   if (r >= m) return;
@@ -341,14 +341,18 @@ __global__ void normalize_kernel(const float* __restrict__ U, int nU, const floa
   for (int s = 0; s < nU; ++s) a += (double)U[MN * s + o];
   for (int s = 0; s < nW; ++s) b += (double)W[MN * s + o];
   for (int s = 0; s < nWx; ++s) b += (double)Wx[MN * s + o];
-  out[(size_t)r + (size_t)m * c] = (float)(a / b);
+  const float v = (float)(a / b);
+  out[(size_t)r + (size_t)m * c] = v;
+  if (out2 != nullptr && r < m2 && c < n2) out2[(size_t)r + (size_t)m2 * c] = v;
 }
 
 int launch_normalize(const float* U, int nU, const float* W, int nW, const float* Wextra, int nWextra, int M, int N,
-                     int m, int n, float* out, hipStream_t st) {
+                     int m, int n, float* out, hipStream_t st, float* out2, int m2, int n2) {
   if (m > M || n > N) return fail(GBM3D_ERR_ARG, "normalize: crop");
+  if (out2 != nullptr && (m2 > m || n2 > n)) return fail(GBM3D_ERR_ARG, "normalize: second crop");
   dim3 grid((m + 255) / 256, n);
-  hipLaunchKernelGGL(normalize_kernel, grid, dim3(256), 0, st, U, nU, W, nW, Wextra, nWextra, M, N, m, n, out);
+  hipLaunchKernelGGL(normalize_kernel, grid, dim3(256), 0, st, U, nU, W, nW, Wextra, nWextra, M, N, m, n, out, out2,
+                     m2, n2);
   return check_launch("normalize");
 }
 
