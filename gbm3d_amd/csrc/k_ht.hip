@@ -21,6 +21,16 @@
 
 namespace gbm3d {
 
+// Raw buffer resource over `bytes` bytes from `base` (word 3: 32-bit data
+// format); a load through it takes the base in scalar registers and a 32-bit
+// byte offset in a vector register (no per-load 64-bit address arithmetic).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, 0, 0));
+}
+
 namespace {
 constexpr float fCa = (float)kCa, fCb = (float)kCb, fCc = (float)kCc;
 
@@ -560,18 +570,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
   const bool coarse = t3r >= 0 && t3r < 2 * C;
   const float tl = tau_of(tau, (b23 << 1) | (zb3 << 2)), th = tau_of(tau, (b23 << 1) | (zb3 << 2) | 1);
   float cur[NIT], prev[NIT], yl[11];
-  int off[NIT];  // in-plane offsets of this thread's window elements e = tid + 256 it (plane < 2^31 elements)
+  // in-plane byte offsets of this thread's window elements e = tid + 256 it
+  // (a plane holds fewer than 2^28 elements: volume < 2^31 elements, r >= 10)
+  uint32_t offb[NIT];
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
     const int e = tid + 256 * it;
-    off[it] = e < W * W ? ridx[e % W] + cidxp[e / W] : 0;
+    offb[it] = e < W * W ? 4u * (uint32_t)(ridx[e % W] + cidxp[e / W]) : 0u;
   }
+  const uint32_t pbytes = (uint32_t)(4 * pq);
   auto fetch_plane = [&](float (&dst)[NIT], int z) {
-    const float* Xz = X + pq * z;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(X + pq * z, pbytes);
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       dst[it] = 0.f;
-      if (tid + 256 * it < W * W) dst[it] = Xz[off[it]];
+      if (tid + 256 * it < W * W) dst[it] = buf_load(rs, offb[it]);
     }
   };
   // coarse (rebuilt level-1) approximation of anchor az: spin s = az & 1, level-1 plane (az + s) / 2
