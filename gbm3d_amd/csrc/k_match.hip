@@ -210,6 +210,10 @@ struct MatchCfg {
   static_assert(NCH % 64 == 0, "chunking");
 };
 
+// Largest compacted candidate set the exact selection takes before falling
+// back to K rounds over every candidate.
+constexpr int kSelMax = 256;
+
 // One wave per reference tile; a workgroup's NW waves take NW tiles adjacent
 // along the tile rows (same tile column), whose (D2+D)^2 search windows are
 // staged in LDS once as their union, (D2+D+(NW-1)D) x (D2+D); wave w's window
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict_
   constexpr int RW = C::WR + (NW - 1) * D;          // union rows
   constexpr int LDW = (RW % 2 == 0) ? RW + 1 : RW;  // odd: conflict-free columns
   __shared__ float win[C::WR * LDW];
-  __shared__ uint64_t scr[NW][64];
+  __shared__ uint64_t scr[NW][kSelMax];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int Mt = M / D, Mb = (Mt + NW - 1) / NW;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -305,19 +309,50 @@ __global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict_
       for (int j = 0; j < C::NL; ++j) cnt += numv[j] <= bound ? 1 : 0;
       const int off = wave_excl_scan(cnt, lane);
       const int total = __shfl(off + cnt, 63, 64);
-      if (total <= 64) {
+      if (total <= kSelMax) {
         int pos = off;
 #pragma unroll
         for (int j = 0; j < C::NL; ++j)
           if (numv[j] <= bound) scr[wave][pos++] = ((uint64_t)__float_as_uint(numv[j]) << 32) | (uint32_t)cand_idx(j);
         wave_sync();
-        uint64_t ek = ~0ull;
-        if (lane < total) {
-          const uint64_t e = scr[wave][lane];
-          ek = cand_key(__uint_as_float((uint32_t)(e >> 32)) / nx, (int)(uint32_t)e);
+        auto key_at = [&](int q) {
+          const uint64_t e = scr[wave][q];
+          return cand_key(__uint_as_float((uint32_t)(e >> 32)) / nx, (int)(uint32_t)e);
+        };
+        if (total <= 64) {
+          const uint64_t ek = lane < total ? key_at(lane) : ~0ull;
+          wave_sync();
+          // at most 32 keys (most tiles): the lower half's sort is the answer
+          best = (total <= 32 && K <= 32) ? wave_sort64<uint64_t, 32>(ek, lane) : wave_sort64<uint64_t>(ek, lane);
+        } else {
+          // up to kSelMax keys (best matches crowded into few lanes): each
+          // lane sorts its four, then K rounds take the wave minimum of the
+          // lanes' heads (keys are distinct: the owner alone pops)
+          uint64_t h[kSelMax / 64];
+#pragma unroll
+          for (int r = 0; r < kSelMax / 64; ++r) h[r] = lane + 64 * r < total ? key_at(lane + 64 * r) : ~0ull;
+          wave_sync();
+#pragma unroll
+          for (int a = 0; a < kSelMax / 64; ++a)
+#pragma unroll
+            for (int b = kSelMax / 64 - 1; b > a; --b)
+              if (h[b] < h[b - 1]) {
+                const uint64_t tmp = h[b];
+                h[b] = h[b - 1];
+                h[b - 1] = tmp;
+              }
+          uint64_t res = ~0ull;
+          for (int r = 0; r < K; ++r) {
+            const uint64_t m = wave_min_u64(h[0]);
+            if (lane == r) res = m;
+            if (h[0] == m) {
+#pragma unroll
+              for (int a = 0; a + 1 < kSelMax / 64; ++a) h[a] = h[a + 1];
+              h[kSelMax / 64 - 1] = ~0ull;
+            }
+          }
+          best = res;
         }
-        wave_sync();
-        best = wave_sort64<uint64_t>(ek, lane);
         done = true;
       }
     }

@@ -41,10 +41,12 @@ __device__ __forceinline__ uint32_t shfl_xor_any(uint32_t v, int m) { return shf
 __device__ __forceinline__ uint64_t shfl_xor_any(uint64_t v, int m) { return shfl_xor_u64(v, m); }
 
 // Ascending bitonic sort of one value per lane across the 64 lanes.
-template <class T>
+// N = 32: each half of the wave sorted on its own, the lower ascending.
+template <class T, int N = 64>
 __device__ __forceinline__ T wave_sort64(T v, int lane) {
+  static_assert(N == 32 || N == 64, "sort width");
 #pragma unroll
-  for (int k = 2; k <= 64; k <<= 1) {
+  for (int k = 2; k <= N; k <<= 1) {
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
       const T p = shfl_xor_any(v, j);
