@@ -143,6 +143,11 @@ constexpr int AT = 16;  // output tile edge per wave
 // accumulator's column stride AS makes a half-wave's 64-bit accesses to one
 // block hit distinct banks.  The next round's index and weight loads are
 // issued before the current round's hits are processed.
+// two 16-bit lanes of one register (v_pk_add_u16)
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+static_assert((AT & (AT - 1)) == 0, "AT a power of two");
+constexpr uint32_t kOutsideAT = ~((uint32_t)(AT - 1) | ((uint32_t)(AT - 1) << 16));
+
 template <int D>
 struct AggCfg {
   static constexpr int AS = D <= 8 ? 24 : 16;
@@ -157,6 +162,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
                                                                 float* __restrict__ W) {
   constexpr int DD = D * D, PER = (DD + 63) / 64, NB = 8, AS = AggCfg<D>::AS;
   __shared__ float2 acc[4][AT * AS];
+  __shared__ int hl[4][4][64];  // per wave: the round's hits (block, packed origin, weight, LDS offset)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tilesR = (M + AT - 1) / AT, tilesC = (N + AT - 1) / AT;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
@@ -165,14 +171,13 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   float2* ac = acc[wave];
   for (int e = lane; e < AT * AS; e += 64) ac[e] = make_float2(0.f, 0.f);
   float mk[PER];
-  int pu[PER], pv[PER];
-  float2* pa[PER];  // this lane's pixel of a block at the tile origin
+  uint32_t puv[PER];  // this lane's pixel (u, v) of a block as two 16-bit halves
+  float2* pa[PER];    // the same pixel of a block at the tile origin
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = lane + 64 * i;
-    pu[i] = e % D;
-    pv[i] = e / D;
-    pa[i] = ac + (pu[i] + AS * pv[i]);
+    puv[i] = (uint32_t)(e % D) | ((uint32_t)(e / D) << 16);
+    pa[i] = ac + (e % D + AS * (e / D));
     mk[i] = e < DD ? masks[(STEP == 1 ? 0 : DD) + e] : 0.f;
   }
   wave_sync();
@@ -225,7 +230,9 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
       tile_of(yx, c.y, c.x);
       const int g = c.y + Mt * c.x;
       c.sidx = S[(size_t)K * g + c.k];
-      c.w = wgt[g];
+      // step 2 uses 1/weight (w / norm(filt)^2): divided here, once per
+      // candidate lane, rather than once per hit
+      c.w = STEP == 1 ? wgt[g] : 1.0f / wgt[g];
     }
   };
   Cand cn;
@@ -235,7 +242,8 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
     const Cand cc0 = cn;
     const float wv = cc0.w;
     if (base + 64 < ncand) fetch(t + 64, cn);
-    int blk = 0, brc = 0;
+    int blk = 0, bob = 0;
+    uint32_t brc = 0;
     bool hit = false;
     if (t < ncand) {
       const uint32_t idx = cc0.sidx - 1u;
@@ -244,42 +252,52 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
       const int bc = cc0.x * D + bq - s - C0;
       hit = br < AT && br + D > 0 && bc < AT && bc + D > 0;
       blk = cc0.k + K * (cc0.y + Mt * cc0.x);
-      brc = (br + 64) | ((bc + 64) << 16);  // both in (-64, 64) for a hit
+      brc = ((uint32_t)br & 0xffffu) | ((uint32_t)bc << 16);  // 16-bit two's complement halves
+      bob = br + AS * bc;                                      // its LDS offset (float2 units)
     }
-    uint64_t mask = __ballot(hit);
-    while (mask) {
-      int src[NB];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        src[j] = mask ? __builtin_ctzll(mask) : -1;
-        mask &= mask ? mask - 1 : 0;
-      }
+    // this round's hits, in candidate order, listed in LDS so that lane j
+    // holds the j-th hit's block, offset and weight
+    const uint64_t mask = __ballot(hit);
+    const int nh = __popcll(mask);
+    if (nh == 0) continue;
+    if (hit) {
+      const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+      hl[wave][0][rank] = blk;
+      hl[wave][1][rank] = (int)brc;
+      hl[wave][2][rank] = __float_as_int(wv);
+      hl[wave][3][rank] = bob;
+    }
+    wave_sync();
+    const int hblk = hl[wave][0][lane], hrc = hl[wave][1][lane], hw = hl[wave][2][lane], hob = hl[wave][3][lane];
+    wave_sync();  // read before the next round rewrites the list
+    for (int j0 = 0; j0 < nh; j0 += NB) {
+      // all NB loads issued (past the last hit: the first hit's block again),
+      // so each hit waits only for its own block
       float val[NB][PER];
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
-        if (src[j] < 0) continue;
-        const float* bp = blocks + (size_t)DD * __builtin_amdgcn_readlane(blk, src[j]);
+        const float* bp = blocks + (size_t)DD * __builtin_amdgcn_readlane(hblk, j0 + j < nh ? j0 + j : 0);
 #pragma unroll
         for (int i = 0; i < PER; ++i) val[j][i] = (lane + 64 * i < DD) ? bp[lane + 64 * i] : 0.f;
       }
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
-        if (src[j] < 0) continue;
-        const int rc = __builtin_amdgcn_readlane(brc, src[j]);
-        const int rr = (rc & 0xffff) - 64, cc = (rc >> 16) - 64;
-        const float wg = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), src[j]));
-        const float rwg = STEP == 1 ? 0.f : 1.0f / wg;  // step 2: one division per block
-        const int ob = rr + AS * cc;
+        if (j0 + j >= nh) break;
+        const uint32_t rc = (uint32_t)__builtin_amdgcn_readlane(hrc, j0 + j);
+        const float wg = __int_as_float(__builtin_amdgcn_readlane(hw, j0 + j));  // step 2: 1/weight
+        const int ob = __builtin_amdgcn_readlane(hob, j0 + j);
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-          if (lane + 64 * i < DD && (unsigned)(rr + pu[i]) < (unsigned)AT && (unsigned)(cc + pv[i]) < (unsigned)AT) {
+          // pixel (r + u, c + v) in the tile: both 16-bit sums (wrapping) below AT
+          const us2 t = __builtin_bit_cast(us2, puv[i]) + __builtin_bit_cast(us2, rc);
+          if (lane + 64 * i < DD && (__builtin_bit_cast(uint32_t, t) & kOutsideAT) == 0u) {
             float2 a = pa[i][ob];
             if (STEP == 1) {
               const float c = (wg * val[j][i]) * mk[i];  // weight*blocks(:,:,i).*mask in single
               a.x = a.x + c;
               a.y = a.y + wg * mk[i];
             } else {
-              const float wt = mk[i] * rwg;  // w / norm(filt)^2
+              const float wt = mk[i] * wg;  // w / norm(filt)^2
               a.x = a.x + wt * val[j][i];
               a.y = a.y + wt;
             }
