@@ -42,21 +42,34 @@ __device__ __forceinline__ int padmap_spin(int r, int L, int s, int spin) {
   return x < 0 ? x + L : x;
 }
 
-// spinR = spin mod M, spinC = spin mod N.
+// spinR = spin mod M, spinC = spin mod N.  A thread fills 4 consecutive rows
+// of one padded column (one 16-byte store when the column allows it).
 __global__ void search_pad_kernel(const float* __restrict__ img, int M, int N, int spinR, int spinC, int d2,
                                   float* __restrict__ pad) {
   const int P = M + d2, s = d2 / 2;
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  int c = blockIdx.y;
-  if (r >= P) return;
-  const int sr = padmap_spin(r, M, s, spinR), sc = padmap_spin(c, N, s, spinC);
-  pad[r + (size_t)P * c] = img[sr + (size_t)M * sc];
+  const int r0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int c = blockIdx.y;
+  if (r0 >= P) return;
+  const float* src = img + (size_t)M * padmap_spin(c, N, s, spinC);
+  float* dst = pad + (size_t)P * c + r0;
+  if ((P & 3) == 0 && (reinterpret_cast<uintptr_t>(pad) & 15) == 0) {  // r0 + 3 < P, 16-byte aligned
+    float4 v;
+    v.x = src[padmap_spin(r0, M, s, spinR)];
+    v.y = src[padmap_spin(r0 + 1, M, s, spinR)];
+    v.z = src[padmap_spin(r0 + 2, M, s, spinR)];
+    v.w = src[padmap_spin(r0 + 3, M, s, spinR)];
+    *reinterpret_cast<float4*>(dst) = v;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (r0 + k < P) dst[k] = src[padmap_spin(r0 + k, M, s, spinR)];
+  }
 }
 
 int launch_search_pad(const float* img, int M, int N, int spin, int d2, float* pad, hipStream_t st) {
   if (M < d2 || N < d2 || (d2 & 1)) return fail(GBM3D_ERR_ARG, "search pad: image smaller than matchSize");
   const int spinR = ((spin % M) + M) % M, spinC = ((spin % N) + N) % N;
-  dim3 grid((M + d2 + 255) / 256, N + d2);
+  dim3 grid((M + d2 + 1023) / 1024, N + d2);
   hipLaunchKernelGGL(search_pad_kernel, grid, dim3(256), 0, st, img, M, N, spinR, spinC, d2, pad);
   return check_launch("search_pad");
 }
