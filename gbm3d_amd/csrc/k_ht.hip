@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
     const float* __restrict__ Ylo2, float* Y2) {
   using namespace fs;
   __shared__ float lds[REG0 + REG1];
-  __shared__ int ridx[R], cidx[R];
+  __shared__ int ridx[R], cidx[R], yrow[C];
   float* Z = lds;
   float* B = lds;
   float* A = lds + REG0;
@@ -340,14 +340,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
     Y = Y2;
   }
   const int tz0 = zc * zpb, tz1 = min(r / 2, tz0 + zpb);
+  const int ph = p >> 1, qh = q >> 1;
   if (tid < R)
     ridx[tid] = pmod_i(x0 - 9 - sh + tid, p);
   else if (tid < 2 * R)
     cidx[tid - R] = pmod_i(y0 - 9 - sh + tid - R, q);
+  else if (tid < 2 * R + C)
+    yrow[tid - 2 * R] = pmod_i(x0 / 2 + tid - 2 * R, ph);  // coarse rows of the tile
   __syncthreads();
   const size_t pq = (size_t)p * q;
-  const int ph = p >> 1, qh = q >> 1;
   constexpr int NIT = (R * R + 255) / 256;
+  // in-plane byte offsets of this thread's window elements, fixed over the z
+  // loop (planes of fewer than 2^30 elements, read through buffer loads;
+  // larger planes take 64-bit addresses)
+  const bool buf = pq < ((size_t)1 << 30);
+  uint32_t offb[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int e = tid + 256 * it;
+    offb[it] = e < R * R ? 4u * ((uint32_t)ridx[e % R] + (uint32_t)p * (uint32_t)cidx[e / R]) : 0u;
+  }
   // phase-3 roles (fixed per thread): coefficient column, run, dim-2 band, z
   // band.  The 6C tasks of the three detail (z band, dim-2 band) pairs run on
   // threads 0 .. 6C-1 (waves 0 and 1), the 2C tasks of the approximation pair,
@@ -359,23 +371,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   const bool coarse = t3r >= 0 && t3r < 2 * C;
   const float tl = tau_of(tau, (b23 << 1) | (zb3 << 2)), th = tau_of(tau, (b23 << 1) | (zb3 << 2) | 1);
   float a[NIT], b[NIT], yl[11];
+  const size_t ycol = coarse ? (size_t)ph * pmod_i(y0 / 2 + c3, qh) : 0;  // this thread's coarse column
   auto fetch = [&](int t3) {
     const float* X0 = X + pq * pmod_i(2 * t3 - sh, r);
     const float* X1 = X + pq * pmod_i(2 * t3 - 1 - sh, r);
+    if (buf) {
+      const __amdgpu_buffer_rsrc_t r0 = buf_rsrc(X0, (uint32_t)(4 * pq)), r1 = buf_rsrc(X1, (uint32_t)(4 * pq));
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int e = tid + 256 * it;
-      a[it] = b[it] = 0.f;
-      if (e < R * R) {
-        const size_t o = (size_t)ridx[e % R] + (size_t)p * cidx[e / R];
-        a[it] = X0[o];
-        b[it] = X1[o];
+      for (int it = 0; it < NIT; ++it) {
+        a[it] = b[it] = 0.f;
+        if (tid + 256 * it < R * R) {
+          a[it] = buf_load(r0, offb[it]);
+          b[it] = buf_load(r1, offb[it]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int e = tid + 256 * it;
+        a[it] = b[it] = 0.f;
+        if (e < R * R) {
+          const size_t o = (size_t)ridx[e % R] + (size_t)p * cidx[e / R];
+          a[it] = X0[o];
+          b[it] = X1[o];
+        }
       }
     }
     if (coarse) {
-      const size_t col = (size_t)ph * ((size_t)pmod_i(y0 / 2 + c3, qh) + (size_t)qh * t3);
+      const float* Yl = Ylo + (size_t)ph * qh * t3 + ycol;
 #pragma unroll
-      for (int i = 0; i < 11; ++i) yl[i] = Ylo[(size_t)pmod_i(x0 / 2 + run3 * 10 + i, ph) + col];
+      for (int i = 0; i < 11; ++i) yl[i] = Yl[yrow[run3 * 10 + i]];
     }
   };
   if (tz0 < tz1) fetch(tz0);
