@@ -21,16 +21,6 @@
 
 namespace gbm3d {
 
-// Raw buffer resource over `bytes` bytes from `base` (word 3: 32-bit data
-// format); a load through it takes the base in scalar registers and a 32-bit
-// byte offset in a vector register (no per-load 64-bit address arithmetic).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, 0, 0));
-}
-
 namespace {
 constexpr float fCa = (float)kCa, fCb = (float)kCb, fCc = (float)kCc;
 

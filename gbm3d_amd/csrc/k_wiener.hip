@@ -52,13 +52,30 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
     const int Mt = M / D;
     const int ti = (int)(g % Mt), tj = (int)(g / Mt);
     const size_t P = (size_t)M + d2;
+    if (P * ((size_t)N + d2) < ((size_t)1 << 30)) {
+      // buffer loads: this lane's pixel offset in a vector register, the
+      // slice's block origin (uniform: one group per wave at D = 8) in a
+      // scalar register
+      const uint32_t pbytes = (uint32_t)(4 * P * ((size_t)N + d2));
+      const __amdgpu_buffer_rsrc_t rP = buf_rsrc(padP, pbytes), rN = buf_rsrc(padN, pbytes);
+      const uint32_t lo = 4u * (uint32_t)(u + (int)P * v);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t idx = S[(size_t)K * g + k] - 1u;
-      const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
-      const size_t o = (size_t)(ti * D + a + u) + P * (size_t)(tj * D + b + v);
-      c1[k] = padP[o];
-      c2[k] = padN[o];
+      for (int k = 0; k < K; ++k) {
+        const uint32_t idx = S[(size_t)K * g + k] - 1u;
+        const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
+        const uint32_t so = 4u * (uint32_t)((ti * D + a) + (int)P * (tj * D + b));
+        c1[k] = buf_load(rP, lo, so);
+        c2[k] = buf_load(rN, lo, so);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint32_t idx = S[(size_t)K * g + k] - 1u;
+        const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
+        const size_t o = (size_t)(ti * D + a + u) + P * (size_t)(tj * D + b + v);
+        c1[k] = padP[o];
+        c2[k] = padN[o];
+      }
     }
   } else {
 #pragma unroll
@@ -95,10 +112,11 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
   else
     page_transform<D, K, true, true>(c2, sm, T, u, v);
   if (active) {
+    float* dst = V2f + (size_t)TPG * K * g;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const float y = c2[k];
-      V2f[(size_t)t + (size_t)TPG * (k + (size_t)K * g)] = y > 0.f ? y : 0.f;  // max(V2, 0)
+      dst[t + TPG * k] = y > 0.f ? y : 0.f;  // max(V2, 0)
     }
   }
   // |filt|_F^2 of the group

@@ -40,6 +40,17 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
 __device__ __forceinline__ uint32_t shfl_xor_any(uint32_t v, int m) { return shfl_xor_u32(v, m); }
 __device__ __forceinline__ uint64_t shfl_xor_any(uint64_t v, int m) { return shfl_xor_u64(v, m); }
 
+// Raw buffer resource over `bytes` bytes from `base` (word 3: 32-bit data
+// format); a load through it takes the base in scalar registers, a 32-bit
+// byte offset in a vector register and, optionally, a uniform byte offset in
+// a scalar register (no per-load 64-bit address arithmetic).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes, uint32_t soff_bytes = 0) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, soff_bytes, 0));
+}
+
 // Ascending bitonic sort of one value per lane across the 64 lanes.
 // N = 32: each half of the wave sorted on its own, the lower ascending.
 template <class T, int N = 64>
