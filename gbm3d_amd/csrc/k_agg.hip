@@ -348,29 +348,72 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
 // --------------------------------------------------------- normalisation
 // out(r,c) = sum_s U_s(r,c) / (sum_s W_s(r,c) + sum_t Wx_t(r,c)), double
 // arithmetic (sum(U,3)./sum(W,3) of source/GBM3D.m:57,82), cropped to m x n.
+// A thread takes 4 consecutive rows of one column; vec: 16-byte loads and
+// stores (M, m, m2 multiples of 4 and 16-byte aligned arrays, checked on the
+// host).  The per-element sums are those of the scalar form.
 __global__ void normalize_kernel(const float* __restrict__ U, int nU, const float* __restrict__ W, int nW,
                                  const float* __restrict__ Wx, int nWx, int M, int N, int m, int n,
-                                 float* __restrict__ out, float* __restrict__ out2, int m2, int n2) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+                                 float* __restrict__ out, float* __restrict__ out2, int m2, int n2, int vec) {
+  const int r0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
   const int c = blockIdx.y;
-  if (r >= m) return;
-  const size_t MN = (size_t)M * N, o = (size_t)r + (size_t)M * c;
-  double a = 0.0, b = 0.0;
-  for (int s = 0; s < nU; ++s) a += (double)U[MN * s + o];
-  for (int s = 0; s < nW; ++s) b += (double)W[MN * s + o];
-  for (int s = 0; s < nWx; ++s) b += (double)Wx[MN * s + o];
-  const float v = (float)(a / b);
-  out[(size_t)r + (size_t)m * c] = v;
-  if (out2 != nullptr && r < m2 && c < n2) out2[(size_t)r + (size_t)m2 * c] = v;
+  if (r0 >= m) return;
+  const bool v4 = vec && r0 + 3 < m;
+  const size_t MN = (size_t)M * N, o = (size_t)r0 + (size_t)M * c;
+  double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+  if (v4) {
+    auto add4 = [&](double (&acc)[4], const float* p) {
+      const float4 x = *reinterpret_cast<const float4*>(p + o);
+      acc[0] += (double)x.x;
+      acc[1] += (double)x.y;
+      acc[2] += (double)x.z;
+      acc[3] += (double)x.w;
+    };
+    for (int s = 0; s < nU; ++s) add4(a, U + MN * s);
+    for (int s = 0; s < nW; ++s) add4(b, W + MN * s);
+    for (int s = 0; s < nWx; ++s) add4(b, Wx + MN * s);
+  } else {
+    auto add1 = [&](double (&acc)[4], const float* p) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (r0 + k < m) acc[k] += (double)p[o + k];
+    };
+    for (int s = 0; s < nU; ++s) add1(a, U + MN * s);
+    for (int s = 0; s < nW; ++s) add1(b, W + MN * s);
+    for (int s = 0; s < nWx; ++s) add1(b, Wx + MN * s);
+  }
+  float v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = (float)(a[k] / b[k]);
+  float* d1 = out + (size_t)r0 + (size_t)m * c;
+  if (v4) {
+    *reinterpret_cast<float4*>(d1) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (r0 + k < m) d1[k] = v[k];
+  }
+  if (out2 != nullptr && c < n2) {
+    float* d2 = out2 + (size_t)r0 + (size_t)m2 * c;
+    if (vec && r0 + 3 < m2) {
+      *reinterpret_cast<float4*>(d2) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (r0 + k < m2) d2[k] = v[k];
+    }
+  }
 }
 
 int launch_normalize(const float* U, int nU, const float* W, int nW, const float* Wextra, int nWextra, int M, int N,
                      int m, int n, float* out, hipStream_t st, float* out2, int m2, int n2) {
   if (m > M || n > N) return fail(GBM3D_ERR_ARG, "normalize: crop");
   if (out2 != nullptr && (m2 > m || n2 > n)) return fail(GBM3D_ERR_ARG, "normalize: second crop");
-  dim3 grid((m + 255) / 256, n);
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  const int vec = M % 4 == 0 && m % 4 == 0 && al(U) && al(W) && (nWextra == 0 || al(Wextra)) && al(out) &&
+                  (out2 == nullptr || (m2 % 4 == 0 && al(out2)));
+  dim3 grid((m + 1023) / 1024, n);
   hipLaunchKernelGGL(normalize_kernel, grid, dim3(256), 0, st, U, nU, W, nW, Wextra, nWextra, M, N, m, n, out, out2,
-                     m2, n2);
+                     m2, n2, vec);
   return check_launch("normalize");
 }
 
