@@ -47,3 +47,32 @@ def test_numerator_bound_keeps_exact_top_k():
         sel = num <= bound
         assert sel.sum() >= K
         np.testing.assert_array_equal(np.sort(key[sel])[:K], np.sort(key)[:K])
+
+
+def test_gpu_match_images_cover_every_selection_size():
+    """The matcher sorts the compacted set over 32 lanes (at most 32 members),
+    over 64 lanes (33-64), by a per-lane sort plus K wave-minimum rounds
+    (65-256) or by K rounds over every candidate (more).  The synthetic images of
+    test_gpu_parity.py::test_match_bit_exact_vs_fp32_restatement hold tiles of
+    the first three sizes (the degenerate images there the last), counted here
+    with the kernel's lane layout (lane l: window column l % 32, rows
+    16 (l // 32) .. +15) on the FP32 restatement's distances (the quotient is
+    monotone in the numerator for |x|^2 > 0)."""
+    import oracle as O
+    from gbm3d_amd import synth
+
+    sizes = []
+    for m, n, d, seed in ((256, 192, 8, 4), (256, 256, 16, 3)):
+        _, noisy = synth.make_pair(seed, m, n, 25.0)
+        _, _, dist = O.match(noisy, d, 32, 16, precision=32, want_dist=True)
+        dist = np.asarray(dist)
+        for i in range(dist.shape[1]):
+            for j in range(dist.shape[2]):
+                dd = dist[:, i, j].reshape(32, 32, order="F")  # [a, b], window index a + 32 b
+                lanes = np.stack([dd[16 * (l // 32):16 * (l // 32) + 16, l % 32] for l in range(64)])
+                theta = np.sort(lanes.min(axis=1))[15]
+                sizes.append(int((lanes <= theta * (1 + 2.0**-20)).sum()))
+    sizes = np.array(sizes)
+    assert (sizes <= 32).sum() > 0
+    assert ((sizes > 32) & (sizes <= 64)).sum() > 0
+    assert ((sizes > 64) & (sizes <= 256)).sum() > 10
