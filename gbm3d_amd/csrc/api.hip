@@ -1,6 +1,7 @@
 // api.hip -- the C ABI of libgbm3d: contexts, the two-step pipeline of
 // source/GBM3D.m:21-83, and the stage entry points.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -307,6 +308,10 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
     GBM3D_TRY(launch_mirror_pad(dI, m, n, p.M, p.N, b.Ip, st));  // source/GBM3D.m:22-25
   }
   // ---- step 1: source/GBM3D.m:50-57
+  // The noisy image's search image of spin s is built into a lane's pad2 and
+  // left there: a step-2 spin with the same s on that lane reuses it.
+  int noisy_spin[kMaxLanes];
+  for (int l = 0; l < kMaxLanes; ++l) noisy_spin[l] = INT_MIN;
   const int l1 = std::max(1, std::min(lanes, p.n1));
   GBM3D_TRY(lanes_fork(c, l1, st));
   for (int ii = 0; ii < p.n1; ++ii) {
@@ -315,21 +320,22 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
     SpinBufs& sb = b.sb[ii % l1];
     {
       StageTimer t(c, GBM3D_STAGE_PAD);
-      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad1, ls));
+      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad2, ls));
+      noisy_spin[ii % l1] = sp;
     }
     if (c->ext.learned) {
       // variant: learned gains on the DCT-Haar coefficients of each group,
       // gathered straight from the search image (no mosaic volume)
-      GBM3D_TRY(do_match(c, sb.pad1, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, nullptr, ls,
+      GBM3D_TRY(do_match(c, sb.pad2, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, nullptr, ls,
                          GBM3D_STAGE_MATCH1));
       StageTimer t(c, GBM3D_STAGE_SHRINK);
       const long G1 = (long)(p.M / p.bs) * (p.N / p.bs);
-      GBM3D_TRY(launch_shrink(sb.pad1, sb.S1, p.M, p.d2, G1, sigma, c->shrink_wb, c->shrink_fb, sb.vavg, nullptr,
+      GBM3D_TRY(launch_shrink(sb.pad2, sb.S1, p.M, p.d2, G1, sigma, c->shrink_wb, c->shrink_fb, sb.vavg, nullptr,
                               sstats + (size_t)2 * kShrinkGrid * ii, nullptr, ls));
     } else {
       HtWork hw;
       ht_work_bind(sb.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
-      GBM3D_TRY(do_match(c, sb.pad1, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, sb.vol, ls,
+      GBM3D_TRY(do_match(c, sb.pad2, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, sb.vol, ls,
                          GBM3D_STAGE_MATCH1));
       if (o.filtType == GBM3D_FILT_MEDIAN) {
         StageTimer t(c, GBM3D_STAGE_HT_SYN0);
@@ -371,7 +377,10 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
     {
       StageTimer t(c, GBM3D_STAGE_PAD);
       GBM3D_TRY(launch_search_pad(b.pilot, p.M, p.N, sp, p.d2, sb.pad1, ls));
-      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad2, ls));
+      if (noisy_spin[ii % l2] != sp) {  // else step 1 left this search image in pad2 (the steps are joined)
+        GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad2, ls));
+        noisy_spin[ii % l2] = sp;
+      }
     }
     GBM3D_TRY(do_match(c, sb.pad1, sb.bn, sb.smap, p.M, p.N, p.bw, p.d2, p.K, sb.S2, nullptr, ls,
                        GBM3D_STAGE_MATCH2));
