@@ -656,20 +656,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
       fetch_coarse(az + 1);
     }
     __syncthreads();
-    // dim-2 analysis on the spin's sub-window (offset s in rows and columns)
-    if (tid < 2 * 2 * R) {
-      const int rr = tid % R, run = (tid / R) & 1, zb = tid / (2 * R);
-      const int c0 = run * 10;
-      const float* Zc = Z + zb * W * ZS + (2 * c0 + s) * ZS + rr + s;
-      float v[30];
+    // dim-2 analysis on the spin's sub-window (offset s in rows and columns):
+    // three runs of C/3 coefficients per (z band, row), 6R tasks over two rounds
+    static_assert(C % 3 == 0 && 6 * R <= 512, "phase-2 runs");
 #pragma unroll
-      for (int j = 0; j < 30; ++j) v[j] = Zc[j * ZS];
-      float lo[11], hi[11];
-      ana_lo_hi<11>(v, lo, hi);
+    for (int r2 = 0; r2 < 2; ++r2) {
+      const int t2 = tid + 256 * r2;
+      if (t2 < 6 * R) {
+        const int rr = t2 % R, run = (t2 / R) % 3, zb = t2 / (3 * R);
+        const int c0 = run * (C / 3);
+        const float* Zc = Z + zb * W * ZS + (2 * c0 + s) * ZS + rr + s;
+        float v[2 * (C / 3) + 8];
 #pragma unroll
-      for (int i = 0; i < 11; ++i) {
-        A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
-        A[((zb * 2 + 1) * C + c0 + i) * AS + rr] = hi[i];
+        for (int j = 0; j < 2 * (C / 3) + 8; ++j) v[j] = Zc[j * ZS];
+        float lo[C / 3], hi[C / 3];
+        ana_lo_hi<C / 3>(v, lo, hi);
+#pragma unroll
+        for (int i = 0; i < C / 3; ++i) {
+          A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
+          A[((zb * 2 + 1) * C + c0 + i) * AS + rr] = hi[i];
+        }
       }
     }
     __syncthreads();
