@@ -135,14 +135,17 @@ int launch_mosaic_to_groups(const float* src, int M, int N, int K, int d, float*
 constexpr int AT = 16;  // output tile edge per wave
 
 // Lane e of the wave owns pixel (u, v) = (e % D, e / D) (+64i) of every block.
-// Hits (candidate blocks that meet the tile) come from a ballot over 64
-// candidates in (y, x, k) order, eight at a time: their loads are issued
-// together (whole blocks, no bounds tests), then each block is added into the
-// LDS tile accumulator in order, predicated on the pixel lying in the tile.
-// U and W are interleaved (one 64-bit read and write per pixel and block); the
-// accumulator's column stride AS makes a half-wave's 64-bit accesses to one
-// block hit distinct banks.  The next round's index and weight loads are
-// issued before the current round's hits are processed.
+// Candidates come 64 per round in (y, x, k) order; the round's hits (blocks
+// that meet the tile) are listed in LDS in that order, so that lane j holds
+// hit j's block, packed origin, weight and accumulator offset.  Hits are taken
+// eight at a time: their loads are issued together (whole blocks, no bounds
+// tests), then each block is added into the LDS tile accumulator in order,
+// predicated on the pixel lying in the tile (one packed 16-bit add and a
+// mask).  U and W are interleaved (one 64-bit read and write per pixel and
+// block); the accumulator's column stride AS makes a half-wave's 64-bit
+// accesses to one block hit distinct banks.  The next round's index and
+// weight loads are issued before the current round's hits are processed.
+
 // two 16-bit lanes of one register (v_pk_add_u16)
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 static_assert((AT & (AT - 1)) == 0, "AT a power of two");
