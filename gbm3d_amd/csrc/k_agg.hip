@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
       bob = br + AS * bc;                                      // its LDS offset (float2 units)
     }
     // this round's hits, in candidate order, listed in LDS so that lane j
-    // holds the j-th hit's block, offset and weight
+    // holds the j-th hit's block, packed origin, weight and accumulator offset
     const uint64_t mask = __ballot(hit);
     const int nh = __popcll(mask);
     if (nh == 0) continue;
