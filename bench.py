@@ -134,10 +134,18 @@ def precision_report(args):
     out["fp64_oracle_s"] = round(time.perf_counter() - t0, 2)
     for name, (U, dt, st) in times.items():
         dev_ms = sum(v[0] for v in st.values())
+        mlaunch = st["match1"][1] + st["match2"][1]
+        mms = st["match1"][0] + st["match2"][0]
+        # distance work 2*m*n*d2^2 per launch (the useful FMAs; the band GEMM issues more)
+        tflops = 2.0 * size * size * 48 * 48 * mlaunch / (mms * 1e-3) / 1e12
+        peak = PEAK_BF16_TFLOPS if name == "bf16_rescored" else PEAK_FP32_TFLOPS
         out[name] = {"psnr_db": round(synth.psnr(clean, U), 4), "psnr_vs_fp64_oracle_db": round(synth.psnr(U64, U), 3),
                      "max_abs_vs_fp64_oracle": float(np.abs(U - U64).max()),
-                     "match_ms": round(st["match1"][0] + st["match2"][0], 3), "device_ms": round(dev_ms, 3),
-                     "mpix_s_device": round(size * size / (dev_ms * 1e-3) / 1e6, 2)}
+                     "match_ms": round(mms, 3), "device_ms": round(dev_ms, 3),
+                     "mpix_s_device": round(size * size / (dev_ms * 1e-3) / 1e6, 2),
+                     "match_kernel": {"launches": mlaunch, "avg_launch_ms": round(mms / mlaunch, 4),
+                                      "achieved_tflops": round(tflops, 2), "peak_tflops": peak,
+                                      "frac": round(tflops / peak, 4)}}
     out["fp64_oracle_psnr_db"] = round(synth.psnr(clean, U64), 4)
     print(json.dumps(out))
 
@@ -499,6 +507,12 @@ def main():
                           "psnr_pilot_db": round(forms[name]["pilot"], 4),
                           "max_abs_vs_plain": float(np.abs(Uv - U).max()),
                           "stages_ms": {k: round(v, 4) for k, v in smv.items() if v > 0}}
+        if kw.get("match_bf16") and "match1" in rlv:
+            # the distance kernel against the BF16 MFMA peak (work: 2*m*n*d2^2 per launch)
+            k = rlv["match1"]
+            variants[name]["match_kernel"] = {"achieved_tflops": round(k["achieved"], 2), "peak": PEAK_BF16_TFLOPS,
+                                              "frac": round(k["achieved"] / PEAK_BF16_TFLOPS, 4),
+                                              "avg_launch_ms": round(k["ms"], 4)}
         if kw.get("learned"):
             el, eh, fell = ctx.shrink_stats(0)
             variants[name].update({"removed_energy_vs_noise": round(el, 4),
