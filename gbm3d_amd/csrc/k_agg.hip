@@ -27,6 +27,49 @@ __device__ __forceinline__ float third_diff(float x0, float x1, float x2, float 
   return f - e;
 }
 
+// 8 x 8 blocks, 16 slices: the group's 4 KB stack is read with 16-byte loads
+// (four per lane), staged in LDS with one exchange, and each lane then takes
+// its pixel's terms slice by slice in the order of the general form below.
+constexpr int kPrepStack = 16;
+
+template <int D>
+__device__ __forceinline__ void agg1_prep_stack(const float* __restrict__ vg, int G, float* __restrict__ wgt) {
+  constexpr int DD = D * D, NF = kPrepStack * DD / 4 / 64;  // float4 per lane
+  static_assert(DD == 64 && NF == 4, "8x8 blocks, 16 slices");
+  __shared__ __attribute__((aligned(16))) float st[4][kPrepStack * DD];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+  if (g >= G) return;
+  const float4* blk = reinterpret_cast<const float4*>(vg + (size_t)DD * kPrepStack * g);
+  float* sw = st[wave];
+  float4 q[NF];
+#pragma unroll
+  for (int r = 0; r < NF; ++r) q[r] = blk[lane + 64 * r];
+#pragma unroll
+  for (int r = 0; r < NF; ++r) reinterpret_cast<float4*>(sw)[lane + 64 * r] = q[r];
+  wave_sync();
+  const int u = lane % D, v = lane / D;
+  float s1 = 0.f, s2 = 0.f, s3 = 0.f, prev = 0.f;
+#pragma unroll
+  for (int k = 0; k < kPrepStack; ++k) {
+    const float* c = sw + DD * k + lane;
+    const float x = c[0];
+    if (k > 0) s3 += fabsf(x - prev);  // |diff(U,1,3)|
+    prev = x;
+    if (v + 3 < D) s2 += fabsf(third_diff(c[0], c[D], c[2 * D], c[3 * D]));
+    if (u + 3 < D) s1 += fabsf(third_diff(c[0], c[1], c[2], c[3]));
+  }
+  s1 = wave_sum_f32(s1);
+  s2 = wave_sum_f32(s2);
+  s3 = wave_sum_f32(s3);
+  if (lane == 0) {
+    float Dv = s2 * 0.25f;
+    Dv = Dv + s1 * 0.25f;
+    Dv = Dv + s3;
+    wgt[g] = 1.0f / (Dv + 2.220446049250313e-16f);  // 1/(D(blocks)+eps)
+  }
+}
+
 // One wave per group; lane e < d*d owns pixel (u, v) = (e % d, e / d) of
 // every slice, slices staged one at a time in LDS for the in-plane differences.
 // The slices are fetched in batches of KB whose loads are all issued before
@@ -35,6 +78,12 @@ template <int D>
 __global__ __launch_bounds__(256) void agg1_prep_wave_kernel(const float* __restrict__ vg, int G, int K,
                                                              float* __restrict__ wgt) {
   constexpr int DD = D * D, PER = (DD + 63) / 64, KB = PER >= 16 ? 2 : 16 / PER;
+  if constexpr (DD == 64) {
+    if (K == kPrepStack) {  // 8x8 blocks, 16 slices: the whole 4 KB stack staged at once
+      agg1_prep_stack<D>(vg, G, wgt);
+      return;
+    }
+  }
   __shared__ float sl[4][DD];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
