@@ -177,7 +177,8 @@ def test_end_to_end_golden(ctx, path):
 
 @pytest.mark.parametrize("opts,m,n", [(None, 256, 256), ({"blockSize": 8, "matchSpinsWie": [0, 4]}, 256, 192),
                                       ({"profile": "accuracy"}, 128, 160), ({"profile": "superFast"}, 256, 256),
-                                      ({"profile": "fast"}, 200, 232)])
+                                      ({"profile": "fast"}, 200, 232),
+                                      ({"blockSize": 8, "matchSpinsWie": [0, 4]}, 70, 66)])  # crop not a multiple of 4
 def test_end_to_end_vs_oracle(ctx, opts, m, n):
     """Both steps, chained: the GPU pilot U1 against the FP32 restatement run on
     the same input (bit-exact indices, so only transform round-off remains),
