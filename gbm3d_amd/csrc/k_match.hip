@@ -232,7 +232,7 @@ constexpr int kSelMax = 256;
 // staged in LDS once as their union, (D2+D+(NW-1)D) x (D2+D); wave w's window
 // starts w*D rows down.  Window index a + D2*b <-> Ipad(i*D + a, j*D + b).
 template <int D, int D2, int NW = 4>
-__global__ __launch_bounds__(64 * NW) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(8))) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
                                                         int M, int N, int K, uint32_t* __restrict__ S,
                                                         float* __restrict__ vol) {
   using C = MatchCfg<D, D2>;
