@@ -231,8 +231,19 @@ constexpr int kSelMax = 256;
 // along the tile rows (same tile column), whose (D2+D)^2 search windows are
 // staged in LDS once as their union, (D2+D+(NW-1)D) x (D2+D); wave w's window
 // starts w*D rows down.  Window index a + D2*b <-> Ipad(i*D + a, j*D + b).
+// Occupancy hint: 8 waves per SIMD (64 VGPRs) where the LDS footprint admits
+// it (160 KB per CU), otherwise none, leaving the register budget to the
+// compiler for the large windows.
+template <int D, int D2, int NW>
+struct MatchOcc {
+  static constexpr int RW = MatchCfg<D, D2>::WR + (NW - 1) * D;
+  static constexpr int LDW = (RW % 2 == 0) ? RW + 1 : RW;
+  static constexpr int kLds = MatchCfg<D, D2>::WR * LDW * 4 + NW * kSelMax * 8;
+  static constexpr int kWaves = (163840 / kLds) * NW / 4 >= 8 ? 8 : 1;
+};
+
 template <int D, int D2, int NW = 4>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(8))) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOcc<D, D2, NW>::kWaves))) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
                                                         int M, int N, int K, uint32_t* __restrict__ S,
                                                         float* __restrict__ vol) {
   using C = MatchCfg<D, D2>;
