@@ -252,7 +252,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
   constexpr int LDW = (RW % 2 == 0) ? RW + 1 : RW;  // odd: conflict-free columns
   __shared__ float win[C::WR * LDW];
   __shared__ uint64_t scr[NW][kSelMax];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index in a scalar register: the tile's addresses are wave-uniform
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int Mt = M / D, Mb = (Mt + NW - 1) / NW;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int ti0 = (blk % Mb) * NW, tj = blk / Mb;
@@ -324,20 +325,29 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
 #pragma unroll
     for (int j = 1; j < C::NL; ++j) fmn = fminf(fmn, numv[j]);
     const uint32_t lmin = ord_bits(fmn);
-    const uint32_t th = (uint32_t)__shfl((int)wave_sort64<uint32_t>(lmin, lane), K - 1, 64);
+    const uint32_t th = wave_kth_smallest(lmin, K);
     if (th != 0xffffffffu) {
       const float tnum = ord_to_f32(th);
       const float bound = fmaf(fabsf(tnum), 0x1p-20f, tnum) + nx * 0x1p-126f;
-      int cnt = 0;
+      // the survivors of each candidate slot j as a lane mask; they are
+      // compacted slot by slot (the selection below depends on the set alone)
+      uint64_t mk[C::NL];
+      int total = 0;
 #pragma unroll
-      for (int j = 0; j < C::NL; ++j) cnt += numv[j] <= bound ? 1 : 0;
-      const int off = wave_excl_scan(cnt, lane);
-      const int total = __shfl(off + cnt, 63, 64);
+      for (int j = 0; j < C::NL; ++j) {
+        mk[j] = __ballot(numv[j] <= bound);
+        total += __popcll(mk[j]);
+      }
       if (total <= kSelMax) {
-        int pos = off;
+        uint32_t base = 0;
 #pragma unroll
-        for (int j = 0; j < C::NL; ++j)
-          if (numv[j] <= bound) scr[wave][pos++] = ((uint64_t)__float_as_uint(numv[j]) << 32) | (uint32_t)cand_idx(j);
+        for (int j = 0; j < C::NL; ++j) {
+          if (numv[j] <= bound) {
+            const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mk[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[j], base));
+            scr[wave][pos] = ((uint64_t)__float_as_uint(numv[j]) << 32) | (uint32_t)cand_idx(j);
+          }
+          base += (uint32_t)__popcll(mk[j]);
+        }
         wave_sync();
         auto key_at = [&](int q) {
           const uint64_t e = scr[wave][q];
@@ -392,14 +402,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
   if (lane < K) S[lane + (size_t)K * g] = (uint32_t)idx + 1u;
   if (vol == nullptr) return;
   // Volume((i-1)d+1:id, (j-1)d+1:jd, k) = Itmp(a:a+d-1, b:b+d-1)  (:107-111)
+  // (the block's (D-1)(M+1)+1 floats of slice k behind a buffer resource: one
+  // 32-bit lane offset for all k)
   const size_t MN = (size_t)M * N;
+  const uint32_t blk_bytes = 4u * (uint32_t)((D - 1) * (M + 1) + 1);
   for (int k = 0; k < K; ++k) {
     const int ik = __builtin_amdgcn_readlane(idx, k);
     const int a = ik % D2, b = ik / D2;
-    float* dst = vol + MN * k + (size_t)(ti * D) + (size_t)M * (tj * D);
+    const float* Wk = W + a + LDW * b;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(vol + MN * k + (size_t)(ti * D) + (size_t)M * (tj * D), blk_bytes);
     for (int e = lane; e < D * D; e += 64) {
       const int u = e % D, v = e / D;
-      dst[u + (size_t)M * v] = W[(a + u) + LDW * (b + v)];
+      buf_store(rs, 4u * (uint32_t)(u + M * v), Wk[u + LDW * v]);
     }
   }
 }

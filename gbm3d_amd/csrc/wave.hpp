@@ -29,8 +29,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// This lane's lane index (0..63).
+__device__ __forceinline__ int lane_id() {
+  return (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// The value of lane (this lane ^ m), the whole wave taking part: one
+// ds_bpermute on an address formed once per m (no clamp to a sub-width).
 __device__ __forceinline__ uint32_t shfl_xor_u32(uint32_t v, int m) {
-  return (uint32_t)__shfl_xor((int)v, m, 64);
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((lane_id() ^ m) << 2, (int)v);
 }
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
   uint32_t lo = shfl_xor_u32((uint32_t)v, m);
@@ -49,6 +56,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base, ui
 }
 __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes, uint32_t soff_bytes = 0) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, soff_bytes, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off_bytes, 0, 0);
 }
 
 // Ascending bitonic sort of one value per lane across the 64 lanes.
@@ -69,6 +79,19 @@ __device__ __forceinline__ T wave_sort64(T v, int lane) {
     }
   }
   return v;
+}
+
+// The K-th smallest (1 <= K <= 64) of the 64 lanes' values, the same in every
+// lane: the largest t with fewer than K values below t, built from the top
+// bit down with one ballot per bit.
+__device__ __forceinline__ uint32_t wave_kth_smallest(uint32_t v, int K) {
+  uint32_t t = 0;
+#pragma unroll
+  for (int b = 31; b >= 0; --b) {
+    const uint32_t c = t | (1u << b);
+    if (__popcll(__ballot(v < c)) < K) t = c;
+  }
+  return t;
 }
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
