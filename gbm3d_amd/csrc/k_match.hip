@@ -261,9 +261,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
   const int P = M + D2, Pm = P - D + 1;
   const int rows = C::WR + (nval - 1) * D;  // union rows inside the padded image
   const float* src = pad + (size_t)(ti0 * D) + (size_t)P * (tj * D);
-  for (int e = threadIdx.x; e < RW * C::WR; e += 64 * NW) {
-    const int rr = e % RW, cc = e / RW;
-    win[rr + LDW * cc] = rr < rows ? src[rr + (size_t)P * cc] : 0.f;
+  if constexpr ((64 * NW) % RW == 0) {
+    // each thread keeps one union row and steps over the columns: 32-bit
+    // offsets from the window's origin, LDS offsets immediate
+    // (rows past the union, masked below, may lie past the padded image: the
+    // resource ends with the image, so such loads return 0 and stay inside it)
+    constexpr int CPR = 64 * NW / RW, NR = C::WR / CPR;
+    static_assert(C::WR % CPR == 0, "window columns per round");
+    const int rr = threadIdx.x % RW, c0 = threadIdx.x / RW;
+    const size_t left = (size_t)P * (N + D2) - (size_t)(src - pad);
+    const uint32_t span = 4u * (uint32_t)(RW + P * (C::WR - 1));
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(src, left < span / 4 ? 4u * (uint32_t)left : span);
+    const uint32_t off = 4u * (uint32_t)(rr + P * c0);
+    float x[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) x[i] = buf_load(rs, off, 4u * (uint32_t)(P * CPR * i));
+#pragma unroll
+    for (int i = 0; i < NR; ++i) win[rr + LDW * (c0 + CPR * i)] = rr < rows ? x[i] : 0.f;
+  } else {
+    for (int e = threadIdx.x; e < RW * C::WR; e += 64 * NW) {
+      const int rr = e % RW, cc = e / RW;
+      win[rr + LDW * cc] = rr < rows ? src[rr + (size_t)P * cc] : 0.f;
+    }
   }
   if (NW > 1)
     __syncthreads();
