@@ -298,9 +298,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
   for (int it = 0; it < C::IT; ++it) {
     const int ch = lane + 64 * it;
     const int b = ch % D2, a0 = (ch / D2) * C::RA;
+    // acc[i] = sum over (v, u) in that order of fmaf(x(u, v), y, acc[i]) from
+    // +0.  With the v loop unrolled (8x8 blocks) the first term is a product:
+    // x*y differs from fmaf(x, y, +0) only in the sign of a zero, and the
+    // numerator (nx + |y|^2) - 2 acc, nx + |y|^2 >= +0, is the same either way.
     float acc[C::RA];
+    constexpr bool kFlat = D <= 8;
+    constexpr int kUnrollV = kFlat ? D : 1;
+    if (!kFlat) {
 #pragma unroll
-    for (int i = 0; i < C::RA; ++i) acc[i] = 0.f;
+      for (int i = 0; i < C::RA; ++i) acc[i] = 0.f;
+    }
+#pragma unroll kUnrollV
     for (int v = 0; v < D; ++v) {
       float xs[D];
 #pragma unroll
@@ -312,7 +321,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
 #pragma unroll
         for (int i = 0; i < C::RA; ++i) {
           const int u = t - i;
-          if (u >= 0 && u < D) acc[i] = fmaf(xs[u], y, acc[i]);
+          if (u >= 0 && u < D) acc[i] = (kFlat && v == 0 && u == 0) ? xs[0] * y : fmaf(xs[u], y, acc[i]);
         }
       }
     }
@@ -362,8 +371,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
 #pragma unroll
         for (int j = 0; j < C::NL; ++j) {
           if (numv[j] <= bound) {
-            const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mk[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[j], base));
-            scr[wave][pos] = ((uint64_t)__float_as_uint(numv[j]) << 32) | (uint32_t)cand_idx(j);
+            const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(mk[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[j], 0u));
+            scr[wave][base + r] = ((uint64_t)__float_as_uint(numv[j]) << 32) | (uint32_t)cand_idx(j);
           }
           base += (uint32_t)__popcll(mk[j]);
         }
