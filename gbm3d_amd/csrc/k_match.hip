@@ -178,18 +178,20 @@ __device__ uint64_t wave_topk(const uint64_t (&key)[NL], int K, uint64_t* scr, i
   uint32_t lmin = 0xffffffffu;
 #pragma unroll
   for (int i = 0; i < NL; ++i) lmin = min(lmin, (uint32_t)(key[i] >> 32));
-  const uint32_t sorted = wave_sort64<uint32_t>(lmin, lane);
-  const uint32_t theta = (uint32_t)__shfl((int)sorted, K - 1, 64);
-  int cnt = 0;
+  const uint32_t theta = wave_kth_smallest(lmin, K);
+  int total = 0;
 #pragma unroll
-  for (int i = 0; i < NL; ++i) cnt += ((uint32_t)(key[i] >> 32) <= theta) ? 1 : 0;
-  const int off = wave_excl_scan(cnt, lane);
-  const int total = __shfl(off + cnt, 63, 64);
+  for (int i = 0; i < NL; ++i) total += __popcll(__ballot((uint32_t)(key[i] >> 32) <= theta));
   if (total <= 64) {
-    int pos = off;
+    // compacted slot by slot (the sort below depends on the set alone)
+    uint32_t base = 0;
 #pragma unroll
-    for (int i = 0; i < NL; ++i)
-      if ((uint32_t)(key[i] >> 32) <= theta) scr[pos++] = key[i];
+    for (int i = 0; i < NL; ++i) {
+      const bool in = (uint32_t)(key[i] >> 32) <= theta;
+      const uint64_t m = __ballot(in);
+      if (in) scr[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = key[i];
+      base += (uint32_t)__popcll(m);
+    }
     wave_sync();
     uint64_t v = lane < total ? scr[lane] : ~0ull;
     wave_sync();
@@ -666,22 +668,26 @@ __global__ __launch_bounds__(256) void match_bf16_kernel(
     uint32_t lmin = 0xffffffffu;
 #pragma unroll
     for (int j = 0; j < C::NC; ++j) lmin = min(lmin, key[j]);
-    const uint32_t theta = (uint32_t)__shfl((int)wave_sort64<uint32_t>(lmin, lane), R - 1, 64);
-    int cnt = 0;
+    const uint32_t theta = wave_kth_smallest(lmin, R);
+    int total = 0;
 #pragma unroll
-    for (int j = 0; j < C::NC; ++j) cnt += key[j] <= theta ? 1 : 0;
-    const int off = wave_excl_scan(cnt, lane);
-    const int total = __shfl(off + cnt, 63, 64);
+    for (int j = 0; j < C::NC; ++j) total += __popcll(__ballot(key[j] <= theta));
     unsigned short* cl = clist[wave];
     if (total <= kRescoreMax) {
-      int pos = off;
+      // compacted slot by slot (the rescored selection depends on the set alone)
+      uint32_t base = 0;
 #pragma unroll
-      for (int j = 0; j < C::NC; ++j)
-        if (key[j] <= theta) {
+      for (int j = 0; j < C::NC; ++j) {
+        const bool in = key[j] <= theta;
+        const uint64_t m = __ballot(in);
+        if (in) {
           int a, b;
           cand_ab(j, a, b);
-          cl[pos++] = (unsigned short)(a + D2 * b);
+          cl[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+              (unsigned short)(a + D2 * b);
         }
+        base += (uint32_t)__popcll(m);
+      }
       wave_sync();
       uint64_t ek[kRescoreMax / 64];
 #pragma unroll

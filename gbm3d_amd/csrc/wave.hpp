@@ -103,16 +103,6 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
-__device__ __forceinline__ int wave_excl_scan(int x, int lane) {
-  int incl = x;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += y;
-  }
-  return incl - x;
-}
-
 __device__ __forceinline__ float wave_sum_f32(float v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
