@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
     if (t == 0 && active) gain2[g] = n2;
   } else if constexpr (TPG < 64) {
 #pragma unroll
-    for (int o = TPG / 2; o >= 1; o >>= 1) n2 += __shfl_xor(n2, o, 64);
+    for (int o = TPG / 2; o >= 1; o >>= 1) n2 += __uint_as_float(shfl_xor_u32(__float_as_uint(n2), o));
     if (t == 0 && active) gain2[g] = n2;
   } else {
     n2 = wave_sum_f32(n2);

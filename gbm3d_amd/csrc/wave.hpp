@@ -105,7 +105,7 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 
 __device__ __forceinline__ float wave_sum_f32(float v) {
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  for (int o = 32; o >= 1; o >>= 1) v += __uint_as_float(shfl_xor_u32(__float_as_uint(v), o));
   return v;
 }
 
