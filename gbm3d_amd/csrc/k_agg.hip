@@ -205,7 +205,9 @@ struct AggCfg {
   static constexpr int AS = D <= 8 ? 24 : 16;
 };
 
-template <int STEP, int D>
+// BUF: the blocks array spans fewer than 2^31 bytes, so a block is read
+// through one buffer resource with its byte offset in a scalar register.
+template <int STEP, int D, bool BUF>
 __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __restrict__ blocks,
                                                                 const float* __restrict__ wgt,
                                                                 const uint32_t* __restrict__ S,
@@ -214,7 +216,9 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
                                                                 float* __restrict__ W) {
   constexpr int DD = D * D, PER = (DD + 63) / 64, NB = 8, AS = AggCfg<D>::AS;
   __shared__ float2 acc[4][AT * AS];
-  __shared__ int hl[4][4][64];  // per wave: the round's hits (block, packed origin, weight, LDS offset)
+  // per wave: the round's hits (block, packed origin, weight, LDS offset),
+  // NB entries of slack for a batch's reads past the last hit
+  __shared__ int4 hl[4][64 + NB];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tilesR = (M + AT - 1) / AT, tilesC = (N + AT - 1) / AT;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
@@ -222,6 +226,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   const int R0 = (tile % tilesR) * AT, C0 = (tile / tilesR) * AT;
   float2* ac = acc[wave];
   for (int e = lane; e < AT * AS; e += 64) ac[e] = make_float2(0.f, 0.f);
+  const __amdgpu_buffer_rsrc_t rB = buf_rsrc(blocks, BUF ? 4u * (uint32_t)DD * (uint32_t)(K * (M / D) * (N / D)) : 0u);
   float mk[PER];
   uint32_t puv[PER];  // this lane's pixel (u, v) of a block as two 16-bit halves
   float2* pa[PER];    // the same pixel of a block at the tile origin
@@ -314,30 +319,39 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
     if (nh == 0) continue;
     if (hit) {
       const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-      hl[wave][0][rank] = blk;
-      hl[wave][1][rank] = (int)brc;
-      hl[wave][2][rank] = __float_as_int(wv);
-      hl[wave][3][rank] = bob;
+      hl[wave][rank] = make_int4(blk, (int)brc, __float_as_int(wv), bob);
     }
     wave_sync();
-    const int hblk = hl[wave][0][lane], hrc = hl[wave][1][lane], hw = hl[wave][2][lane], hob = hl[wave][3][lane];
-    wave_sync();  // read before the next round rewrites the list
+    const int hblk = hl[wave][lane].x;
     for (int j0 = 0; j0 < nh; j0 += NB) {
       // all NB loads issued (past the last hit: the first hit's block again),
       // so each hit waits only for its own block
       float val[NB][PER];
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
-        const float* bp = blocks + (size_t)DD * __builtin_amdgcn_readlane(hblk, j0 + j < nh ? j0 + j : 0);
+        const int b = __builtin_amdgcn_readlane(hblk, j0 + j < nh ? j0 + j : 0);
+        if constexpr (BUF) {
 #pragma unroll
-        for (int i = 0; i < PER; ++i) val[j][i] = (lane + 64 * i < DD) ? bp[lane + 64 * i] : 0.f;
+          for (int i = 0; i < PER; ++i)
+            val[j][i] = (lane + 64 * i < DD) ? buf_load(rB, 4u * (uint32_t)(lane + 64 * i), 4u * (uint32_t)(DD * b)) : 0.f;
+        } else {
+          const float* bp = blocks + (size_t)DD * b;
+#pragma unroll
+          for (int i = 0; i < PER; ++i) val[j][i] = (lane + 64 * i < DD) ? bp[lane + 64 * i] : 0.f;
+        }
       }
+      // the batch's hit fields by broadcast LDS reads (one address in every
+      // lane), all issued before the first is used; entries past the last
+      // hit are read and not used
+      int4 hh[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) hh[j] = hl[wave][j0 + j];
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         if (j0 + j >= nh) break;
-        const uint32_t rc = (uint32_t)__builtin_amdgcn_readlane(hrc, j0 + j);
-        const float wg = __int_as_float(__builtin_amdgcn_readlane(hw, j0 + j));  // step 2: 1/weight
-        const int ob = __builtin_amdgcn_readlane(hob, j0 + j);
+        const uint32_t rc = (uint32_t)hh[j].y;
+        const float wg = __int_as_float(hh[j].z);  // step 2: 1/weight
+        const int ob = hh[j].w;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
           // pixel (r + u, c + v) in the tile: both 16-bit sums (wrapping) below AT
@@ -361,6 +375,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
         // read-modify-writes in order, so each pixel sums in (y, x, k) order
       }
     }
+    wave_sync();  // the hit list read before the next round rewrites it
   }
   wave_sync();
   for (int e = lane; e < AT * AT; e += 64) {
@@ -379,14 +394,24 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
   if (M % d || N % d) return fail(GBM3D_ERR_ARG, "aggregate: size");
   const int tiles = ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
   dim3 grid((tiles + 3) / 4);
+  // the blocks array: K slices of M x N floats
+  const bool buf = (size_t)K * M * N * sizeof(float) < ((size_t)1 << 31);
+#define GBM3D_AGG_LAUNCH(S_, DD, B_)                                                                         \
+  hipLaunchKernelGGL((aggregate_batched_kernel<S_, DD, B_>), grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, \
+                     d2, K, spin, U, W)
 #define GBM3D_AGG_CASE(DD)                                                                                   \
   if (d == DD) {                                                                                             \
-    if (step == 1)                                                                                           \
-      hipLaunchKernelGGL((aggregate_batched_kernel<1, DD>), grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d2, \
-                         K, spin, U, W);                                                                     \
-    else                                                                                                     \
-      hipLaunchKernelGGL((aggregate_batched_kernel<2, DD>), grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, d2, \
-                         K, spin, U, W);                                                                     \
+    if (step == 1) {                                                                                         \
+      if (buf)                                                                                               \
+        GBM3D_AGG_LAUNCH(1, DD, true);                                                                       \
+      else                                                                                                   \
+        GBM3D_AGG_LAUNCH(1, DD, false);                                                                      \
+    } else {                                                                                                 \
+      if (buf)                                                                                               \
+        GBM3D_AGG_LAUNCH(2, DD, true);                                                                       \
+      else                                                                                                   \
+        GBM3D_AGG_LAUNCH(2, DD, false);                                                                      \
+    }                                                                                                        \
     return check_launch("aggregate");                                                                        \
   }
   GBM3D_AGG_CASE(4)
@@ -394,6 +419,7 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
   GBM3D_AGG_CASE(16)
   GBM3D_AGG_CASE(32)
 #undef GBM3D_AGG_CASE
+#undef GBM3D_AGG_LAUNCH
   return fail(GBM3D_ERR_UNSUPPORTED, "aggregate: block size must be 4, 8, 16 or 32");
 }
 
