@@ -12,6 +12,8 @@
 // pixels into an LDS accumulator one block after another.  Every pixel thus
 // receives its contributions in the reference's order, in FP32, with no
 // atomics: results are bitwise reproducible run to run.
+#include <cstdlib>
+
 #include "internal.hpp"
 #include "wave.hpp"
 
@@ -394,8 +396,10 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
   if (M % d || N % d) return fail(GBM3D_ERR_ARG, "aggregate: size");
   const int tiles = ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
   dim3 grid((tiles + 3) / 4);
-  // the blocks array: K slices of M x N floats
-  const bool buf = (size_t)K * M * N * sizeof(float) < ((size_t)1 << 31);
+  // the blocks array: K slices of M x N floats.  GBM3D_AGG_POINTER=1 keeps
+  // the 64-bit pointer loads below 2 GB too, for cross-checking the two.
+  const char* ptr_env = getenv("GBM3D_AGG_POINTER");
+  const bool buf = (size_t)K * M * N * sizeof(float) < ((size_t)1 << 31) && !(ptr_env && ptr_env[0] == '1');
 #define GBM3D_AGG_LAUNCH(S_, DD, B_)                                                                         \
   hipLaunchKernelGGL((aggregate_batched_kernel<S_, DD, B_>), grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, \
                      d2, K, spin, U, W)

@@ -120,6 +120,26 @@ def test_ht_fused_cycle_spins_equal_per_spin_kernels(ctx, monkeypatch, m, n, K, 
     np.testing.assert_array_equal(Ua, Ub)
 
 
+@pytest.mark.parametrize("d", [4, 8, 16])
+def test_aggregate_buffer_and_pointer_loads_agree(ctx, monkeypatch, d):
+    """Hit blocks read through a buffer resource (arrays below 2 GB) and through
+    64-bit pointers (GBM3D_AGG_POINTER=1, the path of larger arrays): bitwise
+    equal, both steps and end to end."""
+    clean, noisy = synth.make_pair(60 + d, 128, 96, 25.0)
+    S, V, _ = O.match(noisy, d, 32, 16, precision=32)
+    Vavg = np.maximum(V, 0).astype(np.float32)
+    a = ctx.aggregate_ht(Vavg, S, d, 32)
+    monkeypatch.setenv("GBM3D_AGG_POINTER", "1")
+    b = ctx.aggregate_ht(Vavg, S, d, 32)
+    Ub, ob = ctx.denoise(noisy, 25.0, {"blockSize": d, "matchSpinsWie": [0, 4]})
+    monkeypatch.delenv("GBM3D_AGG_POINTER")
+    Ua, oa = ctx.denoise(noisy, 25.0, {"blockSize": d, "matchSpinsWie": [0, 4]})
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(oa["U1"], ob["U1"])
+    np.testing.assert_array_equal(Ua, Ub)
+
+
 def test_ht_filter_perfect_reconstruction(ctx):
     """tau = 0 keeps every coefficient: the filter is max(V, 0) (closed form)."""
     rng = np.random.default_rng(3)
