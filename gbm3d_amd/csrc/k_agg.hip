@@ -207,8 +207,10 @@ struct AggCfg {
   static constexpr int AS = D <= 8 ? 24 : 16;
 };
 
-// BUF: the blocks array spans fewer than 2^31 bytes, so a block is read
-// through one buffer resource with its byte offset in a scalar register.
+// BUF: the blocks array spans fewer than 2^31 bytes and the image fewer than
+// 2^26 pixels, so a block is read through one buffer resource with its byte
+// offset in a scalar register, the slot lists and weights through buffer
+// resources with 32-bit offsets, and every index product has 24-bit factors.
 template <int STEP, int D, bool BUF>
 __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __restrict__ blocks,
                                                                 const float* __restrict__ wgt,
@@ -229,6 +231,10 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   float2* ac = acc[wave];
   for (int e = lane; e < AT * AS; e += 64) ac[e] = make_float2(0.f, 0.f);
   const __amdgpu_buffer_rsrc_t rB = buf_rsrc(blocks, BUF ? 4u * (uint32_t)DD * (uint32_t)(K * (M / D) * (N / D)) : 0u);
+  const __amdgpu_buffer_rsrc_t rS = buf_rsrc(reinterpret_cast<const float*>(S), BUF ? 4u * (uint32_t)(K * (M / D) * (N / D)) : 0u);
+  const __amdgpu_buffer_rsrc_t rW = buf_rsrc(wgt, BUF ? 4u * (uint32_t)((M / D) * (N / D)) : 0u);
+  // a * b with 24-bit factors where the sizes allow it (full-rate multiply-add)
+  auto imul = [](int a, int b) { return BUF ? __mul24(a, b) : a * b; };
   float mk[PER];
   uint32_t puv[PER];  // this lane's pixel (u, v) of a block as two 16-bit halves
   float2* pa[PER];    // the same pixel of a block at the tile origin
@@ -271,7 +277,7 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
   auto tile_of = [&](int yx, int& y, int& x) {
     const int qy = (int)(((float)yx + 0.5f) * rnx);
     y = ylo + qy;
-    x = xlo + (yx - qy * nx);
+    x = xlo + (yx - imul(qy, nx));
   };
   // its reference tile (y, x), slot k, slot index and weight
   struct Cand {
@@ -287,11 +293,18 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
       int yx;
       split(t, yx, c.k);
       tile_of(yx, c.y, c.x);
-      const int g = c.y + Mt * c.x;
-      c.sidx = S[(size_t)K * g + c.k];
+      const int g = c.y + imul(Mt, c.x);
+      float wg;
+      if constexpr (BUF) {
+        c.sidx = __float_as_uint(buf_load(rS, 4u * (uint32_t)(imul(K, g) + c.k)));
+        wg = buf_load(rW, 4u * (uint32_t)g);
+      } else {
+        c.sidx = S[(size_t)K * g + c.k];
+        wg = wgt[g];
+      }
       // step 2 uses 1/weight (w / norm(filt)^2): divided here, once per
       // candidate lane, rather than once per hit
-      c.w = STEP == 1 ? wgt[g] : 1.0f / wgt[g];
+      c.w = STEP == 1 ? wg : 1.0f / wg;
     }
   };
   Cand cn;
@@ -307,10 +320,10 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
     if (t < ncand) {
       const uint32_t idx = cc0.sidx - 1u;
       const int bq = (int)(((float)idx + 0.5f) * rd2);
-      const int br = cc0.y * D + ((int)idx - bq * d2) - s - R0;  // block origin relative to the tile
+      const int br = cc0.y * D + ((int)idx - imul(bq, d2)) - s - R0;  // block origin relative to the tile
       const int bc = cc0.x * D + bq - s - C0;
       hit = br < AT && br + D > 0 && bc < AT && bc + D > 0;
-      blk = cc0.k + K * (cc0.y + Mt * cc0.x);
+      blk = cc0.k + imul(K, cc0.y + imul(Mt, cc0.x));
       brc = ((uint32_t)br & 0xffffu) | ((uint32_t)bc << 16);  // 16-bit two's complement halves
       bob = br + AS * bc;                                      // its LDS offset (float2 units)
     }
@@ -399,7 +412,8 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
   // the blocks array: K slices of M x N floats.  GBM3D_AGG_POINTER=1 keeps
   // the 64-bit pointer loads below 2 GB too, for cross-checking the two.
   const char* ptr_env = getenv("GBM3D_AGG_POINTER");
-  const bool buf = (size_t)K * M * N * sizeof(float) < ((size_t)1 << 31) && !(ptr_env && ptr_env[0] == '1');
+  const bool buf = (size_t)K * M * N * sizeof(float) < ((size_t)1 << 31) && (size_t)M * N < ((size_t)1 << 26) &&
+                   !(ptr_env && ptr_env[0] == '1');
 #define GBM3D_AGG_LAUNCH(S_, DD, B_)                                                                         \
   hipLaunchKernelGGL((aggregate_batched_kernel<S_, DD, B_>), grid, dim3(256), 0, st, blocks, wgt, S, masks, M, N, \
                      d2, K, spin, U, W)
