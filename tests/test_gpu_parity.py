@@ -49,6 +49,18 @@ def test_match_bit_exact_vs_fp32_restatement(ctx, m, n, d, d2, seed):
     assert _near_tie_only(S, S64, dist)
 
 
+@pytest.mark.parametrize("m,n,d,d2,K,seed", [(64, 64, 8, 32, 8, 11), (128, 96, 8, 32, 32, 12), (96, 64, 16, 32, 32, 13),
+                                             (64, 96, 4, 32, 8, 14), (96, 128, 8, 48, 32, 15)])
+def test_match_bit_exact_other_group_sizes(ctx, m, n, d, d2, K, seed):
+    """K = 8 and 32 (numMax): the K-th lane minimum, the survivor bound and the
+    32/64-lane sorts at other group sizes, bit for bit against the restatement."""
+    clean, noisy = synth.make_pair(seed, m, n, 25.0)
+    S, V = ctx.match(noisy, d, d2, K)
+    So, Vo, _ = O.match(noisy, d, d2, K, precision=32)
+    np.testing.assert_array_equal(S, So)
+    np.testing.assert_array_equal(V, Vo)
+
+
 @pytest.mark.parametrize("kind", ["zeros", "constant", "ramp", "sparse"])
 def test_match_degenerate_images_vs_fp32_restatement(ctx, kind):
     """Exact ties and 0/0 distances (all-zero reference blocks) resolve as mink does."""
