@@ -20,7 +20,8 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import Opts, check, load_library, opts_to_struct, ptr, struct_to_opts
+from ._abi import (GBM3D_ERR_ARG, GBM3D_ERR_UNSUPPORTED, GBM3DError, Opts, check, load_library, opts_to_struct, ptr,
+                   struct_to_opts)
 
 
 def setBM3Dopts(profile: str = "default") -> dict:
@@ -48,6 +49,32 @@ def resolve_opts(opts: dict | None) -> dict:
     o = Opts()
     check(lib.gbm3d_resolve_opts(C.byref(src) if src is not None else None, C.byref(o)))
     return struct_to_opts(o)
+
+
+def validate(opts: dict | None, m: int, n: int) -> int:
+    """Check opts and an m x n image against GBM3D's rules and the built
+    kernels (host only); returns the device workspace bytes of one image with
+    its match spins run in series.  Raises GBM3DError as gbm3d_denoise would."""
+    lib = load_library()
+    src = opts_to_struct(opts)
+    nb = C.c_size_t(0)
+    check(lib.gbm3d_validate(C.byref(src) if src is not None else None, m, n, C.byref(nb)))
+    return int(nb.value)
+
+
+def check_sigma_wie(opts: dict | None, sigma_wie) -> None:
+    """sigmaWie (getColoredWie's output) scales the blockSizeWie x blockSizeWie
+    DCT coefficients of every step-2 slice (denoiseAndAggregateWieFast.m:49):
+    with the Wiener step on it must be exactly that size."""
+    o = resolve_opts(opts)
+    if not o.get("Wiener"):
+        return
+    if sigma_wie is None:
+        raise GBM3DError(GBM3D_ERR_UNSUPPORTED, "coloured noise: sigmaWie (getColoredWie) must be given")
+    bw = o["blockSizeWie"]
+    if np.shape(sigma_wie) != (bw, bw):
+        raise GBM3DError(GBM3D_ERR_ARG, f"coloured noise: sigmaWie must be blockSizeWie x blockSizeWie "
+                                        f"({bw} x {bw}), got {' x '.join(map(str, np.shape(sigma_wie)))}")
 
 
 def getTau(levels: int, mode: int, sigma: float = 1.0) -> np.ndarray:
@@ -198,12 +225,18 @@ class Context:
         I = np.asfortranarray(I, dtype=np.float32)
         m, n = I.shape
         psd = np.asfortranarray(sigma_psd, dtype=np.float64)
+        if psd.ndim != 2:
+            raise ValueError("sigmaPSD must be a 2-D array")
         sw = None if sigma_wie is None else np.asfortranarray(sigma_wie, dtype=np.float64)
+        check_sigma_wie(opts, sw)
+        wm, wn = (0, 0) if sw is None else sw.shape
         U = np.zeros((m, n), dtype=np.float32, order="F")
         U1 = np.zeros_like(U)
         o = opts_to_struct(opts)
+        # the library checks wm x wn against the resolved blockSizeWie
         check(self.lib.gbm3d_denoise_colored(self.handle, ptr(I), m, n, ptr(psd), psd.shape[0], psd.shape[1],
-                                             ptr(sw), C.byref(o) if o is not None else None, ptr(U), ptr(U1)))
+                                             ptr(sw), wm, wn, C.byref(o) if o is not None else None, ptr(U),
+                                             ptr(U1)))
         return U, {"U1": U1}
 
     def denoise_distributed(self, I, sigma: float, opts: dict | None = None):

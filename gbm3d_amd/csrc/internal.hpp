@@ -62,6 +62,12 @@ struct PadGeom {
   int Q() const { return N + d2; }
 };
 
+// Block / match / group sizes the kernels are built for (checked by the
+// pipeline's option validation before anything is launched).
+bool match_supported(int d, int d2);       // FP32 matcher: d in {4, 8, 16, 32}, d2 in {32, 48}
+bool match_bf16_supported(int d, int d2);  // BF16-MFMA matcher: d in {8, 16}, d2 in {32, 48}
+bool wiener_supported(int d, int K);       // step-2 Wiener: d in {4, 8, 16}, K in {8, 16, 32}
+
 int launch_mirror_pad(const float* src, int m, int n, int M, int N, float* dst, hipStream_t st);
 int launch_search_pad(const float* img, int M, int N, int spin, int d2, float* pad, hipStream_t st);
 int launch_norm_map(const float* pad, int P, int Q, int d, float* bn, hipStream_t st);

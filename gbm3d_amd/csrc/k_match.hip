@@ -480,11 +480,16 @@ int launch_match_w(const float* pad, const float* bn, int M, int N, int d, int d
   GBM3D_MATCH_CASE(8, 32)
   GBM3D_MATCH_CASE(16, 32)
   GBM3D_MATCH_CASE(32, 32)
+  GBM3D_MATCH_CASE(4, 48)
   GBM3D_MATCH_CASE(8, 48)
   GBM3D_MATCH_CASE(16, 48)
+  GBM3D_MATCH_CASE(32, 48)
 #undef GBM3D_MATCH_CASE
-  return fail(GBM3D_ERR_UNSUPPORTED, "match: (blockSize, matchSize) must be (4|8|16|32, 32) or (8|16, 48)");
+  return fail(GBM3D_ERR_UNSUPPORTED, "match: (blockSize, matchSize) must be (4|8|16|32, 32|48)");
 }
+
+bool match_supported(int d, int d2) { return (d == 4 || d == 8 || d == 16 || d == 32) && (d2 == 32 || d2 == 48); }
+bool match_bf16_supported(int d, int d2) { return (d == 8 || d == 16) && (d2 == 32 || d2 == 48); }
 
 int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
                  hipStream_t st) {

@@ -167,10 +167,16 @@ int launch_wiener(const float* pad_pilot, const float* pad_noisy, const uint32_t
   GBM3D_WIE_CASE(8, 16)
   GBM3D_WIE_CASE(8, 8)
   GBM3D_WIE_CASE(8, 32)
+  GBM3D_WIE_CASE(4, 8)
   GBM3D_WIE_CASE(4, 16)
+  GBM3D_WIE_CASE(4, 32)
+  GBM3D_WIE_CASE(16, 8)
   GBM3D_WIE_CASE(16, 16)
+  GBM3D_WIE_CASE(16, 32)
 #undef GBM3D_WIE_CASE
-  return fail(GBM3D_ERR_UNSUPPORTED, "wiener: (blockSizeWie, numMax) must be (8, 8|16|32), (4, 16) or (16, 16)");
+  return fail(GBM3D_ERR_UNSUPPORTED, "wiener: (blockSizeWie, numMax) must be (4|8|16, 8|16|32)");
 }
+
+bool wiener_supported(int d, int K) { return (d == 4 || d == 8 || d == 16) && (K == 8 || K == 16 || K == 32); }
 
 }  // namespace gbm3d

@@ -1,0 +1,218 @@
+// learned.hip -- the learned-shrinkage form (a small BF16 conv network on
+// the DCT-Haar coefficients of each step-1 group replaces the wavelet hard
+// threshold), its per-image fallback judge, and the entry points of its
+// training (features, forward, gradient, Adam) and of the synthetic image
+// generator.  No reference counterpart; the gain targets follow
+// source/denoiseAndAggregateWieFast.m:49.
+#include <algorithm>
+#include <cmath>
+
+#include "ctx.hpp"
+
+using namespace gbm3d;
+
+namespace gbm3d {
+
+// The learned form needs loaded weights, 8 x 8 x 16 step-1 groups and a
+// residual buffer of [images][spins][kShrinkGrid][2].
+int learned_setup(gbm3d_ctx* c, const gbm3d_opts& o, int images) {
+  if (!c->ext.learned) return GBM3D_OK;
+  if (!c->shrink_wb) return fail(GBM3D_ERR_ARG, "learned shrinkage: no weights loaded (gbm3d_shrink_set_weights)");
+  if (o.blockSize != 8 || o.numMax != 16)
+    return fail(GBM3D_ERR_UNSUPPORTED, "learned shrinkage needs blockSize 8 and numMax 16");
+  const size_t need = (size_t)images * GBM3D_MAX_SPINS * 2 * kShrinkGrid;
+  if (need > c->sstats_floats) {
+    if (c->sstats) GBM3D_HIP(hipFree(c->sstats));
+    c->sstats = nullptr;
+    c->sstats_floats = 0;
+    GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->sstats), need * sizeof(float)));
+    c->sstats_floats = need;
+  }
+  return GBM3D_OK;
+}
+
+float* image_stats(gbm3d_ctx* c, int i) {
+  return c->ext.learned ? c->sstats + (size_t)i * GBM3D_MAX_SPINS * 2 * kShrinkGrid : nullptr;
+}
+
+static int shrink_grid_of(const Plan& p) { return shrink_grid((long)(p.M / p.bs) * (p.N / p.bs)); }
+
+// After a learned-form call has finished: per image, the energy the learned
+// gains removed from the noisy coefficients relative to the noise energy
+// (coefficients x sigma^2, all spins).  Removing more than `bound` times the
+// noise energy means the shrinkage also removed signal (the discrepancy
+// principle); such images are recomputed on the reference path (eagerly, on
+// the context stream).
+int learned_judge(gbm3d_ctx* c, const gbm3d_opts& o, int count, const std::vector<Plan>& plans,
+                  const float* const* dI, const float* sigma, float* const* dU, float* const* dU1) {
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  const size_t per = (size_t)GBM3D_MAX_SPINS * 2 * kShrinkGrid;
+  std::vector<float> h((size_t)count * per);
+  GBM3D_HIP(hipMemcpy(h.data(), c->sstats, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  const double bound = (c->ext.learned_bound > 0 ? c->ext.learned_bound : 1000) / 1000.0;
+  c->s_learned.assign(count, 0.0);
+  c->s_ht.assign(count, 0.0);
+  c->s_fell.assign(count, 0);
+  std::vector<int> redo;
+  for (int i = 0; i < count; ++i) {
+    const int grid = shrink_grid_of(plans[i]);
+    for (int sp = 0; sp < plans[i].n1; ++sp)
+      for (int b = 0; b < grid; ++b) {
+        c->s_learned[i] += h[i * per + (size_t)2 * kShrinkGrid * sp + 2 * b];
+        c->s_ht[i] += h[i * per + (size_t)2 * kShrinkGrid * sp + 2 * b + 1];
+      }
+    const double noise = (double)plans[i].n1 * plans[i].M * plans[i].N * plans[i].K * (double)sigma[i] * sigma[i];
+    c->s_learned[i] /= noise;
+    c->s_ht[i] /= noise;
+    if (!(c->s_learned[i] <= bound)) redo.push_back(i);
+  }
+  if (redo.empty()) return GBM3D_OK;
+  const gbm3d_ext keep = c->ext;
+  c->ext.learned = 0;
+  int st = GBM3D_OK;
+  for (int i : redo) {
+    c->s_fell[i] = 1;
+    st = ensure(&c->arena, &c->arena_bytes, pipeline_bytes(plans[i], o));
+    if (st == GBM3D_OK)
+      st = run_pipeline_on(c, dI[i], plans[i], sigma[i], o, dU[i], dU1 ? dU1[i] : nullptr, c->stream, c->arena,
+                           nullptr);
+    if (st != GBM3D_OK) break;
+  }
+  c->ext = keep;
+  return st;
+}
+
+}  // namespace gbm3d
+
+extern "C" {
+
+int gbm3d_shrink_set_weights(gbm3d_ctx* c, const float* params, int count) {
+  GBM3D_TRY(set_dev(c));
+  if (!params || count != GBM3D_SHRINK_PARAMS)
+    return fail(GBM3D_ERR_ARG, "shrink weights: expected " + std::to_string(GBM3D_SHRINK_PARAMS) + " values");
+  for (int i = 0; i < count; ++i)
+    if (!std::isfinite(params[i])) return fail(GBM3D_ERR_ARG, "shrink weights: non-finite value");
+  std::vector<uint16_t> wb((size_t)shrink::NL * shrink::CH * shrink::KW);
+  std::vector<float> fb(shrink::FB);
+  shrink_pack_weights(params, wb.data(), fb.data());
+  if (!c->shrink_wb) GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->shrink_wb), wb.size() * sizeof(uint16_t)));
+  if (!c->shrink_fb) GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->shrink_fb), fb.size() * sizeof(float)));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  GBM3D_HIP(hipMemcpy(c->shrink_wb, wb.data(), wb.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  GBM3D_HIP(hipMemcpy(c->shrink_fb, fb.data(), fb.size() * sizeof(float), hipMemcpyHostToDevice));
+  return GBM3D_OK;
+}
+
+int gbm3d_shrink_stats(gbm3d_ctx* c, int image, double* est_learned, double* est_ht, int* fell_back) {
+  GBM3D_TRY(set_dev(c));
+  if (image < 0 || image >= (int)c->s_learned.size()) return fail(GBM3D_ERR_ARG, "shrink stats: no such image");
+  if (est_learned) *est_learned = c->s_learned[image];
+  if (est_ht) *est_ht = c->s_ht[image];
+  if (fell_back) *fell_back = c->s_fell[image];
+  return GBM3D_OK;
+}
+
+int gbm3d_shrink_groups(gbm3d_ctx* c, const float* V, int G, float sigma, float* Vout, float* gains) {
+  GBM3D_TRY(set_dev(c));
+  if (!V || !Vout || G < 0) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (!c->shrink_wb) return fail(GBM3D_ERR_ARG, "learned shrinkage: no weights loaded (gbm3d_shrink_set_weights)");
+  if (G == 0) return GBM3D_OK;
+  const size_t n = (size_t)64 * 16 * G;
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** a, float** o, float** g) {
+    *a = k.take<float>(n);
+    *o = k.take<float>(n);
+    *g = k.take<float>(gains ? n : 0);
+  };
+  float *a, *o, *g;
+  lay(cv, &a, &o, &g);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &a, &o, &g);
+  GBM3D_TRY(upload(c, a, V, n));
+  GBM3D_TRY(launch_shrink(a, nullptr, 0, 0, G, sigma, c->shrink_wb, c->shrink_fb, o, gains ? g : nullptr, nullptr,
+                          nullptr, c->stream));
+  GBM3D_TRY(download(c, Vout, o, n));
+  if (gains) GBM3D_TRY(download(c, gains, g, n));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_shrink_features(gbm3d_ctx* c, const float* dClean, const float* dNoisy, int m, int n, float sigma,
+                          const uint32_t* dSel, int nsel, float* dX, float* dT, float* dWt) {
+  GBM3D_TRY(set_dev(c));
+  if (!dClean || !dNoisy || !dSel || !dX || !dT || !dWt || nsel < 0) return fail(GBM3D_ERR_ARG, "NULL argument");
+  constexpr int d = 8, d2 = 32, K = 16;
+  if (m % d || n % d || m < d2 || n < d2) return fail(GBM3D_ERR_ARG, "shrink features: m, n multiples of 8, >= 32");
+  if (!(sigma > 0.f)) return fail(GBM3D_ERR_ARG, "shrink features: sigma must be > 0");
+  const size_t P = (size_t)m + d2, Q = (size_t)n + d2, G = (size_t)m * n / (d * d);
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** pc, float** pn, float** bn, uint32_t** S) {
+    *pc = k.take<float>(P * Q);
+    *pn = k.take<float>(P * Q);
+    *bn = k.take<float>((P - d + 1) * (Q - d + 1));
+    *S = k.take<uint32_t>(G * K);
+  };
+  float *pc, *pn, *bn;
+  uint32_t* S;
+  lay(cv, &pc, &pn, &bn, &S);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &pc, &pn, &bn, &S);
+  GBM3D_TRY(launch_search_pad(dClean, m, n, 0, d2, pc, c->stream));
+  GBM3D_TRY(launch_search_pad(dNoisy, m, n, 0, d2, pn, c->stream));
+  GBM3D_TRY(launch_norm_map(pn, (int)P, (int)Q, d, bn, c->stream));
+  GBM3D_TRY(launch_match_w(pn, bn, m, n, d, d2, K, S, nullptr, c->match_waves, c->stream));
+  return launch_shrink_features(pc, pn, S, m, d2, dSel, nsel, sigma, dX, dT, dWt, c->stream);
+}
+
+int gbm3d_shrink_forward(gbm3d_ctx* c, const float* dParams, const float* dX, const float* dSig, int ngroups,
+                         float* dGains) {
+  GBM3D_TRY(set_dev(c));
+  if (!dParams || !dX || !dSig || !dGains || ngroups < 0) return fail(GBM3D_ERR_ARG, "NULL argument");
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, shrink_train_floats(ngroups) * sizeof(float)));
+  return shrink_forward(dParams, dX, dSig, ngroups, reinterpret_cast<float*>(c->arena), dGains, c->stream);
+}
+
+int gbm3d_shrink_grad(gbm3d_ctx* c, const float* dParams, const float* dX, const float* dT, const float* dWt,
+                      const float* dSig, int ngroups, float* dGrad, double* loss) {
+  GBM3D_TRY(set_dev(c));
+  if (!dParams || !dX || !dT || !dWt || !dSig || !dGrad) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (ngroups < 1) return fail(GBM3D_ERR_ARG, "shrink grad: empty batch");
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, shrink_train_floats(ngroups) * sizeof(float)));
+  float* ws = reinterpret_cast<float*>(c->arena);
+  int lb = 0;
+  GBM3D_TRY(shrink_grad(dParams, dX, dT, dWt, dSig, ngroups, ws, dGrad, nullptr, &lb, c->stream));
+  if (loss) {
+    // the loss partials follow the gradient workspace (see shrink_train_floats)
+    const size_t off = shrink_train_floats(ngroups) - 1024;
+    std::vector<float> h(lb);
+    GBM3D_HIP(hipMemcpyAsync(h.data(), ws + off, lb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    GBM3D_HIP(hipStreamSynchronize(c->stream));
+    double acc = 0.0;
+    for (float v : h) acc += v;
+    *loss = acc / ngroups;
+  }
+  return GBM3D_OK;
+}
+
+int gbm3d_shrink_adam(gbm3d_ctx* c, float* dParams, float* dM, float* dV, const float* dGrad, int step, float lr,
+                      float beta1, float beta2, float eps) {
+  GBM3D_TRY(set_dev(c));
+  if (!dParams || !dM || !dV || !dGrad) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (step < 1 || !(beta1 >= 0.f && beta1 < 1.f) || !(beta2 >= 0.f && beta2 < 1.f) || !(eps > 0.f))
+    return fail(GBM3D_ERR_ARG, "adam: step >= 1, 0 <= beta < 1, eps > 0");
+  const double lr_t = lr * std::sqrt(1.0 - std::pow((double)beta2, step)) / (1.0 - std::pow((double)beta1, step));
+  return launch_shrink_adam(dParams, dM, dV, dGrad, GBM3D_SHRINK_PARAMS, (float)lr_t, beta1, beta2, eps, c->stream);
+}
+
+int gbm3d_synth_pair(gbm3d_ctx* c, uint64_t seed, int m, int n, float sigma, float* dClean, float* dNoisy) {
+  GBM3D_TRY(set_dev(c));
+  if (m < 1 || n < 1) return fail(GBM3D_ERR_ARG, "synth: empty image");
+  SynthParams prm;
+  synth_params(seed, m, n, &prm);
+  return launch_synth_pair(prm, seed, m, n, sigma, dClean, dNoisy, c->stream);
+}
+
+}  // extern "C"
+

@@ -193,8 +193,10 @@ void colored(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   }
   ensure_ctx();
   std::vector<float> U(in.size()), U1(in.size());
-  int st = gbm3d_denoise_colored(g_ctx, in.data(), m, n, mxGetPr(prhs[2]), d1, d2, mxGetPr(prhs[3]), op, U.data(),
-                                 U1.data());
+  // the library rejects a sigmaWie that is not blockSizeWie x blockSizeWie
+  const int wm = (int)mxGetM(prhs[3]), wn = (int)mxGetN(prhs[3]);
+  int st = gbm3d_denoise_colored(g_ctx, in.data(), m, n, mxGetPr(prhs[2]), d1, d2, mxGetPr(prhs[3]), wm, wn, op,
+                                 U.data(), U1.data());
   if (st) fail(st);
   plhs[0] = to_double(U, m, n);
   if (nlhs > 1) {

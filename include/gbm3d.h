@@ -126,6 +126,17 @@ int gbm3d_get_ext(gbm3d_ctx* ctx, gbm3d_ext* ext);
 /* --------------------------------------------------------------- hot path */
 
 /*
+ * The checks GBM3D's stages make on opts and the image size (the option rule
+ * of source/GBM3D.m:15-19, the reference's own errors, and the block / match
+ * / group sizes this library is built for), on the host only: no GPU is
+ * touched.  workspace_bytes (nullable) receives the device workspace of one
+ * m x n image with its match spins run one after another.  Every gbm3d_denoise*
+ * call makes the same checks before it launches anything.  Returns
+ * GBM3D_ERR_UNSUPPORTED for a combination the kernels are not built for.
+ */
+int gbm3d_validate(const gbm3d_opts* opts, int m, int n, size_t* workspace_bytes);
+
+/*
  * [U, out] = GBM3D(I, sigma, opts)            -- source/GBM3D.m:1-84
  * I: m x n input (host), U: m x n output, U1 (nullable): m x n pilot
  * (out.U1, source/GBM3D.m:81).  opts may be NULL (profile 'default').
@@ -158,15 +169,16 @@ int gbm3d_denoise_distributed_device(gbm3d_ctx* ctx, const float* dI, int m,
  * Coloured noise: [U, out] = GBM3D(I, sigmaPSD, opts) with a matrix sigma --
  * source/GBM3D.m:29-36.  sigmaPSD: d1 x d2 (sigma^2 x the noise power
  * spectrum, column-major); thresholds from getColoredTau
- * (utilities/getColoredTau.m); sigmaWie: blockSizeWie^2 per-coefficient
- * Wiener sigma of getColoredWie (utilities/getColoredWie.m), required when
- * opts.Wiener (it needs makeGausPSF, which the reference does not contain,
+ * (utilities/getColoredTau.m); sigmaWie: wm x wn per-coefficient Wiener
+ * sigma of getColoredWie (utilities/getColoredWie.m), required when
+ * opts.Wiener and then exactly blockSizeWie x blockSizeWie (any other size is
+ * GBM3D_ERR_ARG; it needs makeGausPSF, which the reference does not contain,
  * so the caller computes it).  Host buffers.
  */
 int gbm3d_denoise_colored(gbm3d_ctx* ctx, const float* I, int m, int n,
                           const double* sigmaPSD, int d1, int d2,
-                          const double* sigmaWie, const gbm3d_opts* opts,
-                          float* U, float* U1);
+                          const double* sigmaWie, int wm, int wn,
+                          const gbm3d_opts* opts, float* U, float* U1);
 /* tau = getColoredTau(levels, sigmaPSD) (levels x 7, column-major). */
 int gbm3d_get_colored_tau(int levels, const double* sigmaPSD, int d1, int d2,
                           double* tau);

@@ -89,3 +89,57 @@ def test_null_arguments_return_status_not_crash():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(ImportError):
         _abi.load_library(str(tmp_path / "libgbm3d.so"))
+
+
+# Option sets GBM3D accepts, and those the built kernels reject before any
+# launch (UNSUPPORTED), checked on the host through gbm3d_validate.
+@pytest.mark.parametrize("opts", [None, {"profile": "superFast"}, {"profile": "accuracy"},
+                                  {"blockSize": 8, "matchSpinsWie": [0, 4]},
+                                  {"blockSizeWie": 4, "numMax": 32, "numMin": 32, "matchSpinsWie": [0]},
+                                  {"blockSizeWie": 16, "numMax": 8, "numMin": 8, "matchSpinsWie": [0]},
+                                  {"blockSize": 32, "matchSize": 48, "matchSpinsWie": [0]},
+                                  {"blockSize": 4, "blockSizeWie": 4, "matchSize": 48, "matchSpinsWie": [0]}])
+def test_validate_accepts_built_combinations(opts):
+    assert gbm3d_amd.validate(opts, 128, 96) > 0
+
+
+@pytest.mark.parametrize("opts,what", [
+    ({"numMax": 24, "numMin": 24, "matchSpinsWie": [0]}, "Wiener step"),        # Wiener group of 24 not built
+    ({"numMax": 64, "numMin": 64, "matchSpinsWie": [0]}, "Wiener step"),
+    ({"blockSizeWie": 32, "matchSpinsWie": [0]}, "blockSizeWie"),
+    ({"matchSize": 64, "matchSpinsWie": [0]}, "matchSize"),
+    ({"numMax": 16, "numMin": 8, "matchSpinsWie": [0]}, "tolerance"),
+])
+def test_validate_rejects_unbuilt_combinations(opts, what):
+    with pytest.raises(gbm3d_amd.GBM3DError, match=what) as e:
+        gbm3d_amd.validate(opts, 128, 96)
+    assert e.value.status == _abi.GBM3D_ERR_UNSUPPORTED
+
+
+def test_validate_reports_reference_errors_and_sizes():
+    with pytest.raises(gbm3d_amd.GBM3DError, match="matchSpinsWie") as e:
+        gbm3d_amd.validate({"blockSize": 8}, 64, 64)              # checkBM3Dopts has no matchSpinsWie default
+    assert e.value.status == _abi.GBM3D_ERR_MISSING_FIELD
+    with pytest.raises(gbm3d_amd.GBM3DError, match="matchSize"):
+        gbm3d_amd.validate(None, 12, 12)                          # pads to 16 x 16 < matchSize
+    # the workspace grows with the image
+    assert gbm3d_amd.validate(None, 256, 256) > 3 * gbm3d_amd.validate(None, 128, 128)
+
+
+def test_sigma_wie_must_be_block_size_wie_square():
+    from gbm3d_amd.api import check_sigma_wie
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0]}
+    check_sigma_wie(opts, np.ones((8, 8)))
+    for bad in (np.ones((4, 4)), np.ones((8, 7)), np.ones(64)):
+        with pytest.raises(gbm3d_amd.GBM3DError, match="blockSizeWie x blockSizeWie") as e:
+            check_sigma_wie(opts, bad)
+        assert e.value.status == _abi.GBM3D_ERR_ARG
+    with pytest.raises(gbm3d_amd.GBM3DError, match="sigmaWie"):
+        check_sigma_wie(opts, None)
+    check_sigma_wie({"profile": "superFast"}, None)               # no Wiener step: not needed
+
+
+def test_colored_abi_carries_sigma_wie_size():
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    decl = re.search(r"int gbm3d_denoise_colored\((.*?)\);", txt, flags=re.S).group(1)
+    assert re.search(r"const double\* sigmaWie,\s*int wm,\s*int wn", decl)
