@@ -137,6 +137,7 @@ SIGNATURES = {
     "gbm3d_resolve_opts": (_I, [C.POINTER(Opts), C.POINTER(Opts)]),
     "gbm3d_get_tau": (_I, [_I, _I, _D, _P]),
     "gbm3d_get_colored_tau": (_I, [_I, _P, _I, _I, _P]),
+    "gbm3d_denoise_device_noalloc": (_I, [_P, _P, _I, _I, C.c_float, C.POINTER(Opts), _P, _P]),
     "gbm3d_validate": (_I, [C.POINTER(Opts), _I, _I, C.POINTER(C.c_size_t)]),
     "gbm3d_denoise_colored": (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _I, _I, C.POINTER(Opts), _P, _P]),
     "gbm3d_last_error": (C.c_char_p, []),

@@ -266,6 +266,13 @@ class Context:
                                             C.byref(o) if o is not None else None, C.c_void_p(dU),
                                             C.c_void_p(dU1) if dU1 else None))
 
+    def denoise_device_noalloc(self, dI: int, m: int, n: int, sigma: float, opts, dU: int, dU1: int | None = None):
+        """gbm3d_denoise_device without any device allocation (static scratch array)."""
+        o = opts_to_struct(opts)
+        check(self.lib.gbm3d_denoise_device_noalloc(self.handle, C.c_void_p(dI), m, n, C.c_float(sigma),
+                                                    C.byref(o) if o is not None else None, C.c_void_p(dU),
+                                                    C.c_void_p(dU1) if dU1 else None))
+
     def denoise_batch_device(self, dIs, ms, ns, sigmas, opts, dUs, dU1s=None, graph=True, tune=False):
         """Batched GBM3D over device buffers (lists of int pointers); asynchronous."""
         cnt = len(dIs)

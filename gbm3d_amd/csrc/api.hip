@@ -69,6 +69,13 @@ int gbm3d_create(int device, gbm3d_ctx** out) {
     delete c;
     return fail(GBM3D_ERR_HIP, "mask table upload failed");
   }
+  if (noalloc_scratch_init(device, &c->noalloc_ws) != GBM3D_OK) {
+    const std::string why = gbm3d_last_error();
+    (void)hipFree(c->masks);
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return fail(GBM3D_ERR_HIP, why);
+  }
   *out = c;
   return GBM3D_OK;
 }

@@ -6,6 +6,7 @@
 //   batch.hip        many images per call (graph capture, launch-shape tuner)
 //   learned.hip      the learned-shrinkage form and its training entry points
 //   distributed.hip  GBM3D_distributed (tiles, ramp-window stitching)
+//   noalloc.hip      the entry point that allocates nothing
 #pragma once
 
 #include <climits>
@@ -24,6 +25,7 @@ struct gbm3d_ctx {
   char* io = nullptr;  // staging for the host-buffer entry points
   size_t io_bytes = 0;
   float* masks = nullptr;  // [4 block sizes][2][32*32]
+  char* noalloc_ws = nullptr;  // this device's static scratch array (noalloc.hip)
   gbm3d_ext ext{};
   bool timing = false;
   bool in_batch = false;  // stage timers off while a batch runs on several streams
@@ -158,6 +160,10 @@ int learned_setup(gbm3d_ctx* c, const gbm3d_opts& o, int images);
 float* image_stats(gbm3d_ctx* c, int i);
 int learned_judge(gbm3d_ctx* c, const gbm3d_opts& o, int count, const std::vector<Plan>& plans,
                   const float* const* dI, const float* sigma, float* const* dU, float* const* dU1);
+
+// noalloc.hip: the device address of this device's scratch array, looked up
+// once per device (the device must be current).
+int noalloc_scratch_init(int device, char** ptr);
 
 int set_dev(gbm3d_ctx* c);
 int resolve_checked(const gbm3d_opts* in, gbm3d_opts* out);

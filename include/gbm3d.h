@@ -183,6 +183,24 @@ int gbm3d_denoise_colored(gbm3d_ctx* ctx, const float* I, int m, int n,
 int gbm3d_get_colored_tau(int levels, const double* sigmaPSD, int d1, int d2,
                           double* tau);
 
+/*
+ * gbm3d_denoise_device with no device allocation: no hipMalloc, hipFree or
+ * allocating library call, for callers that cannot allocate (code under
+ * stream capture, tight serving loops, harnesses that forbid it).  No
+ * counterpart in the reference.  The workspace is a static device array of
+ * GBM3D_NOALLOC_SCRATCH_BYTES per GPU, held from library load to process end
+ * (gbm3d_validate reports what an image needs: 512 x 512 fits with every
+ * profile); an image that needs more is GBM3D_ERR_ARG.  The array is shared
+ * by every call on that GPU: calls that overlap on one GPU race, calls in
+ * order on one stream do not.  Match spins run one after another on the
+ * context stream; the learned form is GBM3D_ERR_UNSUPPORTED.  Results equal
+ * gbm3d_denoise_device's bit for bit.
+ */
+#define GBM3D_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int gbm3d_denoise_device_noalloc(gbm3d_ctx* ctx, const float* dI, int m, int n,
+                                 float sigma, const gbm3d_opts* opts,
+                                 float* dU, float* dU1);
+
 /* ------------------------------------------------------------- batches
  * [U_i, out_i] = GBM3D(I_i, sigma_i, opts) for `count` independent images
  * (device buffers, sizes m[i] x n[i]) sharing one opts.  Images run on
