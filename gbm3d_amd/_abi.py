@@ -61,7 +61,8 @@ class Ext(C.Structure):
     """struct gbm3d_ext (include/gbm3d.h)."""
 
     _fields_ = [("match_bf16", C.c_int32), ("rescore", C.c_int32), ("learned", C.c_int32),
-                ("learned_bound", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("learned_bound", C.c_int32), ("match_stats", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 _INT_FIELDS = ("numMax", "numMin", "levels", "cycleSpin", "tauMode", "blockSize",
@@ -138,6 +139,7 @@ SIGNATURES = {
     "gbm3d_get_tau": (_I, [_I, _I, _D, _P]),
     "gbm3d_get_colored_tau": (_I, [_I, _P, _I, _I, _P]),
     "gbm3d_denoise_device_noalloc": (_I, [_P, _P, _I, _I, C.c_float, C.POINTER(Opts), _P, _P]),
+    "gbm3d_bf16_match_stats": (_I, [_P, C.POINTER(C.c_ulonglong), _I]),
     "gbm3d_validate": (_I, [C.POINTER(Opts), _I, _I, C.POINTER(C.c_size_t)]),
     "gbm3d_denoise_colored": (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _I, _I, C.POINTER(Opts), _P, _P]),
     "gbm3d_last_error": (C.c_char_p, []),

@@ -123,11 +123,13 @@ class Context:
         check(self.lib.gbm3d_synchronize(self.handle))
 
     def set_variant(self, match_bf16: bool = False, rescore: int = 0, learned: bool = False,
-                    learned_bound: float = 1.0, weights=None):
+                    learned_bound: float = 1.0, weights=None, match_stats: bool = False):
         """Forms beside the reference path (all False = reference).
 
-        match_bf16: BF16-MFMA distances + exact FP32 rescoring of the best
-        `rescore` (>= 2*numMax) candidates per tile.
+        match_bf16: BF16-MFMA distances with error bounds + exact FP32
+        rescoring of every candidate the bounds cannot exclude (first set of
+        `rescore` candidates per tile, numMax..64; 0: 2*numMax); index lists
+        equal the FP32 path's bit for bit.
         learned: learned shrinkage replaces the step-1 hard threshold; an image
         from which the learned gains remove more than learned_bound x its
         noise energy is recomputed on the reference path.  Loads
@@ -141,9 +143,19 @@ class Context:
         e.rescore = int(rescore)
         e.learned = int(bool(learned))
         e.learned_bound = int(round(learned_bound * 1000)) if learned else 0
+        e.match_stats = int(bool(match_stats))
         check(self.lib.gbm3d_set_ext(self.handle, C.byref(e)))
 
     # ------------------------------------------------------ learned shrinkage
+    def bf16_match_stats(self, reset: bool = True) -> dict:
+        """Path counters of the BF16 matcher (tiles settled on the first
+        rescored set, on a widened set, by the exact search; candidates rescored)."""
+        out = (C.c_ulonglong * 8)()
+        check(self.lib.gbm3d_bf16_match_stats(self.handle, out, int(reset)))
+        return {"first_set": out[0], "widened": out[1], "exact_all": out[2], "rescored": out[3],
+                "exact_all_why": {"first_set_over_256": out[4], "no_kth_key": out[5], "widened_over_256": out[6],
+                                  "zero_norm_reference": out[7]}}
+
     def load_shrink_weights(self, params):
         p = np.ascontiguousarray(params, dtype=np.float32).ravel()
         check(self.lib.gbm3d_shrink_set_weights(self.handle, ptr(p), p.size))

@@ -88,6 +88,7 @@ int gbm3d_destroy(gbm3d_ctx* c) {
   if (c->io) (void)hipFree(c->io);
   if (c->masks) (void)hipFree(c->masks);
   if (c->shrink_wb) (void)hipFree(c->shrink_wb);
+  if (c->bf16_stats) (void)hipFree(c->bf16_stats);
   if (c->shrink_fb) (void)hipFree(c->shrink_fb);
   if (c->sstats) (void)hipFree(c->sstats);
   for (auto e : c->pool) (void)hipEventDestroy(e);
@@ -106,7 +107,32 @@ int gbm3d_destroy(gbm3d_ctx* c) {
 int gbm3d_set_ext(gbm3d_ctx* c, const gbm3d_ext* e) {
   GBM3D_TRY(set_dev(c));
   if (e && (e->rescore < 0 || e->rescore > 64)) return fail(GBM3D_ERR_ARG, "ext: rescore must be 0..64");
+  if (e && e->match_stats && !c->bf16_stats) {
+    // the BF16 matcher's path counters, allocated and cleared here at set-up
+    const std::vector<uint32_t> z((size_t)kBf16StatSlots * kBf16StatStride, 0u);
+    GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->bf16_stats), z.size() * sizeof(uint32_t)));
+    GBM3D_HIP(hipMemcpy(c->bf16_stats, z.data(), z.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
   c->ext = e ? *e : gbm3d_ext{};
+  return GBM3D_OK;
+}
+
+int gbm3d_bf16_match_stats(gbm3d_ctx* c, unsigned long long* out, int reset) {
+  GBM3D_TRY(set_dev(c));
+  if (!out) return fail(GBM3D_ERR_ARG, "NULL output");
+  std::vector<uint32_t> h((size_t)kBf16StatSlots * kBf16StatStride, 0u);
+  if (c->bf16_stats) {
+    GBM3D_HIP(hipStreamSynchronize(c->stream));
+    GBM3D_HIP(hipMemcpy(h.data(), c->bf16_stats, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (reset) {
+      const std::vector<uint32_t> z(h.size(), 0u);
+      GBM3D_HIP(hipMemcpy(c->bf16_stats, z.data(), z.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+  }
+  for (int i = 0; i < kBf16Stats; ++i) {
+    out[i] = 0;
+    for (int sl = 0; sl < kBf16StatSlots; ++sl) out[i] += h[(size_t)sl * kBf16StatStride + i];
+  }
   return GBM3D_OK;
 }
 

@@ -27,6 +27,7 @@ struct gbm3d_ctx {
   float* masks = nullptr;  // [4 block sizes][2][32*32]
   char* noalloc_ws = nullptr;  // this device's static scratch array (noalloc.hip)
   gbm3d_ext ext{};
+  uint32_t* bf16_stats = nullptr;  // BF16 matcher path counters (gbm3d_bf16_match_stats)
   bool timing = false;
   bool in_batch = false;  // stage timers off while a batch runs on several streams
   // lane streams: match spins / batch images / tile cases in flight at once

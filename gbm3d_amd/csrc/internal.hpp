@@ -77,10 +77,20 @@ int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2,
                  float* vol, hipStream_t st);
 // d x d box sums of the padded image (the centring term of the BF16 matcher).
 int launch_sum_map(const float* pad, int P, int Q, int d, float* sm, hipStream_t st);
-// BF16-MFMA matching: approximate scores, exact FP32 rescoring of at least R
-// (>= K) candidates, same S / volume outputs as launch_match.
+// BF16-MFMA matching: approximate scores with rigorous error bounds; every
+// candidate that can reach the top K is rescored with the FP32 chain, so S
+// and the volume equal launch_match's bit for bit.  R (K..64): size of the
+// first rescored set.  stats (nullable, kBf16Stats uint32 counters): tiles
+// finished on the first set [0], on a widened set [1], by the exact search
+// over all candidates [2]; rescored candidates [3]; why the exact search ran:
+// first set above 256 candidates [4], no finite K-th key in it [5], widened
+// set above 256 [6], reference block of zero or non-finite norm [7].
+constexpr int kBf16Stats = 8;
+// The counters live in kBf16StatSlots copies kBf16StatStride words apart
+// (one cache line each), summed on the host.
+constexpr int kBf16StatSlots = 64, kBf16StatStride = 32;
 int launch_match_bf16(const float* pad, const float* bn, const float* smap, int M, int N, int d, int d2, int K,
-                      int R, uint32_t* S, float* vol, hipStream_t st);
+                      int R, uint32_t* S, float* vol, hipStream_t st, uint32_t* stats = nullptr);
 // Same with nw (1, 2, 4) reference tiles per workgroup.
 int launch_match_w(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
                    int nw, hipStream_t st);

@@ -247,57 +247,17 @@ struct MatchOcc {
   static constexpr int kWaves = (163840 / kLds) * NW / 4 >= 8 ? 8 : 1;
 };
 
-template <int D, int D2, int NW = 4>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOcc<D, D2, NW>::kWaves))) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
-                                                        int M, int N, int K, uint32_t* __restrict__ S,
-                                                        float* __restrict__ vol) {
+// The exact top K of one reference tile over its D2 x D2 window candidates:
+// W the tile's window in LDS (candidate (a, b) at W[a + LDW b], the reference
+// block at (D2/2, D2/2)), bn the block-norm map (row stride Pm) with the
+// tile's window origin at (ti D, tj D), nx = |x|^2, scr kSelMax keys of LDS
+// for this wave.  Lane k (< K) returns the k-th smallest key.  Every
+// candidate is scored with the FMA chain described at match_kernel.
+template <int D, int D2, int LDW>
+__device__ __forceinline__ uint64_t exact_search(const float* W, const float* __restrict__ bn, int Pm, int ti, int tj,
+                                                 float nx, int K, uint64_t* scr, int lane) {
   using C = MatchCfg<D, D2>;
   constexpr int S2 = D2 / 2;
-  constexpr int RW = C::WR + (NW - 1) * D;          // union rows
-  constexpr int LDW = (RW % 2 == 0) ? RW + 1 : RW;  // odd: conflict-free columns
-  __shared__ float win[C::WR * LDW];
-  __shared__ uint64_t scr[NW][kSelMax];
-  // the wave index in a scalar register: the tile's addresses are wave-uniform
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int Mt = M / D, Mb = (Mt + NW - 1) / NW;
-  const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int ti0 = (blk % Mb) * NW, tj = blk / Mb;
-  const int nval = min(NW, Mt - ti0);
-  const int P = M + D2, Pm = P - D + 1;
-  const int rows = C::WR + (nval - 1) * D;  // union rows inside the padded image
-  const float* src = pad + (size_t)(ti0 * D) + (size_t)P * (tj * D);
-  if constexpr ((64 * NW) % RW == 0) {
-    // each thread keeps one union row and steps over the columns: 32-bit
-    // offsets from the window's origin, LDS offsets immediate
-    // (rows past the union, masked below, may lie past the padded image: the
-    // resource ends with the image, so such loads return 0 and stay inside it)
-    constexpr int CPR = 64 * NW / RW, NR = C::WR / CPR;
-    static_assert(C::WR % CPR == 0, "window columns per round");
-    const int rr = threadIdx.x % RW, c0 = threadIdx.x / RW;
-    const size_t left = (size_t)P * (N + D2) - (size_t)(src - pad);
-    const uint32_t span = 4u * (uint32_t)(RW + P * (C::WR - 1));
-    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(src, left < span / 4 ? 4u * (uint32_t)left : span);
-    const uint32_t off = 4u * (uint32_t)(rr + P * c0);
-    float x[NR];
-#pragma unroll
-    for (int i = 0; i < NR; ++i) x[i] = buf_load(rs, off, 4u * (uint32_t)(P * CPR * i));
-#pragma unroll
-    for (int i = 0; i < NR; ++i) win[rr + LDW * (c0 + CPR * i)] = rr < rows ? x[i] : 0.f;
-  } else {
-    for (int e = threadIdx.x; e < RW * C::WR; e += 64 * NW) {
-      const int rr = e % RW, cc = e / RW;
-      win[rr + LDW * cc] = rr < rows ? src[rr + (size_t)P * cc] : 0.f;
-    }
-  }
-  if (NW > 1)
-    __syncthreads();
-  else
-    wave_sync();
-  if (wave >= nval) return;  // no barrier below
-  const int ti = ti0 + wave, g = ti + Mt * tj;
-  const float* W = win + D * wave;
-  const float nx = bn[(ti * D + S2) + (size_t)Pm * (tj * D + S2)];
-
   float numv[C::NL];  // |x|^2 + |y|^2 - 2<x,y> of this lane's candidates
 #pragma unroll
   for (int it = 0; it < C::IT; ++it) {
@@ -377,13 +337,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
         for (int j = 0; j < C::NL; ++j) {
           if (numv[j] <= bound) {
             const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(mk[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[j], 0u));
-            scr[wave][base + r] = ((uint64_t)__float_as_uint(numv[j]) << 32) | (uint32_t)cand_idx(j);
+            scr[base + r] = ((uint64_t)__float_as_uint(numv[j]) << 32) | (uint32_t)cand_idx(j);
           }
           base += (uint32_t)__popcll(mk[j]);
         }
         wave_sync();
         auto key_at = [&](int q) {
-          const uint64_t e = scr[wave][q];
+          const uint64_t e = scr[q];
           return cand_key(__uint_as_float((uint32_t)(e >> 32)) / nx, (int)(uint32_t)e);
         };
         if (total <= 64) {
@@ -428,8 +388,63 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
     uint64_t key[C::NL];
 #pragma unroll
     for (int j = 0; j < C::NL; ++j) key[j] = cand_key(numv[j] / nx, cand_idx(j));
-    best = wave_topk<C::NL>(key, K, scr[wave], lane);
+    best = wave_topk<C::NL>(key, K, scr, lane);
   }
+  return best;
+}
+
+template <int D, int D2, int NW = 4>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOcc<D, D2, NW>::kWaves))) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
+                                                        int M, int N, int K, uint32_t* __restrict__ S,
+                                                        float* __restrict__ vol) {
+  using C = MatchCfg<D, D2>;
+  constexpr int S2 = D2 / 2;
+  constexpr int RW = C::WR + (NW - 1) * D;          // union rows
+  constexpr int LDW = (RW % 2 == 0) ? RW + 1 : RW;  // odd: conflict-free columns
+  __shared__ float win[C::WR * LDW];
+  __shared__ uint64_t scr[NW][kSelMax];
+  // the wave index in a scalar register: the tile's addresses are wave-uniform
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int Mt = M / D, Mb = (Mt + NW - 1) / NW;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti0 = (blk % Mb) * NW, tj = blk / Mb;
+  const int nval = min(NW, Mt - ti0);
+  const int P = M + D2, Pm = P - D + 1;
+  const int rows = C::WR + (nval - 1) * D;  // union rows inside the padded image
+  const float* src = pad + (size_t)(ti0 * D) + (size_t)P * (tj * D);
+  if constexpr ((64 * NW) % RW == 0) {
+    // each thread keeps one union row and steps over the columns: 32-bit
+    // offsets from the window's origin, LDS offsets immediate
+    // (rows past the union, masked below, may lie past the padded image: the
+    // resource ends with the image, so such loads return 0 and stay inside it)
+    constexpr int CPR = 64 * NW / RW, NR = C::WR / CPR;
+    static_assert(C::WR % CPR == 0, "window columns per round");
+    const int rr = threadIdx.x % RW, c0 = threadIdx.x / RW;
+    const size_t left = (size_t)P * (N + D2) - (size_t)(src - pad);
+    const uint32_t span = 4u * (uint32_t)(RW + P * (C::WR - 1));
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(src, left < span / 4 ? 4u * (uint32_t)left : span);
+    const uint32_t off = 4u * (uint32_t)(rr + P * c0);
+    float x[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) x[i] = buf_load(rs, off, 4u * (uint32_t)(P * CPR * i));
+#pragma unroll
+    for (int i = 0; i < NR; ++i) win[rr + LDW * (c0 + CPR * i)] = rr < rows ? x[i] : 0.f;
+  } else {
+    for (int e = threadIdx.x; e < RW * C::WR; e += 64 * NW) {
+      const int rr = e % RW, cc = e / RW;
+      win[rr + LDW * cc] = rr < rows ? src[rr + (size_t)P * cc] : 0.f;
+    }
+  }
+  if (NW > 1)
+    __syncthreads();
+  else
+    wave_sync();
+  if (wave >= nval) return;  // no barrier below
+  const int ti = ti0 + wave, g = ti + Mt * tj;
+  const float* W = win + D * wave;
+  const float nx = bn[(ti * D + S2) + (size_t)Pm * (tj * D + S2)];
+
+  const uint64_t best = exact_search<D, D2, LDW>(W, bn, Pm, ti, tj, nx, K, scr[wave], lane);
   int idx = (int)(uint32_t)best;
   if (lane == 0) idx = S2 + D2 * S2;  // slot 1 := window centre (matchBlocksCPU.m:103)
   if (lane < K) S[lane + (size_t)K * g] = (uint32_t)idx + 1u;
@@ -497,12 +512,16 @@ int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2,
 }
 
 // ------------------------------------------------ BF16-MFMA matching
-// Precision variant.  Cross terms on BF16 MFMA with FP32 accumulation, then
-// every candidate whose approximate score is at most the R-th smallest lane
-// minimum (R >= K; at least R candidates, at most 256 before a slower exact
-// fallback) is rescored with exactly the FP32 chain of match_kernel, so their
-// keys are bit-identical to the FP32 path's; the final top-K is taken over
-// those keys.
+// Precision variant.  Cross terms on BF16 MFMA with FP32 accumulation give an
+// approximate score and, per candidate, a lower bound of its exact FP32
+// numerator up to a tile constant (error bounds below).  The R candidates
+// with the smallest bounds (R >= K; every candidate at or below the R-th
+// smallest lane minimum) are rescored with exactly the FP32 chain of
+// match_kernel; their K-th exact key then excludes every candidate whose
+// bound proves a strictly larger rounded distance, and any candidate not
+// excluded is rescored as well (or, past 256 candidates, all are).  The
+// final top K is taken over exact keys of a set that provably holds the FP32
+// path's top K, so S equals the FP32 path's bit for bit, ties included.
 //
 // corr(a,b) = sum_v sum_r X_v(a,r) Y(r, b+v), X_v(a,r) = x(r-a, v): per block
 // column v a band Toeplitz A operand (one fragment serves every 16-row M tile,
@@ -518,8 +537,8 @@ int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2,
 // BF16's relative precision on the differences that rank the candidates:
 //   <x,y> = <x-c, y-c> + c*(sum y) + (terms common to the tile),
 // so the approximate score is |y|^2 - 2(<x-c,y-c> + c sum y).
-// A reference block of zero norm (every exact distance Inf or NaN) takes the
-// exact FP32 search over all candidates.
+// A reference block of zero (or non-finite) norm takes the exact FP32 search
+// over all candidates.
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
@@ -531,6 +550,15 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {  // round to neares
   return (unsigned short)(b >> 16);
 }
 
+// f = hi + lo + r with hi = bf16(f), lo = bf16(f - hi) (f - hi is exact in
+// FP32), |r| <= 2^-16 |f|.
+__device__ __forceinline__ void split_bf16(float f, unsigned short* hi, unsigned short* lo) {
+  const unsigned short h = to_bf16(f);
+  const float hf = __uint_as_float((uint32_t)h << 16);
+  *hi = h;
+  *lo = to_bf16(f - hf);
+}
+
 __device__ __forceinline__ uint32_t ord_f32(float d) {
   uint32_t b = __float_as_uint(d);
   if ((b & 0x7fffffffu) > 0x7f800000u) return 0xffffffffu;
@@ -538,7 +566,72 @@ __device__ __forceinline__ uint32_t ord_f32(float d) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-constexpr int kRescoreMax = 256;  // rescored candidates per tile before the round-by-round fallback
+constexpr int kRescoreMax = 128;  // rescored candidates per tile before the FP32 search over all
+
+// ---- error bounds of the approximate scores
+// Reals: x, y the reference and candidate blocks (FP32 values), c the FP32
+// window centring constant, cx the FP32 centring constant of x, bn(y) and
+// Sm(y) the FP32 norm and sum maps, nx = bn(x).
+// The FP32 path ranks exnum(y) = fl(fl(nx + bn) - 2 cr), cr the FMA chain of
+// <x, y> (D^2 roundings from 0); with N(y) = nx + bn(y) - 2<x,y> (nx, bn taken
+// as exact), |exnum - N| <= E1 = (D^2 + 4) u (nx + bn), u = 2^-24.
+// Identity (<x,y> = <x-cx, y-c> + c sum(x) + cx sum(y) - cx c D^2):
+//   N(y) = T + Z(y),  Z(y) = bn(y) - 2<x-cx, y-c> - 2 cx sum(y),
+// with the tile constant T = nx - 2c sum(x) + 2 cx c D^2.
+// The centred values f = fl(v - c) (fl(v - cx) for x) are split
+// f = hi + lo + r, hi = bf16(f), lo = bf16(f - hi) (f - hi is exact),
+// |r| <= 2^-16 |f|, and the MFMA cross term is acc = sum (ha hb + ha lb + la hb)
+// with FP32 accumulation (three MFMAs per block column; the dropped la lb and
+// the r terms are at most 3.02 2^-16 |a||b| per product, the subtractions
+// fl(v - c) add 2^-23):
+//   |acc - <x-cx, y-c>| <= kappa_D |x-cx| |y-c|,
+//   kappa_D = 3.1 2^-16 + 6.1 D^2 2^-24
+// (the second term lets each of the 3 D^2 non-zero products' partial sums
+// lose one ulp per addition, rounded or not; zero products add exactly),
+// plus flushed subnormals (at most 2^-126 per product or addition: the
+// slack term).  sc = fl(bn - 2 fl(acc + fl(cx Sm))) then satisfies
+//   |sc - Z| <= E2 = 2 kappa |x-cx| |y-c| + 2|cx| |Sm - sum y| + u (bn + 5|acc| + 7|cx Sm|) + slack,
+// |Sm - sum y| <= D^2 u sum|y| <= D^3 u sqrt(bn) (1.01).  B(y) >= E1 + E2
+// (with the rounding of its own evaluation and of sc - B) gives
+//   exnum(y) >= T + sc(y) - B(y) = T + lb(y).
+constexpr float kU32 = 0x1p-24f;
+template <int D>
+__host__ __device__ constexpr float bf16x3_kappa() {
+  return 3.1f * 0x1p-16f + 6.1f * (float)(D * D) * 0x1p-24f;
+}
+
+// Upper bound of |y - c| from b = bn(y), s = Sm(y) (sqb = sqrt(b)):
+// |y - c|^2 = sum y^2 - 2c sum y + c^2 D^2, with sum y^2 <= b (1 + 2 D^2 u),
+// |sum y - s| <= D^3 u sqrt(b) (1.01) and <= 4u (b + 2|c s| + c^2 D^2) for the
+// FP32 evaluation (9u there also covers the rounding of this bound itself).
+template <int D>
+__device__ __forceinline__ float centred_norm_up(float b, float s, float c, float sqb) {
+  constexpr float DD = (float)(D * D), D3 = (float)(D * D * D);
+  const float cs = c * s, c2 = (c * c) * DD;
+  const float qv = (b - 2.f * cs) + c2;
+  const float tq = (b + 2.f * fabsf(cs)) + c2;
+  const float qu = qv + kU32 * ((2.f * DD + 4.f) * b + 9.f * tq + 2.1f * fabsf(c) * D3 * sqb);
+  return sqrtf(fmaxf(qu, 0.f)) * (1.f + 0x1p-20f);
+}
+
+// Approximate score sc and its bound B >= E1 + E2 (+ the rounding of sc - B)
+// for one candidate: acc the MFMA cross term, bnp = bn(y), smp = Sm(y), nxc
+// an upper bound of |x - cx|.
+template <int D>
+__device__ __forceinline__ void bf16_bound(float acc, float bnp, float smp, float c, float cx, float nx, float nxc,
+                                           float* sc, float* bnd) {
+  constexpr float DD = (float)(D * D), D3 = (float)(D * D * D);
+  const float t1 = cx * smp;
+  const float s = bnp - 2.0f * (acc + t1);
+  const float sqb = sqrtf(fmaxf(bnp, 0.f));
+  const float nyc = centred_norm_up<D>(bnp, smp, c, sqb);
+  const float e = 2.f * bf16x3_kappa<D>() * nxc * nyc +
+                  kU32 * ((DD + 5.f) * (nx + bnp) + 5.f * fabsf(acc) + 7.f * fabsf(t1) +
+                          2.1f * fabsf(cx) * D3 * sqb + 4.f * fabsf(s)) +
+                  DD * 0x1p-100f * (1.f + nxc + nyc);
+  *sc = s;
+  *bnd = e * (1.f + 0x1p-16f);
+}
 
 template <int D, int D2>
 struct Bf16Cfg {
@@ -560,22 +653,24 @@ struct Bf16Cfg {
 template <int D, int D2>
 __global__ __launch_bounds__(256) void match_bf16_kernel(
     const float* __restrict__ pad, const float* __restrict__ bn, const float* __restrict__ smap, int M, int N, int K,
-    int R, uint32_t* __restrict__ S, float* __restrict__ vol) {
+    int R, uint32_t* __restrict__ S, float* __restrict__ vol, uint32_t* __restrict__ stats) {
   using C = Bf16Cfg<D, D2>;
   constexpr int S2 = D2 / 2, NWT = C::NWT, NTH = 64 * NWT;
+  constexpr float DD = (float)(D * D);
   __shared__ float Wf[C::WC * C::LDW];
-  __shared__ __attribute__((aligned(16))) unsigned short Wb[C::WC * C::LB];
-  __shared__ float Bn[C::CC * C::LN], Sm[C::CC * C::LN];
-  __shared__ unsigned short xpad[NWT][D * C::XP];
-  __shared__ unsigned short clist[NWT][kRescoreMax];  // window indices of the rescored candidates
-  __shared__ uint64_t scr[NWT][64];
+  __shared__ __attribute__((aligned(16))) unsigned short Wb[C::WC * C::LB];  // bf16 hi part of the centred window
+  __shared__ __attribute__((aligned(16))) unsigned short Wl[C::WC * C::LB];  // and its lo part
+  __shared__ unsigned short xpad[NWT][2][D * C::XP];  // zero-padded centred reference columns, hi and lo
+  // per wave: the exact search's keys, or the rescored set's (at most 64)
+  // keys below its candidate list
+  __shared__ uint64_t scr[NWT][kSelMax];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int Mt = M / D, Nt = N / D;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int ti = blk % Mt, tj0 = (blk / Mt) * NWT;
   const int nval = min(NWT, Nt - tj0);
   const int P = M + D2, Pm = P - D + 1;
-  // centring constant: mean of the workgroup's reference blocks
+  // window centring constant: mean of the workgroup's reference blocks
   float csum = 0.f;
   for (int w = 0; w < nval; ++w) csum += smap[(ti * D + S2) + (size_t)Pm * ((tj0 + w) * D + S2)];
   const float cm = csum / (float)(nval * D * D);
@@ -585,149 +680,205 @@ __global__ __launch_bounds__(256) void match_bf16_kernel(
     const int rr = e % C::WR, cc = e / C::WR;
     Wf[rr + C::LDW * cc] = cc < wcv ? src[rr + (size_t)P * cc] : 0.f;
   }
-  const int ccv = D2 + (nval - 1) * D;
-  for (int e = tid; e < D2 * C::CC; e += NTH) {
-    const int a = e % D2, cc = e / D2;
-    const size_t o = (size_t)(ti * D + a) + (size_t)Pm * (tj0 * D + cc);
-    Bn[a + C::LN * cc] = cc < ccv ? bn[o] : 0.f;
-    Sm[a + C::LN * cc] = cc < ccv ? smap[o] : 0.f;
-  }
   __syncthreads();
   for (int e = tid; e < C::RB * C::WC; e += NTH) {
     const int rr = e % C::RB, cc = e / C::RB;
-    Wb[rr + C::LB * cc] = rr < C::WR ? to_bf16(Wf[rr + C::LDW * cc] - cm) : (unsigned short)0;
+    unsigned short h = 0, lo = 0;
+    if (rr < C::WR) split_bf16(Wf[rr + C::LDW * cc] - cm, &h, &lo);
+    Wb[rr + C::LB * cc] = h;
+    Wl[rr + C::LB * cc] = lo;
   }
-  unsigned short* xp = xpad[wave];
+  const int tj = tj0 + wave;
+  const size_t xo = (size_t)(ti * D + S2) + (size_t)Pm * (tj * D + S2);  // the reference block in the maps
+  // the reference block is centred on its own mean cx: <x,y> = <x-cx, y-c> + cx sum(y) + (tile constants)
+  const float nx = wave < nval ? bn[xo] : 1.f, smx = wave < nval ? smap[xo] : 0.f;
+  const float cx = smx / DD;
+  unsigned short* xp = xpad[wave][0];
+  unsigned short* xl = xpad[wave][1];
   for (int e = lane; e < D * C::XP; e += 64) {
     const int v = e / C::XP, t = e % C::XP - 16;
-    xp[e] = (t >= 0 && t < D) ? to_bf16(Wf[(S2 + t) + C::LDW * (S2 + v + D * wave)] - cm) : (unsigned short)0;
+    unsigned short h = 0, lo = 0;
+    if (t >= 0 && t < D) split_bf16(Wf[(S2 + t) + C::LDW * (S2 + v + D * wave)] - cx, &h, &lo);
+    xp[e] = h;
+    xl[e] = lo;
   }
   __syncthreads();
   if (wave >= nval) return;  // no barrier below
-  const int g = ti + Mt * (tj0 + wave);
+  const int g = ti + Mt * tj;
   const int q = lane >> 4, l = lane & 15;
   const float* Wt = Wf + C::LDW * (D * wave);  // this tile's window
-  const float nx = Bn[S2 + C::LN * (S2 + D * wave)];
-  // exact key of window candidate (a, b): the FMA chain of match_kernel
-  auto exact_key = [&](int a, int b) {
+  const float* bnt = bn + (size_t)(ti * D) + (size_t)Pm * (tj * D);
+  const float* smt = smap + (size_t)(ti * D) + (size_t)Pm * (tj * D);
+  // exact numerator of window candidate (a, b): the FMA chain of match_kernel
+  auto exact_num = [&](int a, int b) {
     float cr = 0.f;
 #pragma unroll 1
     for (int v = 0; v < D; ++v)
 #pragma unroll
       for (int u = 0; u < D; ++u) cr = fmaf(Wt[(S2 + u) + C::LDW * (S2 + v)], Wt[(a + u) + C::LDW * (b + v)], cr);
-    const float bnp = Bn[a + C::LN * (b + D * wave)];
-    const float num = (nx + bnp) - 2.0f * cr;
-    return cand_key(num / nx, a + D2 * b);
+    return (nx + bnt[a + (size_t)Pm * b]) - 2.0f * cr;
   };
+  auto exact_key = [&](int a, int b) { return cand_key(exact_num(a, b) / nx, a + D2 * b); };
   auto cand_ab = [&](int j, int& a, int& b) {
     const int mt = j / (4 * C::NT), nt = (j / 4) % C::NT, i = j % 4;
     a = 16 * mt + 4 * q + i;
     b = 16 * nt + l;
   };
-  uint64_t best;
-  if (nx == 0.f) {
-    uint64_t ekey[C::NC];
-#pragma unroll 1
+  uint64_t* sk = scr[wave];
+  unsigned short* cl = reinterpret_cast<unsigned short*>(sk + 64);  // the candidate list, above the keys
+  // exact top K over the candidates j with in(j), `total` of them (<= kRescoreMax)
+  auto rescore = [&](auto in, int total) {
+    wave_sync();  // the list and keys may still be read by a previous pass
+    uint32_t base = 0;
+#pragma unroll
     for (int j = 0; j < C::NC; ++j) {
-      int a, b;
-      cand_ab(j, a, b);
-      ekey[j] = exact_key(a, b);
+      const bool f = in(j);
+      const uint64_t m = __ballot(f);
+      if (f) {
+        int a, b;
+        cand_ab(j, a, b);
+        cl[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+            (unsigned short)(a + D2 * b);
+      }
+      base += (uint32_t)__popcll(m);
     }
-    best = wave_topk<C::NC>(ekey, K, scr[wave], lane);
-  } else {
-    bf16x8_t A[D];
+    wave_sync();
+    uint64_t ek[kRescoreMax / 64];
 #pragma unroll
-    for (int v = 0; v < D; ++v) {
-      unsigned short t[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) t[j] = xp[v * C::XP + 8 * q - l + 16 + j];
-      A[v] = __builtin_bit_cast(bf16x8_t, t);
+    for (int rr = 0; rr < kRescoreMax / 64; ++rr) {
+      const int c = lane + 64 * rr;
+      ek[rr] = ~0ull;
+      if (64 * rr < total && c < total) {
+        const int w = cl[c];
+        ek[rr] = exact_key(w % D2, w / D2);
+      }
     }
+    wave_sync();  // every lane has read the list before the keys are written
+    return wave_topk<kRescoreMax / 64>(ek, K, sk, lane);
+  };
+  uint64_t best;
+  int path = 2, nres = 0;  // 0: first set, 1: widened set, 2: every candidate
+  int why = 4;             // for path 2: 4 first set too large, 5 no K-th key, 6 widened set too large
+  bool done = false;
+  if (nx > 0.f && nx <= 1e38f) {  // (2 nx finite: see the centre's numerator below)
     f4_t acc[C::MT][C::NT];
 #pragma unroll
     for (int mt = 0; mt < C::MT; ++mt)
 #pragma unroll
       for (int nt = 0; nt < C::NT; ++nt) acc[mt][nt] = (f4_t){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
+#pragma unroll 2
     for (int v = 0; v < D; ++v) {
+      unsigned short th[8], tl[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        th[j] = xp[v * C::XP + 8 * q - l + 16 + j];
+        tl[j] = xl[v * C::XP + 8 * q - l + 16 + j];
+      }
+      const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, th), Al = __builtin_bit_cast(bf16x8_t, tl);
 #pragma unroll
       for (int nt = 0; nt < C::NT; ++nt) {
-        const unsigned short* col = Wb + C::LB * (16 * nt + l + v + D * wave) + 8 * q;
+        const int co = C::LB * (16 * nt + l + v + D * wave) + 8 * q;
 #pragma unroll
         for (int mt = 0; mt < C::MT; ++mt) {
-          const bf16x8_t B = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(col + 16 * mt));
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[v], B, acc[mt][nt], 0, 0, 0);
+          const bf16x8_t Bh = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(Wb + co + 16 * mt));
+          const bf16x8_t Bl = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(Wl + co + 16 * mt));
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh, acc[mt][nt], 0, 0, 0);
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl, acc[mt][nt], 0, 0, 0);
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, acc[mt][nt], 0, 0, 0);
         }
       }
     }
-    uint32_t key[C::NC];
+    // Approximate score sc(y) = |y|^2 - 2(<x-cx, y-c>_bf16x3 + cx sum(y)) and a
+    // lower bound lb(y) = sc(y) - B(y) such that every candidate's exact FP32
+    // numerator satisfies  exnum(y) >= T_low + lb(y),  T_low a lower bound of
+    // the tile constant T (see bf16_bound).
+    const float nxc = centred_norm_up<D>(nx, smx, cx, sqrtf(nx));
+    float lb[C::NC];
+    float scC = 0.f, bC = 0.f;
+    constexpr int jc = ((S2 / 16) * C::NT + S2 / 16) * 4 + S2 % 4;  // the centre (a, b) = (S2, S2)
+    constexpr int lc = 16 * ((S2 % 16) / 4) + S2 % 16;               // and its lane
 #pragma unroll
     for (int mt = 0; mt < C::MT; ++mt)
 #pragma unroll
       for (int nt = 0; nt < C::NT; ++nt)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int a = 16 * mt + 4 * q + i, cc = 16 * nt + l + D * wave;
-          const float sc = Bn[a + C::LN * cc] - 2.0f * (acc[mt][nt][i] + cm * Sm[a + C::LN * cc]);
-          key[(mt * C::NT + nt) * 4 + i] = ord_f32(sc);
+          const int a = 16 * mt + 4 * q + i, b = 16 * nt + l;
+          const int j = (mt * C::NT + nt) * 4 + i;
+          const size_t o = (size_t)a + (size_t)Pm * b;
+          float sc, bnd;
+          bf16_bound<D>(acc[mt][nt][i], bnt[o], smt[o], cm, cx, nx, nxc, &sc, &bnd);
+          lb[j] = sc - bnd;
+          if (j == jc) {
+            scC = sc;
+            bC = bnd;
+          }
         }
-    // rescored set: every candidate whose approximate key is <= theta, theta =
-    // R-th smallest lane minimum (at least R of them), compacted in LDS
+    // first set: every candidate whose bound is at most the R-th smallest
+    // lane minimum of the bounds (at least R of them)
     uint32_t lmin = 0xffffffffu;
 #pragma unroll
-    for (int j = 0; j < C::NC; ++j) lmin = min(lmin, key[j]);
+    for (int j = 0; j < C::NC; ++j) lmin = min(lmin, ord_f32(lb[j]));
     const uint32_t theta = wave_kth_smallest(lmin, R);
-    int total = 0;
+    auto in0 = [&](int j) { return ord_f32(lb[j]) <= theta; };
+    int total0 = 0;
 #pragma unroll
-    for (int j = 0; j < C::NC; ++j) total += __popcll(__ballot(key[j] <= theta));
-    unsigned short* cl = clist[wave];
-    if (total <= kRescoreMax) {
-      // compacted slot by slot (the rescored selection depends on the set alone)
-      uint32_t base = 0;
-#pragma unroll
-      for (int j = 0; j < C::NC; ++j) {
-        const bool in = key[j] <= theta;
-        const uint64_t m = __ballot(in);
-        if (in) {
-          int a, b;
-          cand_ab(j, a, b);
-          cl[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
-              (unsigned short)(a + D2 * b);
-        }
-        base += (uint32_t)__popcll(m);
-      }
-      wave_sync();
-      uint64_t ek[kRescoreMax / 64];
-#pragma unroll
-      for (int rr = 0; rr < kRescoreMax / 64; ++rr) {
-        const int c = lane + 64 * rr;
-        ek[rr] = ~0ull;
-        if (c < total) {
-          const int w = cl[c];
-          ek[rr] = exact_key(w % D2, w / D2);
-        }
-      }
-      best = wave_topk<kRescoreMax / 64>(ek, K, scr[wave], lane);
-    } else {
-      // long runs of equal scores: the R smallest (score, index) pairs, one round each
-      int mine = -1;
-      uint64_t prev = 0;
-      for (int r = 0; r < R; ++r) {
-        uint64_t lm = ~0ull;
+    for (int j = 0; j < C::NC; ++j) total0 += __popcll(__ballot(in0(j)));
+    if (theta != 0xffffffffu && total0 <= kRescoreMax) {
+      const uint64_t bestA = rescore(in0, total0);
+      nres = total0;
+      // T_low from the centre candidate (x itself): T = N(x) - Z(x) with
+      // N(x) >= exnum(x) - E1 and Z(x) <= sc(x) + E2
+      scC = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(scC), lc));
+      bC = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(bC), lc));
+      // exnum(x) = fl(fl(2 nx) - 2 cr(x, x)) = 0 exactly: the norm map forms
+      // nx with the very FMA chain of cr, so cr(x, x) = nx
+      const float tlow = (-scC - bC) - 4.f * kU32 * (fabsf(scC) + bC);
+      // the K-th exact key of the first set: a candidate whose numerator
+      // exceeds thrN has a strictly larger rounded distance, so it is not in
+      // the top K (the K-th of a subset is >= the K-th of all)
+      const uint32_t kth = (uint32_t)(__builtin_amdgcn_readlane((uint32_t)(bestA >> 32), K - 1));
+      why = kth != 0xffffffffu ? 6 : 5;
+      if (kth != 0xffffffffu) {
+        const float pk = ord_to_f32(kth) * nx;
+        const float thrN = fmaf(fabsf(pk), 0x1p-19f, pk) + nx * 0x1p-123f;
+        const float thr = (thrN - tlow) + 2.f * kU32 * (fabsf(thrN) + fabsf(tlow));
+        // a candidate can enter the top K only if !(lb > thr) (NaN included)
+        auto in1 = [&](int j) { return in0(j) || !(lb[j] > thr); };
+        int extra = 0, total1 = 0;
 #pragma unroll
         for (int j = 0; j < C::NC; ++j) {
-          int a, b;
-          cand_ab(j, a, b);
-          const uint64_t kk = ((uint64_t)key[j] << 32) | (uint32_t)(a + D2 * b);
-          if ((r == 0 || kk > prev) && kk < lm) lm = kk;
+          extra += __popcll(__ballot(!in0(j) && !(lb[j] > thr)));
+          total1 += __popcll(__ballot(in1(j)));
         }
-        lm = wave_min_u64(lm);
-        if (lane == r) mine = (int)(uint32_t)lm;
-        prev = lm;
+        if (extra == 0) {
+          best = bestA;
+          path = 0;
+          done = true;
+        } else if (total1 <= kRescoreMax) {
+          best = rescore(in1, total1);
+          nres += total1;
+          path = 1;
+          done = true;
+        }
       }
-      const uint64_t ek = mine >= 0 ? exact_key(mine % D2, mine / D2) : ~0ull;
-      best = wave_sort64<uint64_t>(ek, lane);
     }
+  } else {
+    why = 7;
+  }
+  if (!done) {
+    // the FP32 search itself (zero-norm reference blocks; tiles whose bounds
+    // leave more than kRescoreMax candidates, e.g. along a straight edge)
+    wave_sync();
+    best = exact_search<D, D2, C::LDW>(Wt, bn, Pm, ti, tj, nx, K, sk, lane);
+    nres += D2 * D2;
+  }
+  if (stats && lane == 0) {
+    // counters spread over kBf16StatSlots cache lines (few tiles share one)
+    uint32_t* sl = stats + (size_t)(blk % kBf16StatSlots) * kBf16StatStride;
+    atomicAdd(&sl[path], 1u);
+    atomicAdd(&sl[3], (uint32_t)nres);
+    if (path == 2) atomicAdd(&sl[why], 1u);
   }
   int idx = (int)(uint32_t)best;
   if (lane == 0) idx = S2 + D2 * S2;  // slot 1 := window centre (matchBlocksCPU.m:103)
@@ -737,7 +888,7 @@ __global__ __launch_bounds__(256) void match_bf16_kernel(
   for (int k = 0; k < K; ++k) {
     const int ik = __builtin_amdgcn_readlane(idx, k);
     const int a = ik % D2, b = ik / D2;
-    float* dst = vol + MN * k + (size_t)(ti * D) + (size_t)M * ((tj0 + wave) * D);
+    float* dst = vol + MN * k + (size_t)(ti * D) + (size_t)M * (tj * D);
     for (int e = lane; e < D * D; e += 64) {
       const int u = e % D, v = e / D;
       dst[u + (size_t)M * v] = Wt[(a + u) + C::LDW * (b + v)];
@@ -746,7 +897,7 @@ __global__ __launch_bounds__(256) void match_bf16_kernel(
 }
 
 int launch_match_bf16(const float* pad, const float* bn, const float* smap, int M, int N, int d, int d2, int K,
-                      int R, uint32_t* S, float* vol, hipStream_t st) {
+                      int R, uint32_t* S, float* vol, hipStream_t st, uint32_t* stats) {
   if (M % d || N % d || M < d2 || N < d2) return fail(GBM3D_ERR_ARG, "match: image size");
   if (K < 1 || K > 64) return fail(GBM3D_ERR_UNSUPPORTED, "match: numMax must be in [1, 64]");
   if (R < K) R = K;
@@ -757,7 +908,7 @@ int launch_match_bf16(const float* pad, const float* bn, const float* smap, int 
     constexpr int nwt = Bf16Cfg<DD, DD2>::NWT;                                                          \
     const dim3 grid(Mt * ((Nt + nwt - 1) / nwt));                                                       \
     hipLaunchKernelGGL((match_bf16_kernel<DD, DD2>), grid, dim3(64 * nwt), 0, st, pad, bn, smap, M, N, K, R, \
-                       S, vol);                                                                         \
+                       S, vol, stats);                                                                  \
     return check_launch("match_bf16");                                                                  \
   }
   GBM3D_BF16_CASE(8, 32)

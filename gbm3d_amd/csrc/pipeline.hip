@@ -167,8 +167,9 @@ int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int 
   }
   StageTimer t(c, stage);
   if (bf16) {
-    const int R = c->ext.rescore > 0 ? std::max(c->ext.rescore, 2 * K) : 2 * K;
-    return launch_match_bf16(pad, bn, smap, M, N, d, d2, K, std::min(R, 64), S, vol, st);
+    const int R = c->ext.rescore > 0 ? std::max(c->ext.rescore, K) : 2 * K;
+    return launch_match_bf16(pad, bn, smap, M, N, d, d2, K, std::min(R, 64), S, vol, st,
+                             c->ext.match_stats ? c->bf16_stats : nullptr);
   }
   return launch_match_w(pad, bn, M, N, d, d2, K, S, vol, c->match_waves, st);
 }

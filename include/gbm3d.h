@@ -109,8 +109,12 @@ int gbm3d_synchronize(gbm3d_ctx* ctx);
  * NULL) is the reference behaviour.  No reference counterpart. */
 typedef struct gbm3d_ext {
   int32_t match_bf16; /* 1: block distances on BF16 MFMA (FP32 accumulation),
-                         then exact FP32 rescoring of the best candidates    */
-  int32_t rescore;    /* candidates rescored per tile, >= 2*numMax (0: 2*numMax) */
+                         then exact FP32 rescoring of every candidate the
+                         error bounds cannot exclude: S equals the FP32
+                         path's bit for bit (8 x 8 and 16 x 16 blocks; other
+                         block sizes run the FP32 search)                    */
+  int32_t rescore;    /* first rescored set per tile, numMax..64 (0: 2*numMax);
+                         a launch-shape choice: results do not depend on it */
   int32_t learned;    /* 1: learned shrinkage replaces the step-1 wavelet hard
                          threshold (blockSize 8, numMax 16; weights from
                          gbm3d_shrink_set_weights)                            */
@@ -118,10 +122,20 @@ typedef struct gbm3d_ext {
                          learned gains remove more than bound x its noise
                          energy (coefficients x sigma^2) is recomputed on the
                          reference path (0: 1000)                             */
-  int32_t reserved[4];
+  int32_t match_stats; /* 1: count the BF16 matcher's paths for
+                         gbm3d_bf16_match_stats (diagnostics: a few atomic
+                         adds per reference tile)                             */
+  int32_t reserved[3];
 } gbm3d_ext;
 int gbm3d_set_ext(gbm3d_ctx* ctx, const gbm3d_ext* ext);
 int gbm3d_get_ext(gbm3d_ctx* ctx, gbm3d_ext* ext);
+/* Path counters of the BF16 matcher since the last reset (synchronises the
+ * context stream), out[8]: [0] tiles settled on the first rescored set, [1]
+ * on a widened set, [2] by the exact search over every candidate, [3]
+ * candidates rescored in total; why the exact search ran: [4] first set above
+ * 256 candidates, [5] no finite K-th key in it, [6] widened set above 256,
+ * [7] reference block of zero or non-finite norm.  reset != 0 clears them. */
+int gbm3d_bf16_match_stats(gbm3d_ctx* ctx, unsigned long long* out, int reset);
 
 /* --------------------------------------------------------------- hot path */
 

@@ -20,7 +20,7 @@ HEADER = os.path.join(ROOT, "include", "gbm3d.h")
 def declared_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(gbm3d_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(gbm3d_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():

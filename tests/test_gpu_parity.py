@@ -266,76 +266,6 @@ def test_error_paths(ctx):
     assert np.isfinite(U).all()
 
 
-@pytest.mark.parametrize("m,n,d,d2,seed", [(256, 256, 8, 32, 91), (256, 192, 16, 32, 92), (192, 256, 8, 48, 93),
-                                           (128, 128, 16, 48, 94), (256, 200, 8, 32, 96), (96, 80, 16, 48, 97),
-                                           (64, 48, 16, 32, 98)])
-def test_match_bf16_rescored_agrees_with_fp32(m, n, d, d2, seed):
-    """BF16-MFMA scores + exact FP32 rescoring of the best 2K: the index sets
-    equal the FP32 path's wherever its top K lie in the rescored set (all
-    tiles here), and the gathered volume follows S."""
-    import gbm3d_amd
-
-    clean, noisy = synth.make_pair(seed, m, n, 25.0)
-    c32 = gbm3d_amd.Context(0)
-    cbf = gbm3d_amd.Context(0)
-    cbf.set_variant(match_bf16=True)
-    S32, V32 = c32.match(noisy, d, d2, 16)
-    Sbf, Vbf = cbf.match(noisy, d, d2, 16)
-    agree = np.mean(np.all(S32 == Sbf, axis=0))
-    assert agree >= 0.999, agree
-    if agree == 1.0:
-        np.testing.assert_array_equal(Vbf, V32)
-    c32.close()
-    cbf.close()
-
-
-@pytest.mark.parametrize("kind", ["zeros", "half"])
-def test_match_bf16_zero_norm_blocks_take_exact_search(kind):
-    """A reference block of zero norm has every exact distance Inf or NaN, which
-    the BF16 scores cannot rank: such tiles take the exact FP32 search, so their
-    index lists equal the FP32 path's bit for bit.  Other tiles may differ only
-    at exact distance ties."""
-    import gbm3d_amd
-
-    clean, noisy = synth.make_pair(99, 128, 144, 25.0)
-    I = np.zeros_like(noisy) if kind == "zeros" else noisy.copy()
-    if kind == "half":
-        I[:, :64] = 0.0
-    c32 = gbm3d_amd.Context(0)
-    cbf = gbm3d_amd.Context(0)
-    cbf.set_variant(match_bf16=True)
-    for d in (8, 16):
-        S32, _ = c32.match(I, d, 32, 16)
-        Sbf, _ = cbf.match(I, d, 32, 16)
-        zero = np.array([[not np.any(O.search_pad(I, 32)[ti * d + 16:ti * d + 16 + d, tj * d + 16:tj * d + 16 + d])
-                          for tj in range(I.shape[1] // d)] for ti in range(I.shape[0] // d)])
-        assert zero.any()
-        np.testing.assert_array_equal(Sbf[:, zero], S32[:, zero])
-        # elsewhere the index lists differ only at exact FP32 distance ties (runs of
-        # identical all-zero candidate blocks longer than the rescored set)
-        _, _, dist = O.match(I, d, 32, 16, precision=32, want_dist=True)
-        for i, j in zip(*np.where(np.any(S32 != Sbf, axis=0))):
-            np.testing.assert_array_equal(np.sort(dist[S32[1:, i, j].astype(int) - 1, i, j]),
-                                          np.sort(dist[Sbf[1:, i, j].astype(int) - 1, i, j]))
-    c32.close()
-    cbf.close()
-
-
-def test_bf16_variant_end_to_end():
-    import gbm3d_amd
-
-    clean, noisy = synth.make_pair(95, 256, 256, 25.0)
-    c32 = gbm3d_amd.Context(0)
-    cbf = gbm3d_amd.Context(0)
-    cbf.set_variant(match_bf16=True)
-    for opts in (None, {"blockSize": 8, "matchSpinsWie": [0, 4]}):
-        U32, _ = c32.denoise(noisy, 25.0, opts)
-        Ubf, _ = cbf.denoise(noisy, 25.0, opts)
-        assert abs(synth.psnr(clean, Ubf) - synth.psnr(clean, U32)) < 0.01
-    c32.close()
-    cbf.close()
-
-
 def _batch_inputs():
     sizes = [(96, 128), (200, 150), (64, 64), (130, 260), (256, 192), (100, 100)]
     sigmas = [10.0, 25.0, 40.0, 5.0, 50.0, 20.0]
