@@ -140,6 +140,41 @@ def ht_filter(V, levels, cycle_spin, tau, precision=64, want_margin=False):
     return (out, mm.value) if want_margin else out
 
 
+def ht_near_mask(V, levels, cycle_spin, tau, tol=1e-3):
+    """Voxels of ht_filter's output in the synthesis support of a detail
+    coefficient within tol (absolute) of its threshold (FP32 restatement):
+    (mask (m, n, K) bool, number of such coefficients)."""
+    V = _f(V, np.float32)
+    m, n, K = V.shape
+    mask = np.zeros((m, n, K), dtype=np.uint8, order="F")
+    tau = np.asfortranarray(tau, dtype=np.float64)
+    f = lib().orc_ht_near_mask_f32
+    f.restype = C.c_long
+    cnt = f(_p(V), m, n, K, levels, cycle_spin, _p(tau), C.c_double(tol), _p(mask))
+    if cnt < 0:
+        raise RuntimeError("oracle ht_near_mask failed")
+    return mask.astype(bool), int(cnt)
+
+
+def u1_near_mask(I, sigma, opts=None, tol=1e-3):
+    """Pixels of GBM3D's first estimate U1 that a step-1 coefficient within
+    tol (absolute) of its threshold can move (FP32 restatement; every pixel covered
+    by a group whose TV weight such a coefficient changes):
+    (mask (m, n) bool, number of such coefficients)."""
+    I = _f(I, np.float32)
+    m, n = I.shape
+    o = Opts()
+    src = opts_to_struct(opts)
+    lib().orc_resolve_opts(C.byref(src) if src is not None else None, C.byref(o))
+    mask = np.zeros((m, n), dtype=np.uint8, order="F")
+    f = lib().orc_u1_near_mask_f32
+    f.restype = C.c_long
+    cnt = f(_p(I), m, n, C.c_float(sigma), C.byref(o), C.c_double(tol), _p(mask))
+    if cnt < 0:
+        raise RuntimeError("oracle u1_near_mask failed")
+    return mask.astype(bool), int(cnt)
+
+
 def median3(V, precision=64):
     """medfilt3(Volume, [3 3 3]) of the step-1 'median' branch."""
     dt, suf = _dt(precision)

@@ -493,6 +493,117 @@ void ht_filter(const T* V, int m, int n, int K, int levels, int cycleSpin, const
   }
 }
 
+// ------------------------------------------------- near-threshold masks
+// Which outputs of ht_filter can move by more than round-off when the
+// transform's operation order changes: every voxel in the synthesis support
+// of a detail coefficient C with | |C| - tau | <= tol (an absolute margin on
+// the coefficients' scale; the hard threshold C .* (abs(C) > tau) may decide
+// either way there).  A level-l
+// coefficient (l = 0 finest) at index s along an axis reaches the full-size
+// positions [2^(l+1) s - (t - 1)(2^(l+1) - 1), 2^(l+1) s] (mod size), t the
+// synthesis filter length (10 in x, y; 2 in z): each synthesis level maps
+// index s to 2s - k, k in [0, t) (myWavRec3DGPUfast.m:19-21,33-37).  The box
+// is taken in the cycle-spun frame and shifted back.  vmask: m x n x K, set
+// to 1 where affected (never cleared).  Returns the number of such
+// coefficients.
+template <class T>
+long ht_near_mask(const T* V, int m, int n, int K, int levels, int cycleSpin, const double* tau, double tol,
+                  uint8_t* vmask) {
+  Bank<T> F;
+  long count = 0;
+  auto mark_box = [&](int l, int si, int sj, int sk, int sp) {
+    const int f = 1 << (l + 1);
+    const int ri = 9 * (f - 1), rk = f - 1;
+    for (int dk = 0; dk <= rk; ++dk) {
+      const int k = pmod((long long)f * sk - dk - sp, K);
+      for (int dj = 0; dj <= ri; ++dj) {
+        const int c = pmod((long long)f * sj - dj - sp, n);
+        for (int di = 0; di <= ri; ++di) {
+          const int r = pmod((long long)f * si - di - sp, m);
+          vmask[r + (size_t)m * (c + (size_t)n * k)] = 1;
+        }
+      }
+    }
+  };
+  for (int sp = 0; sp < cycleSpin; ++sp) {
+    Vol<T> X(m, n, K);
+    for (int k = 0; k < K; ++k)
+      for (int c = 0; c < n; ++c)
+        for (int r = 0; r < m; ++r)
+          X.at(r, c, k) = V[pmod(r - sp, m) + (size_t)m * (pmod(c - sp, n) + (size_t)n * pmod(k - sp, K))];
+    Vol<T> A = X;
+    for (int l = 0; l < levels; ++l) {
+      Vol<T> C[8];
+      analysis_level<T>(A, C, F);
+      for (int b = 1; b < 8; ++b) {
+        const double t = tau[l + levels * (band_j(b) - 1)];
+        if (!(t > 0)) continue;
+        const Vol<T>& Cb = C[b];
+        for (int kk = 0; kk < Cb.r; ++kk)
+          for (int j = 0; j < Cb.q; ++j)
+            for (int i = 0; i < Cb.p; ++i)
+              if (std::fabs(std::fabs((double)Cb.at(i, j, kk)) - t) <= tol) {
+                ++count;
+                mark_box(l, i, j, kk, sp);
+              }
+      }
+      A = std::move(C[0]);
+    }
+  }
+  return count;
+}
+
+// The pixels of GBM3D's first estimate U1 (m x n, after the crop) that a
+// near-threshold coefficient of step 1 can move (ht_near_mask per match
+// spin): a changed voxel changes its group's TV weight, so every pixel that
+// any of the group's K blocks covers is marked (source/GBM3D.m:50-57,
+// denoiseAndAggregateGPU.m:70-89).  Returns the number of near-threshold
+// coefficients.
+template <class T>
+long u1_near_mask(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, double tol, uint8_t* mask_out) {
+  const int bs = o.blockSize, d2 = o.matchSize, K = o.numMax, s = d2 / 2;
+  const int M = (m + bs - 1) / bs * bs, N = (n + bs - 1) / bs * bs;
+  std::vector<T> I = mirror_pad<T>(I0, m, n, M, N);
+  std::vector<double> tau((size_t)o.levels * 7);
+  get_tau(o.levels, o.tauMode, sigma, tau.data());
+  const size_t MN = (size_t)M * N;
+  const int Mt = M / bs, Nt = N / bs;
+  std::vector<uint8_t> pm(MN, 0);
+  std::vector<uint32_t> S((size_t)K * Mt * Nt);
+  std::vector<T> Vol(MN * K);
+  long count = 0;
+  for (int ii = 0; ii < o.nMatchSpins; ++ii) {
+    const int sp = o.matchSpins[ii];
+    std::vector<T> Is = circshift2<T>(I.data(), M, N, sp);
+    match_step1<T>(Is.data(), M, N, bs, d2, K, S.data(), Vol.data(), nullptr);
+    std::vector<uint8_t> vm(MN * K, 0);
+    count += ht_near_mask<T>(Vol.data(), M, N, K, o.levels, o.cycleSpin, tau.data(), tol, vm.data());
+    for (int x = 0; x < Nt; ++x)
+      for (int y = 0; y < Mt; ++y) {
+        bool hit = false;
+        for (int k = 0; k < K && !hit; ++k)
+          for (int v = 0; v < bs && !hit; ++v)
+            for (int u = 0; u < bs && !hit; ++u) hit = vm[(y * bs + u) + (size_t)M * ((x * bs + v) + (size_t)N * k)];
+        if (!hit) continue;
+        const size_t g = y + (size_t)Mt * x;
+        for (int k = 0; k < K; ++k) {
+          const uint32_t idx = S[k + K * g] - 1u;
+          const int r0 = y * bs + (int)(idx % d2) - s, c0 = x * bs + (int)(idx / d2) - s;
+          for (int v = 0; v < bs; ++v)
+            for (int u = 0; u < bs; ++u) {
+              const int r = r0 + u, c = c0 + v;
+              if (r < 0 || r >= M || c < 0 || c >= N) continue;  // crop-drop
+              // this spin's frame is the image circshifted by sp
+              pm[pmod(r - sp, M) + (size_t)M * pmod(c - sp, N)] = 1;
+            }
+        }
+      }
+  }
+  for (int c = 0; c < n; ++c)
+    for (int r = 0; r < m; ++r) mask_out[r + (size_t)m * c] = pm[r + (size_t)M * c];
+  return count;
+}
+
 // ---------------------------------------------------------------- aggregation
 // getBM3DAggWeights(3,1) (utilities/getBM3DAggWeights.m:15-19) on one group
 // stack blk (d x d x K, column-major), summed in column-major order.

@@ -203,6 +203,30 @@ int orc_gbm3d_f32(const float* I, int m, int n, float sigma, const gbm3d_opts* r
 }
 
 
+long orc_ht_near_mask_f32(const float* V, int m, int n, int K, int levels, int cycleSpin, const double* tau,
+                          double tol, uint8_t* vmask) {
+  if (!V || !tau || !vmask || levels < 1 || (m % (1 << levels)) || (n % (1 << levels)) || (K % (1 << levels)) ||
+      cycleSpin < 1)
+    return -1;
+  long r = -1;
+  const int st = guarded([&] {
+    r = ht_near_mask<float>(V, m, n, K, levels, cycleSpin, tau, tol, vmask);
+    return 0;
+  });
+  return st == 0 ? r : -1;
+}
+
+long orc_u1_near_mask_f32(const float* I, int m, int n, float sigma, const gbm3d_opts* resolved, double tol,
+                          uint8_t* mask) {
+  if (!I || !resolved || !mask || m < 1 || n < 1) return -1;
+  long r = -1;
+  const int st = guarded([&] {
+    r = u1_near_mask<float>(I, m, n, (double)sigma, *resolved, tol, mask);
+    return 0;
+  });
+  return st == 0 ? r : -1;
+}
+
 int orc_colored_tau(int levels, const double* psd, int d1, int d2, double* tau) {
   if (levels < 1 || !psd || !tau || d1 < 1 || d2 < 1) return GBM3D_ERR_ARG;
   colored_tau(levels, psd, d1, d2, tau);

@@ -22,5 +22,6 @@ def ctx():
 
 
 def pytest_collection_modifyitems(config, items):
-    # run the quick checks first, the long ones last
-    items.sort(key=lambda it: 1 if it.get_closest_marker("slow") else 0)
+    # the quick checks first, the long ones last, test_zz_* files at the very end
+    items.sort(key=lambda it: (os.path.basename(str(it.fspath)).startswith("test_zz"),
+                               1 if it.get_closest_marker("slow") else 0))

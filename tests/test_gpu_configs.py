@@ -1,0 +1,147 @@
+"""The benchmark workloads at their full sizes against the oracle.
+
+  * 2048 x 2048, 8 x 8 blocks in both steps, sigma 10 / 25 / 50 (the timed
+    workload): U1 against the FP32 restatement outside near-threshold pixels,
+    U against the restatement's step 2 on the GPU's own U1, PSNR against the
+    FP32 restatement and the FP64 oracle.
+  * 64 images of uneven sizes and sigmas in one graph-captured batch: equal to
+    the per-image runs bit for bit; the smallest, the largest and the most
+    elongated image against the restatement as above.
+  * 4096 x 4096, matchSize 48: the FP32 and the BF16 matcher's index lists
+    and blocks against the FP32 restatement on the noisy image (step 1) and on
+    the pilot (step 2), match spins 0 and 4; the BF16 form end to end equal to
+    the FP32 path bit for bit.
+Bars: 1e-3 max abs on the [0, 255] scale, 0.01 dB (see _chained_vs_oracle).
+"""
+import numpy as np
+import pytest
+
+import gbm3d_amd
+import oracle as O
+from gbm3d_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+P8 = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
+
+
+NEAR = 1e-3   # coefficient margin of the near-threshold mask (absolute, [0, 255] scale)
+
+
+def _chained_vs_oracle(ctx, clean, noisy, sigma, opts):
+    """Returns the GPU's U.
+
+    U1 against the FP32 restatement outside the pixels that a step-1
+    coefficient within NEAR of its threshold can move (oracle.u1_near_mask:
+    two FP32 operation orders may decide such a coefficient either way, which
+    moves whole groups by units); U against the restatement's step 2 on the
+    GPU's own U1 everywhere.  PSNR: within 0.01 dB of the FP32 restatement
+    (the reference computes in single precision), and no further from the FP64
+    oracle than the FP32 restatement is, plus 0.01 dB.  An image that is not a
+    block multiple is chained on its mirror-padded form (myMirrorPadImage, the
+    pipeline's own first step), whose U1 holds the full pilot that step 2
+    reads; its crop must be the plain call's U bit for bit."""
+    m, n = noisy.shape
+    M, N = -(-m // 8) * 8, -(-n // 8) * 8
+    U, out = ctx.denoise(noisy, sigma, opts)
+    Ip = O.mirror_pad(noisy, M, N).astype(np.float32) if (M, N) != (m, n) else noisy
+    if Ip is not noisy:
+        Up, outp = ctx.denoise(Ip, sigma, opts)
+        assert np.array_equal(Up[:m, :n], U) and np.array_equal(outp["U1"][:m, :n], out["U1"])
+    else:
+        Up, outp = U, out
+    U32, U132 = O.gbm3d(Ip, sigma, opts, precision=32)
+    mask, ncoef = O.u1_near_mask(Ip, sigma, opts, NEAR)
+    d1 = np.abs(outp["U1"] - U132)
+    e1 = float(np.max(np.where(mask, 0.0, d1)))
+    Uc = O.step2_from_pilot(outp["U1"], Ip, sigma, opts)
+    e2 = float(np.max(np.abs(Up - Uc)))
+    U64, _ = O.gbm3d(noisy, sigma, opts, precision=64)
+    p_gpu, p32, p64 = synth.psnr(clean, U), synth.psnr(clean, U32[:m, :n]), synth.psnr(clean, U64)
+    print(f"{noisy.shape} sigma {sigma}: U1 vs FP32 {e1:.2e} outside {mask.mean():.3f} masked ({ncoef} coefficients; "
+          f"{float(d1.max()):.2e} inside), U vs chained step 2 {e2:.2e}, PSNR gpu {p_gpu:.4f} fp32 {p32:.4f} "
+          f"fp64 {p64:.4f}")
+    assert e1 <= 1e-3 and e2 <= 1e-3
+    assert mask.mean() < 0.25
+    assert abs(p_gpu - p32) <= 0.01
+    assert abs(p_gpu - p64) <= abs(p32 - p64) + 0.01
+    return U
+
+
+@pytest.mark.parametrize("sigma", [10.0, 25.0, 50.0])
+def test_benchmark_workload_2048_vs_oracle(ctx, sigma):
+    clean, noisy = synth.make_pair(2048, 2048, 2048, sigma)
+    _chained_vs_oracle(ctx, clean, noisy, sigma, P8)
+
+
+@pytest.fixture(scope="module")
+def batch():
+    import bench
+
+    return bench.batch_workload()
+
+
+def test_batched_64_images_equal_per_image_runs(ctx, batch):
+    imgs, cleans, sig = batch
+    assert len(imgs) == 64
+    single = [ctx.denoise(x, s, P8)[0] for x, s in zip(imgs, sig)]
+    for graph, tune in ((True, True), (True, False), (False, False)):
+        out = ctx.denoise_batch(imgs, sig, P8, graph=graph, tune=tune)
+        for i, (a, b) in enumerate(zip(out, single)):
+            assert np.array_equal(a, b), (graph, tune, i)
+
+
+@pytest.mark.parametrize("which", ["smallest", "largest", "elongated"])
+def test_batched_images_vs_oracle(ctx, batch, which):
+    imgs, cleans, sig = batch
+    area = [x.size for x in imgs]
+    aspect = [max(x.shape) / min(x.shape) for x in imgs]
+    i = {"smallest": int(np.argmin(area)), "largest": int(np.argmax(area)),
+         "elongated": int(np.argmax(aspect))}[which]
+    out = ctx.denoise_batch([imgs[i]], [sig[i]], P8)[0]
+    U = _chained_vs_oracle(ctx, cleans[i], imgs[i], sig[i], P8)
+    assert np.array_equal(out, U)
+
+
+P48 = {"blockSize": 8, "blockSizeWie": 8, "matchSize": 48, "matchSpinsWie": [0, 4]}
+
+
+@pytest.fixture(scope="module")
+def precision_case(ctx):
+    clean, noisy = synth.make_pair(4096, 4096, 4096, 25.0)
+    _, out = ctx.denoise(noisy, 25.0, P48)
+    return noisy, out["U1"]
+
+
+@pytest.fixture(scope="module")
+def cbf():
+    c = gbm3d_amd.Context(0)
+    c.set_variant(match_bf16=True)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("step,spin", [(1, 0), (1, 4), (2, 0), (2, 4)])
+def test_precision_4096_index_lists_vs_fp32_restatement(ctx, cbf, precision_case, step, spin):
+    noisy, U1 = precision_case
+    In = np.roll(noisy, (spin, spin), (0, 1))
+    if step == 1:
+        So, Vo, _ = O.match(In, 8, 48, 16, precision=32)
+        for c in (ctx, cbf):
+            S, V = c.match(In, 8, 48, 16)
+            np.testing.assert_array_equal(S, So)
+            np.testing.assert_array_equal(V, Vo)
+    else:
+        P = np.roll(U1, (spin, spin), (0, 1)).astype(np.float32)
+        So, _, _, _ = O.match_wie(P, In, 8, 48, 16, precision=32)
+        for c in (ctx, cbf):
+            S, _, _ = c.match_wie(P, In, 8, 48, 16, want_groups=False)
+            np.testing.assert_array_equal(S, So)
+
+
+def test_precision_4096_bf16_form_equals_fp32_path(ctx, cbf, precision_case):
+    noisy, U1 = precision_case
+    Ubf, obf = cbf.denoise(noisy, 25.0, P48)
+    U32, o32 = ctx.denoise(noisy, 25.0, P48)
+    np.testing.assert_array_equal(obf["U1"], U1)
+    np.testing.assert_array_equal(Ubf, U32)

@@ -318,3 +318,58 @@ def test_library_colored_tau_matches_oracle():
     tau = np.zeros((3, 7), order="F")
     assert lib.gbm3d_get_colored_tau(3, psd.ctypes.data_as(C.c_void_p), 64, 56, tau.ctypes.data_as(C.c_void_p)) == 0
     assert np.allclose(tau, O.colored_tau(3, psd), rtol=1e-10)
+
+
+# ---- near-threshold masks: what a changed operation order may flip
+def test_ht_near_mask_covers_every_flip_of_shifted_thresholds():
+    """Moving every threshold by delta < tol flips exactly the coefficients
+    with |C| between tau and tau + delta; every voxel that moves by more than
+    round-off must lie in the mask taken with margin tol, and some must move."""
+    _, noisy = synth.make_pair(811, 128, 128, 25.0)
+    S, V, _ = O.match(noisy, 8, 32, 16, precision=32)
+    tau = O.get_tau(3, 1, 25.0)
+    tol = 0.05
+    mask, cnt = O.ht_near_mask(V, 3, 2, tau, tol)
+    assert cnt > 0 and 0 < mask.mean() < 0.6
+    moved_any = False
+    for delta in (0.04, -0.04):
+        A = O.ht_filter(V, 3, 2, tau, precision=32)
+        B = O.ht_filter(V, 3, 2, tau + delta, precision=32)
+        moved = np.abs(A - B) > 1e-4
+        moved_any |= moved.any()
+        assert not np.any(moved & ~mask)
+    assert moved_any
+
+
+def test_u1_near_mask_covers_every_flip_through_aggregation():
+    """Same through matching and aggregation for one match spin: pixels of
+    U1 = U/W that move when the thresholds move lie in u1_near_mask."""
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpins": [0], "matchSpinsWie": [0]}
+    _, noisy = synth.make_pair(811, 128, 128, 25.0)
+    mask, cnt = O.u1_near_mask(noisy, 25.0, opts, 0.02)
+    assert cnt > 0 and 0 < mask.mean() < 1.0
+    S, V, _ = O.match(noisy, 8, 32, 16, precision=32)
+    tau = O.get_tau(3, 1, 25.0)
+    U0, W0 = O.aggregate_ht(O.ht_filter(V, 3, 2, tau, precision=32), S, 8, 32, precision=32)
+    moved_any = False
+    for delta in (0.015, -0.015):
+        U1, W1 = O.aggregate_ht(O.ht_filter(V, 3, 2, tau + delta, precision=32), S, 8, 32, precision=32)
+        moved = np.abs(U1 / W1 - U0 / W0) > 1e-4
+        moved_any |= moved.any()
+        assert not np.any(moved & ~mask)
+    assert moved_any
+    # and the single-spin U1 of the restatement is that U/W
+    _, U1r = O.gbm3d(noisy, 25.0, opts, precision=32)
+    assert np.max(np.abs(U1r - U0 / W0)) <= 1e-4
+
+
+def test_u1_near_mask_follows_the_match_spins():
+    """A spin's marks are placed in the unshifted frame: the mask of spins
+    [0 4] is the union of the masks of [0] and of [4] on the same image."""
+    _, noisy = synth.make_pair(813, 128, 128, 25.0)
+    base = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0]}
+    m0, _ = O.u1_near_mask(noisy, 25.0, dict(base, matchSpins=[0]), 0.01)
+    m4, _ = O.u1_near_mask(noisy, 25.0, dict(base, matchSpins=[4]), 0.01)
+    m04, _ = O.u1_near_mask(noisy, 25.0, dict(base, matchSpins=[0, 4]), 0.01)
+    assert np.array_equal(m04, m0 | m4)
+    assert not np.array_equal(m0, m4)
