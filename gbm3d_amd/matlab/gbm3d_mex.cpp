@@ -159,7 +159,9 @@ void distributed(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   int m, n;
   std::vector<float> in = to_image(prhs[1], &m, &n);
   if (mxGetNumberOfElements(prhs[2]) != 1)
-    mexErrMsgIdAndTxt("gbm3d:unsupported", "sigma must be a scalar (coloured-noise PSD is not built)");
+    mexErrMsgIdAndTxt("gbm3d:unsupported",
+                      "sigma must be a scalar here; a matrix sigma (noise PSD) goes through "
+                      "gbm3d_mex('GBM3D_colored', I, sigmaPSD, sigmaWie, opts)");
   const float sigma = (float)mxGetScalar(prhs[2]);
   gbm3d_opts o;
   const gbm3d_opts* op = nullptr;
@@ -240,7 +242,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   if (mxIsComplex(I) || mxGetNumberOfDimensions(I) != 2 || !(mxIsDouble(I) || mxIsSingle(I)))
     mexErrMsgIdAndTxt("gbm3d:arg", "I must be a real 2-D double or single image");
   if (mxGetNumberOfElements(prhs[1]) != 1)
-    mexErrMsgIdAndTxt("gbm3d:unsupported", "sigma must be a scalar (coloured-noise PSD is not built)");
+    mexErrMsgIdAndTxt("gbm3d:unsupported",
+                      "sigma must be a scalar here; a matrix sigma (noise PSD) goes through "
+                      "gbm3d_mex('GBM3D_colored', I, sigmaPSD, sigmaWie, opts)");
   const int m = (int)mxGetM(I), n = (int)mxGetN(I);
   const float sigma = (float)mxGetScalar(prhs[1]);
   std::vector<float> in((size_t)m * n), U((size_t)m * n), U1((size_t)m * n);

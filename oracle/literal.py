@@ -392,3 +392,100 @@ def S_to_array(S, K):
 def S_from_array(Sa):
     K, M, N = Sa.shape
     return [[Sa[:, i, j] for j in range(N)] for i in range(M)]
+
+
+def getBM3D_distribution(m, n, matchSpins, k):
+    """utilities/getBM3D_distribution.m:10-48 -> R (cases x 5, 1-based), d0, k."""
+    d0 = 256 + k
+    Xcnt = int(np.ceil((n - k) / (d0 - k)))
+    Ycnt = int(np.ceil((m - k) / (d0 - k)))
+    R = np.zeros((Xcnt * Ycnt * len(matchSpins), 5), dtype=np.int64)
+    cnt = 0
+    for i in range(1, Ycnt + 1):
+        indY = (i - 1) * (d0 - k)
+        for j in range(1, Xcnt + 1):
+            indX = (j - 1) * (d0 - k)
+            for mS in range(len(matchSpins)):
+                R[cnt, 0] = indY + 1
+                R[cnt, 1] = indY + d0
+                R[cnt, 2] = indX + 1
+                R[cnt, 3] = indX + d0
+                R[cnt, 4] = matchSpins[mS]
+                if R[cnt, 3] >= n:
+                    R[cnt, 2] = min(n - int(np.ceil((n - R[cnt, 2] + 1) / k)) * k + 1, n - 2 * k + 1)
+                if R[cnt, 1] >= m:
+                    R[cnt, 0] = min(m - int(np.ceil((m - R[cnt, 0] + 1) / k)) * k + 1, m - 2 * k + 1)
+                cnt += 1
+    R[:, 1] = np.minimum(R[:, 1], m)
+    R[:, 3] = np.minimum(R[:, 3], n)
+    return R, d0, k
+
+
+def makeInterpWindow(a, b, k):
+    """utilities/makeInterpWindow.m:8-13."""
+    w = np.ones((a, b))
+    ramp = _linspace(0.01, 1, k)
+    w[:k, :] = w[:k, :] * np.tile(ramp[:, None], (1, b))
+    w[:, :k] = w[:, :k] * np.tile(ramp[None, :], (a, 1))
+    w[a - k:, :] = w[a - k:, :] * np.flipud(np.tile(ramp[:, None], (1, b)))
+    w[:, b - k:] = w[:, b - k:] * np.fliplr(np.tile(ramp[None, :], (a, 1)))
+    return w
+
+
+def GBM3D_distributed(I, sigma, opts):
+    """source/GBM3D_distributed.m:21-120 on a resolved opts dict (white noise):
+    per case (tile, match spin) the stages on circshift(I(tile), [s, s]), then
+    ramp-window stitching U2 = sum(U1 .* w) ./ sum(w .* W).  Returns (U, U2)."""
+    I = np.asarray(I, dtype=np.float64)
+    m, n = I.shape
+    k = opts["blockSize"]
+    if m < 256 + k or n < 256 + k:  # :24-27
+        return GBM3D(I, sigma, opts)
+    tau = sigma * getTau(opts["levels"], opts["tauMode"])
+    R, d0, K = getBM3D_distribution(m, n, opts["matchSpins"], k)
+    RW, d0, K = getBM3D_distribution(m, n, opts["matchSpinsWie"], k)
+    K_ = opts["numMax"]
+    d2 = opts["matchSize"]
+
+    def tile(A, r):
+        return A[r[0] - 1:r[1], r[2] - 1:r[3]]
+
+    U1, W = [None] * len(R), [None] * len(R)
+    for ii in range(len(R)):  # :54-62
+        sp = int(R[ii, 4])
+        Ilocal = np.roll(tile(I, R[ii]), (sp, sp), (0, 1))
+        S, Volume, _ = matchBlocksCPU(Ilocal, k, d2, K_)
+        Vavg = ht_filter(Volume, tau, opts["levels"], opts["cycleSpin"])
+        u, w = aggregate_ht(Vavg, S, k, d2)
+        U1[ii] = np.roll(u, (-sp, -sp), (0, 1))
+        W[ii] = np.roll(w, (-sp, -sp), (0, 1))
+    U2 = np.zeros((m, n))
+    norm = np.zeros((m, n))
+    mWindow = makeInterpWindow(d0, d0, K)
+    for ii in range(len(R)):  # :69-80
+        a, b = R[ii, 1] - R[ii, 0] + 1, R[ii, 3] - R[ii, 2] + 1
+        tw = makeInterpWindow(a, b, K) if (a != d0 or b != d0) else mWindow
+        U2[R[ii, 0] - 1:R[ii, 1], R[ii, 2] - 1:R[ii, 3]] += U1[ii] * tw
+        norm[R[ii, 0] - 1:R[ii, 1], R[ii, 2] - 1:R[ii, 3]] += tw * W[ii]
+    U2 = U2 / norm
+    if not opts["Wiener"]:
+        return U2, U2
+    d = opts["blockSizeWie"]
+    U1, W = [None] * len(RW), [None] * len(RW)
+    for ii in range(len(RW)):  # :94-104
+        sp = int(RW[ii, 4])
+        Ilocal1 = np.roll(tile(I, RW[ii]), (sp, sp), (0, 1))
+        Ilocal2 = np.roll(tile(U2, RW[ii]), (sp, sp), (0, 1))
+        S, V1, V2 = matchBlocksWie(Ilocal2, Ilocal1, d, d2, K_)
+        V2f, filt = wiener_filter(V1, V2, sigma)
+        u, w = aggregate_wie(V2f, filt, S, d, d2)
+        U1[ii] = np.roll(u, (-sp, -sp), (0, 1))
+        W[ii] = np.roll(w, (-sp, -sp), (0, 1))
+    Uf = np.zeros((m, n))
+    norm = np.zeros((m, n))
+    for ii in range(len(RW)):  # :107-120
+        a, b = RW[ii, 1] - RW[ii, 0] + 1, RW[ii, 3] - RW[ii, 2] + 1
+        tw = makeInterpWindow(a, b, K) if (a != d0 or b != d0) else mWindow
+        Uf[RW[ii, 0] - 1:RW[ii, 1], RW[ii, 2] - 1:RW[ii, 3]] += U1[ii] * tw
+        norm[RW[ii, 0] - 1:RW[ii, 1], RW[ii, 2] - 1:RW[ii, 3]] += tw * W[ii]
+    return Uf / norm, U2

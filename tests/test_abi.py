@@ -143,3 +143,43 @@ def test_colored_abi_carries_sigma_wie_size():
     txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     decl = re.search(r"int gbm3d_denoise_colored\((.*?)\);", txt, flags=re.S).group(1)
     assert re.search(r"const double\* sigmaWie,\s*int wm,\s*int wn", decl)
+
+
+# ---- the MATLAB gateway, compiled against a declarations-only mex.h and run
+# on a minimal fake mxArray runtime (tests/mex_stub/): its option commands and
+# argument checks, none of which needs a GPU.  Not a binding test against
+# MATLAB itself (none is installed here).
+STUB = os.path.join(ROOT, "tests", "mex_stub")
+MEX_SRC = os.path.join(ROOT, "gbm3d_amd", "matlab", "gbm3d_mex.cpp")
+
+
+def test_mex_gateway_compiles_against_the_mex_api():
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), "-I", STUB, MEX_SRC], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_mex_gateway_option_commands_and_errors(tmp_path):
+    import json
+
+    exe = str(tmp_path / "run_gateway")
+    libdir = os.path.dirname(_abi.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", STUB, MEX_SRC,
+                        os.path.join(STUB, "fake_mx.cpp"), os.path.join(STUB, "run_gateway.cpp"), "-L", libdir,
+                        "-lgbm3d", f"-Wl,-rpath,{libdir}", "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    res = {d["case"]: d for d in map(json.loads, out.stdout.splitlines())}
+    for p in ("default", "fast", "superFast", "accuracy", "unknown"):
+        assert res["set:" + p]["opts"] == O.set_opts(p), p
+    assert res["set:none"]["opts"] == O.set_opts("default")
+    assert res["check:blockSize8"]["opts"] == O.check_opts({"blockSize": 8})
+    assert res["check:spins_median"]["opts"] == O.check_opts(
+        {"matchSpins": [0, 2, 4], "Wiener": False, "filtType": "median", "numMax": 8, "numMin": 8})
+    assert res["check:missing"]["error_id"] == "gbm3d:arg"
+    assert res["check:badfilt"]["error_id"] == "gbm3d:filt"        # denoiseAndAggregateGPU.m:54
+    assert res["unknown_command"]["error_id"] == "gbm3d:arg"
+    assert res["denoise:matrix_sigma"]["error_id"] == "gbm3d:unsupported"
+    assert res["denoise:complex"]["error_id"] == "gbm3d:arg"
+    assert res["denoise:too_few"]["error_id"] == "gbm3d:arg"

@@ -373,3 +373,31 @@ def test_u1_near_mask_follows_the_match_spins():
     m04, _ = O.u1_near_mask(noisy, 25.0, dict(base, matchSpins=[0, 4]), 0.01)
     assert np.array_equal(m04, m0 | m4)
     assert not np.array_equal(m0, m4)
+
+
+def test_distribution_and_window_match_literal():
+    for m, n, spins, k in ((600, 520, [0], 8), (300, 280, [0, 4], 8), (1000, 1000, [0, 4, 6], 16)):
+        R, d0, _ = L.getBM3D_distribution(m, n, spins, k)
+        np.testing.assert_array_equal(O.distribution(m, n, spins, k), R)
+    for a, b, k in ((264, 264, 8), (48, 264, 8), (272, 96, 16)):
+        np.testing.assert_allclose(O.interp_window(a, b, k), L.makeInterpWindow(a, b, k), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("m,n,spins", [(300, 300, [0]), (300, 296, [0, 4])])
+def test_distributed_multi_tile_matches_literal(m, n, spins):
+    """Four tiles (two of them cut back to a block multiple at the image
+    edge): the oracle's stitch against the line-by-line transcription of
+    source/GBM3D_distributed.m:54-120, first and final estimates."""
+    opts = {"blockSize": 8, "blockSizeWie": 8, "matchSpins": spins, "matchSpinsWie": spins}
+    clean, noisy = synth.make_pair(821, m, n, 25.0)
+    # lifted off 0: where the first estimate is clipped to exact zeros, step-2
+    # reference blocks of norm 0 rank their candidates by the FFT round-off of
+    # the literal form's block norms (0/0 in the direct form), which no
+    # reference data pins
+    noisy = noisy + 80.0
+    R = O.distribution(m, n, spins, 8)
+    assert len(R) == 4 * len(spins)
+    U, U2 = O.gbm3d_distributed(noisy, 25.0, opts, precision=64)
+    Ul, U2l = L.GBM3D_distributed(noisy, 25.0, O.resolve_opts(opts))
+    np.testing.assert_allclose(U2, U2l, atol=1e-9)
+    np.testing.assert_allclose(U, Ul, atol=1e-9)
