@@ -140,6 +140,7 @@ SIGNATURES = {
     "gbm3d_get_colored_tau": (_I, [_I, _P, _I, _I, _P]),
     "gbm3d_denoise_device_noalloc": (_I, [_P, _P, _I, _I, C.c_float, C.POINTER(Opts), _P, _P]),
     "gbm3d_bf16_match_stats": (_I, [_P, C.POINTER(C.c_ulonglong), _I]),
+    "gbm3d_denoise_aggregate_wie": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _I, C.c_float, _I, _P, _P]),
     "gbm3d_validate": (_I, [C.POINTER(Opts), _I, _I, C.POINTER(C.c_size_t)]),
     "gbm3d_denoise_colored": (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _I, _I, C.POINTER(Opts), _P, _P]),
     "gbm3d_last_error": (C.c_char_p, []),

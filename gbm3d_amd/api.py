@@ -371,6 +371,20 @@ class Context:
         check(self.lib.gbm3d_wiener(self.handle, ptr(V1), ptr(V2), d, K, G, C.c_float(sigma), ptr(V2f), ptr(g2)))
         return V2f, g2
 
+    def denoise_aggregate_wie(self, V1, V2, S, sigma: float, d2: int, clip01: bool = True):
+        """[U, W] = denoiseAndAggregateWie(V1, V2, S, sigma, opts) (clip01, the
+        dct() twin that clips to [0, 1]) or denoiseAndAggregateWieFast."""
+        V1 = np.asfortranarray(V1, dtype=np.float32)
+        V2 = np.asfortranarray(V2, dtype=np.float32)
+        S = np.asfortranarray(S, dtype=np.uint32)
+        d, K, Mt, Nt = V1.shape[0], V1.shape[2], V1.shape[3], V1.shape[4]
+        m, n = Mt * d, Nt * d
+        U = np.zeros((m, n), dtype=np.float32, order="F")
+        W = np.zeros_like(U)
+        check(self.lib.gbm3d_denoise_aggregate_wie(self.handle, ptr(V1), ptr(V2), ptr(S), m, n, d, d2, K,
+                                                   C.c_float(sigma), int(bool(clip01)), ptr(U), ptr(W)))
+        return U, W
+
     def aggregate_wie(self, V2f, gain2, S, m: int, n: int, d2: int):
         V2f = np.asfortranarray(V2f, dtype=np.float32)
         gain2 = np.asfortranarray(gain2, dtype=np.float32)
@@ -443,3 +457,11 @@ def denoiseAndAggregateWieFast(V1, V2, S, sigma: float, opts: dict):
     V2f, g2 = ctx.wiener(V1, V2, sigma)
     d = o["blockSizeWie"]
     return ctx.aggregate_wie(V2f, g2, S, S.shape[1] * d, S.shape[2] * d, o["matchSize"])
+
+
+def denoiseAndAggregateWie(V1, V2, S, sigma: float, opts: dict):
+    """[U, W] = denoiseAndAggregateWie(V1, V2, S, sigma, opts): the dct() twin
+    of denoiseAndAggregateWieFast whose filtered groups are clipped to [0, 1]
+    (source/denoiseAndAggregateWie.m:34-55), so it expects images on [0, 1]."""
+    o = checkBM3Dopts(opts)
+    return default_context().denoise_aggregate_wie(V1, V2, S, sigma, o["matchSize"], clip01=True)

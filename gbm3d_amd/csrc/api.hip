@@ -408,4 +408,43 @@ int gbm3d_aggregate_wie(gbm3d_ctx* c, const float* V2f, const float* gain2, cons
   return GBM3D_OK;
 }
 
+// [U, W] = denoiseAndAggregateWie(V1, V2, S, sigma, opts) (clip01 = 1,
+// source/denoiseAndAggregateWie.m) or denoiseAndAggregateWieFast (clip01 = 0,
+// source/denoiseAndAggregateWieFast.m): Wiener filter, then aggregation.
+int gbm3d_denoise_aggregate_wie(gbm3d_ctx* c, const float* V1, const float* V2, const uint32_t* S, int m, int n,
+                                int d, int d2, int K, float sigma, int clip01, float* U, float* W) {
+  GBM3D_TRY(set_dev(c));
+  if (!V1 || !V2 || !S || !U || !W) return fail(GBM3D_ERR_ARG, "NULL argument");
+  GBM3D_TRY(stage_sizes(m, n, d, d2, K));
+  if (!wiener_supported(d, K)) return fail(GBM3D_ERR_UNSUPPORTED, "Wiener step: (blockSizeWie, numMax) not built");
+  const size_t mn = (size_t)m * n, G = mn / ((size_t)d * d);
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** a, float** b, float** v, float** g, uint32_t** dS, float** dU, float** dW) {
+    *a = k.take<float>(mn * K);
+    *b = k.take<float>(mn * K);
+    *v = k.take<float>(mn * K);
+    *g = k.take<float>(G);
+    *dS = k.take<uint32_t>(G * K);
+    *dU = k.take<float>(mn);
+    *dW = k.take<float>(mn);
+  };
+  float *a, *b, *v, *g, *dU, *dW;
+  uint32_t* dS;
+  lay(cv, &a, &b, &v, &g, &dS, &dU, &dW);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &a, &b, &v, &g, &dS, &dU, &dW);
+  GBM3D_TRY(upload(c, a, V1, mn * K));
+  GBM3D_TRY(upload(c, b, V2, mn * K));
+  GBM3D_TRY(upload(c, dS, S, G * K));
+  GBM3D_TRY(launch_wiener(nullptr, nullptr, nullptr, a, b, 0, 0, d, 0, K, (long)G, sigma, v, g, c->stream, nullptr,
+                          clip01 != 0));
+  GBM3D_TRY(launch_aggregate_m(2, v, g, dS, c->masks + kMaskStride * mask_slot(d), m, n, d, d2, K, 0, dU, dW,
+                               c->stream));
+  GBM3D_TRY(download(c, U, dU, mn));
+  GBM3D_TRY(download(c, W, dW, mn));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
 }  // extern "C"

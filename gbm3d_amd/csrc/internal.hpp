@@ -133,9 +133,11 @@ int launch_aggregate_m(int step, const float* blocks, const float* wgt, const ui
 // Step-2 Wiener filter on groups gathered from the padded pilot / noisy images
 // (pads != nullptr) or read from V1/V2 arrays.
 // sig2tab (host, d x d, nullable): per-coefficient noise variance (coloured noise).
+// clip01: clip the filtered groups to [0, 1] (denoiseAndAggregateWie.m:55)
+// instead of at 0 only (denoiseAndAggregateWieFast.m:64).
 int launch_wiener(const float* pad_pilot, const float* pad_noisy, const uint32_t* S, const float* V1,
                   const float* V2, int M, int N, int d, int d2, int K, long G, float sigma, float* V2f,
-                  float* gain2, hipStream_t st, const float* sig2tab = nullptr);
+                  float* gain2, hipStream_t st, const float* sig2tab = nullptr, bool clip01 = false);
 // out = sum_s U_s / (sum_s W_s + sum_t Wextra_t), cropped to m x n, double math;
 // out2 (nullable) receives the same values cropped to m2 x n2 (m2 <= m, n2 <= n).
 int launch_normalize(const float* U, int nU, const float* W, int nW, const float* Wextra, int nWextra,

@@ -30,8 +30,8 @@ template <int D, int K, bool GATHER>
 __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ padP, const float* __restrict__ padN,
                                                      const uint32_t* __restrict__ S, const float* __restrict__ V1,
                                                      const float* __restrict__ V2, int M, int N, int d2, long G,
-                                                     Sig2Tab<D> sig2, DctTable<D> tab, float* __restrict__ V2f,
-                                                     float* __restrict__ gain2) {
+                                                     Sig2Tab<D> sig2, DctTable<D> tab, float hi,
+                                                     float* __restrict__ V2f, float* __restrict__ gain2) {
   constexpr int TPG = D * D, GPB = 256 / TPG;
   constexpr int L = K < 16 ? 2 : 3;
   __shared__ float smem[GPB][D == 8 ? K * kDct8Slice : K * TPG];
@@ -116,7 +116,8 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const float y = c2[k];
-      dst[t + TPG * k] = y > 0.f ? y : 0.f;  // max(V2, 0)
+      // max(V2, 0) (WieFast :64), or min(max(V2, 0), 1) (the dct() twin :55; hi = 1)
+      dst[t + TPG * k] = fminf(y > 0.f ? y : 0.f, hi);
     }
   }
   // |filt|_F^2 of the group
@@ -141,8 +142,8 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
 
 template <int D, int K>
 static int launch_wiener_t(const float* padP, const float* padN, const uint32_t* S, const float* V1, const float* V2,
-                           int M, int N, int d2, long G, float sigma, const float* sig2tab, float* V2f, float* gain2,
-                           hipStream_t st) {
+                           int M, int N, int d2, long G, float sigma, const float* sig2tab, float hi, float* V2f,
+                           float* gain2, hipStream_t st) {
   const DctTable<D> tab = make_dct_table<D>();
   constexpr int GPB = 256 / (D * D);
   dim3 grid((unsigned)((G + GPB - 1) / GPB));
@@ -150,20 +151,21 @@ static int launch_wiener_t(const float* padP, const float* padN, const uint32_t*
   for (int e = 0; e < D * D; ++e) s2.s[e] = sig2tab ? sig2tab[e] : sigma * sigma;
   if (padP)
     hipLaunchKernelGGL((wiener_kernel<D, K, true>), grid, dim3(256), 0, st, padP, padN, S, V1, V2, M, N, d2, G, s2,
-                       tab, V2f, gain2);
+                       tab, hi, V2f, gain2);
   else
     hipLaunchKernelGGL((wiener_kernel<D, K, false>), grid, dim3(256), 0, st, padP, padN, S, V1, V2, M, N, d2, G, s2,
-                       tab, V2f, gain2);
+                       tab, hi, V2f, gain2);
   return check_launch("wiener");
 }
 
 int launch_wiener(const float* pad_pilot, const float* pad_noisy, const uint32_t* S, const float* V1, const float* V2,
                   int M, int N, int d, int d2, int K, long G, float sigma, float* V2f, float* gain2, hipStream_t st,
-                  const float* sig2tab) {
+                  const float* sig2tab, bool clip01) {
+  const float hi = clip01 ? 1.f : __builtin_huge_valf();
   if (G <= 0) return GBM3D_OK;
 #define GBM3D_WIE_CASE(DD, KK)                                                                                  \
   if (d == DD && K == KK)                                                                                       \
-    return launch_wiener_t<DD, KK>(pad_pilot, pad_noisy, S, V1, V2, M, N, d2, G, sigma, sig2tab, V2f, gain2, st);
+    return launch_wiener_t<DD, KK>(pad_pilot, pad_noisy, S, V1, V2, M, N, d2, G, sigma, sig2tab, hi, V2f, gain2, st);
   GBM3D_WIE_CASE(8, 16)
   GBM3D_WIE_CASE(8, 8)
   GBM3D_WIE_CASE(8, 32)

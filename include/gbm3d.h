@@ -279,6 +279,17 @@ int gbm3d_aggregate_wie(gbm3d_ctx* ctx, const float* V2f, const float* gain2,
                         const uint32_t* S, int m, int n, int d, int d2, int K,
                         float* U, float* W);
 
+/* [U, W] = denoiseAndAggregateWie(V1, V2, S, sigma, opts) -- clip01 = 1:
+ * source/denoiseAndAggregateWie.m (dct()/idct() pages, filtered groups
+ * clipped to [0, 1], so meant for images on [0, 1]); clip01 = 0: the same
+ * call as denoiseAndAggregateWieFast (source/denoiseAndAggregateWieFast.m,
+ * clip at 0 only).  Filter and aggregation in one call; layouts as
+ * gbm3d_wiener / gbm3d_aggregate_wie, U and W m x n (unnormalised). */
+int gbm3d_denoise_aggregate_wie(gbm3d_ctx* ctx, const float* V1, const float* V2,
+                                const uint32_t* S, int m, int n, int d, int d2,
+                                int K, float sigma, int clip01, float* U,
+                                float* W);
+
 /* ------------------------------------------------------- instrumentation
  * Per-stage device time of the last gbm3d_denoise* call, measured with HIP
  * events on the stream the kernels ran on (enabled by gbm3d_set_timing). */

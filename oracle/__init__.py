@@ -205,8 +205,9 @@ def aggregate_ht(Vavg, S, d, d2, precision=64, gather=True):
     return U, W
 
 
-def wiener(V1, V2, sigma, precision=64, sigma_wie=None):
-    """sigma_wie (d x d): per-coefficient sigma (coloured noise) instead of sigma."""
+def wiener(V1, V2, sigma, precision=64, sigma_wie=None, clip01=False):
+    """sigma_wie (d x d): per-coefficient sigma (coloured noise) instead of sigma.
+    clip01: denoiseAndAggregateWie.m's clip to [0, 1] (source/denoiseAndAggregateWie.m:55)."""
     dt, suf = _dt(precision)
     V1 = _f(V1, dt)
     V2 = _f(V2, dt)
@@ -214,7 +215,10 @@ def wiener(V1, V2, sigma, precision=64, sigma_wie=None):
     G = V1.size // (d * d * K)
     V2f = np.zeros_like(V2)
     g2 = np.zeros(V1.shape[3:], dtype=dt, order="F")
-    if sigma_wie is None:
+    if clip01:
+        _check(getattr(lib(), "orc_wiener01_" + suf)(_p(V1), _p(V2), d, K, C.c_long(G), C.c_double(sigma), _p(V2f),
+                                                     _p(g2)))
+    elif sigma_wie is None:
         _check(getattr(lib(), "orc_wiener_" + suf)(_p(V1), _p(V2), d, K, C.c_long(G), C.c_double(sigma), _p(V2f),
                                                    _p(g2)))
     else:

@@ -825,8 +825,11 @@ void haar_rec(T* x, int R, int levels) {
 // sigWie (d x d, nullable): per-coefficient sigma (coloured noise; the d x d
 // sigma of :49 expands over the slices and groups).
 template <class T>
+// clip01: denoiseAndAggregateWie.m (the dct() twin): V2 = min(max(V2,0),1)
+// (:55) instead of max(V2,0); its dct/idct along dims 1 and 2 are the same
+// orthonormal DCT-II pages (:34-35,44-45,53-54).
 void wiener(const T* V1, const T* V2, int d, int K, long G, double sigma, T* V2f, T* gain2,
-            const double* sigWie = nullptr) {
+            const double* sigWie = nullptr, bool clip01 = false) {
   const int levels = K < 16 ? 2 : 3;  // :23-24
   std::vector<T> Tm = dct_matrix<T>(d);
   std::vector<T> s2tab((size_t)d * d);
@@ -883,7 +886,11 @@ void wiener(const T* V1, const T* V2, int d, int K, long G, double sigma, T* V2f
       }
       along_k(B.data(), true);
       page(B.data(), true);
-      for (size_t e = 0; e < gs; ++e) V2f[gs * g + e] = B[e] > 0 ? B[e] : T(0);  // :64
+      for (size_t e = 0; e < gs; ++e) {
+        T y = B[e] > 0 ? B[e] : T(0);  // :64
+        if (clip01 && y > T(1)) y = T(1);
+        V2f[gs * g + e] = y;
+      }
       gain2[g] = n2;
     }
   }

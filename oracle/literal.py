@@ -305,6 +305,29 @@ def wiener_filter(V1, V2, sigma):
     return np.maximum(V2, 0), filt
 
 
+def wiener_filter_dct(V1, V2, sigma):
+    """source/denoiseAndAggregateWie.m:19-55 (the dct() twin of WieFast):
+    dct along dims 1 and 2 (orthonormal type II), Haar along dim 3, the same
+    gains, idct along dims 2 and 1, clip to [0, 1] -> (V2 filtered, filt)."""
+    d, K = V1.shape[0], V1.shape[2]
+    levels = 3 if K >= 16 else 2
+    Psi = np.array([-CC, CC])
+    Phi = np.abs(Psi)
+    V1 = scipy.fft.dct(V1, type=2, axis=0, norm="ortho")
+    V1 = scipy.fft.dct(V1, type=2, axis=1, norm="ortho")
+    V1 = myWavDec1D(Psi, Phi, levels, V1)
+    filt = np.abs(V1) ** 2 / (np.abs(V1) ** 2 + sigma ** 2)
+    filt[0, 0, 14:] = 1
+    V2 = scipy.fft.dct(V2, type=2, axis=0, norm="ortho")
+    V2 = scipy.fft.dct(V2, type=2, axis=1, norm="ortho")
+    V2 = myWavDec1D(Psi, Phi, levels, V2)
+    V2 = V2 * filt
+    V2 = myWavRec1D(Psi, Phi, levels, V2)
+    V2 = scipy.fft.idct(V2, type=2, axis=1, norm="ortho")
+    V2 = scipy.fft.idct(V2, type=2, axis=0, norm="ortho")
+    return np.minimum(np.maximum(V2, 0), 1), filt
+
+
 def aggregate_wie(V2, filt, S, d, d2):
     """source/denoiseAndAggregateWieFast.m:68-100."""
     M, N = len(S), len(S[0])

@@ -146,6 +146,14 @@ MED(f32, float)
       return 0;                                                                                      \
     });                                                                                              \
   }                                                                                                  \
+  int orc_wiener01_##SUF(const T* V1, const T* V2, int d, int K, long G, double sigma, T* V2f,      \
+                         T* gain2) {                                                                 \
+    if (d < 1 || K < 4 || (K % 4)) return GBM3D_ERR_ARG;                                             \
+    return guarded([&] {                                                                             \
+      wiener<T>(V1, V2, d, K, G, sigma, V2f, gain2, nullptr, true);                                  \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }                                                                                                  \
   int orc_wiener_tab_##SUF(const T* V1, const T* V2, int d, int K, long G, const double* sigWie,     \
                            T* V2f, T* gain2) {                                                       \
     if (d < 1 || K < 4 || (K % 4) || !sigWie) return GBM3D_ERR_ARG;                                  \

@@ -120,3 +120,23 @@ def test_colored_sigma_wie_size_checked_by_the_c_abi(ctx):
                                    C.byref(o), _abi.ptr(U), None)
     assert st == _abi.GBM3D_ERR_ARG
     assert b"blockSizeWie x blockSizeWie" in lib.gbm3d_last_error()
+
+
+@pytest.mark.parametrize("d,K", [(8, 16), (4, 8), (16, 32)])
+def test_denoise_and_aggregate_wie_dct_twin_vs_oracle(ctx, d, K):
+    """denoiseAndAggregateWie (source/denoiseAndAggregateWie.m): filter with
+    the [0, 1] clip, then aggregation, against the FP64 oracle on a
+    [0, 1]-scale image; clip01 = 0 is WieFast."""
+    m, n = 96, 128
+    clean, noisy = synth.make_pair(660 + d, m, n, 25.0)
+    clean, noisy = clean / 255.0, (noisy / 255.0).astype(np.float32)
+    pilot = np.maximum(clean + 0.02 * synth.noise(661, m, n), 0)
+    S, V1, V2, _ = O.match_wie(pilot, noisy, d, 32, K, precision=32)
+    s = 25.0 / 255.0
+    for clip in (True, False):
+        U, W = ctx.denoise_aggregate_wie(V1, V2, S, s, 32, clip01=clip)
+        V2o, g2o = O.wiener(V1.astype(np.float64), V2.astype(np.float64), s, precision=64, clip01=clip)
+        Uo, Wo = O.aggregate_wie(V2o, g2o, S, m, n, 32, precision=64)
+        assert np.abs(U / W - Uo / Wo).max() <= 1e-3 / 255.0
+    V2c, _ = O.wiener(V1.astype(np.float64), V2.astype(np.float64), s, precision=64, clip01=True)
+    assert np.mean(V2c == 1.0) > 1e-4        # the clip is active on these inputs

@@ -401,3 +401,22 @@ def test_distributed_multi_tile_matches_literal(m, n, spins):
     Ul, U2l = L.GBM3D_distributed(noisy, 25.0, O.resolve_opts(opts))
     np.testing.assert_allclose(U2, U2l, atol=1e-9)
     np.testing.assert_allclose(U, Ul, atol=1e-9)
+
+
+@pytest.mark.parametrize("d,K", [(8, 16), (4, 8), (16, 32)])
+def test_wiener_dct_twin_matches_literal(d, K):
+    """denoiseAndAggregateWie.m (dct()/idct() along dims 1, 2 and a clip to
+    [0, 1]) on a [0, 1]-scale image, where the upper clip is active."""
+    clean, noisy = synth.make_pair(22 + d, 64, 64, 25.0)
+    clean, noisy = clean / 255.0, noisy / 255.0
+    pilot = clean + 0.02 * synth.noise(98, 64, 64)
+    S, V1, V2, _ = O.match_wie(pilot, noisy, d, 32, K)
+    Sl, V1l, V2l = L.matchBlocksWie(pilot, noisy.astype(np.float64), d, 32, K)
+    V2f, g2 = O.wiener(V1, V2, 25.0 / 255.0, clip01=True)
+    V2fl, filt = L.wiener_filter_dct(V1l, V2l, 25.0 / 255.0)
+    assert np.mean(V2fl == 1.0) > 1e-4                        # the upper clip does work here
+    np.testing.assert_allclose(V2f, V2fl, atol=1e-12)
+    np.testing.assert_allclose(g2, np.sum(filt ** 2, axis=(0, 1, 2)), rtol=1e-12)
+    # without the clip it is WieFast
+    V2w, _ = O.wiener(V1, V2, 25.0 / 255.0)
+    np.testing.assert_allclose(np.minimum(V2w, 1.0), V2f, atol=1e-12)
