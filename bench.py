@@ -30,33 +30,37 @@ SEED = 2048
 # cycleSpin 2, 3 levels, tau = sigma*[3.3 3.0 2.7]
 OPTS = {"blockSize": 8, "blockSizeWie": 8, "matchSpinsWie": [0, 4]}
 PEAK_HBM_GBS = 8000.0
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 with packed (dual) FP32 instructions
+PEAK_FP32_VALU_TFLOPS = 78.6      # FP32 FMA without packed instructions (the build does not emit them)
 PEAK_BF16_TFLOPS = 2500.0
 # learned shrinkage: 4 conv layers x 2 FLOP x 16 positions x 64 out x 192 in, per group
 SHRINK_FLOP_PER_GROUP = 4 * 2 * 16 * 64 * 192
 DPSNR_BOUND = -0.05   # a form is eligible as the chosen form when PSNR >= plain - 0.05 dB
 
 
-def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2):
-    """Algorithmic work per launch of each single-kernel stage."""
+def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2, bw=8):
+    """Algorithmic work per launch of each single-kernel stage (d1: step-1
+    block size, bw: step-2 block size, d2: match size)."""
     MN = M * N
     vol_bytes = MN * K * 4
-    G2 = MN // (d1 * d1)
+    pad_bytes = (M + d2) * (N + d2) * 4
+    G2 = MN // (bw * bw)
     out = {}
     flops_match = 2.0 * MN * d2 * d2  # |x|^2+|y|^2-2<x,y>: d^2 FMAs per candidate, d2^2 candidates per d^2 pixels
+    # the FP32 matcher runs plain FP32 FMAs on the VALU: bound by their issue rate
     if stage_n["match1"]:
-        out["match1"] = dict(bound="mfma", unit="TFLOP/s", work=flops_match, launches=stage_n["match1"],
-                             ms=stage_ms["match1"] / stage_n["match1"], peak=PEAK_FP32_TFLOPS)
+        out["match1"] = dict(bound="valu", unit="TFLOP/s", work=flops_match, launches=stage_n["match1"],
+                             ms=stage_ms["match1"] / stage_n["match1"], peak=PEAK_FP32_VALU_TFLOPS)
     if stage_n["match2"]:
-        out["match2"] = dict(bound="mfma", unit="TFLOP/s", work=flops_match, launches=stage_n["match2"],
-                             ms=stage_ms["match2"] / stage_n["match2"], peak=PEAK_FP32_TFLOPS)
+        out["match2"] = dict(bound="valu", unit="TFLOP/s", work=flops_match, launches=stage_n["match2"],
+                             ms=stage_ms["match2"] / stage_n["match2"], peak=PEAK_FP32_VALU_TFLOPS)
     if stage_n["wiener"]:
-        # V1, V2 in and filtered V2 out, N = M*N*K floats each
-        out["wiener"] = dict(bound="hbm", unit="GB/s", work=3.0 * vol_bytes, launches=stage_n["wiener"],
-                             ms=stage_ms["wiener"] / stage_n["wiener"], peak=PEAK_HBM_GBS)
+        # groups gathered straight from the padded pilot and noisy search images
+        # through S (V1, V2 never materialised): two pads + S in, V2f + gains out
+        out["wiener"] = dict(bound="hbm", unit="GB/s", work=float(2 * pad_bytes + G2 * K * 4 + vol_bytes + G2 * 4),
+                             launches=stage_n["wiener"], ms=stage_ms["wiener"] / stage_n["wiener"], peak=PEAK_HBM_GBS)
     if stage_n["agg2"]:
-        G = MN // (8 * 8)
-        out["agg2"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G * K * 4 + G * 4 + 2 * MN * 4),
+        out["agg2"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G2 * K * 4 + G2 * 4 + 2 * MN * 4),
                            launches=stage_n["agg2"], ms=stage_ms["agg2"] / stage_n["agg2"], peak=PEAK_HBM_GBS)
     if stage_n["ht_syn0"]:
         # finest-level threshold + synthesis, one cycle spin per launch: read the
@@ -85,10 +89,29 @@ def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2):
     return out
 
 
-def cpu_baseline(threads: int):
+def host_cores():
+    """The host cores this process may run on (its CPU affinity set) and the
+    CPU model."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_baseline():
     import oracle
     from gbm3d_amd import synth
 
+    threads, model = host_cores()
     sz = SIZE  # the timed workload itself: about 10 s on 16 host threads
     clean, noisy = synth.make_pair(SEED, sz, sz, SIGMA)
     oracle.set_threads(threads)
@@ -96,8 +119,10 @@ def cpu_baseline(threads: int):
     U, _ = oracle.gbm3d(noisy, SIGMA, OPTS, precision=32, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": sz * sz / dt / 1e6, "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": f"FP32 C++ restatement (OpenMP) of the same two-step GBM3D on the same {sz}x{sz} synthetic "
-                      f"image (seed {SEED}), sigma=25, same options; {dt:.2f} s",
+            "cpu_model": model, "os_cpu_count": os.cpu_count(),
+            "sample": f"FP32 C++ restatement (OpenMP, one thread per core of this process's affinity set) of the "
+                      f"same two-step GBM3D on the same {sz}x{sz} synthetic image (seed {SEED}), sigma=25, same "
+                      f"options; {dt:.2f} s",
             "psnr_db": synth.psnr(clean, U)}
 
 
@@ -113,12 +138,15 @@ def precision_report(args):
     c32, cbf = gbm3d_amd.Context(0), gbm3d_amd.Context(0)
     cbf.set_variant(match_bf16=True)
     out = {"workload": f"precision: {size}x{size}, sigma 25, K 16, matchSize 48, 8x8 blocks", "size": size}
-    # index sets of step 1, spin 0, on the (block-multiple) noisy image
-    S32, _ = c32.match(noisy, 8, 48, 16, want_volume=False)
-    Sbf, _ = cbf.match(noisy, 8, 48, 16, want_volume=False)
-    out["index_tiles"] = int(S32.shape[1] * S32.shape[2])
-    out["index_identical_frac"] = float(np.mean(np.all(S32 == Sbf, axis=0)))
-    out["index_set_equal_frac"] = float(np.mean(np.all(np.sort(S32, 0) == np.sort(Sbf, 0), axis=0)))
+    _, o32 = c32.denoise(noisy, SIGMA, opts)
+    # index lists of every matching call (steps 1 and 2, both match spins)
+    out["index_tiles_per_call"] = (size // 8) ** 2
+    out["index_identical_frac"] = bf16_index_agreement(cbf, noisy, o32["U1"], opts)
+    cbf.set_variant(match_bf16=True, match_stats=True)
+    cbf.bf16_match_stats(reset=True)
+    cbf.denoise(noisy, SIGMA, opts)
+    out["bf16_match_paths"] = cbf.bf16_match_stats(reset=True)
+    cbf.set_variant(match_bf16=True)
     times = {}
     for name, c in (("fp32", c32), ("bf16_rescored", cbf)):
         c.denoise(noisy, SIGMA, opts)
@@ -129,16 +157,18 @@ def precision_report(args):
         st = c.stage_times()
         times[name] = (U, dt, st)
     t0 = time.perf_counter()
-    oracle.set_threads(min(16, os.cpu_count() or 1))
-    U64, _ = oracle.gbm3d(noisy, SIGMA, opts, precision=64, nthreads=min(16, os.cpu_count() or 1))
+    nth, _ = host_cores()
+    oracle.set_threads(nth)
+    U64, _ = oracle.gbm3d(noisy, SIGMA, opts, precision=64, nthreads=nth)
     out["fp64_oracle_s"] = round(time.perf_counter() - t0, 2)
+    out["bf16_max_abs_vs_fp32"] = float(np.abs(times["bf16_rescored"][0] - times["fp32"][0]).max())
     for name, (U, dt, st) in times.items():
         dev_ms = sum(v[0] for v in st.values())
         mlaunch = st["match1"][1] + st["match2"][1]
         mms = st["match1"][0] + st["match2"][0]
         # distance work 2*m*n*d2^2 per launch (the useful FMAs; the band GEMM issues more)
         tflops = 2.0 * size * size * 48 * 48 * mlaunch / (mms * 1e-3) / 1e12
-        peak = PEAK_BF16_TFLOPS if name == "bf16_rescored" else PEAK_FP32_TFLOPS
+        peak = PEAK_BF16_TFLOPS if name == "bf16_rescored" else PEAK_FP32_VALU_TFLOPS
         out[name] = {"psnr_db": round(synth.psnr(clean, U), 4), "psnr_vs_fp64_oracle_db": round(synth.psnr(U64, U), 3),
                      "max_abs_vs_fp64_oracle": float(np.abs(U - U64).max()),
                      "match_ms": round(mms, 3), "device_ms": round(dev_ms, 3),
@@ -148,6 +178,32 @@ def precision_report(args):
                                       "frac": round(tflops / peak, 4)}}
     out["fp64_oracle_psnr_db"] = round(synth.psnr(clean, U64), 4)
     print(json.dumps(out))
+
+
+def bf16_index_agreement(ctx, noisy, pilot, opts=None):
+    """Index lists of every matching call of the timed workload (step 1 on the
+    noisy image, step 2 on the pilot, match spins 0 and 4), BF16 form against
+    the FP32 matcher: the fraction of reference tiles whose lists are equal."""
+    import gbm3d_amd
+
+    ro = gbm3d_amd.resolve_opts(opts or OPTS)
+    c32 = gbm3d_amd.Context(0)
+    out = {}
+    try:
+        for step, spins in ((1, ro["matchSpins"]), (2, ro["matchSpinsWie"])):
+            for sp in spins:
+                I = np.roll(noisy, (sp, sp), (0, 1)).astype(np.float32)
+                if step == 1:
+                    a, _ = c32.match(I, ro["blockSize"], ro["matchSize"], ro["numMax"], want_volume=False)
+                    b, _ = ctx.match(I, ro["blockSize"], ro["matchSize"], ro["numMax"], want_volume=False)
+                else:
+                    P = np.roll(pilot, (sp, sp), (0, 1)).astype(np.float32)
+                    a, _, _ = c32.match_wie(P, I, ro["blockSizeWie"], ro["matchSize"], ro["numMax"], False)
+                    b, _, _ = ctx.match_wie(P, I, ro["blockSizeWie"], ro["matchSize"], ro["numMax"], False)
+                out[f"step{step}_spin{sp}"] = float(np.mean(np.all(a == b, axis=0)))
+    finally:
+        c32.close()
+    return out
 
 
 def batch_workload(seed=64, count=64):
@@ -280,7 +336,7 @@ def learned_report(args):
     qp, ql = np.array(res["plain"]["psnr_pilot"]), np.array(res["learned"]["psnr_pilot"])
     # FP64 oracle runs of the same pipelines on image 0
     params = shrink.load_weights()
-    nth = min(16, os.cpu_count() or 1)
+    nth, _ = host_cores()
     oracle.set_threads(nth)
     U64, U164 = oracle.gbm3d(imgs[0], sig[0], OPTS, precision=64, nthreads=nth)
     L164 = so.learned_pilot(imgs[0].astype(np.float64), sig[0], params, OPTS, precision=64, exact=True)
@@ -482,7 +538,7 @@ def main():
         stage_ms = {k: v[0] for k, v in st.items()}
         stage_n = {k: v[1] for k, v in st.items()}
         rl = kernel_rooflines(stage_ms, stage_n, SIZE, SIZE, ro["numMax"], ro["blockSize"], ro["matchSize"],
-                              len(ro["matchSpins"]), len(ro["matchSpinsWie"]))
+                              len(ro["matchSpins"]), len(ro["matchSpinsWie"]), ro["blockSizeWie"])
         return stage_ms, rl
 
     U, U1 = dU.to_host(), dU1.to_host()
@@ -508,11 +564,24 @@ def main():
                           "max_abs_vs_plain": float(np.abs(Uv - U).max()),
                           "stages_ms": {k: round(v, 4) for k, v in smv.items() if v > 0}}
         if kw.get("match_bf16") and "match1" in rlv:
-            # the distance kernel against the BF16 MFMA peak (work: 2*m*n*d2^2 per launch)
+            # the distance kernel against the BF16 MFMA peak: the useful distance
+            # work 2*m*n*d2^2 per launch, and the MFMA work it issues (three
+            # 16x16x32 MFMAs per 16x16 candidate tile and block column: hi*hi,
+            # hi*lo, lo*hi)
             k = rlv["match1"]
-            variants[name]["match_kernel"] = {"achieved_tflops": round(k["achieved"], 2), "peak": PEAK_BF16_TFLOPS,
-                                              "frac": round(k["achieved"] / PEAK_BF16_TFLOPS, 4),
-                                              "avg_launch_ms": round(k["ms"], 4)}
+            d1 = ro["blockSize"]
+            issued = 3.0 * 16 * 16 * 32 * 2 * (ro["matchSize"] // 16) ** 2 * d1 * (SIZE // d1) ** 2
+            variants[name]["match_kernel"] = {
+                "achieved_tflops": round(k["achieved"], 2), "peak": PEAK_BF16_TFLOPS,
+                "frac": round(k["achieved"] / PEAK_BF16_TFLOPS, 4), "avg_launch_ms": round(k["ms"], 4),
+                "mfma_issued_tflops": round(issued / (k["ms"] * 1e-3) / 1e12, 2),
+                "mfma_issued_frac": round(issued / (k["ms"] * 1e-3) / 1e12 / PEAK_BF16_TFLOPS, 4)}
+            variants[name]["index_agreement"] = bf16_index_agreement(ctx, noisy, U1)
+            ctx.set_variant(**kw, match_stats=True)
+            ctx.bf16_match_stats(reset=True)
+            step()
+            variants[name]["match_paths"] = ctx.bf16_match_stats(reset=True)
+            ctx.set_variant(**kw)
         if kw.get("learned"):
             el, eh, fell = ctx.shrink_stats(0)
             variants[name].update({"removed_energy_vs_noise": round(el, 4),
@@ -552,7 +621,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline()
 
     mpix = SIZE * SIZE / (cf["ms"] * 1e-3) / 1e6
     plain_mpix = SIZE * SIZE / (ms * 1e-3) / 1e6
