@@ -169,7 +169,7 @@ SIGNATURES = {
     "gbm3d_shrink_set_weights": (_I, [_P, _P, _I]),
     "gbm3d_shrink_stats": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I)]),
     "gbm3d_shrink_groups": (_I, [_P, _P, _I, _F, _P, _P]),
-    "gbm3d_shrink_features": (_I, [_P, _P, _P, _I, _I, _F, _P, _I, _P, _P, _P]),
+    "gbm3d_shrink_features": (_I, [_P, _P, _P, _I, _I, _F, _P, _I, _I, _P, _P, _P]),
     "gbm3d_shrink_forward": (_I, [_P, _P, _P, _P, _I, _P]),
     "gbm3d_shrink_grad": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, C.POINTER(_D)]),
     "gbm3d_shrink_adam": (_I, [_P, _P, _P, _P, _P, _I, _F, _F, _F, _F]),

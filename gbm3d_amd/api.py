@@ -183,9 +183,10 @@ class Context:
                                         C.c_void_p(dNoisy) if dNoisy else None))
 
     def shrink_features(self, dClean: int, dNoisy: int, m: int, n: int, sigma: float, dSel: int, nsel: int,
-                        dX: int, dT: int, dWt: int):
+                        dX: int, dT: int, dWt: int, target: str = "wiener"):
+        t = {"wiener": 0, "energy": 1}[target]
         check(self.lib.gbm3d_shrink_features(self.handle, C.c_void_p(dClean), C.c_void_p(dNoisy), m, n,
-                                             C.c_float(sigma), C.c_void_p(dSel), nsel, C.c_void_p(dX),
+                                             C.c_float(sigma), C.c_void_p(dSel), nsel, t, C.c_void_p(dX),
                                              C.c_void_p(dT), C.c_void_p(dWt)))
 
     def shrink_forward(self, dParams: int, dX: int, dSig: int, ngroups: int, dGains: int):

@@ -171,7 +171,7 @@ int launch_shrink(const float* src, const uint32_t* S, int M, int d2, long G, fl
                   const float* fb, float* vout, float* gout, float* stats, int* grid_used, hipStream_t st);
 // Training rows of the selected groups (noisy features, oracle Wiener gains, loss weights).
 int launch_shrink_features(const float* padC, const float* padN, const uint32_t* S, int M, int d2,
-                           const uint32_t* sel, int nsel, float sigma, float* X, float* Tg, float* Wt,
+                           const uint32_t* sel, int nsel, float sigma, int target, float* X, float* Tg, float* Wt,
                            hipStream_t st);
 size_t shrink_train_floats(int ngroups);
 int shrink_forward(const float* par, const float* X, const float* sig, int ngroups, float* ws, float* gains,

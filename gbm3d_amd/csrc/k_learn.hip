@@ -303,8 +303,9 @@ __global__ __launch_bounds__(256) void shrink_features_kernel(const float* __res
                                                               const float* __restrict__ padN,
                                                               const uint32_t* __restrict__ S, int M, int d2,
                                                               const uint32_t* __restrict__ sel, int nsel,
-                                                              float sigma, DctTable<8> tab, float* __restrict__ Xf,
-                                                              float* __restrict__ Tg, float* __restrict__ Wt) {
+                                                              float sigma, int target, DctTable<8> tab,
+                                                              float* __restrict__ Xf, float* __restrict__ Tg,
+                                                              float* __restrict__ Wt) {
   __shared__ float sm[4][POS * 64];
   __shared__ float T[64];
   if (threadIdx.x < 64) T[threadIdx.x] = tab.t[threadIdx.x];
@@ -335,18 +336,29 @@ __global__ __launch_bounds__(256) void shrink_features_kernel(const float* __res
   for (int k = 0; k < POS; ++k) {
     const size_t o = ((size_t)i * POS + k) * CH + lane;
     const float e2 = xc[k] * xc[k];
-    Xf[o] = xn[k] * inv_s;
-    Tg[o] = e2 / (e2 + s2);
-    Wt[o] = (lane == 0 && k >= POS - 2) ? 0.f : fminf(1.f + e2 / s2, 100.f);
+    const float x = xn[k] * inv_s;
+    Xf[o] = x;
+    const bool dc = lane == 0 && k >= POS - 2;  // the fixed DC approximation: no loss
+    if (target == GBM3D_SHRINK_TARGET_ENERGY) {
+      // the gain whose pilot coefficient has the clean coefficient's magnitude,
+      // |g y| = |c| (then step 2's gain |P|^2/(|P|^2 + sigma^2) is the oracle
+      // Wiener gain), weighted by the pilot error it measures: (g - T)^2 y^2/sigma^2
+      const float ay = fabsf(xn[k]);
+      Tg[o] = ay > 0.f ? fminf(fabsf(xc[k]) / ay, 1.f) : 1.f;
+      Wt[o] = dc ? 0.f : fminf(x * x, 100.f);
+    } else {
+      Tg[o] = e2 / (e2 + s2);
+      Wt[o] = dc ? 0.f : fminf(1.f + e2 / s2, 100.f);
+    }
   }
 }
 
 int launch_shrink_features(const float* padC, const float* padN, const uint32_t* S, int M, int d2,
-                           const uint32_t* sel, int nsel, float sigma, float* X, float* Tg, float* Wt,
+                           const uint32_t* sel, int nsel, float sigma, int target, float* X, float* Tg, float* Wt,
                            hipStream_t st) {
   if (nsel <= 0) return GBM3D_OK;
   hipLaunchKernelGGL(shrink_features_kernel, dim3((unsigned)((nsel + 3) / 4)), dim3(256), 0, st, padC, padN, S, M,
-                     d2, sel, nsel, sigma, make_dct_table<8>(), X, Tg, Wt);
+                     d2, sel, nsel, sigma, target, make_dct_table<8>(), X, Tg, Wt);
   return check_launch("shrink_features");
 }
 

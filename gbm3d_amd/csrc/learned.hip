@@ -139,7 +139,7 @@ int gbm3d_shrink_groups(gbm3d_ctx* c, const float* V, int G, float sigma, float*
 }
 
 int gbm3d_shrink_features(gbm3d_ctx* c, const float* dClean, const float* dNoisy, int m, int n, float sigma,
-                          const uint32_t* dSel, int nsel, float* dX, float* dT, float* dWt) {
+                          const uint32_t* dSel, int nsel, int target, float* dX, float* dT, float* dWt) {
   GBM3D_TRY(set_dev(c));
   if (!dClean || !dNoisy || !dSel || !dX || !dT || !dWt || nsel < 0) return fail(GBM3D_ERR_ARG, "NULL argument");
   constexpr int d = 8, d2 = 32, K = 16;
@@ -163,7 +163,9 @@ int gbm3d_shrink_features(gbm3d_ctx* c, const float* dClean, const float* dNoisy
   GBM3D_TRY(launch_search_pad(dNoisy, m, n, 0, d2, pn, c->stream));
   GBM3D_TRY(launch_norm_map(pn, (int)P, (int)Q, d, bn, c->stream));
   GBM3D_TRY(launch_match_w(pn, bn, m, n, d, d2, K, S, nullptr, c->match_waves, c->stream));
-  return launch_shrink_features(pc, pn, S, m, d2, dSel, nsel, sigma, dX, dT, dWt, c->stream);
+  if (target != GBM3D_SHRINK_TARGET_WIENER && target != GBM3D_SHRINK_TARGET_ENERGY)
+    return fail(GBM3D_ERR_ARG, "shrink features: unknown target");
+  return launch_shrink_features(pc, pn, S, m, d2, dSel, nsel, sigma, target, dX, dT, dWt, c->stream);
 }
 
 int gbm3d_shrink_forward(gbm3d_ctx* c, const float* dParams, const float* dX, const float* dSig, int ngroups,

@@ -81,7 +81,7 @@ class Dataset:
     """Training rows of many images resident in device memory."""
 
     def __init__(self, ctx, images: int, groups: int, size: int = 512, seed: int = 1, smin: float = 5.0,
-                 smax: float = 50.0, log=None):
+                 smax: float = 50.0, log=None, target: str = "energy"):
         from .device import DeviceArray
 
         self.images, self.gpi, self.size = images, groups, size
@@ -104,7 +104,8 @@ class Dataset:
         for i in range(images):
             ctx.synth_pair(seed * 1_000_003 + i, size, size, float(self.sigmas[i]), clean.ptr, noisy.ptr)
             ctx.shrink_features(clean.ptr, noisy.ptr, size, size, float(self.sigmas[i]), dsel.ptr + 4 * groups * i,
-                                groups, self.X.ptr + stride * i, self.T.ptr + stride * i, self.Wt.ptr + stride * i)
+                                groups, self.X.ptr + stride * i, self.T.ptr + stride * i, self.Wt.ptr + stride * i,
+                                target=target)
             if log and (i + 1) % 2000 == 0:
                 ctx.synchronize()
                 log(json.dumps({"images": i + 1}))
@@ -182,20 +183,23 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=WEIGHTS_PATH)
+    ap.add_argument("--target", default="energy", choices=["energy", "wiener"],
+                    help="gain targets: 'energy' min(|c|/|y|, 1), the pilot magnitudes step 2's Wiener gains "
+                         "need; 'wiener' c^2/(c^2 + sigma^2)")
     a = ap.parse_args(argv)
     from .api import Context
 
     ctx = Context(0)
     t0 = time.time()
-    data = Dataset(ctx, a.images, a.groups, a.size, seed=1, log=lambda s: print(s, flush=True))
-    val = Dataset(ctx, a.val_images, a.groups, a.size, seed=2)
+    data = Dataset(ctx, a.images, a.groups, a.size, seed=1, log=lambda s: print(s, flush=True), target=a.target)
+    val = Dataset(ctx, a.val_images, a.groups, a.size, seed=2, target=a.target)
     t_data = time.time() - t0
     print(json.dumps({"data_s": round(t_data, 1), "groups": data.ngroups, "val_groups": val.ngroups}), flush=True)
     base = baseline_losses(val)
     print(json.dumps({"baseline_val_loss": base}), flush=True)
     params, hist = train(ctx, data, val, a.epochs, a.batch, a.lr, a.seed,
                          log=lambda s: print(s, flush=True))
-    meta = {"images": a.images, "groups_per_image": a.groups, "size": a.size, "sigma": [5, 50],
+    meta = {"images": a.images, "groups_per_image": a.groups, "size": a.size, "sigma": [5, 50], "target": a.target,
             "val_images": a.val_images, "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "seed": a.seed,
             "data_seconds": round(t_data, 1), "baseline_val_loss": base, "history": hist}
     save_weights(a.out, params, meta)

@@ -343,14 +343,20 @@ int gbm3d_shrink_groups(gbm3d_ctx* ctx, const float* V, int G, float sigma,
 /* Training on device buffers (context stream).  Rows r = 16 i + k of group
  * i, 64 columns (coefficient t).
  * features: step-1 matching (spin 0, 8x8 blocks, matchSize 32, K 16) on the
- * noisy image, then for the nsel groups listed in dSel: X = c_noisy/sigma,
- * T = Wiener gains of the clean group, Wt = loss weights
- * min(1 + c_clean^2/sigma^2, 100) (0 on the fixed DC approximation).
- * m, n: multiples of 8, >= 32. */
+ * noisy image, then for the nsel groups listed in dSel: X = c_noisy/sigma and
+ * per target
+ *   GBM3D_SHRINK_TARGET_WIENER: T = Wiener gains of the clean group,
+ *     Wt = min(1 + c_clean^2/sigma^2, 100);
+ *   GBM3D_SHRINK_TARGET_ENERGY: T = min(|c_clean| / |c_noisy|, 1) (the gain
+ *     whose pilot coefficient has the clean magnitude, so that step 2's gain
+ *     computed from the pilot is the oracle Wiener gain),
+ *     Wt = min(c_noisy^2/sigma^2, 100);
+ * Wt is 0 on the fixed DC approximation.  m, n: multiples of 8, >= 32. */
+enum { GBM3D_SHRINK_TARGET_WIENER = 0, GBM3D_SHRINK_TARGET_ENERGY = 1 };
 int gbm3d_shrink_features(gbm3d_ctx* ctx, const float* dClean,
                           const float* dNoisy, int m, int n, float sigma,
-                          const uint32_t* dSel, int nsel, float* dX,
-                          float* dT, float* dWt);
+                          const uint32_t* dSel, int nsel, int target,
+                          float* dX, float* dT, float* dWt);
 /* Gains of ngroups groups (sigma per group in dSig). */
 int gbm3d_shrink_forward(gbm3d_ctx* ctx, const float* dParams, const float* dX,
                          const float* dSig, int ngroups, float* dGains);

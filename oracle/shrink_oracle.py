@@ -226,15 +226,23 @@ def shrink_groups(V, sigma, params, exact=False):
     return np.maximum(out, 0.0), g
 
 
-def features(Vc, Vn, sigma):
-    """Training rows of groups (clean Vc, noisy Vn): X, T, Wt each (G, 16, 64)."""
+def features(Vc, Vn, sigma, target="wiener"):
+    """Training rows of groups (clean Vc, noisy Vn): X, T, Wt each (G, 16, 64).
+    target "wiener": T = c^2/(c^2 + sigma^2), Wt = min(1 + c^2/sigma^2, 100);
+    "energy": T = min(|c|/|y|, 1), Wt = min(y^2/sigma^2, 100) (c clean, y noisy)."""
     cc = coefficients(Vc).astype(np.float32)
     cn = coefficients(Vn).astype(np.float32)
     s2 = np.float32(sigma) * np.float32(sigma)
     X = cn * np.float32(1.0 / np.float32(sigma))
     e2 = cc * cc
-    T = e2 / (e2 + s2)
-    Wt = np.minimum(1 + e2 / s2, 100).astype(np.float32)
+    if target == "energy":
+        ay = np.abs(cn)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            T = np.where(ay > 0, np.minimum(np.abs(cc) / np.where(ay > 0, ay, 1), 1), 1).astype(np.float32)
+        Wt = np.minimum(X * X, 100).astype(np.float32)
+    else:
+        T = e2 / (e2 + s2)
+        Wt = np.minimum(1 + e2 / s2, 100).astype(np.float32)
     Wt[:, 0, 14:] = 0
     tr = (0, 2, 1)
     return X.transpose(tr), T.transpose(tr), Wt.transpose(tr)

@@ -39,7 +39,8 @@ def test_synth_pair_matches_host_construction(ctx, m, n, seed, sigma):
 _groups = so.gather_groups
 
 
-def test_features_vs_restatement(ctx):
+@pytest.mark.parametrize("target", ["wiener", "energy"])
+def test_features_vs_restatement(ctx, target):
     m, n, sigma = 96, 128, 20.0
     dc, dn = DeviceArray((m, n)), DeviceArray((m, n))
     ctx.synth_pair(11, m, n, sigma, dc.ptr, dn.ptr)
@@ -50,18 +51,24 @@ def test_features_vs_restatement(ctx):
     dsel = DeviceArray.from_host(sel)
     rows = sel.size * 16 * 64
     X, T, W = DeviceArray((rows,)), DeviceArray((rows,)), DeviceArray((rows,))
-    ctx.shrink_features(dc.ptr, dn.ptr, m, n, sigma, dsel.ptr, sel.size, X.ptr, T.ptr, W.ptr)
+    ctx.shrink_features(dc.ptr, dn.ptr, m, n, sigma, dsel.ptr, sel.size, X.ptr, T.ptr, W.ptr, target=target)
     ctx.synchronize()
     S, _ = ctx.match(noisy, 8, 32, 16, want_volume=False)   # the same matcher (spin 0)
     Vc = _groups(oracle.search_pad(clean, 32), S)[..., sel]
     Vn = _groups(oracle.search_pad(noisy, 32), S)[..., sel]
-    Xr, Tr, Wr = so.features(Vc, Vn, sigma)
+    Xr, Tr, Wr = so.features(Vc, Vn, sigma, target=target)
     Xg = X.to_host().reshape(sel.size, 16, 64)
     Tg = T.to_host().reshape(sel.size, 16, 64)
     Wg = W.to_host().reshape(sel.size, 16, 64)
     assert np.max(np.abs(Xg - Xr)) <= 1e-3 * max(1.0, np.abs(Xr).max() / 100)
-    assert np.max(np.abs(Tg - Tr)) <= 1e-4
-    assert np.max(np.abs(Wg - Wr) / Wr.clip(1)) <= 1e-4
+    if target == "wiener":
+        assert np.max(np.abs(Tg - Tr)) <= 1e-4
+        assert np.max(np.abs(Wg - Wr) / Wr.clip(1)) <= 1e-4
+    else:
+        # |c|/|y| is ill-conditioned where the noisy coefficient is tiny; such
+        # rows carry the weight y^2/sigma^2 ~ 0, so compare the weighted target
+        assert np.max(np.sqrt(Wr) * np.abs(Tg - Tr)) <= 1e-3
+        assert np.max(np.abs(Wg - Wr)) <= 1e-3 * max(1.0, Wr.max())
 
 
 def _gains_close(a, b):

@@ -117,3 +117,9 @@ def test_features_targets_and_weights():
     assert np.allclose(X, cn / 25.0, rtol=1e-5, atol=1e-5)
     assert np.allclose(T, cc ** 2 / (cc ** 2 + 625.0), rtol=1e-5, atol=1e-6)
     assert np.all(Wt[:, 14:, 0] == 0) and np.all(Wt[:, :14, :] >= 1) and Wt.max() <= 100
+    # energy-matched targets: |T y| = |c| where |c| <= |y|, else 1; weight y^2/sigma^2
+    X, T, Wt = so.features(Vc, Vn, 25.0, target="energy")
+    y = cn
+    assert np.allclose(T, np.minimum(np.abs(cc) / np.abs(y), 1), rtol=1e-4, atol=1e-6)
+    assert np.allclose(Wt[:, :14, :], np.minimum(y[:, :14, :] ** 2 / 625.0, 100), rtol=1e-4, atol=1e-6)
+    assert np.all(Wt[:, 14:, 0] == 0)
