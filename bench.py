@@ -460,6 +460,8 @@ def main():
                          "the schedule whose kernel durations the roofline figures use")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="with --profile-only: also run one device copy of MIB MiB (byte-counter calibration)")
+    ap.add_argument("--form", default="plain", choices=["plain", "bf16_match", "learned"],
+                    help="with --profile-only: the form whose steps run (for kernel profiles)")
     ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched", "learned",
                                                           "distributed"])
     ap.add_argument("--size", type=int, default=0, help="image side for --workload precision")
@@ -516,6 +518,8 @@ def main():
         device.synchronize()
         return e0.elapsed_ms(e1) / steps, time.perf_counter() - t0
 
+    if args.profile_only and args.form != "plain":
+        ctx.set_variant(**{"bf16_match": {"match_bf16": True}, "learned": {"learned": True}}[args.form])
     if args.profile_only and args.stage_timing:
         ctx.set_timing(True)  # the stage-timed (serial) schedule the roofline figures come from
     ms, wall = timed(args.steps)
