@@ -409,6 +409,35 @@ def distributed_report(args):
                                         "dpsnr_db": round(synth.psnr(c2, U) - synth.psnr(c2, U64), 4)}}))
 
 
+def global_match_report(args):
+    """matchBlocksGPU (source/matchBlocksGPU.m:18-91) on a size x size synthetic
+    image (default 512), sigma 25 (the sym4 prefilter runs), 8 x 8 blocks,
+    numMax = numMin = 16, through the host entry point (upload, prefilter,
+    norms, search, download).  The search's algorithmic work is 2 d^2 flops
+    per (tile, position) pair, G = (size / 8)^2 tiles times size^2 positions;
+    divided by the whole call's time it is a lower bound on the search
+    kernel's rate (rocprofv3 --kernel-trace gives the kernel's own time)."""
+    import gbm3d_amd
+    from gbm3d_amd import synth
+
+    size, d, K = args.size or 512, 8, 16
+    _, noisy = synth.make_pair(SEED, size, size, SIGMA)
+    ctx = gbm3d_amd.Context(0)
+    for _ in range(args.warmup):
+        ctx.match_global(noisy, SIGMA, d, K)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.match_global(noisy, SIGMA, d, K)
+    dt = (time.perf_counter() - t0) / args.steps
+    G = (size // d) ** 2
+    pairs = G * size * size
+    print(json.dumps({"workload": f"matchBlocksGPU, {size}x{size} synthetic image, sigma 25, 8x8 blocks, numMax 16",
+                      "ms_per_call": round(dt * 1e3, 3), "tile_position_pairs": pairs,
+                      "pairs_per_s": round(pairs / dt, 1),
+                      "search_tflops_lower_bound": round(2 * d * d * pairs / dt / 1e12, 3),
+                      "peak_fp32_valu_tflops": PEAK_FP32_VALU_TFLOPS}))
+
+
 # kernel symbol prefix -> roofline key; the largest dispatch of a kernel is the
 # one measured (the finest level for the wavelet synthesis)
 TRAFFIC_KERNELS = {"gbm3d::ht_syn0_cs2_kernel": "ht_syn0", "void gbm3d::match_kernel": "match2",
@@ -463,8 +492,8 @@ def main():
     ap.add_argument("--form", default="plain", choices=["plain", "bf16_match", "learned"],
                     help="with --profile-only: the form whose steps run (for kernel profiles)")
     ap.add_argument("--workload", default="plain", choices=["plain", "precision", "batched", "learned",
-                                                          "distributed"])
-    ap.add_argument("--size", type=int, default=0, help="image side for --workload precision")
+                                                          "distributed", "global_match"])
+    ap.add_argument("--size", type=int, default=0, help="image side for --workload precision / global_match")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="derive profiles/traffic.json from two rocprofv3 --pmc runs made with --calibrate-copy 256")
     args = ap.parse_args()
@@ -482,6 +511,9 @@ def main():
         return
     if args.workload == "distributed":
         distributed_report(args)
+        return
+    if args.workload == "global_match":
+        global_match_report(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (one image per launch); --gpus must be 1", file=sys.stderr)

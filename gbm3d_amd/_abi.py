@@ -157,6 +157,7 @@ SIGNATURES = {
     "gbm3d_denoise_batch_device": (_I, [_P, _I, _P, _P, _P, _P, C.POINTER(Opts), _P, _P, _I]),
     "gbm3d_batch_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "gbm3d_match": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
+    "gbm3d_match_global": (_I, [_P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P]),
     "gbm3d_ht_filter": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "gbm3d_median_filter": (_I, [_P, _P, _I, _I, _I, _P]),
     "gbm3d_aggregate_ht": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _P, _P]),

@@ -325,6 +325,24 @@ class Context:
         check(self.lib.gbm3d_match(self.handle, ptr(I), m, n, d, d2, K, ptr(S), ptr(V)))
         return S, V
 
+    def match_global(self, I, sigma: float, d: int, numMax: int, numMin: int | None = None,
+                     want_filtered: bool = False):
+        """matchBlocksGPU's whole-image search (source/matchBlocksGPU.m:18-91).
+
+        Returns (S, counts[, filtered]): S (numMax, floor(m/d), floor(n/d))
+        uint32 1-based column-major indices into m x n, each tile's first
+        counts[ti, tj] entries its list and the rest 0; filtered, the image
+        searched (the sym4 prefilter when sigma > 30/255)."""
+        I = np.asfortranarray(I, dtype=np.float32)
+        m, n = I.shape
+        numMin = numMax if numMin is None else numMin
+        S = np.zeros((numMax, m // d, n // d), dtype=np.uint32, order="F")
+        cnt = np.zeros((m // d, n // d), dtype=np.uint32, order="F")
+        F = np.zeros_like(I) if want_filtered else None
+        check(self.lib.gbm3d_match_global(self.handle, ptr(I), m, n, float(sigma), d, numMax, numMin, ptr(S),
+                                          ptr(cnt), ptr(F)))
+        return (S, cnt, F) if want_filtered else (S, cnt)
+
     def ht_filter(self, V, levels: int, cycle_spin: int, tau):
         V = np.asfortranarray(V, dtype=np.float32)
         m, n, K = V.shape
@@ -430,6 +448,19 @@ def matchBlocksCPU(I, sigma: float, opts: dict):
     """[S, Volume] = matchBlocksCPU(I, sigma, opts) on the GPU (numMax == numMin)."""
     o = checkBM3Dopts(opts)
     return default_context().match(I, o["blockSize"], o["matchSize"], o["numMax"])
+
+
+def matchBlocksGPU(I, sigma: float, opts: dict):
+    """S = matchBlocksGPU(I, sigma, opts): an object array (floor(m/d),
+    floor(n/d)) of uint32 index columns (1-based, into m x n), like the
+    reference's cell array; the tolerance branch when numMax != numMin."""
+    o = checkBM3Dopts(opts)
+    S, cnt = default_context().match_global(I, sigma, o["blockSize"], o["numMax"], o["numMin"])
+    out = np.empty(cnt.shape, dtype=object)
+    for i in range(cnt.shape[0]):
+        for j in range(cnt.shape[1]):
+            out[i, j] = S[:cnt[i, j], i, j].copy()
+    return out
 
 
 def denoiseAndAggregateGPU(Volume, S, tau, opts: dict):

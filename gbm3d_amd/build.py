@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgbm3d.so")
-SOURCES = ["host.cpp", "api.hip", "pipeline.hip", "batch.hip", "learned.hip", "distributed.hip", "noalloc.hip", "k_match.hip", "k_ht.hip", "k_agg.hip", "k_wiener.hip", "k_learn.hip",
+SOURCES = ["host.cpp", "api.hip", "pipeline.hip", "batch.hip", "learned.hip", "distributed.hip", "noalloc.hip", "k_match.hip", "k_gmatch.hip", "k_ht.hip", "k_agg.hip", "k_wiener.hip", "k_learn.hip",
            "k_synth.hip", "k_median.hip"]
 ARCH = os.environ.get("GBM3D_ARCH", "gfx950")
 

@@ -226,6 +226,50 @@ int gbm3d_match(gbm3d_ctx* c, const float* I, int m, int n, int d, int d2, int K
   return GBM3D_OK;
 }
 
+int gbm3d_match_global(gbm3d_ctx* c, const float* I, int m, int n, double sigma, int d, int numMax, int numMin,
+                       uint32_t* S, uint32_t* counts, float* Ifilt) {
+  GBM3D_TRY(set_dev(c));
+  if (!I || !S) return fail(GBM3D_ERR_ARG, "NULL argument");
+  if (!match_global_supported(d)) return fail(GBM3D_ERR_UNSUPPORTED, "matchBlocksGPU: block size must be 4, 8, 16 or 32");
+  if (m < d || n < d) return fail(GBM3D_ERR_ARG, "matchBlocksGPU: image smaller than one block");
+  if (numMax < 1 || numMax > 64 || numMin < 1 || numMin > numMax || (size_t)numMax > (size_t)m * n)
+    return fail(GBM3D_ERR_ARG, "matchBlocksGPU: need 1 <= numMin <= numMax <= min(64, m n)");
+  if (!(sigma >= 0.0) || !std::isfinite(sigma)) return fail(GBM3D_ERR_ARG, "matchBlocksGPU: sigma must be finite and >= 0");
+  const bool pre = sigma > 30.0 / 255.0;  // matchBlocksGPU.m:34
+  if (pre && (m % 8 || n % 8))
+    return fail(GBM3D_ERR_ARG, "matchBlocksGPU: the prefilter's 3-level transform needs m and n multiples of 8");
+  const size_t mn = (size_t)m * n, G = (size_t)(m / d) * (n / d);
+  const size_t wpre = pre ? (mn / 4) * 4 + (mn / 16) * 4 + (mn / 64) * 4 + mn / 4 + mn / 16 : 0;
+  const size_t wmatch = (size_t)(m + d - 1) * (n + d - 1) + mn;
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** img, float** filt, float** work, uint32_t** dS, uint32_t** dC) {
+    *img = k.take<float>(mn);
+    *filt = k.take<float>(pre ? mn : 0);
+    *work = k.take<float>(std::max(wpre, wmatch));
+    *dS = k.take<uint32_t>(G * numMax);
+    *dC = k.take<uint32_t>(G);
+  };
+  float *img, *filt, *work;
+  uint32_t *dS, *dC;
+  lay(cv, &img, &filt, &work, &dS, &dC);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &img, &filt, &work, &dS, &dC);
+  GBM3D_TRY(upload(c, img, I, mn));
+  const float* src = img;
+  if (pre) {
+    GBM3D_TRY(launch_wav2_prefilter(img, m, n, 2.0 * sigma, filt, work, c->stream));  // tau = 2 sigma (:35)
+    src = filt;
+  }
+  const double tol = 0.005 * sigma * sigma * d * d;  // :19-23
+  GBM3D_TRY(launch_match_global(src, m, n, d, numMax, numMin, tol, dS, dC, work, c->stream));
+  GBM3D_TRY(download(c, S, dS, G * numMax));
+  if (counts) GBM3D_TRY(download(c, counts, dC, G));
+  if (Ifilt) GBM3D_TRY(download(c, Ifilt, src, mn));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
 int gbm3d_ht_filter(gbm3d_ctx* c, const float* Volume, int m, int n, int K, int levels, int cycleSpin,
                     const double* tau, float* Vavg) {
   GBM3D_TRY(set_dev(c));

@@ -89,6 +89,14 @@ constexpr int kBf16Stats = 8;
 // The counters live in kBf16StatSlots copies kBf16StatStride words apart
 // (one cache line each), summed on the host.
 constexpr int kBf16StatSlots = 64, kBf16StatStride = 32;
+// matchBlocksGPU (k_gmatch.hip).  Prefilter: Iavg (m x n) from I, work of
+// (m n / 4)(4 (1 + 1/4 + 1/16) + 1 + 1/4) floats.  Search: S (K x G), counts
+// (G, nullable), work of (m + d - 1)(n + d - 1) + m n floats.
+int launch_wav2_prefilter(const float* I, int m, int n, double tau, float* Iavg, float* work, hipStream_t st);
+int launch_match_global(const float* I, int m, int n, int d, int K, int Kmin, double tol, uint32_t* S,
+                        uint32_t* counts, float* work, hipStream_t st);
+bool match_global_supported(int d);
+
 int launch_match_bf16(const float* pad, const float* bn, const float* smap, int M, int N, int d, int d2, int K,
                       int R, uint32_t* S, float* vol, hipStream_t st, uint32_t* stats = nullptr);
 // Same with nw (1, 2, 4) reference tiles per workgroup.

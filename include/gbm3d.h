@@ -243,6 +243,25 @@ int gbm3d_batch_info(gbm3d_ctx* ctx, int* streams, int* match_waves, double* tun
 int gbm3d_match(gbm3d_ctx* ctx, const float* I, int m, int n, int d, int d2,
                 int K, uint32_t* S, float* Volume);
 
+/* S = matchBlocksGPU(I, sigma, opts) -- source/matchBlocksGPU.m:18-91, the
+ * whole-image search (not on the GBM3D path, which matches in windows): for
+ * each reference tile (stride d, floor(m/d) x floor(n/d), column-major) the
+ * positions of the m x n image whose d x d block, circularly wrapped, is
+ * nearest by (|x|^2 + |y|^2 - 2<x,y>) / |x|^2.  S: numMax entries per tile,
+ * 1-based column-major indices into m x n (the block's top-left pixel), the
+ * tile's first counts[tile] of them its list, the rest 0.  numMax == numMin:
+ * the numMax nearest (mink; counts all numMax).  numMin < numMax: the
+ * positions with distance < 0.005 sigma^2 d^2 in index order, or the
+ * numMax / numMin nearest when there are more / fewer (:71-81).  When
+ * sigma > 30/255 the search runs on the image prefiltered by the mean over 5
+ * cycle shifts of a 3-level 2-D sym4 wavelet hard threshold at 2 sigma
+ * (:34-50; m and n multiples of 8).  counts, Ifilt (m x n, the image
+ * searched): nullable.  d in {4, 8, 16, 32}; 1 <= numMin <= numMax <= 64.
+ * Host buffers. */
+int gbm3d_match_global(gbm3d_ctx* ctx, const float* I, int m, int n,
+                       double sigma, int d, int numMax, int numMin,
+                       uint32_t* S, uint32_t* counts, float* Ifilt);
+
 /* Vavg = the 'ht' filter of denoiseAndAggregateGPU (cycle-spun 3-D
  * bior1.5 x Haar hard threshold, mean over spins, clip at 0)
  * -- source/denoiseAndAggregateGPU.m:19-47.  tau: levels x 7 (see

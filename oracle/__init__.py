@@ -128,6 +128,42 @@ def match_wie(I, In, d, d2, K, precision=64, want_dist=False):
     return S, V1, V2, dist
 
 
+def sym4_taps():
+    """The sym4 (Lo_d, Hi_d) the oracle's prefilter is built from."""
+    lo, hi = np.zeros(8), np.zeros(8)
+    lib().orc_sym4_taps(_p(lo), _p(hi))
+    return lo, hi
+
+
+def wav2_prefilter(I, tau, precision=64):
+    """matchBlocksGPU.m:34-50 with threshold tau: the mean over 5 cycle shifts
+    of the 3-level 2-D sym4 analysis, hard threshold, synthesis."""
+    dt, suf = _dt(precision)
+    I = _f(I, dt)
+    out = np.zeros_like(I)
+    _check(getattr(lib(), "orc_wav2_prefilter_" + suf)(_p(I), I.shape[0], I.shape[1], C.c_double(tau), _p(out)))
+    return out
+
+
+def match_global(I, sigma, d, numMax, numMin=None, precision=64, want_filtered=False):
+    """S = matchBlocksGPU(I, sigma, opts) (source/matchBlocksGPU.m:18-91).
+
+    Returns (S, counts[, filtered]): S (numMax, floor(m/d), floor(n/d)) uint32,
+    1-based column-major indices into m x n, each tile's first counts[ti, tj]
+    entries its list (the rest 0); filtered is the image the search ran on
+    (the sym4 prefilter when sigma > 30/255)."""
+    dt, suf = _dt(precision)
+    I = _f(I, dt)
+    m, n = I.shape
+    numMin = numMax if numMin is None else numMin
+    S = np.zeros((numMax, m // d, n // d), dtype=np.uint32, order="F")
+    cnt = np.zeros((m // d, n // d), dtype=np.uint32, order="F")
+    F = np.zeros_like(I)
+    _check(getattr(lib(), "orc_match_global_" + suf)(_p(I), m, n, C.c_double(sigma), d, numMax, numMin, _p(S),
+                                                      _p(cnt), _p(F)))
+    return (S, cnt, F) if want_filtered else (S, cnt)
+
+
 def ht_filter(V, levels, cycle_spin, tau, precision=64, want_margin=False):
     dt, suf = _dt(precision)
     V = _f(V, dt)

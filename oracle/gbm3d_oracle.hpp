@@ -24,6 +24,11 @@
 //                   numpy literal loop (oracle/literal.py).
 //   wiener        : scipy orthonormal DCT-II, numpy FFT Haar, PR of the 1-D Haar.
 //   end to end    : numpy literal GBM3D on 64x64 inputs; scale equivariance.
+//   whole-image   : sym4 taps == archive/waveletFilters/sym4Filters.mat
+//   matching        (tests/golden/sym4_taps.json) and their orthonormality;
+//                   prefilter PR without threshold; numpy fftn restatement of
+//                   myWavDec2DFFT / myWavRec2DGPUfast / matchBlocksGPU (all three
+//                   list branches); brute-force |x - y|^2 / |x|^2 ranking.
 #pragma once
 
 #include <algorithm>
@@ -1103,6 +1108,217 @@ int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout,
   return 0;
 }
 
+
+// ------------------------------------------------- whole-image matching
+// S = matchBlocksGPU(I, sigma, opts) (source/matchBlocksGPU.m:18-91), off the
+// GBM3D() path: for every reference tile (stride d, floor(m/d) x floor(n/d))
+// the normalised distance (|x|^2 + |y|^2 - 2<x,y>)/|x|^2 to the block at
+// every one of the m*n positions of the image, circularly wrapped, and the
+// numMax smallest by mink (ascending, NaN last, ties to the lower index; no
+// centre override).  Indices are 1-based column-major into m x n (the block's
+// top-left pixel).  When sigma > 30/255 the search runs on a prefiltered
+// image (:34-50): the mean over 5 cycle shifts of a 3-level 2-D sym4 wavelet
+// hard threshold at 2 sigma.
+//
+// Sums, in this order (the GPU forms the same chains): block norms and
+// correlations column by column (v outer, u inner) with fused multiply-adds;
+// the 2-D filters Psi_b(u, v) = h1[u] h2[v] rounded to T, analysis taps and
+// synthesis taps likewise v outer, u inner; the synthesised bands added in
+// the order of myWavRec2DGPUfast.m:15-31.
+
+// sym4 decomposition filters, archive/waveletFilters/sym4Filters.mat (Lo_d, Hi_d)
+constexpr double kSym4Lo[8] = {-0.07576571478927333, -0.02963552764599851, 0.49761866763201545, 0.8037387518059161,
+                               0.29785779560527736,  -0.09921954357684722, -0.012603967262037833, 0.0322231006040427};
+constexpr double kSym4Hi[8] = {-0.0322231006040427, -0.012603967262037833, 0.09921954357684722, 0.29785779560527736,
+                               -0.8037387518059161, 0.49761866763201545,   0.02963552764599851,  -0.07576571478927333};
+
+// Psi(:,:,b+1) of getWaveFilters2D.m:12-16: b = 0 Lo'Lo, 1 Hi'Lo (dim 1 high),
+// 2 Lo'Hi (dim 2 high), 3 Hi'Hi; entry (u, v) = h1[u] h2[v], stored as T.
+template <class T>
+struct Sym4Bank {
+  T psi[4][64];
+  Sym4Bank() {
+    for (int b = 0; b < 4; ++b)
+      for (int v = 0; v < 8; ++v)
+        for (int u = 0; u < 8; ++u) {
+          const double h1 = (b & 1) ? kSym4Hi[u] : kSym4Lo[u];
+          const double h2 = (b & 2) ? kSym4Hi[v] : kSym4Lo[v];
+          psi[b][u + 8 * v] = (T)(h1 * h2);
+        }
+  }
+};
+
+// One level of myWavDec2DFFT.m:13-21: band b at (i, j) = sum_{u,v} Psi_b(u,v)
+// X((2i - u) mod L1, (2j - v) mod L2) (circular convolution kept at the even
+// samples; filters truncated to the level size like fftn(Psi, size(U))).
+template <class T>
+void wav2_analysis(const std::vector<T>& X, int L1, int L2, const Sym4Bank<T>& F, std::vector<T> out[4]) {
+  const int h1 = L1 / 2, h2 = L2 / 2, n1 = std::min(8, L1), n2 = std::min(8, L2);
+  for (int b = 0; b < 4; ++b) out[b].assign((size_t)h1 * h2, T(0));
+#pragma omp parallel for schedule(static) if (h1 * h2 > 4096)
+  for (int j = 0; j < h2; ++j)
+    for (int i = 0; i < h1; ++i)
+      for (int b = 0; b < 4; ++b) {
+        T acc = 0;
+        for (int v = 0; v < n2; ++v)
+          for (int u = 0; u < n1; ++u)
+            acc = madd<T>(F.psi[b][u + 8 * v], X[pmod(2 * i - u, L1) + (size_t)L1 * pmod(2 * j - v, L2)], acc);
+        out[b][i + (size_t)h1 * j] = acc;
+      }
+}
+
+// Circular correlation of the zero-upsampled band D (Lc1 x Lc2, values at the
+// even positions of the 2Lc1 x 2Lc2 grid) with Psi_b (myWavRec2DGPUfast.m:16-18, 26-31).
+template <class T>
+T wav2_corr_up(const std::vector<T>& D, int Lc1, int Lc2, const T* psi, int t1, int t2) {
+  const int L1 = 2 * Lc1, L2 = 2 * Lc2, n1 = std::min(8, L1), n2 = std::min(8, L2);
+  T acc = 0;
+  for (int v = 0; v < n2; ++v) {
+    const int c = pmod(t2 + v, L2);
+    if (c & 1) continue;
+    for (int u = 0; u < n1; ++u) {
+      const int r = pmod(t1 + u, L1);
+      if (r & 1) continue;
+      acc = madd<T>(psi[u + 8 * v], D[r / 2 + (size_t)Lc1 * (c / 2)], acc);
+    }
+  }
+  return acc;
+}
+
+// I (m x n) -> the mean over shifts k = 0..4 of circshift(rec(thr(dec(circshift(I,[k k])))), [-k -k])
+// (matchBlocksGPU.m:35-49), 3 levels, hard threshold at tau on the details.
+template <class T>
+std::vector<T> wav2_prefilter(const T* I, int m, int n, double tau) {
+  const int levels = 3, shifts = 5;
+  Sym4Bank<T> F;
+  std::vector<T> Iavg((size_t)m * n, T(0));
+  for (int k = 0; k < shifts; ++k) {
+    std::vector<std::vector<T>> D(levels * 3);
+    std::vector<int> L1s(levels + 1), L2s(levels + 1);
+    std::vector<T> A((size_t)m * n);
+    for (int c = 0; c < n; ++c)
+      for (int r = 0; r < m; ++r) A[r + (size_t)m * c] = I[pmod(r - k, m) + (size_t)m * pmod(c - k, n)];
+    int L1 = m, L2 = n;
+    for (int l = 0; l < levels; ++l) {
+      L1s[l] = L1;
+      L2s[l] = L2;
+      std::vector<T> out[4];
+      wav2_analysis<T>(A, L1, L2, F, out);
+      for (int b = 1; b < 4; ++b) {
+        for (auto& x : out[b])
+          if (!(std::fabs((double)x) > tau)) x = 0;  // C.*(abs(C)>tau) (:44)
+        D[l * 3 + b - 1] = std::move(out[b]);
+      }
+      A = std::move(out[0]);
+      L1 /= 2;
+      L2 /= 2;
+    }
+    // Cup{levels+1} = approximation; Cup{l} = sum_j corr(up(C{l,j})) + corr_lo(up(Cup{l+1}))
+    std::vector<T> cup = std::move(A);
+    for (int l = levels - 1; l >= 0; --l) {
+      const int Lc1 = L1s[l] / 2, Lc2 = L2s[l] / 2, Lf1 = L1s[l], Lf2 = L2s[l];
+      std::vector<T> nxt((size_t)Lf1 * Lf2);
+#pragma omp parallel for schedule(static) if (Lf1 * Lf2 > 4096)
+      for (int t2 = 0; t2 < Lf2; ++t2)
+        for (int t1 = 0; t1 < Lf1; ++t1) {
+          T a = wav2_corr_up<T>(D[l * 3 + 0], Lc1, Lc2, F.psi[1], t1, t2);
+          a = a + wav2_corr_up<T>(D[l * 3 + 1], Lc1, Lc2, F.psi[2], t1, t2);
+          a = a + wav2_corr_up<T>(D[l * 3 + 2], Lc1, Lc2, F.psi[3], t1, t2);
+          a = a + wav2_corr_up<T>(cup, Lc1, Lc2, F.psi[0], t1, t2);
+          nxt[t1 + (size_t)Lf1 * t2] = a;
+        }
+      cup = std::move(nxt);
+    }
+    for (int c = 0; c < n; ++c)
+      for (int r = 0; r < m; ++r) {
+        T& o = Iavg[r + (size_t)m * c];
+        o = o + cup[pmod(r + k, m) + (size_t)m * pmod(c + k, n)] / (T)shifts;
+      }
+  }
+  return Iavg;
+}
+
+// Circular top-left-anchored d x d box sums of I^2 (blockNorms, matchBlocksGPU.m:58-60).
+template <class T>
+std::vector<T> circ_norm_map(const T* I, int m, int n, int d) {
+  std::vector<T> bn((size_t)m * n);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < n; ++c)
+    for (int r = 0; r < m; ++r) {
+      T acc = 0;
+      for (int v = 0; v < d; ++v)
+        for (int u = 0; u < d; ++u) {
+          const T y = I[pmod(r + u, m) + (size_t)m * pmod(c + v, n)];
+          acc = madd<T>(y, y, acc);
+        }
+      bn[r + (size_t)m * c] = acc;
+    }
+  return bn;
+}
+
+// S (numMax x floor(m/d) x floor(n/d)), 1-based column-major indices into
+// m x n; counts (floor(m/d) x floor(n/d)) entries of each tile's list, the
+// slots after them 0.  numMax == numMin: the numMax smallest distances
+// (matchBlocksGPU.m:69-70).  Otherwise the tolerance branch (:71-81): the
+// positions with distance < tol = 0.005 sigma^2 d^2 in index order, or the
+// numMax / numMin smallest when there are more / fewer of them.  Comparisons
+// between the single distance or coefficient and the double tol / tau are
+// exact (MATLAB compares the values, single(0.1) == 0.1 is false).
+// filtered (nullable): receives the image the search ran on.
+template <class T>
+int match_global(const T* I0, int m, int n, double sigma, int d, int numMax, int numMin, uint32_t* S,
+                 uint32_t* counts, T* filtered) {
+  if (m < d || n < d || d < 1 || numMax < 1 || numMin < 1 || numMin > numMax || (size_t)numMax > (size_t)m * n)
+    return GBM3D_ERR_ARG;
+  std::vector<T> I(I0, I0 + (size_t)m * n);
+  if (sigma > 30.0 / 255.0) {
+    if (m % 8 || n % 8) return GBM3D_ERR_ARG;  // myWavRec2DGPUfast.m:27 size mismatch otherwise
+    I = wav2_prefilter<T>(I0, m, n, 2.0 * sigma);
+  }
+  if (filtered) std::copy(I.begin(), I.end(), filtered);
+  const double tol = 0.005 * sigma * sigma * d * d;
+  const std::vector<T> bn = circ_norm_map<T>(I.data(), m, n, d);
+  const int Mt = m / d, Nt = n / d;
+  const size_t mn = (size_t)m * n;
+#pragma omp parallel
+  {
+    std::vector<T> dist(mn);
+    std::vector<int> idx(mn);
+#pragma omp for collapse(2) schedule(dynamic)
+    for (int tj = 0; tj < Nt; ++tj)
+      for (int ti = 0; ti < Mt; ++ti) {
+        const T nx = bn[(size_t)ti * d + (size_t)m * tj * d];
+        size_t below = 0;
+        for (int pc = 0; pc < n; ++pc)
+          for (int pr = 0; pr < m; ++pr) {
+            T acc = 0;
+            for (int v = 0; v < d; ++v)
+              for (int u = 0; u < d; ++u)
+                acc = madd<T>(I[(ti * d + u) + (size_t)m * (tj * d + v)],
+                              I[pmod(pr + u, m) + (size_t)m * pmod(pc + v, n)], acc);
+            const size_t e = pr + (size_t)m * pc;
+            dist[e] = ((nx + bn[e]) - T(2) * acc) / nx;
+            below += (double)dist[e] < tol;
+          }
+        for (size_t e = 0; e < mn; ++e) idx[e] = (int)e;
+        std::partial_sort(idx.begin(), idx.begin() + numMax, idx.end(),
+                          [&](int a, int b) { return dist_less<T>(dist[a], a, dist[b], b); });
+        int cnt = numMax;
+        if (numMax != numMin) {
+          if (below < (size_t)numMin)
+            cnt = numMin;
+          else if (below <= (size_t)numMax) {
+            cnt = (int)below;  // find(): index order
+            std::sort(idx.begin(), idx.begin() + cnt);
+          }
+        }
+        uint32_t* out = S + (size_t)numMax * (ti + (size_t)Mt * tj);
+        for (int k = 0; k < numMax; ++k) out[k] = k < cnt ? (uint32_t)idx[k] + 1u : 0u;
+        if (counts) counts[ti + (size_t)Mt * tj] = (uint32_t)cnt;
+      }
+  }
+  return 0;
+}
 
 // ------------------------------------------------------ distributed driver
 // getBM3D_distribution (utilities/getBM3D_distribution.m:10-48): one case per

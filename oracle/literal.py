@@ -512,3 +512,102 @@ def GBM3D_distributed(I, sigma, opts):
         Uf[RW[ii, 0] - 1:RW[ii, 1], RW[ii, 2] - 1:RW[ii, 3]] += U1[ii] * tw
         norm[RW[ii, 0] - 1:RW[ii, 1], RW[ii, 2] - 1:RW[ii, 3]] += tw * W[ii]
     return Uf / norm, U2
+
+
+# ------------------------------------------------ whole-image matching (2-D)
+def getWaveFilters2D(Lo_d, Hi_d):
+    """waveTransforms2D/getWaveFilters2D.m:12-16 -> Psi (N, N, 4); the taps
+    are the data file's Lo_d, Hi_d (row vectors, so Lo_d'*Hi_d is an outer
+    product with the first factor along dimension 1)."""
+    Lo, Hi = np.asarray(Lo_d, float), np.asarray(Hi_d, float)
+    Psi = np.zeros((Lo.size, Lo.size, 4))
+    Psi[:, :, 0] = np.outer(Lo, Lo)
+    Psi[:, :, 1] = np.outer(Hi, Lo)
+    Psi[:, :, 2] = np.outer(Lo, Hi)
+    Psi[:, :, 3] = np.outer(Hi, Hi)
+    return Psi
+
+
+def myWavDec2DFFT(Psi, levels, U):
+    """waveTransforms2D/myWavDec2DFFT.m:10-23; C[i][j] 0-based (C{i,j})."""
+    C = [[None] * 3 for _ in range(levels + 1)]
+    for i in range(levels):
+        FU = np.fft.fftn(U)
+        for j in range(3):
+            c = np.fft.ifftn(FU * np.fft.fftn(Psi[:, :, j + 1], s=U.shape, axes=(0, 1)))
+            C[i][j] = np.real(c[::2, ::2])
+        U = np.real(np.fft.ifftn(FU * np.fft.fftn(Psi[:, :, 0], s=U.shape, axes=(0, 1))))
+        U = U[::2, ::2]
+    C[levels][0] = U
+    return C
+
+
+def myWavRec2DGPUfast(Psi, levels, C):
+    """waveTransforms2D/myWavRec2DGPUfast.m:10-34."""
+    Cup = [None] * (levels + 1)
+    for i in range(levels):
+        Cup[i] = np.zeros((2 * C[i][0].shape[0], 2 * C[i][0].shape[1]))
+        tmp = Cup[i].copy()
+        for j in range(3):
+            tmp[::2, ::2] = C[i][j]
+            Cup[i] = Cup[i] + np.real(np.fft.ifftn(np.fft.fftn(tmp) *
+                                                   np.conj(np.fft.fftn(Psi[:, :, j + 1], s=tmp.shape, axes=(0, 1)))))
+    Cup[levels] = C[levels][0]
+    for i in range(levels, 0, -1):          # i = levels+1:-1:2
+        tmp = np.zeros(Cup[i - 1].shape)
+        tmp[::2, ::2] = Cup[i]
+        FPsi = np.conj(np.fft.fftn(Psi[:, :, 0], s=tmp.shape, axes=(0, 1)))
+        Cup[i - 1] = Cup[i - 1] + np.real(np.fft.ifftn(np.fft.fftn(tmp) * FPsi))
+    return Cup[0]
+
+
+def matchBlocksGPU_prefilter(I, sigma, Lo_d, Hi_d):
+    """source/matchBlocksGPU.m:34-50 (the sigma > 30/255 branch)."""
+    tau = 2 * sigma
+    Psi = getWaveFilters2D(Lo_d, Hi_d)
+    Iavg = np.zeros(I.shape)
+    shiftCnt, levels = 5, 3
+    for k in range(1, shiftCnt + 1):
+        C = myWavDec2DFFT(Psi, levels, np.roll(I, (k - 1, k - 1), (0, 1)))
+        for i in range(levels):
+            for j in range(3):
+                C[i][j] = C[i][j] * (np.abs(C[i][j]) > tau)
+        Iavg = Iavg + np.roll(myWavRec2DGPUfast(Psi, levels, C), (1 - k, 1 - k), (0, 1)) / shiftCnt
+    return Iavg
+
+
+def matchBlocksGPU(I, sigma, opts, Lo_d, Hi_d):
+    """source/matchBlocksGPU.m:18-91 -> (S as a list of lists of 1-based
+    uint32 index arrays, the distance maps, the image searched)."""
+    tol = .005 * sigma ** 2
+    d = opts["blockSize"]
+    m, n = I.shape
+    tol = tol * d ** 2
+    M, N = m // d, n // d
+    I = np.asarray(I, float)
+    if sigma > 30 / 255:
+        I = matchBlocksGPU_prefilter(I, sigma, Lo_d, Hi_d)
+    g = np.zeros((m, n))
+    g[:d, :d] = 1
+    blockNorms = np.real(np.roll(np.fft.ifft2(np.fft.fft2(g) * np.fft.fft2(I ** 2)), (-d + 1, -d + 1), (0, 1)))
+    FI = np.fft.fft2(I)
+    S = [[None] * N for _ in range(M)]
+    dists = {}
+    for i in range(M):
+        for j in range(N):
+            xR = I[i * d:(i + 1) * d, j * d:(j + 1) * d]
+            bd = np.real(np.fft.ifft2(np.conj(np.fft.fft2(xR, s=(m, n))) * FI))
+            nb = blockNorms[i * d, j * d]
+            bd = (nb + blockNorms - 2 * bd) / nb
+            dists[(i, j)] = bd
+            flat = bd.flatten(order="F")
+            if opts["numMax"] == opts["numMin"]:
+                s = _mink_idx(flat, opts["numMax"])
+            else:
+                s = np.flatnonzero(flat < tol) + 1
+                if s.size > opts["numMax"]:
+                    s = _mink_idx(flat, opts["numMax"])
+                elif s.size < opts["numMin"]:
+                    s = _mink_idx(flat, opts["numMin"])
+            S[i][j] = s.astype(np.uint32)
+    return S, dists, I

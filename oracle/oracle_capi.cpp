@@ -211,6 +211,39 @@ int orc_gbm3d_f32(const float* I, int m, int n, float sigma, const gbm3d_opts* r
 }
 
 
+void orc_sym4_taps(double* lo, double* hi) {
+  for (int k = 0; k < 8; ++k) {
+    lo[k] = kSym4Lo[k];
+    hi[k] = kSym4Hi[k];
+  }
+}
+
+#define PREFILT(SUF, T)                                                                              \
+  int orc_wav2_prefilter_##SUF(const T* I, int m, int n, double tau, T* out) {                       \
+    if (!I || !out || m % 8 || n % 8) return GBM3D_ERR_ARG;                                          \
+    return guarded([&] {                                                                             \
+      const std::vector<T> r = wav2_prefilter<T>(I, m, n, tau);                                      \
+      std::copy(r.begin(), r.end(), out);                                                            \
+      return 0;                                                                                      \
+    });                                                                                              \
+  }
+PREFILT(f64, double)
+PREFILT(f32, float)
+
+#define GMATCH(SUF, T)                                                                               \
+  int orc_match_global_##SUF(const T* I, int m, int n, double sigma, int d, int numMax, int numMin,     \
+                             uint32_t* S, uint32_t* counts, T* filtered) {                             \
+    if (!I || !S) return GBM3D_ERR_ARG;                                                              \
+    int st = GBM3D_OK;                                                                               \
+    const int g = guarded([&] {                                                                      \
+      st = match_global<T>(I, m, n, sigma, d, numMax, numMin, S, counts, filtered);                  \
+      return 0;                                                                                      \
+    });                                                                                              \
+    return g != 0 ? g : st;                                                                          \
+  }
+GMATCH(f64, double)
+GMATCH(f32, float)
+
 long orc_ht_near_mask_f32(const float* V, int m, int n, int K, int levels, int cycleSpin, const double* tau,
                           double tol, uint8_t* vmask) {
   if (!V || !tau || !vmask || levels < 1 || (m % (1 << levels)) || (n % (1 << levels)) || (K % (1 << levels)) ||

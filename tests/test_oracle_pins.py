@@ -420,3 +420,101 @@ def test_wiener_dct_twin_matches_literal(d, K):
     # without the clip it is WieFast
     V2w, _ = O.wiener(V1, V2, 25.0 / 255.0)
     np.testing.assert_allclose(np.minimum(V2w, 1.0), V2f, atol=1e-12)
+
+
+# ------------------------------------------------ matchBlocksGPU (whole image)
+def _sym4():
+    t = json.load(open(os.path.join(GOLD, "sym4_taps.json")))
+    return np.asarray(t["Lo_d"]), np.asarray(t["Hi_d"])
+
+
+def test_sym4_taps_match_reference_data_file_and_are_orthonormal():
+    """archive/waveletFilters/sym4Filters.mat (as JSON) against the oracle's
+    constants, and the closed forms of an orthonormal 8-tap filter pair:
+    sum Lo = sqrt(2), Lo orthonormal to its even shifts, Hi the alternating
+    flip of Lo (the data file's taps hold them to about 1e-12)."""
+    lo, hi = _sym4()
+    olo, ohi = O.sym4_taps()
+    np.testing.assert_array_equal(olo, lo)
+    np.testing.assert_array_equal(ohi, hi)
+    assert abs(lo.sum() - np.sqrt(2)) < 1e-11 and abs(hi.sum()) < 1e-11
+    for s in range(0, 8, 2):
+        assert abs(np.dot(lo[s:], lo[:8 - s]) - (s == 0)) < 1e-11
+        assert abs(np.dot(hi[s:], hi[:8 - s]) - (s == 0)) < 1e-11
+        assert abs(np.dot(lo[s:], hi[:8 - s])) < 1e-11 and abs(np.dot(hi[s:], lo[:8 - s])) < 1e-11
+    np.testing.assert_allclose(hi, [(-1) ** (k + 1) * lo[7 - k] for k in range(8)], atol=1e-15)
+
+
+@pytest.mark.parametrize("shape", [(32, 40), (64, 32), (32, 32)])
+def test_wav2_prefilter_without_threshold_reconstructs(shape):
+    """Analysis then synthesis is the identity (orthonormal filter bank), for
+    every shift, so tau below 0 returns the image.  (Below 32 pixels a level
+    is shorter than the filters, which fftn(Psi, size) cuts: no longer an
+    orthonormal bank; the literal pin below covers those sizes.)"""
+    _, x = synth.make_pair(31, *shape, 25.0)
+    assert np.abs(O.wav2_prefilter(x, -1.0) - x).max() < 1e-8
+    assert np.abs(O.wav2_prefilter(x.astype(np.float32), -1.0, precision=32) - x).max() < 1e-3
+
+
+@pytest.mark.parametrize("shape,tau", [((32, 40), 50.0), ((48, 16), 20.0), ((8, 24), 50.0), ((32, 32), np.inf)])
+def test_wav2_prefilter_matches_fft_literal(shape, tau):
+    """myWavDec2DFFT / myWavRec2DGPUfast / matchBlocksGPU.m:35-49 transcribed
+    with fftn (oracle.literal): shift directions, band order, thresholds."""
+    lo, hi = _sym4()
+    _, x = synth.make_pair(32, *shape, 25.0)
+    I = x.astype(np.float64)
+    ref = L.matchBlocksGPU_prefilter(I, tau / 2, lo, hi)
+    out = O.wav2_prefilter(I, tau)
+    assert np.abs(out - ref).max() < 1e-9
+    if np.isfinite(tau):
+        assert np.abs(out - I).max() > 1.0            # the threshold acts
+
+
+MG_CASES = [  # m, n, d, sigma, numMax, numMin, scale
+    (32, 48, 4, 25.0, 16, 16, 255.0),     # prefilter (sigma > 30/255)
+    (48, 40, 8, 0.05, 8, 8, 1.0),         # [0, 1] image, no prefilter
+    (40, 37, 8, 0.1, 16, 16, 1.0),        # not a block multiple, no prefilter
+    (48, 48, 8, 0.4, 16, 4, 255.0),       # tolerance branch: lists of 4..16
+    (40, 48, 4, 0.3, 16, 4, 255.0),
+    (64, 64, 16, 25.0, 8, 8, 255.0),
+    (64, 64, 32, 25.0, 8, 2, 255.0),
+]
+
+
+@pytest.mark.parametrize("case", MG_CASES)
+def test_match_global_matches_fft_literal(case):
+    """The oracle's whole-image search against oracle.literal.matchBlocksGPU
+    (fft2 correlations and norms, mink / find on the distance maps)."""
+    m, n, d, sigma, nmax, nmin, scale = case
+    lo, hi = _sym4()
+    _, x = synth.make_pair(33 + d, m, n, 10.0)
+    I = x.astype(np.float64) * (scale / 255.0)
+    opts = {"blockSize": d, "numMax": nmax, "numMin": nmin}
+    Sl, dl, Fl = L.matchBlocksGPU(I, sigma, opts, lo, hi)
+    S, cnt, F = O.match_global(I, sigma, d, nmax, nmin, want_filtered=True)
+    assert np.abs(F - Fl).max() < 1e-9
+    for i in range(m // d):
+        for j in range(n // d):
+            np.testing.assert_array_equal(S[:cnt[i, j], i, j], Sl[i][j])
+            assert not S[cnt[i, j]:, i, j].any()
+    if nmin < nmax and sigma < 1:                   # tol comparable to the distances
+        assert ((cnt > nmin) & (cnt < nmax)).any()    # the find() order is exercised
+        assert (cnt == nmin).any() or (cnt == nmax).any()
+
+
+def test_match_global_distances_equal_direct_differences():
+    """Brute force on a tiny image: |x - y|^2 / |x|^2 over every circular
+    position, ranked, against the oracle's lists (FP64)."""
+    rng = np.random.default_rng(34)
+    I = rng.uniform(0, 1, (12, 16))
+    d, K = 4, 10
+    S, _ = O.match_global(I, 0.05, d, K)
+    for ti in range(3):
+        for tj in range(4):
+            x = I[ti * d:(ti + 1) * d, tj * d:(tj + 1) * d]
+            dist = np.empty(I.shape)
+            for c in range(16):
+                for r in range(12):
+                    y = np.roll(I, (-r, -c), (0, 1))[:d, :d]
+                    dist[r, c] = ((x - y) ** 2).sum() / (x ** 2).sum()
+            np.testing.assert_array_equal(S[:, ti, tj], np.argsort(dist.ravel(order="F"), kind="stable")[:K] + 1)
