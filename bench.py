@@ -278,6 +278,17 @@ def batched_report(args):
         "bitwise_equal_to_plain": all(np.array_equal(a, b) for a, b in zip(U_b, U_plain))}))
 
 
+def learned_workload():
+    """64 synthetic 512x512 images, sigma ~ U[5, 50] -> (noisy, clean, sigmas)."""
+    from gbm3d_amd import synth
+
+    count, size = 64, 512
+    u = synth.uniform(0x5EED5, np.arange(count))
+    sig = [float(5 + 45 * x) for x in u]
+    pairs = [synth.make_pair(5000 + i, size, size, sig[i]) for i in range(count)]
+    return [p[1] for p in pairs], [p[0] for p in pairs], sig
+
+
 def learned_report(args):
     """64 synthetic 512x512 images, sigma ~ U[5,50]: one graph-captured batch
     launch with the learned shrinkage replacing the step-1 hard threshold, vs
@@ -288,11 +299,8 @@ def learned_report(args):
     from gbm3d_amd import device, shrink, synth
     from oracle import shrink_oracle as so
 
-    count, size = 64, 512
-    u = synth.uniform(0x5EED5, np.arange(count))
-    sig = [float(5 + 45 * x) for x in u]
-    pairs = [synth.make_pair(5000 + i, size, size, sig[i]) for i in range(count)]
-    cleans, imgs = [p[0] for p in pairs], [p[1] for p in pairs]
+    imgs, cleans, sig = learned_workload()
+    count, size = len(imgs), imgs[0].shape[0]
     mp = count * size * size / 1e6
     ctx = gbm3d_amd.Context(0)
     stream = device.Stream()
@@ -328,7 +336,7 @@ def learned_report(args):
              "psnr_pilot": [synth.psnr(c, x) for c, x in zip(cleans, U1)], "U0": U[0], "U10": U1[0]}
         if kw.get("learned"):
             st = [ctx.shrink_stats(i) for i in range(count)]
-            r["fell_back"] = int(sum(x[2] for x in st))
+            r["fell_back_spins"] = int(sum(x[2] for x in st))
             r["removed_energy_vs_noise"] = [round(x[0], 4) for x in st]
         res[name] = r
     ctx.set_variant()
@@ -356,7 +364,7 @@ def learned_report(args):
         "workload": "learned shrinkage: 64 synthetic 512x512 images, sigma U[5,50], 8x8 blocks, one graph launch",
         "images": count, "mpix": round(mp, 3),
         "plain_batched": {k: res["plain"][k] for k in ("ms", "mpix_s", "tuned")},
-        "learned_batched": {k: res["learned"][k] for k in ("ms", "mpix_s", "tuned", "fell_back")},
+        "learned_batched": {k: res["learned"][k] for k in ("ms", "mpix_s", "tuned", "fell_back_spins")},
         "speedup": round(res["plain"]["ms"] / res["learned"]["ms"], 3),
         "psnr_plain_mean_db": round(float(pp.mean()), 4), "psnr_learned_mean_db": round(float(pl.mean()), 4),
         "dpsnr_mean_db": round(float((pl - pp).mean()), 4), "dpsnr_min_db": round(float((pl - pp).min()), 4),
@@ -621,7 +629,7 @@ def main():
         if kw.get("learned"):
             el, eh, fell = ctx.shrink_stats(0)
             variants[name].update({"removed_energy_vs_noise": round(el, 4),
-                                   "hard_threshold_removed_energy_vs_noise": round(eh, 4), "fell_back": fell})
+                                   "hard_threshold_removed_energy_vs_noise": round(eh, 4), "fell_back_spins": fell})
             if "shrink" in rlv:
                 k = rlv["shrink"]
                 variants[name]["shrink_kernel"] = {"achieved_tflops": round(k["achieved"], 2), "peak": k["peak"],

@@ -130,10 +130,13 @@ class Context:
         rescoring of every candidate the bounds cannot exclude (first set of
         `rescore` candidates per tile, numMax..64; 0: 2*numMax); index lists
         equal the FP32 path's bit for bit.
-        learned: learned shrinkage replaces the step-1 hard threshold; an image
-        from which the learned gains remove more than learned_bound x its
-        noise energy is recomputed on the reference path.  Loads
-        `weights` (parameter vector) or the shipped trained weights."""
+        learned: learned shrinkage replaces the step-1 hard threshold.  Per
+        match spin a device-side judge compares the energy the learned gains
+        removed with learned_bound x the noise energy; above it the spin's
+        reference filter runs instead (gated launches, no host round trip), so
+        an image whose spins all fall back equals the reference path bit for
+        bit.  Loads `weights` (parameter vector) or the shipped trained
+        weights."""
         if learned:
             from . import shrink
 
@@ -162,10 +165,12 @@ class Context:
 
     def shrink_stats(self, image: int = 0):
         """(energy removed by the learned gains, by a 2.7 sigma hard threshold,
-        both / noise energy; fell back) of image `image` of the last call."""
+        both / noise energy and averaged over the match spins; the number of
+        spins that fell back) of image `image` of the last learned-form call
+        (waits for the context stream)."""
         a, b, f = C.c_double(), C.c_double(), C.c_int()
         check(self.lib.gbm3d_shrink_stats(self.handle, image, C.byref(a), C.byref(b), C.byref(f)))
-        return a.value, b.value, bool(f.value)
+        return a.value, b.value, int(f.value)
 
     def shrink_groups(self, V, sigma: float, want_gains: bool = False):
         """Fused learned shrinkage of groups V (8, 8, 16, ...) -> (Vout, gains (G, 64, 16) or None)."""

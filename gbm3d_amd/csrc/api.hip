@@ -90,7 +90,7 @@ int gbm3d_destroy(gbm3d_ctx* c) {
   if (c->shrink_wb) (void)hipFree(c->shrink_wb);
   if (c->bf16_stats) (void)hipFree(c->bf16_stats);
   if (c->shrink_fb) (void)hipFree(c->shrink_fb);
-  if (c->sstats) (void)hipFree(c->sstats);
+  if (c->lslots) (void)hipFree(c->lslots);
   for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->dexec) (void)hipGraphExecDestroy(c->dexec);

@@ -60,8 +60,9 @@ int issue_batch(gbm3d_ctx* c, const BatchJob& j, const gbm3d_opts& o, int S, int
     if (hipStreamWaitEvent(c->bstreams[s], c->bfork, 0) != hipSuccess) st = fail(GBM3D_ERR_HIP, "batch fork wait");
   for (int i = 0; i < j.count && st == GBM3D_OK; ++i) {
     const int s = assign[i];
+    const LearnedSlot slot = c->ext.learned ? learned_slot(c, i) : LearnedSlot{};
     st = run_pipeline_on(c, j.dI[i], j.plans[i], j.sigma[i], o, j.dU[i], j.dU1 ? j.dU1[i] : nullptr, c->bstreams[s],
-                         c->arena + off[s], image_stats(c, i));
+                         c->arena + off[s], c->ext.learned ? &slot : nullptr);
   }
   for (int s = 0; s < S; ++s) {
     // join every branch even after an error so the capture stays well formed
@@ -118,12 +119,7 @@ int gbm3d_denoise_batch_device(gbm3d_ctx* c, int count, const float* const* dI, 
     if (!dI[i] || !dU[i]) return fail(GBM3D_ERR_ARG, "batch: NULL image " + std::to_string(i));
     GBM3D_TRY(make_plan(o, m[i], n[i], &j.plans[i]));
   }
-  GBM3D_TRY(learned_setup(c, o, count));
-  if (c->ext.learned) {
-    // learned form: run, then judge every image against the bound
-    GBM3D_TRY(batch_run(c, j, o, flags));
-    return learned_judge(c, o, count, j.plans, dI, sigma, dU, dU1);
-  }
+  GBM3D_TRY(learned_setup(c, o, j.plans));
   return batch_run(c, j, o, flags);
 }
 
@@ -184,7 +180,7 @@ int batch_run(gbm3d_ctx* c, BatchJob& j, const gbm3d_opts& o, int flags) {
   if (!(flags & GBM3D_BATCH_GRAPH)) return issue_batch(c, j, o, S, mw);
   std::vector<int64_t> key = batch_key(j, o, c->ext, S, mw);
   key.push_back((int64_t)(intptr_t)c->arena);  // a grown arena invalidates the captured addresses
-  key.push_back((int64_t)(intptr_t)c->sstats);
+  key.push_back((int64_t)(intptr_t)c->lslots);
   if (!c->gexec || key != c->gkey) {
     if (c->gexec) {
       (void)hipGraphExecDestroy(c->gexec);

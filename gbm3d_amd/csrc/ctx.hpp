@@ -49,10 +49,9 @@ struct gbm3d_ctx {
   // learned shrinkage
   uint16_t* shrink_wb = nullptr;  // BF16 weights [4][64][192]
   float* shrink_fb = nullptr;     // FP32 biases and sigma-channel weights
-  float* sstats = nullptr;        // per image and spin: [kShrinkGrid][2] residual partials
-  size_t sstats_floats = 0;
-  std::vector<double> s_learned, s_ht;  // last call, per image
-  std::vector<int> s_fell;
+  char* lslots = nullptr;         // per image: gbm3d::LearnedSlot's device arrays
+  size_t lslots_images = 0;
+  std::vector<int> l_spins;       // last call: match spins per image
   struct Mark {
     int stage;
     hipEvent_t a, b;
@@ -150,17 +149,26 @@ int on_lanes(gbm3d_ctx* c, int count, int nl, hipStream_t st, F&& body) {
 // One image through both steps on stream st, workspace at ws
 // (pipeline_bytes(p, o, lanes)); with lanes > 1 the match spins of each step
 // run on the context's lane streams.
+// The learned form's device state of one image: per match spin the shrink
+// kernel's residual partials (stats[spin][kShrinkGrid][2]), the gate of the
+// reference filter (run[spin], 1 = fall back) and the judge's figures
+// (res[spin][3]: removed / noise energy of the learned gains and of a 2.7
+// sigma hard threshold, the gate).
+struct LearnedSlot {
+  float* stats;
+  int* run;
+  double* res;
+};
 int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, const gbm3d_opts& o, float* dU,
-                    float* dU1, hipStream_t st, char* ws, float* sstats, int lanes = 1);
+                    float* dU1, hipStream_t st, char* ws, const LearnedSlot* ls, int lanes = 1);
 // One image on the context stream with the context arena.
 int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const gbm3d_opts& o, float* dU,
                  float* dU1);
 
-// learned.hip: the learned form's prerequisites and its per-image judge.
-int learned_setup(gbm3d_ctx* c, const gbm3d_opts& o, int images);
-float* image_stats(gbm3d_ctx* c, int i);
-int learned_judge(gbm3d_ctx* c, const gbm3d_opts& o, int count, const std::vector<Plan>& plans,
-                  const float* const* dI, const float* sigma, float* const* dU, float* const* dU1);
+// learned.hip: the learned form's prerequisites and device state.  setup
+// records each image's match spin count for gbm3d_shrink_stats.
+int learned_setup(gbm3d_ctx* c, const gbm3d_opts& o, const std::vector<Plan>& plans);
+LearnedSlot learned_slot(gbm3d_ctx* c, int image);
 
 // noalloc.hip: the device address of this device's scratch array, looked up
 // once per device (the device must be current).

@@ -103,6 +103,8 @@ int launch_match_bf16(const float* pad, const float* bn, const float* smap, int 
 int launch_match_w(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
                    int nw, hipStream_t st);
 // Gather V (d x d x K x G) from a padded image through S.
+int launch_mosaic_gather(const float* pad, const uint32_t* S, int M, int N, int d, int d2, int K, float* V,
+                         const int* run, hipStream_t st);
 int launch_gather_groups(const float* pad, const uint32_t* S, int M, int N, int d, int d2, int K,
                          float* V, hipStream_t st);
 
@@ -120,8 +122,10 @@ typedef void (*StageHook)(void* user, int stage, int begin);
 // Writes Vavg = max(mean over cycle spins of the rebuilt volume, 0), in the
 // mosaic layout (gd == 0) or group-contiguous with gd x gd blocks
 // (Vavg[u + gd*(v + gd*(k + K*g))], the aggregation's layout).
+// run (nullable, device): when it holds 0 every kernel returns at once.
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau,
-           const HtWork& w, float* vavg, int gd, hipStream_t st, StageHook hook = nullptr, void* user = nullptr);
+           const HtWork& w, float* vavg, int gd, hipStream_t st, StageHook hook = nullptr, void* user = nullptr,
+           const int* run = nullptr);
 
 // The 'median' step-1 filter: Vavg = medfilt3(vol, [3 3 3]) (edge values
 // repeated), mosaic (gd == 0) or group-contiguous output.
@@ -178,6 +182,8 @@ void shrink_pack_weights(const float* params, uint16_t* wb, float* fb);
 int launch_shrink(const float* src, const uint32_t* S, int M, int d2, long G, float sigma, const uint16_t* wb,
                   const float* fb, float* vout, float* gout, float* stats, int* grid_used, hipStream_t st);
 // Training rows of the selected groups (noisy features, oracle Wiener gains, loss weights).
+int launch_learned_judge(const float* stats, int grid, double noise, double bound, int* run, double* res,
+                         hipStream_t st);
 int launch_shrink_features(const float* padC, const float* padN, const uint32_t* S, int M, int d2,
                            const uint32_t* sel, int nsel, float sigma, int target, float* X, float* Tg, float* Wt,
                            hipStream_t st);

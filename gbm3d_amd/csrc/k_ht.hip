@@ -61,7 +61,8 @@ constexpr int LR = 2 * LT + 8;   // input rows/cols per tile
 constexpr int LRP = LR + 1;
 
 __global__ __launch_bounds__(256) void ht_lo_kernel(const float* __restrict__ X, int p, int q, int r, int sh,
-                                                    float* __restrict__ Y) {
+                                                    float* __restrict__ Y, const int* __restrict__ run) {
+  if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   __shared__ float Z[LR * LRP];   // z-lowpass plane, [col][row]
   __shared__ float T2[LT * LRP];  // after dim-2 filtering, [t2][row]
   const int t1b = blockIdx.x * LT, t2b = blockIdx.y * LT, t3 = blockIdx.z;
@@ -139,7 +140,8 @@ __device__ __forceinline__ void put(float* Y, const float* acc, int pr, int pc, 
 
 __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X, const float* __restrict__ Ylo,
                                                      int p, int q, int r, int sh, Tau7 tau,
-                                                     float* Y, const float* acc, int mode, float cs, int gd) {
+                                                     float* Y, const float* acc, int mode, float cs, int gd, const int* __restrict__ run) {
+  if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   __shared__ float Z[2][SR * SRP];   // z-lo / z-hi planes, [col][row]
   __shared__ float A[2][CP * SRP];   // dim-2 lo / hi, [c][row]
   __shared__ float B[4][CP * CPP];   // bands b1 | b2<<1, [c][t]
@@ -310,7 +312,8 @@ static_assert(4 * C * ES <= REG1, "E fits in A");
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_syn_fast_kernel(
     const float* __restrict__ X, const float* __restrict__ Ylo, int p, int q, int r, int sh, Tau7 tau, float* Y,
     const float* acc, int mode, float cs, int gd, int zpb, int nsp, const float* __restrict__ X2,
-    const float* __restrict__ Ylo2, float* Y2) {
+    const float* __restrict__ Ylo2, float* Y2, const int* __restrict__ run) {
+  if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   using namespace fs;
   __shared__ float lds[REG0 + REG1];
   __shared__ int ridx[R], cidx[R], yrow[C];
@@ -546,7 +549,8 @@ static_assert(4 * C * ES <= REG1, "E fits in A");
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ht_syn0_cs2_kernel(const float* __restrict__ X, const float* __restrict__ Ylo0,
                                                           const float* __restrict__ Ylo1, int p, int q, int r,
-                                                          Tau7 tau, float* __restrict__ Y, int gd) {
+                                                          Tau7 tau, float* __restrict__ Y, int gd, const int* __restrict__ run) {
+  if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   using namespace f2;
   __shared__ float lds[REG0 + REG1];
   __shared__ int ridx[W], cidxp[W];  // window rows, p x window columns (circular)
@@ -797,7 +801,8 @@ constexpr int P = 32, R = 2 * P + 8, ZS = R + 1, OS = P + 1;
 // nsp = 2: both cycle spins of a coarse level in one launch (see ht_syn_fast_kernel).
 __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict__ X, int p, int q, int r, int sh,
                                                          float* __restrict__ Y, int nsp, const float* __restrict__ X2,
-                                                         float* __restrict__ Y2) {
+                                                         float* __restrict__ Y2, const int* __restrict__ run) {
+  if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   using namespace fl;
   __shared__ float Z[R * ZS];   // z-lowpass plane [col][row]; later the output tile [c][t]
   __shared__ float T2[P * ZS];  // dim-2 lowpass [c][row]
@@ -882,7 +887,8 @@ constexpr int NIT = (U * U + 255) / 256;
 }  // namespace fl2
 
 __global__ __launch_bounds__(256) void ht_lo0_cs2_kernel(const float* __restrict__ X, int p, int q, int r,
-                                                         float* __restrict__ Y0, float* __restrict__ Y1) {
+                                                         float* __restrict__ Y0, float* __restrict__ Y1, const int* __restrict__ run) {
+  if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   using namespace fl2;
   __shared__ float Z[U * ZS];    // z-lowpass plane [col][row] of the union window
   __shared__ float T2[P * ZS];   // dim-2 lowpass [c][row]
@@ -990,7 +996,7 @@ void ht_work_bind(float* base, int M, int N, int K, int levels, int cycleSpin, H
 }
 
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau, const HtWork& w,
-           float* vavg, int gd, hipStream_t st, StageHook hook, void* user) {
+           float* vavg, int gd, hipStream_t st, StageHook hook, void* user, const int* run) {
   if (gd != 0 && (M % gd || N % gd)) return fail(GBM3D_ERR_ARG, "ht: group layout block size");
   auto mark = [&](int stage, int begin) {
     if (hook) hook(user, stage, begin);
@@ -1015,18 +1021,18 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
     // both spins in one launch each (no shift below level 0)
     mark(GBM3D_STAGE_HT_LO, 1);
     const int nb = ((M / 2 + fl::P - 1) / fl::P) * ((N / 2 + fl::P - 1) / fl::P) * (K / 2);
-    hipLaunchKernelGGL(ht_lo0_cs2_kernel, dim3(nb), dim3(256), 0, st, vol, M, N, K, w.lo[0][0], w.lo[0][1]);
+    hipLaunchKernelGGL(ht_lo0_cs2_kernel, dim3(nb), dim3(256), 0, st, vol, M, N, K, w.lo[0][0], w.lo[0][1], run);
     GBM3D_TRY(check_launch("ht_lo0_cs2"));
     for (int l = 1; l < levels; ++l) {
       const int p = M >> l, q = N >> l, r = K >> l;
       dim3 grid((p / 2 + LT - 1) / LT, (q / 2 + LT - 1) / LT, 2 * (r / 2));
       if (p >= 10 && q >= 10) {
         hipLaunchKernelGGL(ht_lo_fast_kernel, grid, dim3(256), 0, st, w.lo[l - 1][0], p, q, r, 0, w.lo[l][0], 2,
-                           w.lo[l - 1][1], w.lo[l][1]);
+                           w.lo[l - 1][1], w.lo[l][1], run);
       } else {
         for (int s = 0; s < 2; ++s) {
           dim3 g1(grid.x, grid.y, r / 2);
-          hipLaunchKernelGGL(ht_lo_kernel, g1, dim3(256), 0, st, w.lo[l - 1][s], p, q, r, 0, w.lo[l][s]);
+          hipLaunchKernelGGL(ht_lo_kernel, g1, dim3(256), 0, st, w.lo[l - 1][s], p, q, r, 0, w.lo[l][s], run);
         }
       }
       GBM3D_TRY(check_launch("ht_lo"));
@@ -1041,12 +1047,12 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
         const int zpb = kSynPairsPerBlock, nzc = (r / 2 + zpb - 1) / zpb;
         dim3 gf((p + fs::P - 1) / fs::P, (q + fs::P - 1) / fs::P, 2 * nzc);
         hipLaunchKernelGGL(ht_syn_fast_kernel, gf, dim3(256), 0, st, w.lo[l - 1][0], ylo0, p, q, r, 0, tl[l],
-                           w.rec[l - 1][0], nullptr, kStore, 2.f, 0, zpb, 2, w.lo[l - 1][1], ylo1, w.rec[l - 1][1]);
+                           w.rec[l - 1][0], nullptr, kStore, 2.f, 0, zpb, 2, w.lo[l - 1][1], ylo1, w.rec[l - 1][1], run);
       } else {
         dim3 grid((p + SP - 1) / SP, (q + SP - 1) / SP, r / 2);
         for (int s = 0; s < 2; ++s)
           hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, w.lo[l - 1][s], s ? ylo1 : ylo0, p, q, r, 0,
-                             tl[l], w.rec[l - 1][s], nullptr, kStore, 2.f, 0);
+                             tl[l], w.rec[l - 1][s], nullptr, kStore, 2.f, 0, run);
       }
       GBM3D_TRY(check_launch("ht_syn"));
       mark(GBM3D_STAGE_HT_SYN, 0);
@@ -1061,9 +1067,9 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
       dim3 grid((p / 2 + LT - 1) / LT, (q / 2 + LT - 1) / LT, r / 2);
       if (p >= 10 && q >= 10 && !g_ht_generic)
         hipLaunchKernelGGL(ht_lo_fast_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s], 1,
-                           nullptr, nullptr);
+                           nullptr, nullptr, run);
       else
-        hipLaunchKernelGGL(ht_lo_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s]);
+        hipLaunchKernelGGL(ht_lo_kernel, grid, dim3(256), 0, st, in, p, q, r, l == 0 ? s : 0, w.lo[l][s], run);
       GBM3D_TRY(check_launch("ht_lo"));
     }
     mark(GBM3D_STAGE_HT_LO, 0);
@@ -1087,10 +1093,10 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
         const int zpb = kSynPairsPerBlock;
         dim3 gf((p + fs::P - 1) / fs::P, (q + fs::P - 1) / fs::P, (r / 2 + zpb - 1) / zpb);
         hipLaunchKernelGGL(ht_syn_fast_kernel, gf, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out,
-                           acc, mode, cs, lgd, zpb, 1, nullptr, nullptr, nullptr);
+                           acc, mode, cs, lgd, zpb, 1, nullptr, nullptr, nullptr, run);
       } else
         hipLaunchKernelGGL(ht_syn_kernel, grid, dim3(256), 0, st, in, ylo, p, q, r, l == 0 ? s : 0, tl[l], out, acc,
-                           mode, cs, lgd);
+                           mode, cs, lgd, run);
       GBM3D_TRY(check_launch("ht_syn"));
       mark(stage, 0);
     }
@@ -1100,7 +1106,7 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
     const float* y1 = levels == 1 ? w.lo[0][1] : w.rec[0][1];
     dim3 grid((M + f2::P - 1) / f2::P, (N + f2::P - 1) / f2::P, 1);
     mark(GBM3D_STAGE_HT_SYN0, 1);
-    hipLaunchKernelGGL(ht_syn0_cs2_kernel, grid, dim3(256), 0, st, vol, y0, y1, M, N, K, tl[0], vavg, gd);
+    hipLaunchKernelGGL(ht_syn0_cs2_kernel, grid, dim3(256), 0, st, vol, y0, y1, M, N, K, tl[0], vavg, gd, run);
     GBM3D_TRY(check_launch("ht_syn0_cs2"));
     mark(GBM3D_STAGE_HT_SYN0, 0);
   }

@@ -295,6 +295,45 @@ int launch_shrink(const float* src, const uint32_t* S, int M, int d2, long G, fl
   return check_launch("shrink");
 }
 
+// ------------------------------------------------------------ the judge
+// One block after the shrink kernel of one match spin: r = (energy the
+// learned gains removed) / noise and r_ht, the same for the 2.7 sigma hard
+// threshold, each the sum of the kernel's per-block partials in a fixed
+// order; run[0] = 1 (fall back) when !(r <= bound), else 0; res = {r, r_ht,
+// run}.  The gated launches that follow read run on the device: no host
+// round trip.
+__global__ __launch_bounds__(256) void learned_judge_kernel(const float* __restrict__ stats, int grid, double noise,
+                                                            double bound, int* __restrict__ run,
+                                                            double* __restrict__ res) {
+  __shared__ double sa[256], sb[256];
+  const int t = threadIdx.x;
+  sa[t] = t < grid ? (double)stats[2 * t] : 0.0;
+  sb[t] = t < grid ? (double)stats[2 * t + 1] : 0.0;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      sa[t] += sa[t + o];
+      sb[t] += sb[t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double r = sa[0] / noise, rh = sb[0] / noise;
+    const int f = (r <= bound) ? 0 : 1;
+    run[0] = f;
+    res[0] = r;
+    res[1] = rh;
+    res[2] = (double)f;
+  }
+}
+
+int launch_learned_judge(const float* stats, int grid, double noise, double bound, int* run, double* res,
+                         hipStream_t st) {
+  if (grid < 0 || grid > kShrinkGrid) return fail(GBM3D_ERR_ARG, "learned judge: grid");
+  hipLaunchKernelGGL(learned_judge_kernel, dim3(1), dim3(256), 0, st, stats, grid, noise, bound, run, res);
+  return check_launch("learned_judge");
+}
+
 // ------------------------------------------------------ training features
 // One wave per selected group: coefficients of the noisy and clean groups at
 // the same matched positions; X = c_noisy / sigma, T = g*, Wt = loss weight.

@@ -872,6 +872,31 @@ __global__ void gather_groups_kernel(const float* __restrict__ pad, const uint32
   V[e] = pad[(size_t)(ti * d + a + u) + (size_t)P * (tj * d + b + v)];
 }
 
+// Volume(ti d + u, tj d + v, k) = Ipad(ti d + a_k + u, tj d + b_k + v): the
+// mosaic volume the matcher writes (matchBlocksCPU.m:105-111), rebuilt from
+// S and the search image; only when *run != 0 (a gated launch).
+__global__ void mosaic_gather_kernel(const float* __restrict__ pad, const uint32_t* __restrict__ S, int M, int N,
+                                     int d, int d2, int K, float* __restrict__ V, const int* __restrict__ run) {
+  if (*run == 0) return;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t MN = (size_t)M * N;
+  if (e >= MN * K) return;
+  const int r = (int)(e % M), c = (int)((e / M) % N), k = (int)(e / MN);
+  const int ti = r / d, tj = c / d;
+  const uint32_t idx = S[k + (size_t)K * (ti + (size_t)(M / d) * tj)] - 1u;
+  const int a = (int)(idx % d2), b = (int)(idx / d2);
+  V[e] = pad[(size_t)(ti * d + a + r % d) + (size_t)(M + d2) * (tj * d + b + c % d)];
+}
+
+int launch_mosaic_gather(const float* pad, const uint32_t* S, int M, int N, int d, int d2, int K, float* V,
+                         const int* run, hipStream_t st) {
+  if (!run || M % d || N % d) return fail(GBM3D_ERR_ARG, "mosaic gather: geometry");
+  const size_t total = (size_t)M * N * K;
+  hipLaunchKernelGGL(mosaic_gather_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, pad, S, M, N, d,
+                     d2, K, V, run);
+  return check_launch("mosaic_gather");
+}
+
 int launch_gather_groups(const float* pad, const uint32_t* S, int M, int N, int d, int d2, int K, float* V,
                          hipStream_t st) {
   const size_t total = (size_t)d * d * K * (M / d) * (N / d);

@@ -1,6 +1,7 @@
 // learned.hip -- the learned-shrinkage form (a small BF16 conv network on
 // the DCT-Haar coefficients of each step-1 group replaces the wavelet hard
-// threshold), its per-image fallback judge, and the entry points of its
+// threshold), its per-image device state (residual partials, the per-spin
+// fallback gate the judge kernel sets), and the entry points of its
 // training (features, forward, gradient, Adam) and of the synthetic image
 // generator.  No reference counterpart; the gain targets follow
 // source/denoiseAndAggregateWieFast.m:49.
@@ -13,73 +14,41 @@ using namespace gbm3d;
 
 namespace gbm3d {
 
-// The learned form needs loaded weights, 8 x 8 x 16 step-1 groups and a
-// residual buffer of [images][spins][kShrinkGrid][2].
-int learned_setup(gbm3d_ctx* c, const gbm3d_opts& o, int images) {
+namespace {
+
+constexpr size_t kSlotStats = (size_t)GBM3D_MAX_SPINS * 2 * kShrinkGrid * sizeof(float);
+constexpr size_t kSlotRun = ((GBM3D_MAX_SPINS * sizeof(int) + 255) / 256) * 256;
+constexpr size_t kSlotRes = (size_t)GBM3D_MAX_SPINS * 3 * sizeof(double);
+constexpr size_t kSlotBytes = kSlotStats + kSlotRun + kSlotRes;
+
+}  // namespace
+
+// The learned form needs loaded weights, 8 x 8 x 16 step-1 groups, the hard
+// threshold filter (its fallback) and one LearnedSlot per image.
+int learned_setup(gbm3d_ctx* c, const gbm3d_opts& o, const std::vector<Plan>& plans) {
   if (!c->ext.learned) return GBM3D_OK;
   if (!c->shrink_wb) return fail(GBM3D_ERR_ARG, "learned shrinkage: no weights loaded (gbm3d_shrink_set_weights)");
   if (o.blockSize != 8 || o.numMax != 16)
     return fail(GBM3D_ERR_UNSUPPORTED, "learned shrinkage needs blockSize 8 and numMax 16");
-  const size_t need = (size_t)images * GBM3D_MAX_SPINS * 2 * kShrinkGrid;
-  if (need > c->sstats_floats) {
-    if (c->sstats) GBM3D_HIP(hipFree(c->sstats));
-    c->sstats = nullptr;
-    c->sstats_floats = 0;
-    GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->sstats), need * sizeof(float)));
-    c->sstats_floats = need;
+  if (o.filtType != GBM3D_FILT_HT)
+    return fail(GBM3D_ERR_UNSUPPORTED, "learned shrinkage falls back to the 'ht' filter: filtType must be 'ht'");
+  const size_t images = plans.size();
+  if (images > c->lslots_images) {
+    if (c->lslots) GBM3D_HIP(hipFree(c->lslots));
+    c->lslots = nullptr;
+    c->lslots_images = 0;
+    GBM3D_HIP(hipMalloc(reinterpret_cast<void**>(&c->lslots), images * kSlotBytes));
+    c->lslots_images = images;
   }
+  c->l_spins.resize(images);
+  for (size_t i = 0; i < images; ++i) c->l_spins[i] = plans[i].n1;
   return GBM3D_OK;
 }
 
-float* image_stats(gbm3d_ctx* c, int i) {
-  return c->ext.learned ? c->sstats + (size_t)i * GBM3D_MAX_SPINS * 2 * kShrinkGrid : nullptr;
-}
-
-static int shrink_grid_of(const Plan& p) { return shrink_grid((long)(p.M / p.bs) * (p.N / p.bs)); }
-
-// After a learned-form call has finished: per image, the energy the learned
-// gains removed from the noisy coefficients relative to the noise energy
-// (coefficients x sigma^2, all spins).  Removing more than `bound` times the
-// noise energy means the shrinkage also removed signal (the discrepancy
-// principle); such images are recomputed on the reference path (eagerly, on
-// the context stream).
-int learned_judge(gbm3d_ctx* c, const gbm3d_opts& o, int count, const std::vector<Plan>& plans,
-                  const float* const* dI, const float* sigma, float* const* dU, float* const* dU1) {
-  GBM3D_HIP(hipStreamSynchronize(c->stream));
-  const size_t per = (size_t)GBM3D_MAX_SPINS * 2 * kShrinkGrid;
-  std::vector<float> h((size_t)count * per);
-  GBM3D_HIP(hipMemcpy(h.data(), c->sstats, h.size() * sizeof(float), hipMemcpyDeviceToHost));
-  const double bound = (c->ext.learned_bound > 0 ? c->ext.learned_bound : 1000) / 1000.0;
-  c->s_learned.assign(count, 0.0);
-  c->s_ht.assign(count, 0.0);
-  c->s_fell.assign(count, 0);
-  std::vector<int> redo;
-  for (int i = 0; i < count; ++i) {
-    const int grid = shrink_grid_of(plans[i]);
-    for (int sp = 0; sp < plans[i].n1; ++sp)
-      for (int b = 0; b < grid; ++b) {
-        c->s_learned[i] += h[i * per + (size_t)2 * kShrinkGrid * sp + 2 * b];
-        c->s_ht[i] += h[i * per + (size_t)2 * kShrinkGrid * sp + 2 * b + 1];
-      }
-    const double noise = (double)plans[i].n1 * plans[i].M * plans[i].N * plans[i].K * (double)sigma[i] * sigma[i];
-    c->s_learned[i] /= noise;
-    c->s_ht[i] /= noise;
-    if (!(c->s_learned[i] <= bound)) redo.push_back(i);
-  }
-  if (redo.empty()) return GBM3D_OK;
-  const gbm3d_ext keep = c->ext;
-  c->ext.learned = 0;
-  int st = GBM3D_OK;
-  for (int i : redo) {
-    c->s_fell[i] = 1;
-    st = ensure(&c->arena, &c->arena_bytes, pipeline_bytes(plans[i], o));
-    if (st == GBM3D_OK)
-      st = run_pipeline_on(c, dI[i], plans[i], sigma[i], o, dU[i], dU1 ? dU1[i] : nullptr, c->stream, c->arena,
-                           nullptr);
-    if (st != GBM3D_OK) break;
-  }
-  c->ext = keep;
-  return st;
+LearnedSlot learned_slot(gbm3d_ctx* c, int image) {
+  char* base = c->lslots + (size_t)image * kSlotBytes;
+  return {reinterpret_cast<float*>(base), reinterpret_cast<int*>(base + kSlotStats),
+          reinterpret_cast<double*>(base + kSlotStats + kSlotRun)};
 }
 
 }  // namespace gbm3d
@@ -105,10 +74,21 @@ int gbm3d_shrink_set_weights(gbm3d_ctx* c, const float* params, int count) {
 
 int gbm3d_shrink_stats(gbm3d_ctx* c, int image, double* est_learned, double* est_ht, int* fell_back) {
   GBM3D_TRY(set_dev(c));
-  if (image < 0 || image >= (int)c->s_learned.size()) return fail(GBM3D_ERR_ARG, "shrink stats: no such image");
-  if (est_learned) *est_learned = c->s_learned[image];
-  if (est_ht) *est_ht = c->s_ht[image];
-  if (fell_back) *fell_back = c->s_fell[image];
+  if (image < 0 || image >= (int)c->l_spins.size()) return fail(GBM3D_ERR_ARG, "shrink stats: no such image");
+  const int n1 = c->l_spins[image];
+  double h[GBM3D_MAX_SPINS * 3];
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  GBM3D_HIP(hipMemcpy(h, learned_slot(c, image).res, sizeof(double) * 3 * n1, hipMemcpyDeviceToHost));
+  double a = 0.0, b = 0.0;
+  int f = 0;
+  for (int s = 0; s < n1; ++s) {
+    a += h[3 * s] / n1;
+    b += h[3 * s + 1] / n1;
+    f += h[3 * s + 2] != 0.0;
+  }
+  if (est_learned) *est_learned = a;
+  if (est_ht) *est_ht = b;
+  if (fell_back) *fell_back = f;
   return GBM3D_OK;
 }
 

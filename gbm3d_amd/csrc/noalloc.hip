@@ -54,7 +54,7 @@ int gbm3d_denoise_device_noalloc(gbm3d_ctx* c, const float* dI, int m, int n, fl
   if (!dI || !dU) return fail(GBM3D_ERR_ARG, "NULL image");
   if (!c->noalloc_ws) return fail(GBM3D_ERR_ARG, "noalloc: the scratch array of this device was not looked up");
   if (c->ext.learned)
-    return fail(GBM3D_ERR_UNSUPPORTED, "noalloc: the learned form needs its residual buffer and host judge");
+    return fail(GBM3D_ERR_UNSUPPORTED, "noalloc: the learned form needs its per-image device state");
   gbm3d_opts o;
   GBM3D_TRY(resolve_checked(opts, &o));
   Plan p;

@@ -194,7 +194,7 @@ int lanes_join(gbm3d_ctx* c, int lanes, hipStream_t st) {
 }
 
 int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, const gbm3d_opts& o, float* dU,
-                    float* dU1, hipStream_t st, char* ws, float* sstats, int lanes) {
+                    float* dU1, hipStream_t st, char* ws, const LearnedSlot* lslot, int lanes) {
   const int m = p.m, n = p.n;
   Carve cv{ws};
   Bufs b;
@@ -230,13 +230,29 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
     }
     if (c->ext.learned) {
       // variant: learned gains on the DCT-Haar coefficients of each group,
-      // gathered straight from the search image (no mosaic volume)
+      // gathered straight from the search image (no mosaic volume); then the
+      // judge on the device, and the reference filter as gated launches that
+      // run only when the judge falls back (they rebuild the volume and
+      // overwrite vavg exactly as the reference path computes them)
+      if (!lslot) return fail(GBM3D_ERR_ARG, "learned shrinkage: no device state for this image");
       GBM3D_TRY(do_match(c, sb.pad2, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, nullptr, ls,
                          GBM3D_STAGE_MATCH1));
-      StageTimer t(c, GBM3D_STAGE_SHRINK);
-      const long G1 = (long)(p.M / p.bs) * (p.N / p.bs);
-      GBM3D_TRY(launch_shrink(sb.pad2, sb.S1, p.M, p.d2, G1, sigma, c->shrink_wb, c->shrink_fb, sb.vavg, nullptr,
-                              sstats + (size_t)2 * kShrinkGrid * ii, nullptr, ls));
+      {
+        StageTimer t(c, GBM3D_STAGE_SHRINK);
+        const long G1 = (long)(p.M / p.bs) * (p.N / p.bs);
+        float* stats = lslot->stats + (size_t)2 * kShrinkGrid * ii;
+        int grid = 0;
+        GBM3D_TRY(launch_shrink(sb.pad2, sb.S1, p.M, p.d2, G1, sigma, c->shrink_wb, c->shrink_fb, sb.vavg, nullptr,
+                                stats, &grid, ls));
+        const double noise = (double)p.M * p.N * p.K * (double)sigma * sigma;
+        const double bound = (c->ext.learned_bound > 0 ? c->ext.learned_bound : 1000) / 1000.0;
+        GBM3D_TRY(launch_learned_judge(stats, grid, noise, bound, lslot->run + ii, lslot->res + 3 * ii, ls));
+      }
+      HtWork hw;
+      ht_work_bind(sb.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
+      GBM3D_TRY(launch_mosaic_gather(sb.pad2, sb.S1, p.M, p.N, p.bs, p.d2, p.K, sb.vol, lslot->run + ii, ls));
+      GBM3D_TRY(run_ht(sb.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau, hw, sb.vavg, p.bs, ls, ht_hook, c,
+                       lslot->run + ii));
     } else {
       HtWork hw;
       ht_work_bind(sb.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
@@ -304,14 +320,11 @@ int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const
                  float* dU1) {
   Plan p;
   GBM3D_TRY(make_plan(o, m, n, &p));
-  GBM3D_TRY(learned_setup(c, o, 1));
+  GBM3D_TRY(learned_setup(c, o, std::vector<Plan>{p}));
   const int lanes = spin_lanes(c, p);
   GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, pipeline_bytes(p, o, lanes)));
-  GBM3D_TRY(run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena, image_stats(c, 0), lanes));
-  if (!c->ext.learned) return GBM3D_OK;
-  std::vector<Plan> plans{p};
-  float* u1[1] = {dU1};
-  return learned_judge(c, o, 1, plans, &dI, &sigma, &dU, dU1 ? u1 : nullptr);
+  const LearnedSlot slot = c->ext.learned ? learned_slot(c, 0) : LearnedSlot{};
+  return run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena, c->ext.learned ? &slot : nullptr, lanes);
 }
 
 int set_dev(gbm3d_ctx* c) {

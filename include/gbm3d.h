@@ -118,10 +118,11 @@ typedef struct gbm3d_ext {
   int32_t learned;    /* 1: learned shrinkage replaces the step-1 wavelet hard
                          threshold (blockSize 8, numMax 16; weights from
                          gbm3d_shrink_set_weights)                            */
-  int32_t learned_bound; /* fallback bound in 1/1000: an image from which the
-                         learned gains remove more than bound x its noise
-                         energy (coefficients x sigma^2) is recomputed on the
-                         reference path (0: 1000)                             */
+  int32_t learned_bound; /* fallback bound in 1/1000: per match spin, when
+                         the learned gains remove more than bound x the noise
+                         energy (coefficients x sigma^2) the reference filter
+                         runs for that spin instead, decided on the device
+                         (the call stays asynchronous; 0: 1000)               */
   int32_t match_stats; /* 1: count the BF16 matcher's paths for
                          gbm3d_bf16_match_stats (diagnostics: a few atomic
                          adds per reference tile)                             */
@@ -348,10 +349,11 @@ int gbm3d_get_stage_times(gbm3d_ctx* ctx, double* ms, int* launches);
  * position k-1. */
 #define GBM3D_SHRINK_PARAMS 49600
 int gbm3d_shrink_set_weights(gbm3d_ctx* ctx, const float* params, int count);
-/* Image `image` of the last denoise call with the learned form: energy the
- * learned gains removed from the noisy coefficients and energy a 2.7 sigma
- * hard threshold would remove, both relative to the noise energy
- * (coefficients x sigma^2, all groups and spins), and whether it fell back. */
+/* Image `image` of the last denoise call with the learned form (waits for
+ * the context stream): energy the learned gains removed from the noisy
+ * coefficients and energy a 2.7 sigma hard threshold would remove, both
+ * relative to the noise energy (coefficients x sigma^2) and averaged over the
+ * match spins, and the number of spins that fell back. */
 int gbm3d_shrink_stats(gbm3d_ctx* ctx, int image, double* est_learned,
                        double* est_ht, int* fell_back);
 /* Learned shrinkage of G groups (host buffers, 8 x 8 x 16 x G): filtered

@@ -11,6 +11,9 @@
     and blocks against the FP32 restatement on the noisy image (step 1) and on
     the pilot (step 2), match spins 0 and 4; the BF16 form end to end equal to
     the FP32 path bit for bit.
+  * 64 images of 512 x 512 with the learned shrinkage in one graph-captured
+    batch: two pilots against the learned-pilot restatement, the device
+    judge's flags, and its fallback equal to the plain batch bit for bit.
 Bars: 1e-3 max abs on the [0, 255] scale, 0.01 dB (see _chained_vs_oracle).
 """
 import numpy as np
@@ -145,3 +148,67 @@ def test_precision_4096_bf16_form_equals_fp32_path(ctx, cbf, precision_case):
     U32, o32 = ctx.denoise(noisy, 25.0, P48)
     np.testing.assert_array_equal(obf["U1"], U1)
     np.testing.assert_array_equal(Ubf, U32)
+
+
+@pytest.fixture(scope="module")
+def learned_case():
+    import bench
+    from gbm3d_amd import shrink
+
+    imgs, cleans, sig = bench.learned_workload()
+    c = gbm3d_amd.Context(0)
+    c.set_variant(learned=True)
+    U, U1 = _batch_with_pilots(c, imgs, sig)
+    stats = [c.shrink_stats(i) for i in range(len(imgs))]
+    yield imgs, cleans, sig, U, U1, stats, c, shrink.load_weights()
+    c.close()
+
+
+def _batch_with_pilots(c, imgs, sig):
+    from gbm3d_amd import device
+
+    dI = [device.DeviceArray.from_host(x) for x in imgs]
+    dU = [device.DeviceArray(x.shape) for x in imgs]
+    dU1 = [device.DeviceArray(x.shape) for x in imgs]
+    c.denoise_batch_device([d.ptr for d in dI], [x.shape[0] for x in imgs], [x.shape[1] for x in imgs], sig, P8,
+                           [d.ptr for d in dU], [d.ptr for d in dU1], graph=True, tune=False)
+    c.synchronize()
+    return [d.to_host() for d in dU], [d.to_host() for d in dU1]
+
+
+@pytest.mark.parametrize("which", ["lowest_sigma", "highest_sigma"])
+def test_learned_64x512_batch_vs_restatement(learned_case, which):
+    """64 x 512^2 learned inference in one graph-captured batch: the pilot of
+    the images with the lowest and the highest sigma against the learned-pilot
+    restatement (oracle.shrink_oracle.learned_pilot, FP32 stages, BF16
+    network), PSNR within 0.01 dB and mean abs within 1e-3; U against the
+    restatement's step 2 on the GPU's pilot; no spin fell back at the
+    default bound (removed energy between 0.5 and 1 noise energies)."""
+    from oracle import shrink_oracle as so
+
+    imgs, cleans, sig, U, U1, stats, c, params = learned_case
+    i = int(np.argmin(sig)) if which == "lowest_sigma" else int(np.argmax(sig))
+    ref = so.learned_pilot(imgs[i].astype(np.float64), sig[i], params, P8, precision=32)
+    err = np.abs(U1[i] - ref)
+    print(f"image {i} sigma {sig[i]:.2f}: pilot PSNR gpu {synth.psnr(cleans[i], U1[i]):.4f} "
+          f"restatement {synth.psnr(cleans[i], ref):.4f}, mean abs {err.mean():.2e}")
+    assert abs(synth.psnr(cleans[i], U1[i]) - synth.psnr(cleans[i], ref)) <= 0.01
+    assert err.mean() <= 1e-3
+    assert np.max(np.abs(U[i] - O.step2_from_pilot(U1[i], imgs[i], sig[i], P8))) <= 1e-3
+    assert all(f == 0 and 0.5 < el < 1.0 for el, _, f in stats)
+
+
+def test_learned_64x512_fallback_equals_plain_batch(ctx, learned_case):
+    """With a bound below every spin's removed energy the device judge makes
+    every spin fall back: the batch equals the plain batch bit for bit."""
+    imgs, cleans, sig, U, U1, stats, c, params = learned_case
+    Up, U1p = _batch_with_pilots(ctx, imgs, sig)
+    try:
+        c.set_variant(learned=True, learned_bound=1e-3, weights=params)
+        Uf, U1f = _batch_with_pilots(c, imgs, sig)
+        fell = [c.shrink_stats(i)[2] for i in range(len(imgs))]
+    finally:
+        c.set_variant(learned=True, weights=params)
+    assert fell == [len(gbm3d_amd.resolve_opts(P8)["matchSpins"])] * len(imgs)
+    for i in range(len(imgs)):
+        assert np.array_equal(Uf[i], Up[i]) and np.array_equal(U1f[i], U1p[i]), i
