@@ -158,6 +158,7 @@ SIGNATURES = {
     "gbm3d_batch_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "gbm3d_match": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "gbm3d_match_global": (_I, [_P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P]),
+    "gbm3d_match_tol": (_I, [_P, _P, _I, _I, _I, _I, _I, _I, _D, _P, _P]),
     "gbm3d_ht_filter": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "gbm3d_median_filter": (_I, [_P, _P, _I, _I, _I, _P]),
     "gbm3d_aggregate_ht": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _P, _P]),

@@ -330,6 +330,18 @@ class Context:
         check(self.lib.gbm3d_match(self.handle, ptr(I), m, n, d, d2, K, ptr(S), ptr(V)))
         return S, V
 
+    def match_tol(self, I, d: int, d2: int, numMax: int, numMin: int, sigma: float, want_volume: bool = True):
+        """matchBlocksCPU's tolerance branch (numMax != numMin): (S, Volume);
+        raises GBM3DError (ERR_ARG) where the reference fails (a tile with
+        fewer than numMax matches below tol = 0.005 sigma^2 d^2)."""
+        I = np.asfortranarray(I, dtype=np.float32)
+        m, n = I.shape
+        S = np.zeros((numMax, m // d, n // d), dtype=np.uint32, order="F")
+        V = np.zeros((m, n, numMax), dtype=np.float32, order="F") if want_volume else None
+        check(self.lib.gbm3d_match_tol(self.handle, ptr(I), m, n, d, d2, numMax, numMin, float(sigma), ptr(S),
+                                       ptr(V)))
+        return S, V
+
     def match_global(self, I, sigma: float, d: int, numMax: int, numMin: int | None = None,
                      want_filtered: bool = False):
         """matchBlocksGPU's whole-image search (source/matchBlocksGPU.m:18-91).
@@ -450,8 +462,11 @@ def GBM3D_distributed(I, sigma: float, opts: dict | None = None):
 
 
 def matchBlocksCPU(I, sigma: float, opts: dict):
-    """[S, Volume] = matchBlocksCPU(I, sigma, opts) on the GPU (numMax == numMin)."""
+    """[S, Volume] = matchBlocksCPU(I, sigma, opts) on the GPU (the tolerance
+    branch when numMax != numMin)."""
     o = checkBM3Dopts(opts)
+    if o["numMax"] != o["numMin"]:
+        return default_context().match_tol(I, o["blockSize"], o["matchSize"], o["numMax"], o["numMin"], sigma)
     return default_context().match(I, o["blockSize"], o["matchSize"], o["numMax"])
 
 

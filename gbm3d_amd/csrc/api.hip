@@ -63,12 +63,14 @@ int gbm3d_create(int device, gbm3d_ctx** out) {
         for (int u = 0; u < d; ++u) h[i * kMaskStride + which * d * d + u + d * v] = (float)(s[u] * s[v]);
     }
   }
-  if (hipMalloc(reinterpret_cast<void**>(&c->masks), h.size() * sizeof(float)) != hipSuccess ||
+  // (64 more words after the table: the tolerance branch's flag)
+  if (hipMalloc(reinterpret_cast<void**>(&c->masks), (h.size() + 64) * sizeof(float)) != hipSuccess ||
       hipMemcpy(c->masks, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipStreamDestroy(c->own);
     delete c;
     return fail(GBM3D_ERR_HIP, "mask table upload failed");
   }
+  c->tolflag = reinterpret_cast<int*>(c->masks + h.size());
   if (noalloc_scratch_init(device, &c->noalloc_ws) != GBM3D_OK) {
     const std::string why = gbm3d_last_error();
     (void)hipFree(c->masks);
@@ -223,6 +225,49 @@ int gbm3d_match(gbm3d_ctx* c, const float* I, int m, int n, int d, int d2, int K
   GBM3D_TRY(download(c, S, dS, G * K));
   if (Volume) GBM3D_TRY(download(c, Volume, vol, mn * K));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
+  return GBM3D_OK;
+}
+
+int gbm3d_match_tol(gbm3d_ctx* c, const float* I, int m, int n, int d, int d2, int numMax, int numMin, double sigma,
+                    uint32_t* S, float* Volume) {
+  GBM3D_TRY(set_dev(c));
+  if (!I || !S) return fail(GBM3D_ERR_ARG, "NULL argument");
+  GBM3D_TRY(stage_sizes(m, n, d, d2, numMax));
+  if (numMin < 1 || numMin > numMax) return fail(GBM3D_ERR_ARG, "stage: numMin must be in [1, numMax]");
+  if (!(sigma >= 0.0) || !std::isfinite(sigma)) return fail(GBM3D_ERR_ARG, "stage: sigma must be finite and >= 0");
+  const size_t mn = (size_t)m * n, P = (size_t)m + d2, Q = (size_t)n + d2, G = mn / ((size_t)d * d);
+  Carve cv{nullptr};
+  auto lay = [&](Carve& k, float** img, float** pad, float** bn, uint32_t** dS, float** vol) {
+    *img = k.take<float>(mn);
+    *pad = k.take<float>(P * Q);
+    *bn = k.take<float>((P - d + 1) * (Q - d + 1));
+    *dS = k.take<uint32_t>(G * numMax);
+    *vol = k.take<float>(Volume ? mn * numMax : 0);
+  };
+  float *img, *pad, *bn, *vol;
+  uint32_t* dS;
+  lay(cv, &img, &pad, &bn, &dS, &vol);
+  GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, cv.off));
+  Carve k2{c->arena};
+  lay(k2, &img, &pad, &bn, &dS, &vol);
+  const double t = 0.005 * sigma * sigma * d * d;  // matchBlocksCPU.m:22-28
+  float tu = (float)t;
+  if ((double)tu < t) tu = nextafterf(tu, __builtin_inff());
+  const TolArgs ta{numMin == numMax ? 0 : numMin, tu, c->tolflag};
+  GBM3D_TRY(upload(c, img, I, mn));
+  GBM3D_TRY(launch_set_i32(c->tolflag, 0, c->stream));
+  GBM3D_TRY(launch_search_pad(img, m, n, 0, d2, pad, c->stream));
+  GBM3D_TRY(launch_norm_map(pad, (int)P, (int)Q, d, bn, c->stream));
+  GBM3D_TRY(launch_match_w(pad, bn, m, n, d, d2, numMax, dS, Volume ? vol : nullptr, 4, c->stream,
+                           ta.Kmin ? &ta : nullptr));
+  int short_tile = 0;
+  GBM3D_HIP(hipMemcpyAsync(&short_tile, c->tolflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  GBM3D_TRY(download(c, S, dS, G * numMax));
+  if (Volume) GBM3D_TRY(download(c, Volume, vol, mn * numMax));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  if (short_tile)
+    return fail(GBM3D_ERR_ARG, "tolerance matching: a reference block has fewer than numMax matches below tol "
+                               "(matchBlocksCPU fails here, matchBlocksCPU.m:111); S holds the numMax nearest");
   return GBM3D_OK;
 }
 

@@ -113,6 +113,8 @@ int gbm3d_denoise_batch_device(gbm3d_ctx* c, int count, const float* const* dI, 
   if (count < 1 || !dI || !m || !n || !sigma || !dU) return fail(GBM3D_ERR_ARG, "batch: NULL or empty");
   gbm3d_opts o;
   GBM3D_TRY(resolve(opts, &o));
+  if (o.numMin != o.numMax)
+    return fail(GBM3D_ERR_UNSUPPORTED, "batch: the tolerance branch (numMax ~= numMin) is single-image only");
   BatchJob j{count, dI, nullptr, m, n, sigma, dU, dU1, {}};
   j.plans.resize(count);
   for (int i = 0; i < count; ++i) {

@@ -44,6 +44,8 @@ struct gbm3d_ctx {
   int match_waves = 4;  // reference tiles per matching workgroup (single-image path)
   // coloured noise of the current call (gbm3d_denoise_colored)
   bool colored = false;
+  double tol_sigma = 0.0;       // coloured noise: mean(sqrt(PSD)), the tolerance branch's sigma
+  int* tolflag = nullptr;       // device: a tile of the tolerance branch came out short
   std::vector<double> ctau;   // levels x 7
   std::vector<float> wie_s2;  // blockSizeWie^2: sigmaWie.^2
   // learned shrinkage
@@ -128,7 +130,7 @@ int spin_lanes(const gbm3d_ctx* c, const Plan& p);
 
 // Block matching on a padded image with the context's distance variant.
 int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int N, int d, int d2, int K, uint32_t* S,
-             float* vol, hipStream_t st, int stage);
+             float* vol, hipStream_t st, int stage, const TolArgs* ta = nullptr);
 
 // Fork `lanes` lane streams from st and join them back (no-ops for lanes <= 1).
 // A fork is always joined, also after an error between the two, so that a

@@ -127,6 +127,8 @@ int run_distributed(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, co
     return run_pipeline(c, dI, m, n, sigma, o, dU, dU2);
   }
   if (c->ext.learned) return fail(GBM3D_ERR_UNSUPPORTED, "distributed: the learned form is not supported");
+  if (o.numMin != o.numMax)
+    return fail(GBM3D_ERR_UNSUPPORTED, "distributed: the tolerance branch (numMax ~= numMin) is single-image only");
   int dm = 0, dn = 0;
   GBM3D_TRY(validate(o, 256 + k, 256 + k, &dm, &dn));
   int ny1, nx1, ny2 = 0, nx2 = 0;

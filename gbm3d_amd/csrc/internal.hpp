@@ -100,8 +100,18 @@ bool match_global_supported(int d);
 int launch_match_bf16(const float* pad, const float* bn, const float* smap, int M, int N, int d, int d2, int K,
                       int R, uint32_t* S, float* vol, hipStream_t st, uint32_t* stats = nullptr);
 // Same with nw (1, 2, 4) reference tiles per workgroup.
+// matchBlocksCPU's tolerance branch (numMax ~= numMin, :89-99) for the FP32
+// matcher: Kmin = numMin (0: off), tolu = the smallest float >= tol; a tile
+// whose list would be shorter than numMax (where the reference fails) sets
+// *flag = 1.
+struct TolArgs {
+  int Kmin;
+  float tolu;
+  int* flag;
+};
+int launch_set_i32(int* p, int v, hipStream_t st);
 int launch_match_w(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
-                   int nw, hipStream_t st);
+                   int nw, hipStream_t st, const TolArgs* ta = nullptr);
 // Gather V (d x d x K x G) from a padded image through S.
 int launch_mosaic_gather(const float* pad, const uint32_t* S, int M, int N, int d, int d2, int K, float* V,
                          const int* run, hipStream_t st);

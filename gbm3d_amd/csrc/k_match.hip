@@ -190,7 +190,8 @@ struct MatchOcc {
 // candidate is scored with the FMA chain described at match_kernel.
 template <int D, int D2, int LDW>
 __device__ __forceinline__ uint64_t exact_search(const float* W, const float* __restrict__ bn, int Pm, int ti, int tj,
-                                                 float nx, int K, uint64_t* scr, int lane) {
+                                                 float nx, int K, uint64_t* scr, int lane, float tolu = 0.f,
+                                                 uint32_t* below = nullptr) {
   using C = MatchCfg<D, D2>;
   constexpr int S2 = D2 / 2;
   float numv[C::NL];  // |x|^2 + |y|^2 - 2<x,y> of this lane's candidates
@@ -236,6 +237,27 @@ __device__ __forceinline__ uint64_t exact_search(const float* W, const float* __
     const int ch = lane + 64 * (j / C::RA);
     return (ch / D2) * C::RA + j % C::RA + D2 * (ch % D2);
   };
+  if (below) {
+    // candidates with distance num / |x|^2 < tolu (the same for every lane):
+    // num against tolu |x|^2 where that decides (2^-20 of room for the
+    // roundings of the quotient and the product), the quotient otherwise
+    const float tnx = tolu * nx;
+    const float tz = fmaf(fabsf(tnx), 0x1p-20f, nx * 0x1p-126f);
+    const bool fast = nx > 0.f && nx <= 3.4028235e38f && fabsf(tnx) <= 3.4028235e38f;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int j = 0; j < C::NL; ++j) {
+      bool b;
+      if (fast && numv[j] < tnx - tz)
+        b = true;
+      else if (fast && !(numv[j] <= tnx + tz))
+        b = false;
+      else
+        b = numv[j] / nx < tolu;
+      cnt += (uint32_t)__popcll(__ballot(b));
+    }
+    *below = cnt;
+  }
 
   // The distance is num / |x|^2 with one |x|^2 per tile.  For a finite positive
   // |x|^2 the rounded quotient is non-decreasing in num, so with theta = the
@@ -331,7 +353,7 @@ __device__ __forceinline__ uint64_t exact_search(const float* W, const float* __
 template <int D, int D2, int NW = 4>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOcc<D, D2, NW>::kWaves))) void match_kernel(const float* __restrict__ pad, const float* __restrict__ bn,
                                                         int M, int N, int K, uint32_t* __restrict__ S,
-                                                        float* __restrict__ vol) {
+                                                        float* __restrict__ vol, TolArgs ta) {
   using C = MatchCfg<D, D2>;
   constexpr int S2 = D2 / 2;
   constexpr int RW = C::WR + (NW - 1) * D;          // union rows
@@ -379,8 +401,20 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
   const float* W = win + D * wave;
   const float nx = bn[(ti * D + S2) + (size_t)Pm * (tj * D + S2)];
 
-  const uint64_t best = exact_search<D, D2, LDW>(W, bn, Pm, ti, tj, nx, K, scr[wave], lane);
+  uint32_t below = 0;
+  const uint64_t best =
+      exact_search<D, D2, LDW>(W, bn, Pm, ti, tj, nx, K, scr[wave], lane, ta.tolu, ta.Kmin ? &below : nullptr);
   int idx = (int)(uint32_t)best;
+  if (ta.Kmin) {
+    // tolerance branch (matchBlocksCPU.m:89-99): more than K below tol keeps
+    // the K nearest; exactly K, those K in index order (find); fewer, the
+    // reference fails (its list is shorter than numMax, :111)
+    if (below < (uint32_t)K) {
+      if (lane == 0) ta.flag[0] = 1;
+    } else if (below == (uint32_t)K) {
+      idx = (int)wave_sort64<uint32_t>(lane < K ? (uint32_t)idx : 0xffffffffu, lane);
+    }
+  }
   if (lane == 0) idx = S2 + D2 * S2;  // slot 1 := window centre (matchBlocksCPU.m:103)
   if (lane < K) S[lane + (size_t)K * g] = (uint32_t)idx + 1u;
   if (vol == nullptr) return;
@@ -403,27 +437,29 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(MatchOc
 
 template <int D, int D2>
 static void launch_match_t(const float* pad, const float* bn, int M, int N, int K, uint32_t* S, float* vol, int nw,
-                           hipStream_t st) {
+                           hipStream_t st, TolArgs ta) {
   const int Mt = M / D, Nt = N / D;
   const unsigned blocks = (unsigned)(((Mt + nw - 1) / nw) * Nt);
   if (nw == 1)
-    hipLaunchKernelGGL((match_kernel<D, D2, 1>), dim3(blocks), dim3(64), 0, st, pad, bn, M, N, K, S, vol);
+    hipLaunchKernelGGL((match_kernel<D, D2, 1>), dim3(blocks), dim3(64), 0, st, pad, bn, M, N, K, S, vol, ta);
   else if (nw == 2)
-    hipLaunchKernelGGL((match_kernel<D, D2, 2>), dim3(blocks), dim3(128), 0, st, pad, bn, M, N, K, S, vol);
+    hipLaunchKernelGGL((match_kernel<D, D2, 2>), dim3(blocks), dim3(128), 0, st, pad, bn, M, N, K, S, vol, ta);
   else
-    hipLaunchKernelGGL((match_kernel<D, D2, 4>), dim3(blocks), dim3(256), 0, st, pad, bn, M, N, K, S, vol);
+    hipLaunchKernelGGL((match_kernel<D, D2, 4>), dim3(blocks), dim3(256), 0, st, pad, bn, M, N, K, S, vol, ta);
 }
 
 // nw: reference tiles (waves) per workgroup, 1, 2 or 4 (a launch-shape choice
 // of the batch autotuner; results do not depend on it).
 int launch_match_w(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S, float* vol,
-                   int nw, hipStream_t st) {
+                   int nw, hipStream_t st, const TolArgs* tap) {
   if (M % d || N % d || M < d2 || N < d2) return fail(GBM3D_ERR_ARG, "match: image size");
+  const TolArgs ta = tap ? *tap : TolArgs{0, 0.f, nullptr};
+  if (ta.Kmin && (ta.Kmin > K || !ta.flag)) return fail(GBM3D_ERR_ARG, "match: tolerance arguments");
   if (K < 1 || K > 64 || K > d2 * d2) return fail(GBM3D_ERR_UNSUPPORTED, "match: numMax must be in [1, 64]");
   if (nw != 1 && nw != 2 && nw != 4) return fail(GBM3D_ERR_ARG, "match: waves per workgroup must be 1, 2 or 4");
 #define GBM3D_MATCH_CASE(DD, DD2)                                     \
   if (d == DD && d2 == DD2) {                                         \
-    launch_match_t<DD, DD2>(pad, bn, M, N, K, S, vol, nw, st);        \
+    launch_match_t<DD, DD2>(pad, bn, M, N, K, S, vol, nw, st, ta);    \
     return check_launch("match");                                     \
   }
   GBM3D_MATCH_CASE(4, 32)
@@ -436,6 +472,15 @@ int launch_match_w(const float* pad, const float* bn, int M, int N, int d, int d
   GBM3D_MATCH_CASE(32, 48)
 #undef GBM3D_MATCH_CASE
   return fail(GBM3D_ERR_UNSUPPORTED, "match: (blockSize, matchSize) must be (4|8|16|32, 32|48)");
+}
+
+__global__ void set_i32_kernel(int* p, int v) {
+  if (threadIdx.x == 0) p[0] = v;
+}
+
+int launch_set_i32(int* p, int v, hipStream_t st) {
+  hipLaunchKernelGGL(set_i32_kernel, dim3(1), dim3(64), 0, st, p, v);
+  return check_launch("set_i32");
 }
 
 bool match_supported(int d, int d2) { return (d == 4 || d == 8 || d == 16 || d == 32) && (d2 == 32 || d2 == 48); }

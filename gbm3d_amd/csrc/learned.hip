@@ -28,8 +28,8 @@ constexpr size_t kSlotBytes = kSlotStats + kSlotRun + kSlotRes;
 int learned_setup(gbm3d_ctx* c, const gbm3d_opts& o, const std::vector<Plan>& plans) {
   if (!c->ext.learned) return GBM3D_OK;
   if (!c->shrink_wb) return fail(GBM3D_ERR_ARG, "learned shrinkage: no weights loaded (gbm3d_shrink_set_weights)");
-  if (o.blockSize != 8 || o.numMax != 16)
-    return fail(GBM3D_ERR_UNSUPPORTED, "learned shrinkage needs blockSize 8 and numMax 16");
+  if (o.blockSize != 8 || o.numMax != 16 || o.numMin != 16)
+    return fail(GBM3D_ERR_UNSUPPORTED, "learned shrinkage needs blockSize 8 and numMax = numMin = 16");
   if (o.filtType != GBM3D_FILT_HT)
     return fail(GBM3D_ERR_UNSUPPORTED, "learned shrinkage falls back to the 'ht' filter: filtType must be 'ht'");
   const size_t images = plans.size();

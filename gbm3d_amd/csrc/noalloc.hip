@@ -57,6 +57,8 @@ int gbm3d_denoise_device_noalloc(gbm3d_ctx* c, const float* dI, int m, int n, fl
     return fail(GBM3D_ERR_UNSUPPORTED, "noalloc: the learned form needs its per-image device state");
   gbm3d_opts o;
   GBM3D_TRY(resolve_checked(opts, &o));
+  if (o.numMin != o.numMax)
+    return fail(GBM3D_ERR_UNSUPPORTED, "noalloc: the tolerance branch (numMax ~= numMin) waits for the stream");
   Plan p;
   GBM3D_TRY(make_plan(o, m, n, &p));
   const size_t need = pipeline_bytes(p, o, 1);

@@ -66,7 +66,7 @@ int validate(const gbm3d_opts& o, int m, int n, int* M, int* N) {
   if (m < 1 || n < 1) return fail(GBM3D_ERR_ARG, "image must be non-empty");
   if (o.filtType != GBM3D_FILT_HT && o.filtType != GBM3D_FILT_MEDIAN)
     return fail(GBM3D_ERR_ARG, "unrecognized filtering");  // denoiseAndAggregateGPU.m:54
-  if (o.numMax != o.numMin) return fail(GBM3D_ERR_UNSUPPORTED, "numMax ~= numMin (tolerance matching) is not built");
+  if (o.numMin < 1 || o.numMin > o.numMax) return fail(GBM3D_ERR_ARG, "numMin must be in [1, numMax]");
   if (o.nMatchSpins < 1 || o.nMatchSpins > GBM3D_MAX_SPINS) return fail(GBM3D_ERR_ARG, "matchSpins: 1..16 entries");
   if (o.Wiener) {
     if (o.nMatchSpinsWie == GBM3D_UNSET) return fail(GBM3D_ERR_MISSING_FIELD, "Reference to non-existent field 'matchSpinsWie'");
@@ -155,11 +155,12 @@ int spin_lanes(const gbm3d_ctx* c, const Plan& p) {
 }
 
 int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int N, int d, int d2, int K, uint32_t* S,
-             float* vol, hipStream_t st, int stage) {
+             float* vol, hipStream_t st, int stage, const TolArgs* ta) {
   const int P = M + d2, Q = N + d2;
-  // the BF16 form is built for 8 x 8 and 16 x 16 blocks; other block sizes
-  // take the FP32 search, whose index lists the BF16 form reproduces anyway
-  const bool bf16 = c->ext.match_bf16 && match_bf16_supported(d, d2);
+  // the BF16 form is built for 8 x 8 and 16 x 16 blocks and the windowed
+  // mink; other block sizes and the tolerance branch take the FP32 search,
+  // whose index lists the BF16 form reproduces anyway
+  const bool bf16 = c->ext.match_bf16 && match_bf16_supported(d, d2) && !(ta && ta->Kmin);
   {
     StageTimer t(c, GBM3D_STAGE_PAD);
     GBM3D_TRY(launch_norm_map(pad, P, Q, d, bn, st));
@@ -171,7 +172,7 @@ int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int 
     return launch_match_bf16(pad, bn, smap, M, N, d, d2, K, std::min(R, 64), S, vol, st,
                              c->ext.match_stats ? c->bf16_stats : nullptr);
   }
-  return launch_match_w(pad, bn, M, N, d, d2, K, S, vol, c->match_waves, st);
+  return launch_match_w(pad, bn, M, N, d, d2, K, S, vol, c->match_waves, st, ta);
 }
 
 int lanes_fork(gbm3d_ctx* c, int lanes, hipStream_t st) {
@@ -218,6 +219,18 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
   // left there: a step-2 spin with the same s on that lane reuses it.
   int noisy_spin[kMaxLanes];
   for (int l = 0; l < kMaxLanes; ++l) noisy_spin[l] = INT_MIN;
+  // numMax ~= numMin: matchBlocksCPU's tolerance branch in step 1 (step 2,
+  // matchBlocksWie, always keeps the numMax nearest), tol = 0.005 sigma^2 d^2
+  // with sigma = mean(sqrt(PSD)) for coloured noise (matchBlocksCPU.m:21-28)
+  TolArgs tol{0, 0.f, nullptr};
+  if (o.numMin != o.numMax) {
+    const double sg = c->colored ? c->tol_sigma : (double)sigma;
+    const double t = 0.005 * sg * sg * p.bs * p.bs;
+    float tu = (float)t;
+    if ((double)tu < t) tu = nextafterf(tu, __builtin_inff());
+    tol = TolArgs{o.numMin, tu, c->tolflag};
+    GBM3D_TRY(launch_set_i32(c->tolflag, 0, st));
+  }
   const int l1 = std::max(1, std::min(lanes, p.n1));
   GBM3D_TRY(on_lanes(c, p.n1, l1, st, [&](int ii) {
     const int sp = o.matchSpins[ii];
@@ -257,7 +270,7 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
       HtWork hw;
       ht_work_bind(sb.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
       GBM3D_TRY(do_match(c, sb.pad2, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, sb.vol, ls,
-                         GBM3D_STAGE_MATCH1));
+                         GBM3D_STAGE_MATCH1, tol.Kmin ? &tol : nullptr));
       if (o.filtType == GBM3D_FILT_MEDIAN) {
         StageTimer t(c, GBM3D_STAGE_HT_SYN0);
         GBM3D_TRY(launch_median3(sb.vol, p.M, p.N, p.K, p.bs, sb.vavg, ls));  // denoiseAndAggregateGPU.m:49-52
@@ -324,7 +337,18 @@ int run_pipeline(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, const
   const int lanes = spin_lanes(c, p);
   GBM3D_TRY(ensure(&c->arena, &c->arena_bytes, pipeline_bytes(p, o, lanes)));
   const LearnedSlot slot = c->ext.learned ? learned_slot(c, 0) : LearnedSlot{};
-  return run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena, c->ext.learned ? &slot : nullptr, lanes);
+  GBM3D_TRY(run_pipeline_on(c, dI, p, sigma, o, dU, dU1, c->stream, c->arena, c->ext.learned ? &slot : nullptr, lanes));
+  if (o.numMin == o.numMax) return GBM3D_OK;
+  // the tolerance branch: whether a tile came out short is known only after
+  // matching, so this call waits for the stream
+  int short_tile = 0;
+  GBM3D_HIP(hipMemcpyAsync(&short_tile, c->tolflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  GBM3D_HIP(hipStreamSynchronize(c->stream));
+  if (short_tile)
+    return fail(GBM3D_ERR_ARG,
+                "tolerance matching: a reference block has fewer than numMax matches below tol, where "
+                "matchBlocksCPU fails (Volume(.., 1:numel(S)) = blocks, matchBlocksCPU.m:111)");
+  return GBM3D_OK;
 }
 
 int set_dev(gbm3d_ctx* c) {
@@ -410,11 +434,14 @@ int gbm3d_denoise_colored(gbm3d_ctx* c, const float* I, int m, int n, const doub
     s2.resize((size_t)bw * bw);
     for (int e = 0; e < bw * bw; ++e) s2[e] = (float)(sigmaWie[e] * sigmaWie[e]);
   }
+  double ssum = 0.0;
+  for (int e = 0; e < d1 * d2; ++e) ssum += std::sqrt(sigmaPSD[e]);
   c->colored = true;
   c->ctau = tau;
   c->wie_s2 = s2;
-  // the scalar sigma is not read on this path (matching uses it only for the
-  // inactive tolerance branch, matchBlocksCPU.m:20-22)
+  c->tol_sigma = ssum / (d1 * d2);  // mean(sqrt(sigma(:))), matchBlocksCPU.m:21
+  // the scalar sigma is not read on this path (thresholds and Wiener weights
+  // come from the PSD and sigmaWie)
   const int st = gbm3d_denoise(c, I, m, n, 1.0f, &o, U, U1);
   c->colored = false;
   return st;

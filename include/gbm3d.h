@@ -244,6 +244,19 @@ int gbm3d_batch_info(gbm3d_ctx* ctx, int* streams, int* match_waves, double* tun
 int gbm3d_match(gbm3d_ctx* ctx, const float* I, int m, int n, int d, int d2,
                 int K, uint32_t* S, float* Volume);
 
+/* [S, Volume] = matchBlocksCPU(I, sigma, opts) with numMax ~= numMin: the
+ * tolerance branch (source/matchBlocksCPU.m:21-28, 89-111), tol = 0.005
+ * sigma^2 d^2.  Per tile, with `below` the window positions of distance <
+ * tol: more than numMax -> the numMax nearest; exactly numMax -> those in
+ * index order (find); then slot 1 := the window centre.  Fewer than numMax
+ * (mink of numMin or find is shorter than numMax) is where the reference
+ * fails (it assigns numMax blocks into numel(S) slices, :111): returns
+ * GBM3D_ERR_ARG, S and Volume then holding the numMax nearest.  numMin ==
+ * numMax is gbm3d_match.  Shapes as gbm3d_match. */
+int gbm3d_match_tol(gbm3d_ctx* ctx, const float* I, int m, int n, int d, int d2,
+                    int numMax, int numMin, double sigma, uint32_t* S,
+                    float* Volume);
+
 /* S = matchBlocksGPU(I, sigma, opts) -- source/matchBlocksGPU.m:18-91, the
  * whole-image search (not on the GBM3D path, which matches in windows): for
  * each reference tile (stride d, floor(m/d) x floor(n/d), column-major) the

@@ -112,6 +112,20 @@ def match(I, d, d2, K, precision=64, want_dist=False):
     return S, V, dist
 
 
+def match_tol(I, d, d2, numMax, numMin, sigma, precision=64):
+    """matchBlocksCPU's tolerance branch (numMax != numMin, tol = 0.005
+    sigma^2 d^2) -> (S, Volume); raises where the reference fails (a tile
+    with fewer than numMax distances below tol)."""
+    dt, suf = _dt(precision)
+    I = _f(I, dt)
+    m, n = I.shape
+    S = np.zeros((numMax, m // d, n // d), dtype=np.uint32, order="F")
+    V = np.zeros((m, n, numMax), dtype=dt, order="F")
+    _check(getattr(lib(), "orc_match_tol_" + suf)(_p(I), m, n, d, d2, numMax, numMin, C.c_double(sigma), _p(S),
+                                                   _p(V)))
+    return S, V
+
+
 def match_wie(I, In, d, d2, K, precision=64, want_dist=False):
     """matchBlocksWie -> (S, V1, V2, dist); V: (d, d, K, Mt, Nt)."""
     dt, suf = _dt(precision)

@@ -234,9 +234,16 @@ std::vector<T> block_norm_map(const std::vector<T>& Ipad, int P, int Q, int d) {
 // lower linear index), then slot 1 := the window centre (:103).
 // S[k + K*(i + Mt*j)] = 1-based window index; dist (nullable) receives all
 // d2*d2 distances of every tile.
+// Tolerance branch (Kmin != K, matchBlocksCPU.m:89-99): with `below` the
+// number of window positions whose distance is < tol (the single value
+// compared with the double tol exactly), below > K keeps the K smallest,
+// below == K the K positions in index order (find), and below < K (mink of
+// numMin, or find, returns fewer than numMax) is where the reference fails
+// (Volume(.., 1:numel(S)) = blocks, :111, blocks has numMax slices): *short_out
+// is set and the K smallest are kept.
 template <class T>
 void match_core(const T* I, int m, int n, int d, int d2, int K, uint32_t* S, T* dist_out,
-                const std::vector<T>& Ipad) {
+                const std::vector<T>& Ipad, int Kmin = 0, double tol = 0.0, int* short_out = nullptr) {
   const int Mt = m / d, Nt = n / d, s = d2 / 2, P = m + d2, Q = n + d2;
   const int W2 = d2 * d2;
   std::vector<T> bn = block_norm_map<T>(Ipad, P, Q, d);
@@ -266,6 +273,16 @@ void match_core(const T* I, int m, int n, int d, int d2, int K, uint32_t* S, T* 
         for (int t = 0; t < W2; ++t) order[t] = t;
         std::partial_sort(order.begin(), order.begin() + K, order.end(),
                           [&](int p, int q) { return dist_less<T>(dist[p], p, dist[q], q); });
+        if (Kmin != 0 && Kmin != K) {
+          int below = 0;
+          for (int t = 0; t < W2; ++t) below += (double)dist[t] < tol;
+          if (below == K) {
+            std::sort(order.begin(), order.begin() + K);
+          } else if (below < K && short_out) {
+#pragma omp atomic write
+            *short_out = 1;
+          }
+        }
         uint32_t* Sg = S + (size_t)K * (i + (size_t)Mt * j);
         for (int k = 0; k < K; ++k) Sg[k] = (uint32_t)order[k] + 1u;
         Sg[0] = (uint32_t)(s + d2 * s + 1);  // d2^2/2 + d2/2 + 1
@@ -274,12 +291,17 @@ void match_core(const T* I, int m, int n, int d, int d2, int K, uint32_t* S, T* 
   }
 }
 
-// [S, Volume] = matchBlocksCPU(I, sigma, opts), numMax == numMin branch.
+// [S, Volume] = matchBlocksCPU(I, sigma, opts); Kmin != K: the tolerance
+// branch with tol = 0.005 sigma^2 d^2 (:21-28), returning GBM3D_ERR_ARG where
+// the reference fails (see match_core).
 template <class T>
-void match_step1(const T* I, int m, int n, int d, int d2, int K, uint32_t* S, T* Volume, T* dist) {
+int match_step1(const T* I, int m, int n, int d, int d2, int K, uint32_t* S, T* Volume, T* dist, int Kmin = 0,
+                double sigma = 0.0) {
   std::vector<T> Ipad = search_pad<T>(I, m, n, d2);
-  match_core<T>(I, m, n, d, d2, K, S, dist, Ipad);
-  if (!Volume) return;
+  int short_tile = 0;
+  match_core<T>(I, m, n, d, d2, K, S, dist, Ipad, Kmin, 0.005 * sigma * sigma * d * d, &short_tile);
+  if (short_tile) return GBM3D_ERR_ARG;
+  if (!Volume) return 0;
   const int Mt = m / d, Nt = n / d, P = m + d2;
   // Volume(tile, k) = Itmp(a:a+d-1, b:b+d-1)  (:107-111)
 #pragma omp parallel for collapse(2)
@@ -293,6 +315,7 @@ void match_step1(const T* I, int m, int n, int d, int d2, int K, uint32_t* S, T*
             Volume[(i * d + u) + (size_t)m * ((j * d + v) + (size_t)n * k)] =
                 Ipad[(i * d + a + u) + (size_t)P * (j * d + b + v)];
       }
+  return 0;
 }
 
 // [S, V1, V2] = matchBlocksWie(I, In, sigma, opts); V: d x d x K x Mt x Nt.
@@ -1020,7 +1043,7 @@ template <class T>
 int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout, T* U1out, bool gather,
           const double* tau_in = nullptr, const double* sigWie = nullptr) {
   if (o.filtType != GBM3D_FILT_HT && o.filtType != GBM3D_FILT_MEDIAN) return GBM3D_ERR_UNSUPPORTED;
-  if (o.numMax != o.numMin) return GBM3D_ERR_UNSUPPORTED;
+  if (o.numMin > o.numMax) return GBM3D_ERR_ARG;
   if (o.Wiener && o.nMatchSpinsWie == GBM3D_UNSET) return GBM3D_ERR_MISSING_FIELD;
   const int bs = o.blockSize, d2 = o.matchSize, K = o.numMax;
   const int M = (m + bs - 1) / bs * bs, N = (n + bs - 1) / bs * bs;  // :23-24
@@ -1041,7 +1064,8 @@ int gbm3d(const T* I0, int m, int n, double sigma, const gbm3d_opts& o, T* Uout,
     for (int ii = 0; ii < n1; ++ii) {  // :50-55
       int sp = o.matchSpins[ii];
       std::vector<T> Is = circshift2<T>(I.data(), M, N, sp);
-      match_step1<T>(Is.data(), M, N, bs, d2, K, S.data(), Vol.data(), nullptr);
+      const int st1 = match_step1<T>(Is.data(), M, N, bs, d2, K, S.data(), Vol.data(), nullptr, o.numMin, sigma);
+      if (st1) return st1;
       if (o.filtType == GBM3D_FILT_MEDIAN)
         median3<T>(Vol.data(), M, N, K, Vavg.data());
       else

@@ -611,3 +611,49 @@ def matchBlocksGPU(I, sigma, opts, Lo_d, Hi_d):
                     s = _mink_idx(flat, opts["numMin"])
             S[i][j] = s.astype(np.uint32)
     return S, dists, I
+
+
+def matchBlocksCPU_tol(I, sigma, d, d2, numMax, numMin):
+    """source/matchBlocksCPU.m:21-28, 56-111 with numMax ~= numMin (the
+    tolerance branch).  Returns (S as a list of lists, Volume); raises
+    ValueError where MATLAB's `Volume(.., 1:numel(S{i,j})) = blocks` (:111,
+    blocks = zeros(d, d, numMax), :70) fails on a size mismatch."""
+    tol = .005 * sigma ** 2
+    tol = tol * d ** 2
+    m, n = I.shape
+    M, N = m // d, n // d
+    Ipad = _search_pad(I, d2)
+    g = np.ones((d, d))
+    blockNorms = _corr_fft(g, I ** 2, (m, n))
+    blockNormsPad = _corr_fft(g, Ipad ** 2, (m + d2, n + d2))
+    Volume = np.zeros((m, n, numMax))
+    blocks = np.zeros((d, d, numMax))
+    S = [[None] * N for _ in range(M)]
+    for i in range(1, M + 1):
+        indI = (i - 1) * d
+        for j in range(1, N + 1):
+            indJ = (j - 1) * d
+            xR = I[(i - 1) * d:i * d, (j - 1) * d:j * d]
+            Itmp = Ipad[indI:indI + d + d2, indJ:indJ + d + d2]
+            bd = _corr_fft(xR, Itmp, (d + d2, d + d2))
+            nb = blockNorms[(i - 1) * d, (j - 1) * d]
+            bd = (nb + blockNormsPad[indI:indI + d2, indJ:indJ + d2] - 2 * bd[:d2, :d2]) / nb
+            flat = bd.flatten(order="F")
+            s = np.flatnonzero(flat < tol) + 1                      # find(blockDistances<tol)
+            if s.size > numMax:
+                s = _mink_idx(flat, numMax)
+            elif s.size < numMin:
+                s = _mink_idx(flat, numMin)
+            s = s.astype(np.uint32)
+            s[0] = d2 * d2 // 2 + d2 // 2 + 1                        # S{i,j}(1) = centre (:103)
+            for k in range(s.size):
+                a, b = _ind2sub(d2, s[k])
+                if k >= blocks.shape[2]:                             # MATLAB grows blocks
+                    blocks = np.concatenate([blocks, np.zeros((d, d, k + 1 - blocks.shape[2]))], axis=2)
+                blocks[:, :, k] = Itmp[a - 1:a - 1 + d, b - 1:b - 1 + d]
+            if blocks.shape[2] != s.size:
+                raise ValueError(f"Unable to perform assignment: left side {d}x{d}x{s.size}, "
+                                 f"right side {d}x{d}x{blocks.shape[2]}")
+            Volume[(i - 1) * d:i * d, (j - 1) * d:j * d, :s.size] = blocks
+            S[i - 1][j - 1] = s
+    return S, Volume

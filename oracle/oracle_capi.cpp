@@ -78,6 +78,16 @@ int orc_search_pad_f64(const double* X, int m, int n, int d2, double* out) {
       return 0;                                                                                      \
     });                                                                                              \
   }                                                                                                  \
+  int orc_match_tol_##SUF(const T* I, int m, int n, int d, int d2, int K, int Kmin, double sigma,    \
+                          uint32_t* S, T* Volume) {                                                  \
+    if (!ok_sizes(m, n, d, d2) || K < 1 || K > d2 * d2 || Kmin < 1 || Kmin > K) return GBM3D_ERR_ARG; \
+    int st = 0;                                                                                      \
+    const int g = guarded([&] {                                                                      \
+      st = match_step1<T>(I, m, n, d, d2, K, S, Volume, nullptr, Kmin, sigma);                       \
+      return 0;                                                                                      \
+    });                                                                                              \
+    return g != 0 ? g : st;                                                                          \
+  }                                                                                                  \
   int orc_match_wie_##SUF(const T* I, const T* In, int m, int n, int d, int d2, int K, uint32_t* S, \
                           T* V1, T* V2, T* dist) {                                                   \
     if (!ok_sizes(m, n, d, d2) || K < 1 || K > d2 * d2) return GBM3D_ERR_ARG;                     \
@@ -280,7 +290,10 @@ int orc_gbm3d_colored_f64(const double* I, int m, int n, const double* psd, int 
   return guarded([&] {
     std::vector<double> tau((size_t)resolved->levels * 7);
     colored_tau(resolved->levels, psd, d1, d2, tau.data());
-    return gbm3d<double>(I, m, n, 1.0, *resolved, U, U1, true, tau.data(), sigWie);
+    // sigma enters only the tolerance branch there: mean(sqrt(sigma(:))) (matchBlocksCPU.m:21)
+    double ss = 0.0;
+    for (int e = 0; e < d1 * d2; ++e) ss += std::sqrt(psd[e]);
+    return gbm3d<double>(I, m, n, ss / (d1 * d2), *resolved, U, U1, true, tau.data(), sigWie);
   });
 }
 int orc_gbm3d_colored_f32(const float* I, int m, int n, const double* psd, int d1, int d2, const double* sigWie,
@@ -289,7 +302,10 @@ int orc_gbm3d_colored_f32(const float* I, int m, int n, const double* psd, int d
   return guarded([&] {
     std::vector<double> tau((size_t)resolved->levels * 7);
     colored_tau(resolved->levels, psd, d1, d2, tau.data());
-    return gbm3d<float>(I, m, n, 1.0, *resolved, U, U1, true, tau.data(), sigWie);
+    // sigma enters only the tolerance branch there: mean(sqrt(sigma(:))) (matchBlocksCPU.m:21)
+    double ss = 0.0;
+    for (int e = 0; e < d1 * d2; ++e) ss += std::sqrt(psd[e]);
+    return gbm3d<float>(I, m, n, ss / (d1 * d2), *resolved, U, U1, true, tau.data(), sigWie);
   });
 }
 

@@ -98,7 +98,8 @@ def test_missing_library_fails_loudly(tmp_path):
                                   {"blockSizeWie": 4, "numMax": 32, "numMin": 32, "matchSpinsWie": [0]},
                                   {"blockSizeWie": 16, "numMax": 8, "numMin": 8, "matchSpinsWie": [0]},
                                   {"blockSize": 32, "matchSize": 48, "matchSpinsWie": [0]},
-                                  {"blockSize": 4, "blockSizeWie": 4, "matchSize": 48, "matchSpinsWie": [0]}])
+                                  {"blockSize": 4, "blockSizeWie": 4, "matchSize": 48, "matchSpinsWie": [0]},
+                                  {"numMax": 16, "numMin": 8, "matchSpinsWie": [0]}])   # tolerance branch
 def test_validate_accepts_built_combinations(opts):
     assert gbm3d_amd.validate(opts, 128, 96) > 0
 
@@ -108,12 +109,17 @@ def test_validate_accepts_built_combinations(opts):
     ({"numMax": 64, "numMin": 64, "matchSpinsWie": [0]}, "Wiener step"),
     ({"blockSizeWie": 32, "matchSpinsWie": [0]}, "blockSizeWie"),
     ({"matchSize": 64, "matchSpinsWie": [0]}, "matchSize"),
-    ({"numMax": 16, "numMin": 8, "matchSpinsWie": [0]}, "tolerance"),
 ])
 def test_validate_rejects_unbuilt_combinations(opts, what):
     with pytest.raises(gbm3d_amd.GBM3DError, match=what) as e:
         gbm3d_amd.validate(opts, 128, 96)
     assert e.value.status == _abi.GBM3D_ERR_UNSUPPORTED
+
+
+def test_validate_rejects_num_min_above_num_max():
+    with pytest.raises(gbm3d_amd.GBM3DError, match="numMin") as e:
+        gbm3d_amd.validate({"numMax": 8, "numMin": 16, "matchSpinsWie": [0]}, 128, 96)
+    assert e.value.status == _abi.GBM3D_ERR_ARG
 
 
 def test_validate_reports_reference_errors_and_sizes():

@@ -140,3 +140,45 @@ def test_denoise_and_aggregate_wie_dct_twin_vs_oracle(ctx, d, K):
         assert np.abs(U / W - Uo / Wo).max() <= 1e-3 / 255.0
     V2c, _ = O.wiener(V1.astype(np.float64), V2.astype(np.float64), s, precision=64, clip01=True)
     assert np.mean(V2c == 1.0) > 1e-4        # the clip is active on these inputs
+
+
+def _periodic(seed=3, m=64, n=64):
+    rng = np.random.default_rng(seed)
+    tile = rng.uniform(20, 235, (8, 8))
+    return (np.tile(tile, (m // 8, n // 8)) + 0.5 * rng.standard_normal((m, n))).astype(np.float32)
+
+
+@pytest.mark.parametrize("nmax,nmin,sigma", [(16, 8, 0.177), (12, 4, 0.177), (16, 8, 25.0), (16, 4, 0.05)])
+def test_match_tolerance_branch_bit_exact(ctx, nmax, nmin, sigma):
+    """matchBlocksCPU's tolerance branch (numMax ~= numMin, matchBlocksCPU.m:
+    89-111) against the FP32 restatement: find order when exactly numMax
+    distances are below tol, mink otherwise; where the reference fails (fewer
+    than numMax) both report it."""
+    I = _periodic()
+    try:
+        So, Vo = O.match_tol(I, 8, 32, nmax, nmin, sigma, precision=32)
+    except RuntimeError:
+        with pytest.raises(GBM3DError) as e:
+            ctx.match_tol(I, 8, 32, nmax, nmin, sigma)
+        assert e.value.status == _abi.GBM3D_ERR_ARG
+        return
+    S, V = ctx.match_tol(I, 8, 32, nmax, nmin, sigma)
+    np.testing.assert_array_equal(S, So)
+    np.testing.assert_array_equal(V, Vo)
+
+
+def test_tolerance_branch_end_to_end_vs_oracle(ctx):
+    opts = {"blockSize": 8, "blockSizeWie": 8, "numMax": 16, "numMin": 8, "matchSpinsWie": [0, 4]}
+    I = _periodic(5, 96, 64)
+    sigma = 0.177
+    U, out = ctx.denoise(I, sigma, opts)
+    U32, U132 = O.gbm3d(I, sigma, opts, precision=32)
+    assert np.max(np.abs(out["U1"] - U132)) <= 1e-3
+    assert np.max(np.abs(U - O.step2_from_pilot(out["U1"], I, sigma, opts))) <= 1e-3
+    Um, _ = ctx.denoise(I, sigma, dict(opts, numMin=16))
+    assert not np.array_equal(U, Um)                        # the find order changes the groups
+    with pytest.raises(GBM3DError) as e:                    # numMax 32: every tile is short
+        ctx.denoise(I, sigma, dict(opts, numMax=32, numMin=8))
+    assert e.value.status == _abi.GBM3D_ERR_ARG
+    V, _ = ctx.denoise(I, sigma, opts)                      # the context still works
+    assert np.array_equal(U, V)

@@ -518,3 +518,34 @@ def test_match_global_distances_equal_direct_differences():
                     y = np.roll(I, (-r, -c), (0, 1))[:d, :d]
                     dist[r, c] = ((x - y) ** 2).sum() / (x ** 2).sum()
             np.testing.assert_array_equal(S[:, ti, tj], np.argsort(dist.ravel(order="F"), kind="stable")[:K] + 1)
+
+
+# ------------------------------------------- matchBlocksCPU tolerance branch
+def _periodic(seed=3, m=64, n=64):
+    """A period-8 image plus small noise: each 32 x 32 window holds exactly 16
+    near copies of its reference block (offsets multiple of 8)."""
+    rng = np.random.default_rng(seed)
+    tile = rng.uniform(20, 235, (8, 8))
+    return np.tile(tile, (m // 8, n // 8)) + 0.5 * rng.standard_normal((m, n))
+
+
+@pytest.mark.parametrize("nmax,nmin,sigma,branch", [
+    (16, 8, 0.177, "find"),        # exactly numMax below tol: index order, then the centre
+    (12, 4, 0.177, "mink"),        # more than numMax below tol
+    (16, 8, 25.0, "mink"),         # tol = 200: every candidate below tol
+    (20, 8, 0.177, "error"),       # fewer than numMax: the reference's assignment fails
+])
+def test_match_tolerance_branch_matches_literal(nmax, nmin, sigma, branch):
+    I = _periodic()
+    if branch == "error":
+        with pytest.raises(ValueError):
+            L.matchBlocksCPU_tol(I, sigma, 8, 32, nmax, nmin)
+        with pytest.raises(RuntimeError):
+            O.match_tol(I, 8, 32, nmax, nmin, sigma)
+        return
+    Sl, Vl = L.matchBlocksCPU_tol(I, sigma, 8, 32, nmax, nmin)
+    S, V = O.match_tol(I, 8, 32, nmax, nmin, sigma)
+    np.testing.assert_array_equal(S, L.S_to_array(Sl, nmax))
+    np.testing.assert_array_equal(V, Vl)
+    Sm, _, _ = O.match(I, 8, 32, nmax)
+    assert np.array_equal(S, Sm) == (branch == "mink")
