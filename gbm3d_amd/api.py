@@ -156,7 +156,7 @@ class Context:
         out = (C.c_ulonglong * 8)()
         check(self.lib.gbm3d_bf16_match_stats(self.handle, out, int(reset)))
         return {"first_set": out[0], "widened": out[1], "exact_all": out[2], "rescored": out[3],
-                "exact_all_why": {"first_set_over_256": out[4], "no_kth_key": out[5], "widened_over_256": out[6],
+                "exact_all_why": {"first_set_over_128": out[4], "no_kth_key": out[5], "widened_over_128": out[6],
                                   "zero_norm_reference": out[7]}}
 
     def load_shrink_weights(self, params):

@@ -83,8 +83,8 @@ int launch_sum_map(const float* pad, int P, int Q, int d, float* sm, hipStream_t
 // first rescored set.  stats (nullable, kBf16Stats uint32 counters): tiles
 // finished on the first set [0], on a widened set [1], by the exact search
 // over all candidates [2]; rescored candidates [3]; why the exact search ran:
-// first set above 256 candidates [4], no finite K-th key in it [5], widened
-// set above 256 [6], reference block of zero or non-finite norm [7].
+// first set above 128 candidates [4], no finite K-th key in it [5], widened
+// set above 128 [6], reference block of zero or non-finite norm [7].
 constexpr int kBf16Stats = 8;
 // The counters live in kBf16StatSlots copies kBf16StatStride words apart
 // (one cache line each), summed on the host.

@@ -134,7 +134,7 @@ int gbm3d_get_ext(gbm3d_ctx* ctx, gbm3d_ext* ext);
  * context stream), out[8]: [0] tiles settled on the first rescored set, [1]
  * on a widened set, [2] by the exact search over every candidate, [3]
  * candidates rescored in total; why the exact search ran: [4] first set above
- * 256 candidates, [5] no finite K-th key in it, [6] widened set above 256,
+ * 128 candidates, [5] no finite K-th key in it, [6] widened set above 128,
  * [7] reference block of zero or non-finite norm.  reset != 0 clears them. */
 int gbm3d_bf16_match_stats(gbm3d_ctx* ctx, unsigned long long* out, int reset);
 
