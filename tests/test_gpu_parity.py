@@ -258,8 +258,14 @@ def test_error_paths(ctx):
         ctx.denoise(noisy, 25.0, {"filtType": 7, "matchSpinsWie": [0]})
     with pytest.raises(GBM3DError, match="matchSpinsWie"):
         ctx.denoise(noisy, 25.0, {"blockSize": 8})
-    with pytest.raises(GBM3DError, match="numMax"):
-        ctx.denoise(noisy, 25.0, {"numMax": 16, "numMin": 8, "matchSpinsWie": [0]})
+    with pytest.raises(GBM3DError, match="numMin"):
+        ctx.denoise(noisy, 25.0, {"numMax": 8, "numMin": 16, "matchSpinsWie": [0]})
+    # tolerance branch at sigma 25 on the [0, 255] scale: tol = 0.005 sigma^2 d^2
+    # = 800 (16 x 16 blocks): here more than numMax candidates lie below it, so it
+    # keeps the numMax nearest
+    Ut, _ = ctx.denoise(noisy, 25.0, {"numMax": 16, "numMin": 8, "matchSpinsWie": [0]})
+    Um, _ = ctx.denoise(noisy, 25.0, {"numMax": 16, "numMin": 16, "matchSpinsWie": [0]})
+    assert np.array_equal(Ut, Um)
     with pytest.raises(GBM3DError):
         ctx.denoise(noisy[:12, :12], 25.0, None)  # pads to 16x16, smaller than matchSize
     U, _ = ctx.denoise(noisy, 25.0, None)  # the context still works afterwards
