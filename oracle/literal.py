@@ -54,9 +54,9 @@ def myWavDec3DFFT(Psi, levels, U):
     for i in range(levels):
         FU = np.fft.fftn(U)
         for j in range(7):
-            c = np.fft.ifftn(FU * np.fft.fftn(Psi[:, :, :, j + 1], s=U.shape))
+            c = np.fft.ifftn(FU * np.fft.fftn(Psi[:, :, :, j + 1], s=U.shape, axes=(0, 1, 2)))
             C[i][j] = np.real(c[::2, ::2, ::2])
-        U = np.real(np.fft.ifftn(FU * np.fft.fftn(Psi[:, :, :, 0], s=U.shape)))
+        U = np.real(np.fft.ifftn(FU * np.fft.fftn(Psi[:, :, :, 0], s=U.shape, axes=(0, 1, 2))))
         U = U[::2, ::2, ::2]
     C[levels][0] = U
     return C
@@ -71,12 +71,12 @@ def myWavRec3DGPUfast(Psi, levels, C):
         tmp = np.zeros(sh)
         for j in range(7):
             tmp[::2, ::2, ::2] = C[i][j]
-            Cup[i] = Cup[i] + np.real(np.fft.ifftn(np.fft.fftn(tmp) * np.conj(np.fft.fftn(Psi[:, :, :, j + 1], s=sh))))
+            Cup[i] = Cup[i] + np.real(np.fft.ifftn(np.fft.fftn(tmp) * np.conj(np.fft.fftn(Psi[:, :, :, j + 1], s=sh, axes=(0, 1, 2)))))
     Cup[levels] = C[levels][0]
     for i in range(levels, 0, -1):
         tmp = np.zeros(Cup[i - 1].shape)
         tmp[::2, ::2, ::2] = Cup[i]
-        FPsi = np.conj(np.fft.fftn(Psi[:, :, :, 0], s=tmp.shape))
+        FPsi = np.conj(np.fft.fftn(Psi[:, :, :, 0], s=tmp.shape, axes=(0, 1, 2)))
         Cup[i - 1] = Cup[i - 1] + np.real(np.fft.ifftn(np.fft.fftn(tmp) * FPsi))
     return Cup[0]
 
