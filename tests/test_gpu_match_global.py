@@ -76,3 +76,29 @@ def test_match_global_repeat_calls_and_arguments(ctx):
         assert e.value.status == status
     S3, _ = ctx.match_global(b, 25.0, 4, 8)
     np.testing.assert_array_equal(S3, S2)
+
+
+def _degenerate(kind, m, n):
+    if kind == "zeros":
+        return np.zeros((m, n), np.float32)
+    if kind == "constant":
+        return np.full((m, n), 100.0, np.float32)
+    if kind == "ramp":
+        r, c = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+        return (r + 3.0 * c).astype(np.float32)
+    I = np.zeros((m, n), np.float32)              # sparse: a few bright pixels
+    I[::13, ::7] = 200.0
+    return I
+
+
+@pytest.mark.parametrize("kind", ["zeros", "constant", "ramp", "sparse"])
+@pytest.mark.parametrize("nmin", [16, 4])
+def test_match_global_degenerate_images(ctx, kind, nmin):
+    """Exact ties, 0/0 and Inf distances (zero-norm reference blocks), all-tied
+    windows: the running bound never excludes a candidate that can still tie,
+    and ties resolve to the lower index as mink does."""
+    I = _degenerate(kind, 64, 48)
+    S, cnt = ctx.match_global(I, 0.05, 8, 16, nmin)
+    So, cnto = O.match_global(I, 0.05, 8, 16, nmin, precision=32)
+    np.testing.assert_array_equal(cnt, cnto)
+    np.testing.assert_array_equal(S, So)
