@@ -182,3 +182,18 @@ def test_tolerance_branch_end_to_end_vs_oracle(ctx):
     assert e.value.status == _abi.GBM3D_ERR_ARG
     V, _ = ctx.denoise(I, sigma, opts)                      # the context still works
     assert np.array_equal(U, V)
+
+
+def test_tolerance_branch_coloured_noise_uses_mean_sqrt_psd(ctx):
+    """With a matrix sigma the tolerance branch takes sigma = mean(sqrt(PSD(:)))
+    (matchBlocksCPU.m:21): a flat PSD of 0.177^2 puts exactly numMax copies
+    below tol on the period-8 image, so the lists come in find order."""
+    I = _periodic(7, 64, 64)
+    psd = np.full((16, 16), 0.177 ** 2)
+    sw = np.full((8, 8), 0.177)
+    opts = {"blockSize": 8, "blockSizeWie": 8, "numMax": 16, "numMin": 8, "matchSpinsWie": [0, 4]}
+    U, out = ctx.denoise_colored(I, psd, sw, opts)
+    U32, U132 = O.gbm3d_colored(I, psd, sw, opts, precision=32)
+    assert np.max(np.abs(out["U1"] - U132)) <= 1e-3
+    Um, outm = ctx.denoise_colored(I, psd, sw, dict(opts, numMin=16))
+    assert not np.array_equal(out["U1"], outm["U1"])
