@@ -196,15 +196,43 @@ def test_wiener_vs_fp64_oracle(ctx, seed):
     assert np.abs(U / W - Uo / Wo).max() <= 1e-3
 
 
+def _chain_step2(ctx, noisy, sigma, opts, U):
+    """U against the FP32 restatement's step 2 run on the GPU's own pilot; an
+    image that is not a block multiple is chained on its mirror-padded form
+    (the pipeline's first step), whose crop must be the plain call's U."""
+    m, n = noisy.shape
+    bs = O.resolve_opts(opts)["blockSize"]
+    M, N = -(-m // bs) * bs, -(-n // bs) * bs
+    Ip = O.mirror_pad(noisy, M, N).astype(np.float32) if (M, N) != (m, n) else np.asarray(noisy, np.float32)
+    Up, outp = ctx.denoise(Ip, sigma, opts)
+    assert np.array_equal(Up[:m, :n], U)
+    U2 = O.step2_from_pilot(outp["U1"], Ip, sigma, opts, precision=32)
+    assert np.abs(Up - U2).max() <= 1e-3
+
+
+def _step2_chainable(opts):
+    # 'fast'-like option sets (fewer Wiener spins than step-1 spins) keep
+    # step-1 weights in the final quotient (source/GBM3D.m:26-27,82)
+    ro = O.resolve_opts(opts)
+    return ro["Wiener"] and len(ro["matchSpinsWie"]) >= len(ro["matchSpins"])
+
+
 @pytest.mark.parametrize("path", FIX, ids=[os.path.basename(p) for p in FIX])
 def test_end_to_end_golden(ctx, path):
-    """PSNR of U and of the pilot U1 within 0.01 dB of the FP64 golden run."""
+    """PSNR of U and of the pilot U1 within 0.01 dB of the FP64 golden run;
+    chained: U1 within 1e-3 of the FP32 restatement, U within 1e-3 of the
+    restatement's step 2 on the GPU's own pilot."""
     f = np.load(path, allow_pickle=False)
     opts = json.loads(str(f["opts"]))
-    U, out = ctx.denoise(f["noisy"], float(f["sigma"]), opts)
+    sigma = float(f["sigma"])
+    U, out = ctx.denoise(f["noisy"], sigma, opts)
     clean = f["clean"]
     assert abs(synth.psnr(clean, U) - synth.psnr(clean, f["U"])) < 0.01
     assert abs(synth.psnr(clean, out["U1"]) - synth.psnr(clean, f["U1"])) < 0.01
+    _, U132 = O.gbm3d(f["noisy"], sigma, opts, precision=32)
+    assert np.abs(out["U1"] - U132).max() <= 1e-3
+    if _step2_chainable(opts):
+        _chain_step2(ctx, f["noisy"], sigma, opts, U)
 
 
 @pytest.mark.parametrize("opts,m,n", [(None, 256, 256), ({"blockSize": 8, "matchSpinsWie": [0, 4]}, 256, 192),
@@ -220,10 +248,8 @@ def test_end_to_end_vs_oracle(ctx, opts, m, n):
     U, out = ctx.denoise(noisy, 25.0, opts)
     Uf, U1f = O.gbm3d(noisy, 25.0, opts, precision=32)
     assert np.abs(out["U1"] - U1f).max() <= 1e-3
-    ro = O.resolve_opts(opts)
-    if ro["Wiener"] and len(ro["matchSpinsWie"]) >= len(ro["matchSpins"]) and m % 16 == 0 and n % 16 == 0:
-        U2 = O.step2_from_pilot(out["U1"], noisy, 25.0, opts, precision=32)
-        assert np.abs(U - U2).max() <= 1e-3
+    if _step2_chainable(opts):
+        _chain_step2(ctx, noisy, 25.0, opts, U)
     Uo, U1o = O.gbm3d(noisy, 25.0, opts, precision=64)
     assert abs(synth.psnr(clean, U) - synth.psnr(clean, Uo)) < 0.01
     assert abs(synth.psnr(clean, out["U1"]) - synth.psnr(clean, U1o)) < 0.01
