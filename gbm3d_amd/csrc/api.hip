@@ -277,6 +277,7 @@ int gbm3d_match_global(gbm3d_ctx* c, const float* I, int m, int n, double sigma,
   if (!I || !S) return fail(GBM3D_ERR_ARG, "NULL argument");
   if (!match_global_supported(d)) return fail(GBM3D_ERR_UNSUPPORTED, "matchBlocksGPU: block size must be 4, 8, 16 or 32");
   if (m < d || n < d) return fail(GBM3D_ERR_ARG, "matchBlocksGPU: image smaller than one block");
+  if ((size_t)m * n >= ((size_t)1 << 31)) return fail(GBM3D_ERR_ARG, "matchBlocksGPU: image of 2^31 pixels or more");
   if (numMax < 1 || numMax > 64 || numMin < 1 || numMin > numMax || (size_t)numMax > (size_t)m * n)
     return fail(GBM3D_ERR_ARG, "matchBlocksGPU: need 1 <= numMin <= numMax <= min(64, m n)");
   if (!(sigma >= 0.0) || !std::isfinite(sigma)) return fail(GBM3D_ERR_ARG, "matchBlocksGPU: sigma must be finite and >= 0");

@@ -326,6 +326,7 @@ int launch_match_global(const float* I, int m, int n, int d, int K, int Kmin, do
                         uint32_t* counts, float* work, hipStream_t st) {
   if (!match_global_supported(d)) return fail(GBM3D_ERR_UNSUPPORTED, "matchBlocksGPU: block size must be 4, 8, 16 or 32");
   if (m < d || n < d) return fail(GBM3D_ERR_ARG, "matchBlocksGPU: image smaller than one block");
+  if ((size_t)m * n >= ((size_t)1 << 31)) return fail(GBM3D_ERR_ARG, "matchBlocksGPU: image of 2^31 pixels or more");
   if (K < 1 || K > 64 || Kmin < 1 || Kmin > K || (size_t)K > (size_t)m * n)
     return fail(GBM3D_ERR_ARG, "matchBlocksGPU: need 1 <= numMin <= numMax <= min(64, m n)");
   float tolu = (float)tol;
