@@ -242,55 +242,97 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
 // window, computing the low and the high band from the same loads; zero taps
 // are skipped at compile time; circular indices come from per-tile tables.
 
-// out[i] = sum_k a_lo(k) v[2i+9-k], hi[i] = sum_k a_hi(k) v[2i+9-k]
-template <int R>
-__device__ __forceinline__ void ana_lo_hi(const float (&v)[2 * R + 8], float (&lo)[R], float (&hi)[R]) {
+// The filters in lifting form.  With the input pairs D_j = v[2j] - v[2j+1],
+// S_j = v[2j] + v[2j+1] (a_lo and s_hi are symmetric up to sign),
+//   lo[i] = Ca (D_i - D_{i+4}) + Cb (D_{i+3} - D_{i+1}) + Cc S_{i+2},
+//   hi[i] = Cc D_{i+2},
+// and with Q_j = Ca (hi[j+4] - hi[j]) + Cb (hi[j+1] - hi[j+3]) the synthesis
+// outputs of coefficient window position j are Q_j + Cc (lo[j+2] - hi[j+2])
+// (even output 2j) and Q_j + Cc (lo[j+2] + hi[j+2]) (odd output 2j-1): 8
+// operations per coefficient (12 taps) and 4 per output (6 taps).  Every
+// value is a fixed function of its own inputs, whatever the run it is
+// computed in, so kernels that tile the axes differently agree bitwise.
+
+// lo[i] = sum_k a_lo(k) v[2i+9-k], hi[i] = sum_k a_hi(k) v[2i+9-k], i < R
+template <int R, int NV>
+__device__ __forceinline__ void ana_lo_hi(const float (&v)[NV], float (&lo)[R], float (&hi)[R]) {
+  static_assert(NV >= 2 * R + 8, "analysis window");
+  float D[R + 4], S[R + 4];
+#pragma unroll
+  for (int j = 0; j < R + 4; ++j) {
+    D[j] = v[2 * j] - v[2 * j + 1];
+    S[j] = v[2 * j] + v[2 * j + 1];
+  }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
-    float a = 0.f;
-#pragma unroll
-    for (int k = 0; k < 10; ++k) a = fmaf(a_lo(k), v[2 * i + 9 - k], a);
-    lo[i] = a;
-    hi[i] = fCc * v[2 * i + 4] - fCc * v[2 * i + 5];
+    lo[i] = fmaf(fCa, D[i] - D[i + 4], fmaf(fCb, D[i + 3] - D[i + 1], fCc * S[i + 2]));
+    hi[i] = fCc * D[i + 2];
   }
 }
 
-template <int R>
-__device__ __forceinline__ void ana_lo(const float (&v)[2 * R + 8], float (&lo)[R]) {
+template <int R, int NV>
+__device__ __forceinline__ void ana_lo(const float (&v)[NV], float (&lo)[R]) {
+  static_assert(NV >= 2 * R + 8, "analysis window");
+  float D[R + 4], S[R + 4];
 #pragma unroll
-  for (int i = 0; i < R; ++i) {
-    float a = 0.f;
-#pragma unroll
-    for (int k = 0; k < 10; ++k) a = fmaf(a_lo(k), v[2 * i + 9 - k], a);
-    lo[i] = a;
+  for (int j = 0; j < R + 4; ++j) {
+    D[j] = v[2 * j] - v[2 * j + 1];
+    S[j] = v[2 * j] + v[2 * j + 1];
   }
+#pragma unroll
+  for (int i = 0; i < R; ++i) lo[i] = fmaf(fCa, D[i] - D[i + 4], fmaf(fCb, D[i + 3] - D[i + 1], fCc * S[i + 2]));
 }
 
 // the high band of ana_lo_hi alone (reads v[4 .. 2R+3])
-template <int R>
-__device__ __forceinline__ void ana_hi(const float (&v)[2 * R + 8], float (&hi)[R]) {
+template <int R, int NV>
+__device__ __forceinline__ void ana_hi(const float (&v)[NV], float (&hi)[R]) {
+  static_assert(NV >= 2 * R + 8, "analysis window");
 #pragma unroll
-  for (int i = 0; i < R; ++i) hi[i] = fCc * v[2 * i + 4] - fCc * v[2 * i + 5];
+  for (int i = 0; i < R; ++i) hi[i] = fCc * (v[2 * i + 4] - v[2 * i + 5]);
 }
 
-// out[i] = sum_{k: i+k even} s_lo(k) lo[(i+k)/2] + s_hi(k) hi[(i+k)/2], i < R2 (window starts at an even output)
-template <int R2>
-__device__ __forceinline__ void syn_lo_hi(const float (&lo)[R2 / 2 + 5], const float (&hi)[R2 / 2 + 5],
-                                          float (&out)[R2]) {
+__device__ __forceinline__ float syn_q(const float* hi, int j) {
+  return fmaf(fCa, hi[j + 4] - hi[j], fCb * (hi[j + 1] - hi[j + 3]));
+}
+
+// out[i] = sum_{k: i+k even} s_lo(k) lo[(i+k)/2] + s_hi(k) hi[(i+k)/2], i < R2
+// (the window starts at an even output)
+template <int R2, int NC>
+__device__ __forceinline__ void syn_lo_hi(const float (&lo)[NC], const float (&hi)[NC], float (&out)[R2]) {
+  static_assert(NC >= R2 / 2 + 5 && R2 % 2 == 0, "synthesis window");
 #pragma unroll
-  for (int i = 0; i < R2; ++i) {
-    float a = 0.f;
-#pragma unroll
-    for (int k = 0; k < 10; ++k) {
-      if (((i + k) & 1) == 0) {
-        const int j = (i + k) >> 1;
-        a = fmaf(s_hi(k), hi[j], a);
-        if (k == 4 || k == 5) a = fmaf(fCc, lo[j], a);
-      }
-    }
-    out[i] = a;
+  for (int j = 0; j <= R2 / 2; ++j) {
+    const float q = syn_q(hi, j);
+    if (j < R2 / 2) out[2 * j] = fmaf(fCc, lo[j + 2] - hi[j + 2], q);
+    if (j >= 1) out[2 * j - 1] = fmaf(fCc, lo[j + 2] + hi[j + 2], q);
   }
 }
+
+// the same synthesis for a window that starts at an odd output 2m-1, its
+// coefficient window starting at m: out[i] = sum_{k: i+k odd} s_lo(k)
+// lo[(i+k-1)/2] + s_hi(k) hi[(i+k-1)/2]
+template <int R2, int NC>
+__device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[NC], const float (&hi)[NC], float (&out)[R2]) {
+  static_assert(NC >= R2 / 2 + 4 && R2 % 2 == 0, "synthesis window");
+#pragma unroll
+  for (int j = 0; j < R2 / 2; ++j) {
+    const float q = syn_q(hi, j);
+    out[2 * j] = fmaf(fCc, lo[j + 2] + hi[j + 2], q);
+    out[2 * j + 1] = fmaf(fCc, lo[j + 2] - hi[j + 2], q);
+  }
+}
+
+// C .* (abs(C) > tau) (denoiseAndAggregateGPU.m:37) as one compare and one
+// select; NaN is kept, as the product keeps it (a coefficient at or below
+// tau becomes +0 instead of a signed zero, which no later sum or the final
+// max(., 0) can tell apart)
+__device__ __forceinline__ float hard_thr(float c, float t) { return fabsf(c) <= t ? 0.f : c; }
+
+// z analysis of the pair (a = plane 2t, b = plane 2t-1) and its synthesis
+__device__ __forceinline__ float z_lo(float a, float b) { return fCc * (a + b); }
+__device__ __forceinline__ float z_hi(float a, float b) { return fCc * (b - a); }
+__device__ __forceinline__ float z_even(float lo, float hi) { return fCc * (lo - hi); }  // plane 2t
+__device__ __forceinline__ float z_odd(float lo, float hi) { return fCc * (lo + hi); }   // plane 2t-1
 
 constexpr int kSynPairsPerBlock = 4;  // z-pairs per block of the register-blocked synthesis
 
@@ -404,8 +446,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       const int e = tid + 256 * it;
       if (e < R * R) {
         const int rr = e % R, cc = e / R;
-        Z[cc * ZS + rr] = fCc * a[it] + fCc * b[it];
-        Z[R * ZS + cc * ZS + rr] = fCc * b[it] - fCc * a[it];
+        Z[cc * ZS + rr] = z_lo(a[it], b[it]);
+        Z[R * ZS + cc * ZS + rr] = z_hi(a[it], b[it]);
       }
     }
     float ycur[11];
@@ -448,10 +490,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
         float lo[11];
         ana_lo_hi<11>(v, lo, hi);
 #pragma unroll
-        for (int i = 0; i < 11; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
+        for (int i = 0; i < 11; ++i) Bl[i] = hard_thr(lo[i], tl);
       }
 #pragma unroll
-      for (int i = 0; i < 11; ++i) Bh[i] = hi[i] * (fabsf(hi[i]) > th ? 1.f : 0.f);
+      for (int i = 0; i < 11; ++i) Bh[i] = hard_thr(hi[i], th);
     }
     __syncthreads();
     // dim-1 synthesis: runs of 16 outputs per (z band, dim-2 band, coefficient column)
@@ -492,8 +534,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
         for (int i = 0; i < 4; ++i) {
           if (y0 + yq + i >= q) continue;
           const int pc = cidx[yq + i + 9];
-          put(Y, acc, pr, pc, ze, fCc * o[0][i] - fCc * o[1][i], p, q, r, mode, cs, gd);
-          put(Y, acc, pr, pc, zo, fCc * o[0][i] + fCc * o[1][i], p, q, r, mode, cs, gd);
+          put(Y, acc, pr, pc, ze, z_even(o[0][i], o[1][i]), p, q, r, mode, cs, gd);
+          put(Y, acc, pr, pc, zo, z_odd(o[0][i], o[1][i]), p, q, r, mode, cs, gd);
         }
       }
     }
@@ -508,289 +550,275 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
 // decimated transform and the inverse shift amount to the same analysis and
 // synthesis with the coefficient anchors at parity s in x, y and z:
 //   C(a) = sum_k h[k] V(a - k),  a = s (mod 2);  Vrec_s(o) = sum_k g[k] C(o + k).
-// So one 51 x 51 input window per plane serves both spins: spin 0 reads its
-// 50 x 50 sub-window at (0, 0), spin 1 at (1, 1), and spin 1's synthesis runs
-// start one output earlier (odd output parity).  The block owns a 32 x 32
-// output column over all r planes and walks the z anchors az = 0..r-1 (spin
-// az & 1, planes az-1 and az), so each output plane is complete after two
-// consecutive anchors; Vavg = max((Vrec_0 + Vrec_1) / 2, 0) is stored once.
-// The per-anchor arithmetic is that of ht_syn_fast_kernel with sh = s.
-
-// out[i] = sum_{k: i+k odd} s_lo(k) lo[(i+k-1)/2] + s_hi(k) hi[(i+k-1)/2]: a run of
-// R2 outputs starting at an odd output 2m-1, coefficient window starting at m.
-// Per output the taps are taken in the same order as in syn_lo_hi.
-template <int R2>
-__device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[R2 / 2 + 4], const float (&hi)[R2 / 2 + 4],
-                                              float (&out)[R2]) {
-#pragma unroll
-  for (int i = 0; i < R2; ++i) {
-    float a = 0.f;
-#pragma unroll
-    for (int k = 0; k < 10; ++k) {
-      if (((i + k) & 1) == 1) {
-        const int j = (i + k - 1) >> 1;
-        a = fmaf(s_hi(k), hi[j], a);
-        if (k == 4 || k == 5) a = fmaf(fCc, lo[j], a);
-      }
-    }
-    out[i] = a;
-  }
-}
-
+// One workgroup owns a 64 x 32 output column (rows x0.., columns y0..) over
+// all r planes and walks the z anchors az = 0..r-1 (spin az & 1, planes az-1
+// and az); each output plane is complete after two consecutive anchors and
+// Vavg = max((Vrec_0 + Vrec_1) / 2, 0) is stored once.  Spin s's x coefficient
+// j has its anchor at x0 + s + 2j and reads window rows s + 2j .. s + 2j + 9
+// (window row w = image row x0 - 9 + w, 83 rows); y likewise (51 columns).
+// Per anchor, three phases and two barriers:
+//   A  (thread = window row w, run of 7 y coefficients): the thread's 23
+//      window columns of both planes are in registers; z analysis, dim-2
+//      analysis of both z bands -> A[zb][b2][c][w];
+//   B  (thread = (zb, b2, c), run of 22 output rows): dim-1 analysis of 16
+//      coefficients, hard threshold (or, for the approximation band, the
+//      rebuilt coarser level), dim-1 synthesis -> E[zb][b2][c][x];
+//   C  (thread = output row x, 8 output columns): dim-2 synthesis of both z
+//      bands, z synthesis, spin combine, store.
+// The planes of the next anchor and the coarse values it needs are loaded
+// during the current one.  The per-coefficient and per-output arithmetic is
+// that of the per-spin kernels (lifting helpers), so the two agree bitwise.
 namespace f2 {
-constexpr int P = 32, C = 21, R = 50, W = R + 1;  // output tile, coefficients, sub-window, window
-constexpr int ZS = W, AS = R + 1, BS = C + 1, ES = P + 17;  // odd strides: conflict-free column walks
-constexpr int REG0 = 2 * W * ZS;  // Z (later B)
-constexpr int REG1 = 4 * C * AS;  // A (later E)
-constexpr int NIT = (W * W + 255) / 256;
-static_assert(8 * C * BS <= REG0, "B fits in Z");
-static_assert(4 * C * ES <= REG1, "E fits in A");
+constexpr int TX = 64, TY = 32;            // output tile: rows x columns
+constexpr int WX = TX + 19, WY = TY + 19;  // window rows, columns (both spins)
+constexpr int C = TY / 2 + 5;              // y coefficients per spin (21)
+constexpr int JX = TX / 2 + 5;             // x coefficients per spin (37)
+constexpr int NJ = 23;                     // window columns per phase-A thread (spin offset + 22)
+constexpr int NA = 3 * WX;                 // phase-A tasks: 83 rows x 3 runs of 7 coefficients
+constexpr int ND = 3 * 3 * C;              // phase-B detail tasks: 63 (zb, b2, c) x 3 runs
+constexpr int NCO = 3 * C;                 // phase-B approximation-band tasks (wave 3)
+constexpr int RB = 22, NB = 16;            // phase-B run: output rows, coefficients
+// A's row stride: odd (consecutive c in distinct banks), and 7 AS = 19 (mod 64)
+// so that the two runs a phase-A wave straddles fill disjoint banks
+constexpr int AS = 85, ES = TX + 1;
+constexpr int NYC = C * JX;                // coarse values staged per anchor
+constexpr int NYT = (NYC + 255) / 256;
+static_assert(NA <= 256 && ND <= 192 && 192 + NCO <= 256, "thread roles");
+static_assert(42 + 1 + 2 * (NB - 1) + 9 < WX, "phase-B rows inside the window");
+static_assert(14 * 2 + NJ <= WY, "phase-A columns inside the window");
 }  // namespace f2
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ht_syn0_cs2_kernel(const float* __restrict__ X, const float* __restrict__ Ylo0,
-                                                          const float* __restrict__ Ylo1, int p, int q, int r,
-                                                          Tau7 tau, float* __restrict__ Y, int gd, const int* __restrict__ run) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_syn0_cs2_kernel(
+    const float* __restrict__ X, const float* __restrict__ Ylo0, const float* __restrict__ Ylo1, int p, int q, int r,
+    Tau7 tau, float* __restrict__ Y, int gd, const int* __restrict__ run) {
   if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   using namespace f2;
-  __shared__ float lds[REG0 + REG1];
-  __shared__ int ridx[W], cidxp[W];  // window rows, p x window columns (circular)
-  __shared__ int yrow[2][C], ycol[2][C];  // per spin: coarse rows, ph x coarse columns (circular)
-  __shared__ float ycs[2 * C * 11];  // the anchor's coarse values, staged per coarse-role thread
-  float* Z = lds;
-  float* B = lds;
-  float* A = lds + REG0;
-  float* E = lds + REG0;
+  __shared__ float A[4 * C * AS];
+  __shared__ float E[4 * C * ES];
+  __shared__ float Yc[NYC];
+  __shared__ int cidxp[WY];             // p x window column (circular)
+  __shared__ int yrow[2][JX], ycol[2][C];  // per spin: coarse rows, ph x coarse columns (circular)
   const int tid = threadIdx.x;
-  __builtin_assume(tid < 256);  // 256-thread blocks: drops the guards of full element rounds
+  __builtin_assume(tid < 256);
   const int nbx = gridDim.x, nby = gridDim.y;
   const int lb = xcd_remap(blockIdx.x + nbx * blockIdx.y, nbx * nby);
-  const int x0 = (lb % nbx) * P, y0 = (lb / nbx) * P;  // output tile origin (original coordinates)
+  const int x0 = (lb % nbx) * TX, y0 = (lb / nbx) * TY;
   const int ph = p >> 1, qh = q >> 1, rh = r >> 1;
-  if (tid < W) {
-    ridx[tid] = pmod_i(x0 - 9 + tid, p);
-  } else if (tid < 2 * W) {
-    cidxp[tid - W] = p * pmod_i(y0 - 9 + tid - W, q);
-  } else if (tid < 2 * W + 2 * C) {
-    const int k = tid - 2 * W, sp = k / C;
-    yrow[sp][k % C] = pmod_i((x0 >> 1) + sp + k % C, ph);
-  } else if (tid < 2 * W + 4 * C) {
-    const int k = tid - 2 * W - 2 * C, sp = k / C;
+  if (tid < WY) {
+    cidxp[tid] = p * pmod_i(y0 - 9 + tid, q);
+  } else if (tid < WY + 2 * JX) {
+    const int k = tid - WY, sp = k / JX;
+    yrow[sp][k % JX] = pmod_i((x0 >> 1) + sp + k % JX, ph);
+  } else if (tid < WY + 2 * JX + 2 * C) {
+    const int k = tid - WY - 2 * JX, sp = k / C;
     ycol[sp][k % C] = ph * pmod_i((y0 >> 1) + sp + k % C, qh);
   }
   __syncthreads();
   const size_t pq = (size_t)p * q;
-  // phase-3 roles: the 6C tasks of the three detail (z band, dim-2 band) pairs
-  // on threads 0 .. 6C-1 (waves 0 and 1), the 2C tasks of the approximation
-  // pair, whose low band is the rebuilt coarser level (no low-pass analysis),
-  // on threads 128 .. 128+2C-1 (wave 2): no wave runs both kinds
-  static_assert(6 * C <= 128 && 128 + 2 * C <= 256, "phase-3 roles");
-  const int t3r = tid < 128 ? (tid < 6 * C ? tid + 2 * C : -1) : (tid < 128 + 2 * C ? tid - 128 : -1);
-  const int c3 = t3r % C, run3 = (t3r / C) & 1, b23 = (t3r / (2 * C)) & 1, zb3 = t3r / (4 * C);
-  const bool coarse = t3r >= 0 && t3r < 2 * C;
-  const float tl = tau_of(tau, (b23 << 1) | (zb3 << 2)), th = tau_of(tau, (b23 << 1) | (zb3 << 2) | 1);
-  float cur[NIT], prev[NIT], yl[11];
-  // in-plane byte offsets of this thread's window elements e = tid + 256 it
-  // (a plane holds fewer than 2^28 elements: volume < 2^31 elements, r >= 10)
-  uint32_t offb[NIT];
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int e = tid + 256 * it;
-    offb[it] = e < W * W ? 4u * (uint32_t)(ridx[e % W] + cidxp[e / W]) : 0u;
-  }
+  // ---- phase-A role: window row wa, first y coefficient 7 ra
+  const int wa = tid % WX, ra = tid / WX;
+  const bool actA = tid < NA;
+  const int rowA = pmod_i(x0 - 9 + wa, p);
+  // interior column range (no wrap): column offsets p * j on a per-thread base
+  const bool colin = y0 - 9 >= 0 && y0 + TY + 9 < q;
+  const uint32_t voffA = 4u * (uint32_t)(rowA + (colin ? p * (y0 - 9 + 14 * ra) : 0));
   const uint32_t pbytes = (uint32_t)(4 * pq);
-  auto fetch_plane = [&](float (&dst)[NIT], int z) {
+  auto fetch_plane = [&](float (&dst)[NJ], int z) {
     const __amdgpu_buffer_rsrc_t rs = buf_rsrc(X + pq * z, pbytes);
+    if (!actA) return;
+    if (colin) {
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      dst[it] = 0.f;
-      if (tid + 256 * it < W * W) dst[it] = buf_load(rs, offb[it]);
+      for (int j = 0; j < NJ; ++j) dst[j] = buf_load(rs, voffA, (uint32_t)(4 * p * j));
+    } else {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) dst[j] = buf_load(rs, 4u * (uint32_t)(rowA + cidxp[14 * ra + j]));
     }
   };
-  // coarse (rebuilt level-1) approximation of anchor az: spin s = az & 1, level-1 plane (az + s) / 2
+  // ---- phase-B role: band row kb = (zb * 2 + b2) * C + c, output rows xb .. xb + 21
+  // detail bands on threads 0 .. 188 (waves 0-2), the approximation band on
+  // threads 192 .. 254 (wave 3): no wave runs both kinds
+  int kb = -1, xr = 0;
+  if (tid < ND) {
+    kb = C + tid % (3 * C);
+    xr = tid / (3 * C);
+  } else if (tid >= 192 && tid < 192 + NCO) {
+    kb = (tid - 192) % C;
+    xr = (tid - 192) / C;
+  }
+  const bool actB = kb >= 0, coarseB = kb >= 0 && kb < C;
+  const int xb = xr == 0 ? 0 : (xr == 1 ? RB : TX - RB);  // runs at 0, 22, 42 (42, 43 stored by the middle run)
+  const int kbc = kb < 0 ? 0 : kb;
+  const int zbB = kbc / (2 * C), b2B = (kbc / C) & 1, cB = kbc % C;
+  const float tl = tau_of(tau, (b2B << 1) | (zbB << 2)), th = tau_of(tau, (b2B << 1) | (zbB << 2) | 1);
+  // ---- phase-C role: output row xo, output columns 8 yc .. 8 yc + 7 (yc = wave index)
+  const int xo = tid & (TX - 1);
+  const int yc = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool rowC = x0 + xo < p;
+  // output offsets: voffset ob0 (per lane) + a uniform part per column i
+  // (mosaic: p i; group layout gd x gd: gd (i % 8) within a group column for
+  // gd >= 8, and for gd = 4 the next group column after 4)
+  const int pc0 = y0 + 8 * yc;
+  int ob0 = 0, zstride, cst, gst = 0, cm = 7, csh = 3;
+  if (gd == 0) {
+    ob0 = x0 + xo + p * pc0;
+    zstride = p * q;
+    cst = p;
+  } else {
+    const int pr = x0 + xo;
+    const int g = pr / gd + (p / gd) * (pc0 / gd);
+    ob0 = pr % gd + gd * (pc0 % gd + gd * r * g);
+    zstride = gd * gd;
+    cst = gd;
+    if (gd == 4) {
+      cm = 3;
+      csh = 2;
+      gst = gd * gd * r * (p / gd);
+    }
+  }
+  const __amdgpu_buffer_rsrc_t ys = buf_rsrc(Y, (uint32_t)(4 * pq * r));
+  auto store_plane = [&](int z, const float (&u)[8], const float (&v)[8]) {
+    if (!rowC) return;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (pc0 + i >= q) continue;
+      const float mean = (u[i] + v[i]) / 2.f;  // mean(Vrec,4), then max(.,0) (denoiseAndAggregateGPU.m:45)
+      const int so = (i & cm) * cst + (i >> csh) * gst + z * zstride;
+      buf_store(ys, 4u * (uint32_t)ob0, mean > 0.f ? mean : 0.f, 4u * (uint32_t)so);
+    }
+  };
+  // ---- coarse (rebuilt level-1) approximation of anchor az, staged through registers
+  float yv[NYT];
   auto fetch_coarse = [&](int az) {
-    if (!coarse) return;
     const int s = az & 1;
     const int t3 = ((az + s) >> 1) % rh;
-    const float* Yl = (s ? Ylo1 : Ylo0) + (size_t)ph * qh * t3 + ycol[s][c3];
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(s ? Ylo1 : Ylo0, (uint32_t)(4 * (size_t)ph * qh * rh));
+    const uint32_t so = 4u * (uint32_t)(ph * qh * t3);
 #pragma unroll
-    for (int i = 0; i < 11; ++i) yl[i] = Yl[yrow[s][run3 * 10 + i]];
+    for (int i = 0; i < NYT; ++i) {
+      const int e = tid + 256 * i;
+      if (e < NYC) yv[i] = buf_load(rs, 4u * (uint32_t)(yrow[s][e % JX] + ycol[s][e / JX]), so);
+    }
   };
-  fetch_plane(prev, r - 1);
-  fetch_plane(cur, 0);
+  float P0[NJ], P1[NJ];  // planes of even / odd index
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) P0[j] = P1[j] = 0.f;
+  fetch_plane(P1, r - 1);
+  fetch_plane(P0, 0);
   fetch_coarse(0);
-  // this thread's outputs: row x, columns yq..yq+3 of every plane; offsets of
-  // plane 0 (-1: outside the image) and the plane stride, mosaic or grouped
-  // (volume < 2^31 elements)
-  const int xo = tid % P, yq = (tid / P) * 4;
-  int obase[4];
-  const int zstride = gd ? gd * gd : p * q;
+  float first[8], pend[8];
+  auto anchor = [&](auto sc, int az, float (&cur)[NJ], float (&prev)[NJ]) {
+    constexpr int s = decltype(sc)::value;
+    // phase A: z analysis of the pair (az-1, az), dim-2 analysis of both z bands
+    if (actA) {
+      float zl[NJ - 1], zh[NJ - 1];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int pr = x0 + xo, pc = y0 + yq + i;
-    obase[i] = -1;
-    if (pr < p && pc < q) {
-      if (gd == 0) {
-        obase[i] = pr + p * pc;
-      } else {
-        const int g = pr / gd + (p / gd) * (pc / gd);
-        obase[i] = pr % gd + gd * (pc % gd + gd * r * g);
+      for (int i = 0; i < NJ - 1; ++i) {
+        zl[i] = z_lo(cur[i + s], prev[i + s]);
+        zh[i] = z_hi(cur[i + s], prev[i + s]);
+      }
+      if (az + 1 < r) fetch_plane(prev, az + 1);  // in flight during the phases below
+      float lo[7], hi[7];
+      float* Aw = A + 7 * ra * AS + wa;
+      ana_lo_hi<7>(zl, lo, hi);
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        Aw[(0 * C + i) * AS] = lo[i];
+        Aw[(1 * C + i) * AS] = hi[i];
+      }
+      ana_lo_hi<7>(zh, lo, hi);
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        Aw[(2 * C + i) * AS] = lo[i];
+        Aw[(3 * C + i) * AS] = hi[i];
       }
     }
-  }
-  float first[4], pend[4];
-  for (int az = 0; az < r; ++az) {
-    const int s = az & 1;
-    // z analysis of the pair (az-1, az): a = plane az, b = plane az-1
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int e = tid + 256 * it;  // = column * W + row, ZS == W
-      if (e < W * W) {
-        Z[e] = fCc * cur[it] + fCc * prev[it];
-        Z[W * ZS + e] = fCc * prev[it] - fCc * cur[it];
-      }
+    for (int i = 0; i < NYT; ++i) {
+      const int e = tid + 256 * i;
+      if (e < NYC) Yc[e] = yv[i];
     }
-    if (coarse) {
-#pragma unroll
-      for (int i = 0; i < 11; ++i) ycs[t3r * 11 + i] = yl[i];
-    }
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) prev[it] = cur[it];
-    if (az + 1 < r) {  // in flight during the phases below
-      fetch_plane(cur, az + 1);
-      fetch_coarse(az + 1);
-    }
+    if (az + 1 < r) fetch_coarse(az + 1);
     __syncthreads();
-    // dim-2 analysis on the spin's sub-window (offset s in rows and columns):
-    // three runs of C/3 coefficients per (z band, row), 6R tasks over two rounds
-    static_assert(C % 3 == 0 && 6 * R <= 512, "phase-2 runs");
+    // phase B: dim-1 analysis, threshold, dim-1 synthesis
+    if (actB) {
+      const float* Ar = A + kbc * AS + xb + s;
+      float v[2 * NB + 8];
+      float lo[NB], hi[NB];
+      if (coarseB) {
 #pragma unroll
-    for (int r2 = 0; r2 < 2; ++r2) {
-      const int t2 = tid + 256 * r2;
-      if (t2 < 6 * R) {
-        const int rr = t2 % R, run = (t2 / R) % 3, zb = t2 / (3 * R);
-        const int c0 = run * (C / 3);
-        const float* Zc = Z + zb * W * ZS + (2 * c0 + s) * ZS + rr + s;
-        float v[2 * (C / 3) + 8];
+        for (int j = 4; j < 2 * NB + 4; ++j) v[j] = Ar[j];
+        ana_hi<NB>(v, hi);
+        const float* Yr = Yc + cB * JX + (xb >> 1);
 #pragma unroll
-        for (int j = 0; j < 2 * (C / 3) + 8; ++j) v[j] = Zc[j * ZS];
-        float lo[C / 3], hi[C / 3];
-        ana_lo_hi<C / 3>(v, lo, hi);
-#pragma unroll
-        for (int i = 0; i < C / 3; ++i) {
-          A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
-          A[((zb * 2 + 1) * C + c0 + i) * AS + rr] = hi[i];
+        for (int i = 0; i < NB; ++i) {
+          lo[i] = Yr[i];
+          hi[i] = hard_thr(hi[i], th);
         }
-      }
-    }
-    __syncthreads();
-    // dim-1 analysis + hard threshold (the approximation is the rebuilt coarser level)
-    if (t3r >= 0) {
-      const int t0 = run3 * 10;
-      const float* Ac = A + ((zb3 * 2 + b23) * C + c3) * AS + 2 * t0;
-      float* Bl = B + ((zb3 * 4 + (b23 << 1)) * C + c3) * BS + t0;
-      float* Bh = Bl + C * BS;
-      float v[30], hi[11];
-      if (coarse) {
-#pragma unroll
-        for (int j = 4; j < 26; ++j) v[j] = Ac[j];
-        ana_hi<11>(v, hi);
-#pragma unroll
-        for (int i = 0; i < 11; ++i) Bl[i] = ycs[t3r * 11 + i];
       } else {
 #pragma unroll
-        for (int j = 0; j < 30; ++j) v[j] = Ac[j];
-        float lo[11];
-        ana_lo_hi<11>(v, lo, hi);
+        for (int j = 0; j < 2 * NB + 8; ++j) v[j] = Ar[j];
+        ana_lo_hi<NB>(v, lo, hi);
 #pragma unroll
-        for (int i = 0; i < 11; ++i) Bl[i] = lo[i] * (fabsf(lo[i]) > tl ? 1.f : 0.f);
+        for (int i = 0; i < NB; ++i) {
+          lo[i] = hard_thr(lo[i], tl);
+          hi[i] = hard_thr(hi[i], th);
+        }
       }
+      float out[RB];
+      if constexpr (s == 0) {
+        syn_lo_hi<RB>(lo, hi, out);
+      } else {
+        syn_lo_hi_odd<RB>(lo, hi, out);
+      }
+      float* Er = E + kbc * ES + xb;
 #pragma unroll
-      for (int i = 0; i < 11; ++i) Bh[i] = hi[i] * (fabsf(hi[i]) > th ? 1.f : 0.f);
+      for (int i = 0; i < RB; ++i)
+        if (i >= 2 || xr != 2) Er[i] = out[i];
     }
     __syncthreads();
-    // dim-1 synthesis: 16 outputs per (z band, dim-2 band, coefficient column),
-    // E indexed by the output row of the tile (sub-window row + s)
-    if (tid < 2 * 2 * 2 * C) {
-      const int c = tid % C, xr = (tid / C) & 1, b2 = (tid / (2 * C)) & 1, zb = tid / (4 * C);
-      const int xl = xr * 16;
-      const float* Bl = B + ((zb * 4 + (b2 << 1)) * C + c) * BS + xl / 2;
-      const float* Bh = Bl + C * BS;
-      float* Eo = E + ((zb * 2 + b2) * C + c) * ES + xl;
-      float out[16];
-      if (s == 0) {
-        float lo[13], hi[13];
-#pragma unroll
-        for (int j = 0; j < 13; ++j) {
-          lo[j] = Bl[j];
-          hi[j] = Bh[j];
-        }
-        syn_lo_hi<16>(lo, hi, out);
-      } else {  // sub-window outputs xl-1 .. xl+14: coefficients xl/2 .. xl/2+11
-        float lo[12], hi[12];
-#pragma unroll
-        for (int j = 0; j < 12; ++j) {
-          lo[j] = Bl[j];
-          hi[j] = Bh[j];
-        }
-        syn_lo_hi_odd<16>(lo, hi, out);
-      }
-#pragma unroll
-      for (int i = 0; i < 16; ++i) Eo[i] = out[i];
-    }
-    __syncthreads();
-    // dim-2 synthesis (runs of 4 tile columns), z synthesis, spin combine, store
+    // phase C: dim-2 synthesis, z synthesis, spin combine, store
     {
-      float o[2][4];
+      constexpr int NCC = s == 0 ? 9 : 8;
+      float o[2][8];
 #pragma unroll
       for (int zb = 0; zb < 2; ++zb) {
-        const float* El = E + ((zb * 2 + 0) * C + yq / 2) * ES + xo;
-        const float* Eh = E + ((zb * 2 + 1) * C + yq / 2) * ES + xo;
-        if (s == 0) {
-          float lo[7], hi[7];
+        const float* El = E + ((zb * 2 + 0) * C + 4 * yc) * ES + xo;
+        const float* Eh = El + C * ES;
+        float lo[NCC], hi[NCC];
 #pragma unroll
-          for (int j = 0; j < 7; ++j) {
-            lo[j] = El[j * ES];
-            hi[j] = Eh[j * ES];
-          }
-          syn_lo_hi<4>(lo, hi, o[zb]);
+        for (int j = 0; j < NCC; ++j) {
+          lo[j] = El[j * ES];
+          hi[j] = Eh[j * ES];
+        }
+        if constexpr (s == 0) {
+          syn_lo_hi<8>(lo, hi, o[zb]);
         } else {
-          float lo[6], hi[6];
-#pragma unroll
-          for (int j = 0; j < 6; ++j) {
-            lo[j] = El[j * ES];
-            hi[j] = Eh[j * ES];
-          }
-          syn_lo_hi_odd<4>(lo, hi, o[zb]);
+          syn_lo_hi_odd<8>(lo, hi, o[zb]);
         }
       }
-      float ve[4], vo[4];  // plane az (even tap g[0]) and plane az-1 (odd tap g[1])
+      float ve[8], vo[8];  // plane az (even tap g[0]) and plane az-1 (odd tap g[1])
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        ve[i] = fCc * o[0][i] - fCc * o[1][i];
-        vo[i] = fCc * o[0][i] + fCc * o[1][i];
+      for (int i = 0; i < 8; ++i) {
+        ve[i] = z_even(o[0][i], o[1][i]);
+        vo[i] = z_odd(o[0][i], o[1][i]);
       }
-      auto store_plane = [&](int z, const float (&u)[4], const float (&v)[4]) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if (obase[i] < 0) continue;
-          const float mean = (u[i] + v[i]) / 2.f;  // mean(Vrec,4), then max(.,0) (denoiseAndAggregateGPU.m:45)
-          Y[obase[i] + z * zstride] = mean > 0.f ? mean : 0.f;
-        }
-      };
       if (az == 0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) first[i] = vo[i];  // plane r-1, completed by anchor r-1
+        for (int i = 0; i < 8; ++i) first[i] = vo[i];  // plane r-1, completed by anchor r-1
       } else {
         store_plane(az - 1, pend, vo);
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) pend[i] = ve[i];
+      for (int i = 0; i < 8; ++i) pend[i] = ve[i];
       if (az == r - 1) store_plane(r - 1, first, pend);
     }
-    // No barrier here: the next anchor first writes Z (over B, last read in
-    // phase 4, behind the barrier above) and the coarse staging (read in phase
-    // 3), and writes A (over E, read in phase 5) only after its own first
-    // barrier, which every thread reaches after phase 5.
+    // No barrier here: the next anchor writes A (read in phase B, behind the
+    // barrier above) and the coarse staging, and writes E only after its own
+    // first barrier, which every thread reaches after this phase.
+  };
+  for (int az = 0; az < r; az += 2) {  // r is even: anchors in spin pairs, plane az in P0, az+1 in P1
+    anchor(std::integral_constant<int, 0>{}, az, P0, P1);
+    anchor(std::integral_constant<int, 1>{}, az + 1, P1, P0);
   }
 }
 
@@ -840,7 +868,7 @@ __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int e = tid + 256 * it;
-      if (e < R * R) Z[(e / R) * ZS + e % R] = fCc * a[it] + fCc * b[it];
+      if (e < R * R) Z[(e / R) * ZS + e % R] = z_lo(a[it], b[it]);
     }
   }
   __syncthreads();
@@ -928,7 +956,7 @@ __global__ __launch_bounds__(256) void ht_lo0_cs2_kernel(const float* __restrict
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int e = tid + 256 * it;  // = column * U + row
-      if (e < U * U) Z[e] = s ? fCc * pb[it] + fCc * pa[it] : fCc * pc[it] + fCc * pb[it];
+      if (e < U * U) Z[e] = s ? z_lo(pb[it], pa[it]) : z_lo(pc[it], pb[it]);
     }
     __syncthreads();
     for (int e = tid; e < 4 * R; e += 256) {
@@ -1014,7 +1042,9 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
   // pass over the volume.  GBM3D_HT_UNFUSED=1 keeps the per-spin kernels, for
   // cross-checking (the two give bitwise equal results).
   const char* unf = getenv("GBM3D_HT_UNFUSED");
-  const bool fused = cycleSpin == 2 && M >= 10 && N >= 10 && (size_t)M * N * K < ((size_t)1 << 31) &&
+  // (the fused kernel addresses the volume through 32-bit byte offsets: < 2^30 elements)
+  const bool fused = cycleSpin == 2 && M >= 10 && N >= 10 && (size_t)M * N * K < ((size_t)1 << 30) &&
+                     (gd == 0 || gd == 4 || gd == 8 || gd == 16 || gd == 32) &&
                      !g_ht_generic && !(unf && unf[0] == '1');
   if (fused) {
     // level-0 approximation of both spins from one read; coarser levels of
@@ -1104,7 +1134,7 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
   if (fused) {
     const float* y0 = levels == 1 ? w.lo[0][0] : w.rec[0][0];
     const float* y1 = levels == 1 ? w.lo[0][1] : w.rec[0][1];
-    dim3 grid((M + f2::P - 1) / f2::P, (N + f2::P - 1) / f2::P, 1);
+    dim3 grid((M + f2::TX - 1) / f2::TX, (N + f2::TY - 1) / f2::TY, 1);
     mark(GBM3D_STAGE_HT_SYN0, 1);
     hipLaunchKernelGGL(ht_syn0_cs2_kernel, grid, dim3(256), 0, st, vol, y0, y1, M, N, K, tl[0], vavg, gd, run);
     GBM3D_TRY(check_launch("ht_syn0_cs2"));

@@ -57,8 +57,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base, ui
 __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes, uint32_t soff_bytes = 0) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, soff_bytes, 0));
 }
-__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off_bytes, 0, 0);
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes, float v, uint32_t soff_bytes = 0) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off_bytes, soff_bytes, 0);
 }
 
 // Ascending bitonic sort of one value per lane across the 64 lanes.

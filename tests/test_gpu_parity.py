@@ -451,7 +451,13 @@ def test_colored_noise_vs_oracle(ctx):
     U32, U132 = O.gbm3d_colored(noisy, psd, sigma_wie, opts, precision=32)
     assert np.max(np.abs(out["U1"] - U132)) <= 1e-3
     U64, _ = O.gbm3d_colored(noisy, psd, sigma_wie, opts, precision=64)
-    assert abs(synth.psnr(clean, U) - synth.psnr(clean, U64)) <= 0.01
+    # the end-to-end PSNR bar of the other chained tests: within 0.01 dB of the
+    # FP32 restatement, and no further from the FP64 oracle than it is + 0.01 dB
+    # (a U1 within round-off of the restatement's can still flip near-tied
+    # step-2 matches on a 96 x 80 image, where 0.01 dB is 0.2 % of the MSE)
+    p_gpu, p32, p64 = synth.psnr(clean, U), synth.psnr(clean, U32), synth.psnr(clean, U64)
+    assert abs(p_gpu - p32) <= 0.01
+    assert abs(p_gpu - p64) <= abs(p32 - p64) + 0.01
     U2 = O.step2_from_pilot(out["U1"], noisy, 1.0, opts, sigma_wie=sigma_wie)   # chained step 2
     assert np.max(np.abs(U - U2)) <= 1e-3
     with pytest.raises(GBM3DError, match="sigmaWie"):
