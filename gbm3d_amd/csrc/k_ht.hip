@@ -252,11 +252,23 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
 // operations per coefficient (12 taps) and 4 per output (6 taps).  Every
 // value is a fixed function of its own inputs, whatever the run it is
 // computed in, so kernels that tile the axes differently agree bitwise.
+//
+// The z axis is Haar: lo = Cc (a + b), hi = Cc (b - a) and back.  Its factor
+// Cc is folded into the dim-2 filters next to it (ZF = true: constants Cc Ca,
+// Cc Cb and Cc Cc = 1/2, applied to the unscaled z sums and differences), so
+// the z steps are one addition or subtraction each.
+constexpr float zCa = (float)(kCc * kCa), zCb = (float)(kCc * kCb), zCc = 0.5f;
+
+template <bool ZF>
+struct Taps {
+  static constexpr float a = ZF ? zCa : fCa, b = ZF ? zCb : fCb, c = ZF ? zCc : fCc;
+};
 
 // lo[i] = sum_k a_lo(k) v[2i+9-k], hi[i] = sum_k a_hi(k) v[2i+9-k], i < R
-template <int R, int NV>
+template <int R, bool ZF = false, int NV>
 __device__ __forceinline__ void ana_lo_hi(const float (&v)[NV], float (&lo)[R], float (&hi)[R]) {
   static_assert(NV >= 2 * R + 8, "analysis window");
+  using T = Taps<ZF>;
   float D[R + 4], S[R + 4];
 #pragma unroll
   for (int j = 0; j < R + 4; ++j) {
@@ -265,14 +277,15 @@ __device__ __forceinline__ void ana_lo_hi(const float (&v)[NV], float (&lo)[R], 
   }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
-    lo[i] = fmaf(fCa, D[i] - D[i + 4], fmaf(fCb, D[i + 3] - D[i + 1], fCc * S[i + 2]));
-    hi[i] = fCc * D[i + 2];
+    lo[i] = fmaf(T::a, D[i] - D[i + 4], fmaf(T::b, D[i + 3] - D[i + 1], T::c * S[i + 2]));
+    hi[i] = T::c * D[i + 2];
   }
 }
 
-template <int R, int NV>
+template <int R, bool ZF = false, int NV>
 __device__ __forceinline__ void ana_lo(const float (&v)[NV], float (&lo)[R]) {
   static_assert(NV >= 2 * R + 8, "analysis window");
+  using T = Taps<ZF>;
   float D[R + 4], S[R + 4];
 #pragma unroll
   for (int j = 0; j < R + 4; ++j) {
@@ -280,7 +293,7 @@ __device__ __forceinline__ void ana_lo(const float (&v)[NV], float (&lo)[R]) {
     S[j] = v[2 * j] + v[2 * j + 1];
   }
 #pragma unroll
-  for (int i = 0; i < R; ++i) lo[i] = fmaf(fCa, D[i] - D[i + 4], fmaf(fCb, D[i + 3] - D[i + 1], fCc * S[i + 2]));
+  for (int i = 0; i < R; ++i) lo[i] = fmaf(T::a, D[i] - D[i + 4], fmaf(T::b, D[i + 3] - D[i + 1], T::c * S[i + 2]));
 }
 
 // the high band of ana_lo_hi alone (reads v[4 .. 2R+3])
@@ -291,34 +304,38 @@ __device__ __forceinline__ void ana_hi(const float (&v)[NV], float (&hi)[R]) {
   for (int i = 0; i < R; ++i) hi[i] = fCc * (v[2 * i + 4] - v[2 * i + 5]);
 }
 
+template <bool ZF>
 __device__ __forceinline__ float syn_q(const float* hi, int j) {
-  return fmaf(fCa, hi[j + 4] - hi[j], fCb * (hi[j + 1] - hi[j + 3]));
+  using T = Taps<ZF>;
+  return fmaf(T::a, hi[j + 4] - hi[j], T::b * (hi[j + 1] - hi[j + 3]));
 }
 
 // out[i] = sum_{k: i+k even} s_lo(k) lo[(i+k)/2] + s_hi(k) hi[(i+k)/2], i < R2
 // (the window starts at an even output)
-template <int R2, int NC>
+template <int R2, bool ZF = false, int NC>
 __device__ __forceinline__ void syn_lo_hi(const float (&lo)[NC], const float (&hi)[NC], float (&out)[R2]) {
   static_assert(NC >= R2 / 2 + 5 && R2 % 2 == 0, "synthesis window");
+  using T = Taps<ZF>;
 #pragma unroll
   for (int j = 0; j <= R2 / 2; ++j) {
-    const float q = syn_q(hi, j);
-    if (j < R2 / 2) out[2 * j] = fmaf(fCc, lo[j + 2] - hi[j + 2], q);
-    if (j >= 1) out[2 * j - 1] = fmaf(fCc, lo[j + 2] + hi[j + 2], q);
+    const float q = syn_q<ZF>(hi, j);
+    if (j < R2 / 2) out[2 * j] = fmaf(T::c, lo[j + 2] - hi[j + 2], q);
+    if (j >= 1) out[2 * j - 1] = fmaf(T::c, lo[j + 2] + hi[j + 2], q);
   }
 }
 
 // the same synthesis for a window that starts at an odd output 2m-1, its
 // coefficient window starting at m: out[i] = sum_{k: i+k odd} s_lo(k)
 // lo[(i+k-1)/2] + s_hi(k) hi[(i+k-1)/2]
-template <int R2, int NC>
+template <int R2, bool ZF = false, int NC>
 __device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[NC], const float (&hi)[NC], float (&out)[R2]) {
   static_assert(NC >= R2 / 2 + 4 && R2 % 2 == 0, "synthesis window");
+  using T = Taps<ZF>;
 #pragma unroll
   for (int j = 0; j < R2 / 2; ++j) {
-    const float q = syn_q(hi, j);
-    out[2 * j] = fmaf(fCc, lo[j + 2] + hi[j + 2], q);
-    out[2 * j + 1] = fmaf(fCc, lo[j + 2] - hi[j + 2], q);
+    const float q = syn_q<ZF>(hi, j);
+    out[2 * j] = fmaf(T::c, lo[j + 2] + hi[j + 2], q);
+    out[2 * j + 1] = fmaf(T::c, lo[j + 2] - hi[j + 2], q);
   }
 }
 
@@ -328,11 +345,12 @@ __device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[NC], const float
 // max(., 0) can tell apart)
 __device__ __forceinline__ float hard_thr(float c, float t) { return fabsf(c) <= t ? 0.f : c; }
 
-// z analysis of the pair (a = plane 2t, b = plane 2t-1) and its synthesis
-__device__ __forceinline__ float z_lo(float a, float b) { return fCc * (a + b); }
-__device__ __forceinline__ float z_hi(float a, float b) { return fCc * (b - a); }
-__device__ __forceinline__ float z_even(float lo, float hi) { return fCc * (lo - hi); }  // plane 2t
-__device__ __forceinline__ float z_odd(float lo, float hi) { return fCc * (lo + hi); }   // plane 2t-1
+// z analysis of the pair (a = plane 2t, b = plane 2t-1) and its synthesis,
+// unscaled (the factor Cc sits in the dim-2 filters, ZF = true)
+__device__ __forceinline__ float z_sum(float a, float b) { return a + b; }
+__device__ __forceinline__ float z_dif(float a, float b) { return b - a; }
+__device__ __forceinline__ float z_even(float lo, float hi) { return lo - hi; }  // plane 2t
+__device__ __forceinline__ float z_odd(float lo, float hi) { return lo + hi; }   // plane 2t-1
 
 constexpr int kSynPairsPerBlock = 4;  // z-pairs per block of the register-blocked synthesis
 
@@ -446,8 +464,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       const int e = tid + 256 * it;
       if (e < R * R) {
         const int rr = e % R, cc = e / R;
-        Z[cc * ZS + rr] = z_lo(a[it], b[it]);
-        Z[R * ZS + cc * ZS + rr] = z_hi(a[it], b[it]);
+        Z[cc * ZS + rr] = z_sum(a[it], b[it]);
+        Z[R * ZS + cc * ZS + rr] = z_dif(a[it], b[it]);
       }
     }
     float ycur[11];
@@ -463,7 +481,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
 #pragma unroll
       for (int j = 0; j < 30; ++j) v[j] = Z[zb * R * ZS + (2 * c0 + j) * ZS + rr];
       float lo[11], hi[11];
-      ana_lo_hi<11>(v, lo, hi);
+      ana_lo_hi<11, true>(v, lo, hi);
 #pragma unroll
       for (int i = 0; i < 11; ++i) {
         A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
@@ -525,7 +543,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
           lo[j] = E[((zb * 2 + 0) * C + yq / 2 + j) * ES + x];
           hi[j] = E[((zb * 2 + 1) * C + yq / 2 + j) * ES + x];
         }
-        syn_lo_hi<4>(lo, hi, o[zb]);
+        syn_lo_hi<4, true>(lo, hi, o[zb]);
       }
       const int ze = pmod_i(2 * t3 - sh, r), zo = pmod_i(2 * t3 - 1 - sh, r);
       if (x0 + x < p) {
@@ -622,13 +640,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   // interior column range (no wrap): column offsets p * j on a per-thread base
   const bool colin = y0 - 9 >= 0 && y0 + TY + 9 < q;
   const uint32_t voffA = 4u * (uint32_t)(rowA + (colin ? p * (y0 - 9 + 14 * ra) : 0));
-  const uint32_t pbytes = (uint32_t)(4 * pq);
+  const uint32_t pbytes = (uint32_t)(4 * pq), cstride = (uint32_t)(4 * p);
   auto fetch_plane = [&](float (&dst)[NJ], int z) {
     const __amdgpu_buffer_rsrc_t rs = buf_rsrc(X + pq * z, pbytes);
     if (!actA) return;
     if (colin) {
+      // column j at scalar offset 4 p j, stepped by an opaque scalar add so
+      // that the 23 offsets are not all kept live across the anchor loop
+      uint32_t so = 0;
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) dst[j] = buf_load(rs, voffA, (uint32_t)(4 * p * j));
+      for (int j = 0; j < NJ; ++j) {
+        dst[j] = buf_load(rs, voffA, so);
+        so = sadd_u32(so, cstride);
+      }
     } else {
 #pragma unroll
       for (int j = 0; j < NJ; ++j) dst[j] = buf_load(rs, 4u * (uint32_t)(rowA + cidxp[14 * ra + j]));
@@ -713,19 +737,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       float zl[NJ - 1], zh[NJ - 1];
 #pragma unroll
       for (int i = 0; i < NJ - 1; ++i) {
-        zl[i] = z_lo(cur[i + s], prev[i + s]);
-        zh[i] = z_hi(cur[i + s], prev[i + s]);
+        zl[i] = z_sum(cur[i + s], prev[i + s]);
+        zh[i] = z_dif(cur[i + s], prev[i + s]);
       }
       if (az + 1 < r) fetch_plane(prev, az + 1);  // in flight during the phases below
       float lo[7], hi[7];
       float* Aw = A + 7 * ra * AS + wa;
-      ana_lo_hi<7>(zl, lo, hi);
+      ana_lo_hi<7, true>(zl, lo, hi);
 #pragma unroll
       for (int i = 0; i < 7; ++i) {
         Aw[(0 * C + i) * AS] = lo[i];
         Aw[(1 * C + i) * AS] = hi[i];
       }
-      ana_lo_hi<7>(zh, lo, hi);
+      ana_lo_hi<7, true>(zh, lo, hi);
 #pragma unroll
       for (int i = 0; i < 7; ++i) {
         Aw[(2 * C + i) * AS] = lo[i];
@@ -791,9 +815,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
           hi[j] = Eh[j * ES];
         }
         if constexpr (s == 0) {
-          syn_lo_hi<8>(lo, hi, o[zb]);
+          syn_lo_hi<8, true>(lo, hi, o[zb]);
         } else {
-          syn_lo_hi_odd<8>(lo, hi, o[zb]);
+          syn_lo_hi_odd<8, true>(lo, hi, o[zb]);
         }
       }
       float ve[8], vo[8];  // plane az (even tap g[0]) and plane az-1 (odd tap g[1])
@@ -868,7 +892,7 @@ __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int e = tid + 256 * it;
-      if (e < R * R) Z[(e / R) * ZS + e % R] = z_lo(a[it], b[it]);
+      if (e < R * R) Z[(e / R) * ZS + e % R] = z_sum(a[it], b[it]);
     }
   }
   __syncthreads();
@@ -877,7 +901,7 @@ __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict
     float v[24], lo[8];
 #pragma unroll
     for (int j = 0; j < 24; ++j) v[j] = Z[(2 * c0 + j) * ZS + rr];
-    ana_lo<8>(v, lo);
+    ana_lo<8, true>(v, lo);
 #pragma unroll
     for (int i = 0; i < 8; ++i) T2[(c0 + i) * ZS + rr] = lo[i];
   }
@@ -956,7 +980,7 @@ __global__ __launch_bounds__(256) void ht_lo0_cs2_kernel(const float* __restrict
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int e = tid + 256 * it;  // = column * U + row
-      if (e < U * U) Z[e] = s ? z_lo(pb[it], pa[it]) : z_lo(pc[it], pb[it]);
+      if (e < U * U) Z[e] = s ? z_sum(pb[it], pa[it]) : z_sum(pc[it], pb[it]);
     }
     __syncthreads();
     for (int e = tid; e < 4 * R; e += 256) {
@@ -964,7 +988,7 @@ __global__ __launch_bounds__(256) void ht_lo0_cs2_kernel(const float* __restrict
       float v[24], lo[8];
 #pragma unroll
       for (int j = 0; j < 24; ++j) v[j] = Z[(2 * c0 + j + off) * ZS + rr + off];
-      ana_lo<8>(v, lo);
+      ana_lo<8, true>(v, lo);
 #pragma unroll
       for (int i = 0; i < 8; ++i) T2[(c0 + i) * ZS + rr] = lo[i];
     }

@@ -61,6 +61,14 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off_bytes, soff_bytes, 0);
 }
 
+// a + b in a scalar register, opaque to the optimiser (which would otherwise
+// hoist every multiple of a loop-invariant stride and keep them all live)
+__device__ __forceinline__ uint32_t sadd_u32(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm volatile("s_add_u32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b));
+  return r;
+}
+
 // Ascending bitonic sort of one value per lane across the 64 lanes.
 // N = 32: each half of the wave sorted on its own, the lower ascending.
 template <class T, int N = 64>
