@@ -448,7 +448,7 @@ def global_match_report(args):
 
 # kernel symbol prefix -> roofline key; the largest dispatch of a kernel is the
 # one measured (the finest level for the wavelet synthesis)
-TRAFFIC_KERNELS = {"gbm3d::ht_syn0_cs2_kernel": "ht_syn0", "void gbm3d::match_kernel": "match2",
+TRAFFIC_KERNELS = {"void gbm3d::ht_fine_kernel<2>": "ht_syn0", "void gbm3d::match_kernel": "match2",
                    "void gbm3d::wiener_kernel": "wiener", "void gbm3d::aggregate_batched_kernel<1": "agg1",
                    "void gbm3d::aggregate_batched_kernel<2": "agg2", "void gbm3d::agg1_prep_wave_kernel": "agg1_prep"}
 

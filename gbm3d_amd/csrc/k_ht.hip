@@ -606,11 +606,25 @@ static_assert(42 + 1 + 2 * (NB - 1) + 9 < WX, "phase-B rows inside the window");
 static_assert(14 * 2 + NJ <= WY, "phase-A columns inside the window");
 }  // namespace f2
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_syn0_cs2_kernel(
-    const float* __restrict__ X, const float* __restrict__ Ylo0, const float* __restrict__ Ylo1, int p, int q, int r,
-    Tau7 tau, float* __restrict__ Y, int gd, const int* __restrict__ run) {
+// CS = 2: the finest level, both cycle spins fused (above).  CS = 1: one
+// level of one cycle spin with no shift (the coarse levels), every anchor at
+// parity 0 (z pairs (2t-1, 2t)), both rebuilt planes stored as they are
+// (Y = Vrec, mosaic layout); grid z selects the spin (X/Ylo0/Y or
+// X1/Ylo1/Y1).  The arithmetic per coefficient and per output is that of
+// ht_syn_fast_kernel, so the two agree bitwise.
+template <int CS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_fine_kernel(
+    const float* __restrict__ X, const float* __restrict__ X1, const float* __restrict__ Ylo0,
+    const float* __restrict__ Ylo1, int p, int q, int r, Tau7 tau, float* __restrict__ Y, float* __restrict__ Y1,
+    int gd, const int* __restrict__ run) {
   if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   using namespace f2;
+  static_assert(CS == 1 || CS == 2, "cycle spins per launch");
+  if (CS == 1 && blockIdx.z) {  // the second spin of a coarse level
+    X = X1;
+    Ylo0 = Ylo1;
+    Y = Y1;
+  }
   __shared__ float A[4 * C * AS];
   __shared__ float E[4 * C * ES];
   __shared__ float Yc[NYC];
@@ -700,6 +714,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
     }
   }
   const __amdgpu_buffer_rsrc_t ys = buf_rsrc(Y, (uint32_t)(4 * pq * r));
+  // CS = 1: the rebuilt plane as it is
+  auto store_vrec = [&](int z, const float (&v)[8]) {
+    if (!rowC) return;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (pc0 + i >= q) continue;
+      const int so = (i & cm) * cst + (i >> csh) * gst + z * zstride;
+      buf_store(ys, 4u * (uint32_t)ob0, v[i], 4u * (uint32_t)so);
+    }
+  };
   auto store_plane = [&](int z, const float (&u)[8], const float (&v)[8]) {
     if (!rowC) return;
 #pragma unroll
@@ -740,7 +764,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
         zl[i] = z_sum(cur[i + s], prev[i + s]);
         zh[i] = z_dif(cur[i + s], prev[i + s]);
       }
-      if (az + 1 < r) fetch_plane(prev, az + 1);  // in flight during the phases below
+      // in flight during the phases below: CS = 2, the next plane into the
+      // registers of plane az-1; CS = 1, the next pair into both
+      if constexpr (CS == 2) {
+        if (az + 1 < r) fetch_plane(prev, az + 1);
+      } else if (az + 2 < r) {
+        fetch_plane(prev, az + 1);
+        fetch_plane(cur, az + 2);
+      }
       float lo[7], hi[7];
       float* Aw = A + 7 * ra * AS + wa;
       ana_lo_hi<7, true>(zl, lo, hi);
@@ -761,7 +792,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       const int e = tid + 256 * i;
       if (e < NYC) Yc[e] = yv[i];
     }
-    if (az + 1 < r) fetch_coarse(az + 1);
+    constexpr int step = CS == 2 ? 1 : 2;  // the next anchor: both spins alternate, or one spin's pairs
+    if (az + step < r) fetch_coarse(az + step);
     __syncthreads();
     // phase B: dim-1 analysis, threshold, dim-1 synthesis
     if (actB) {
@@ -826,15 +858,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
         ve[i] = z_even(o[0][i], o[1][i]);
         vo[i] = z_odd(o[0][i], o[1][i]);
       }
-      if (az == 0) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) first[i] = vo[i];  // plane r-1, completed by anchor r-1
+      if constexpr (CS == 1) {
+        store_vrec(az, ve);
+        store_vrec(az == 0 ? r - 1 : az - 1, vo);
       } else {
-        store_plane(az - 1, pend, vo);
-      }
+        if (az == 0) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) pend[i] = ve[i];
-      if (az == r - 1) store_plane(r - 1, first, pend);
+          for (int i = 0; i < 8; ++i) first[i] = vo[i];  // plane r-1, completed by anchor r-1
+        } else {
+          store_plane(az - 1, pend, vo);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) pend[i] = ve[i];
+        if (az == r - 1) store_plane(r - 1, first, pend);
+      }
     }
     // No barrier here: the next anchor writes A (read in phase B, behind the
     // barrier above) and the coarse staging, and writes E only after its own
@@ -842,7 +879,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   };
   for (int az = 0; az < r; az += 2) {  // r is even: anchors in spin pairs, plane az in P0, az+1 in P1
     anchor(std::integral_constant<int, 0>{}, az, P0, P1);
-    anchor(std::integral_constant<int, 1>{}, az + 1, P1, P0);
+    if constexpr (CS == 2) anchor(std::integral_constant<int, 1>{}, az + 1, P1, P0);
   }
 }
 
@@ -1098,10 +1135,10 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
       const float* ylo1 = (l == levels - 1) ? w.lo[l][1] : w.rec[l][1];
       mark(GBM3D_STAGE_HT_SYN, 1);
       if (p >= 10 && q >= 10) {
-        const int zpb = kSynPairsPerBlock, nzc = (r / 2 + zpb - 1) / zpb;
-        dim3 gf((p + fs::P - 1) / fs::P, (q + fs::P - 1) / fs::P, 2 * nzc);
-        hipLaunchKernelGGL(ht_syn_fast_kernel, gf, dim3(256), 0, st, w.lo[l - 1][0], ylo0, p, q, r, 0, tl[l],
-                           w.rec[l - 1][0], nullptr, kStore, 2.f, 0, zpb, 2, w.lo[l - 1][1], ylo1, w.rec[l - 1][1], run);
+        // both spins of the level in one launch (grid z)
+        dim3 gf((p + f2::TX - 1) / f2::TX, (q + f2::TY - 1) / f2::TY, 2);
+        hipLaunchKernelGGL(ht_fine_kernel<1>, gf, dim3(256), 0, st, w.lo[l - 1][0], w.lo[l - 1][1], ylo0, ylo1, p, q,
+                           r, tl[l], w.rec[l - 1][0], w.rec[l - 1][1], 0, run);
       } else {
         dim3 grid((p + SP - 1) / SP, (q + SP - 1) / SP, r / 2);
         for (int s = 0; s < 2; ++s)
@@ -1160,7 +1197,8 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
     const float* y1 = levels == 1 ? w.lo[0][1] : w.rec[0][1];
     dim3 grid((M + f2::TX - 1) / f2::TX, (N + f2::TY - 1) / f2::TY, 1);
     mark(GBM3D_STAGE_HT_SYN0, 1);
-    hipLaunchKernelGGL(ht_syn0_cs2_kernel, grid, dim3(256), 0, st, vol, y0, y1, M, N, K, tl[0], vavg, gd, run);
+    hipLaunchKernelGGL(ht_fine_kernel<2>, grid, dim3(256), 0, st, vol, nullptr, y0, y1, M, N, K, tl[0], vavg, nullptr,
+                       gd, run);
     GBM3D_TRY(check_launch("ht_syn0_cs2"));
     mark(GBM3D_STAGE_HT_SYN0, 0);
   }
