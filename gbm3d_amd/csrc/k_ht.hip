@@ -16,6 +16,8 @@
 // X_{l+1}.  At level 0 cycle spin s reads the volume at (x - s) and writes
 // the rebuilt voxel back at (x - s), which is circshift(., [s s s]) and its
 // inverse.
+#include <cmath>
+
 #include "internal.hpp"
 #include "wave.hpp"
 
@@ -340,9 +342,9 @@ __device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[NC], const float
 }
 
 // C .* (abs(C) > tau) (denoiseAndAggregateGPU.m:37) as one compare and one
-// select; NaN is kept, as the product keeps it (a coefficient at or below
-// tau becomes +0 instead of a signed zero, which no later sum or the final
-// max(., 0) can tell apart)
+// select; a NaN coefficient is kept, as the product keeps it (a coefficient
+// at or below tau becomes +0 instead of a signed zero, which no later sum or
+// the final max(., 0) can tell apart).  A NaN tau arrives as +inf (run_ht).
 __device__ __forceinline__ float hard_thr(float c, float t) { return fabsf(c) <= t ? 0.f : c; }
 
 // z analysis of the pair (a = plane 2t, b = plane 2t-1) and its synthesis,
@@ -1097,7 +1099,13 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
   std::vector<Tau7> tl(levels);
   for (int l = 0; l < levels; ++l) {
     tl[l].t[0] = 0.f;
-    for (int j = 1; j <= 7; ++j) tl[l].t[j] = (float)tau[l + levels * (j - 1)];
+    // a NaN threshold (getColoredTau on a level coarser than its PSD) zeroes
+    // the band: C .* (abs(C) > NaN) is 0 for every finite C and NaN for a NaN
+    // C, which is what +inf gives with the kernels' compare-and-select
+    for (int j = 1; j <= 7; ++j) {
+      const double t = tau[l + levels * (j - 1)];
+      tl[l].t[j] = std::isnan(t) ? INFINITY : (float)t;
+    }
   }
   // Two cycle spins (every profile): the finest level of both spins in one
   // pass over the volume.  GBM3D_HT_UNFUSED=1 keeps the per-spin kernels, for
