@@ -10,9 +10,10 @@ Wiener gains p^2 / (p^2 + sigma^2) come from the pilot's grouped
 coefficients (source/denoiseAndAggregateWieFast.m:49).  This study runs step
 2 with the groups matched on one pilot and the gains taken from another
 (hard-threshold pilot U1 of the plain path, learned pilot, clean image +
-1e-3), so the two uses can be told apart, and prints the final PSNR of every
-combination plus the mean bias of each pilot's gains against the gains of
-the clean image on the same groups.
+1e-3, the noisy image itself), so the two uses can be told apart, and prints
+the final PSNR of every combination plus the mean bias of each pilot's gains
+against the gains of the clean image on the same groups; on the synthetic
+construction and on a textured image without flat regions.
 """
 from __future__ import annotations
 
@@ -47,14 +48,34 @@ def step2_mixed(group_pilot, gain_pilot, noisy, sigma):
     return Us / Ws
 
 
+def textured(seed: int, n: int) -> np.ndarray:
+    """A clean image without flat regions (two sinusoidal patterns and a
+    smooth random field), for contrast with the synthetic construction, whose
+    rectangles and disks are exactly flat."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:n, 0:n] / n
+    img = 128 + 60 * np.sin(2 * np.pi * (3 * x + 2 * y)) + 40 * np.cos(2 * np.pi * 7 * x * y)
+    f = np.fft.fft2(rng.standard_normal((n, n)))
+    ky, kx = np.meshgrid(np.fft.fftfreq(n), np.fft.fftfreq(n), indexing="ij")
+    field = np.real(np.fft.ifft2(f * np.exp(-(kx ** 2 + ky ** 2) / (2 * 0.03 ** 2))))
+    return np.clip(img + 30 * field / np.std(field), 0, 255)
+
+
 def main(size: int = 512) -> dict:
     params = load_weights()
     out = {}
-    for sigma, seed in ((10.0, 2048), (25.0, 2048), (25.0, 7), (50.0, 2048)):
-        clean, noisy = synth.make_pair(seed, size, size, sigma)
+    cases = [("synthetic", 10.0, 2048), ("synthetic", 25.0, 2048), ("synthetic", 25.0, 7), ("synthetic", 50.0, 2048),
+             ("textured", 25.0, 5)]
+    for kind, sigma, seed in cases:
+        if kind == "synthetic":
+            clean, noisy = synth.make_pair(seed, size, size, sigma)
+        else:
+            clean = textured(seed, size)
+            noisy = clean + sigma * np.random.default_rng(seed + 1).standard_normal((size, size))
         clean, noisy = clean.astype(np.float64), noisy.astype(np.float64)
         _, U1 = O.gbm3d(noisy, sigma, OPTS, precision=64)
-        pilots = {"ht": U1, "learned": so.learned_pilot(noisy, sigma, params, OPTS, precision=64), "clean": clean + 1e-3}
+        pilots = {"ht": U1, "learned": so.learned_pilot(noisy, sigma, params, OPTS, precision=64), "clean": clean + 1e-3,
+                  "noisy": noisy}
         final = {(a, b): synth.psnr(clean, step2_mixed(pilots[a], pilots[b], noisy, sigma)) for a in pilots for b in pilots}
         _, _, Vc, _ = O.match_wie(U1, clean, 8, 32, 16, precision=64)
         cc = so.coefficients(Vc)
@@ -66,8 +87,8 @@ def main(size: int = 512) -> dict:
             bias[name] = float(np.mean(cp ** 2 / (cp ** 2 + sigma ** 2) - wc))
         res = {"pilot_psnr": {k: synth.psnr(clean, v) for k, v in pilots.items()},
                "final_psnr": {f"{a}/{b}": v for (a, b), v in final.items()}, "gain_bias": bias}
-        out[(sigma, seed)] = res
-        print(f"sigma {sigma} seed {seed}: pilots", {k: round(v, 3) for k, v in res["pilot_psnr"].items()})
+        out[(kind, sigma, seed)] = res
+        print(f"{kind} sigma {sigma} seed {seed}: pilots", {k: round(v, 3) for k, v in res["pilot_psnr"].items()})
         print("  final PSNR, groups from the row's pilot, gains from the column's:")
         print("  " + " " * 9 + "".join(f"{b:>10s}" for b in pilots))
         for a in pilots:
