@@ -1207,7 +1207,7 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
     mark(GBM3D_STAGE_HT_SYN0, 1);
     hipLaunchKernelGGL(ht_fine_kernel<2>, grid, dim3(256), 0, st, vol, nullptr, y0, y1, M, N, K, tl[0], vavg, nullptr,
                        gd, run);
-    GBM3D_TRY(check_launch("ht_syn0_cs2"));
+    GBM3D_TRY(check_launch("ht_fine (finest level, both spins)"));
     mark(GBM3D_STAGE_HT_SYN0, 0);
   }
   return GBM3D_OK;
