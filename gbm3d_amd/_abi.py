@@ -138,6 +138,7 @@ SIGNATURES = {
     "gbm3d_resolve_opts": (_I, [C.POINTER(Opts), C.POINTER(Opts)]),
     "gbm3d_get_tau": (_I, [_I, _I, _D, _P]),
     "gbm3d_get_colored_tau": (_I, [_I, _P, _I, _I, _P]),
+    "gbm3d_get_colored_wie": (_I, [_P, _I, _I, _I, _P]),
     "gbm3d_denoise_device_noalloc": (_I, [_P, _P, _I, _I, C.c_float, C.POINTER(Opts), _P, _P]),
     "gbm3d_bf16_match_stats": (_I, [_P, C.POINTER(C.c_ulonglong), _I]),
     "gbm3d_denoise_aggregate_wie": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _I, C.c_float, _I, _P, _P]),

@@ -67,10 +67,8 @@ def check_sigma_wie(opts: dict | None, sigma_wie) -> None:
     DCT coefficients of every step-2 slice (denoiseAndAggregateWieFast.m:49):
     with the Wiener step on it must be exactly that size."""
     o = resolve_opts(opts)
-    if not o.get("Wiener"):
+    if not o.get("Wiener") or sigma_wie is None:  # None: the library computes getColoredWie
         return
-    if sigma_wie is None:
-        raise GBM3DError(GBM3D_ERR_UNSUPPORTED, "coloured noise: sigmaWie (getColoredWie) must be given")
     bw = o["blockSizeWie"]
     if np.shape(sigma_wie) != (bw, bw):
         raise GBM3DError(GBM3D_ERR_ARG, f"coloured noise: sigmaWie must be blockSizeWie x blockSizeWie "
@@ -83,6 +81,26 @@ def getTau(levels: int, mode: int, sigma: float = 1.0) -> np.ndarray:
     t = np.zeros((levels, 7), dtype=np.float64, order="F")
     check(lib.gbm3d_get_tau(levels, mode, C.c_double(sigma), ptr(t)))
     return t
+
+
+def getColoredTau(levels: int, sigmaPSD) -> np.ndarray:
+    """utilities/getColoredTau.m: thresholds (levels, 7) from a d1 x d2 sigma^2 x PSD."""
+    lib = load_library()
+    psd = np.asfortranarray(sigmaPSD, dtype=np.float64)
+    t = np.zeros((levels, 7), dtype=np.float64, order="F")
+    check(lib.gbm3d_get_colored_tau(levels, ptr(psd), psd.shape[0], psd.shape[1], ptr(t)))
+    return t
+
+
+def getColoredWie(sigmaPSD, d: int) -> np.ndarray:
+    """utilities/getColoredWie.m: the d x d per-DCT-coefficient Wiener sigma of
+    a d1 x d2 sigma^2 x PSD (makeGausPSF, absent from the reference, taken as
+    the unit-sum circular Gaussian of standard deviation 3 px)."""
+    lib = load_library()
+    psd = np.asfortranarray(sigmaPSD, dtype=np.float64)
+    w = np.zeros((d, d), dtype=np.float64, order="F")
+    check(lib.gbm3d_get_colored_wie(ptr(psd), psd.shape[0], psd.shape[1], d, ptr(w)))
+    return w
 
 
 def myPSNR(Iref, Irec, maxI=None) -> float:
@@ -237,9 +255,10 @@ class Context:
                                      C.byref(o) if o is not None else None, ptr(U), ptr(U1)))
         return U, {"U1": U1}
 
-    def denoise_colored(self, I, sigma_psd, sigma_wie, opts: dict | None = None):
+    def denoise_colored(self, I, sigma_psd, sigma_wie=None, opts: dict | None = None):
         """[U, out] = GBM3D(I, sigmaPSD, opts) for coloured noise; sigma_wie is
-        getColoredWie(sigmaPSD, blockSizeWie) (blockSizeWie x blockSizeWie)."""
+        getColoredWie(sigmaPSD, blockSizeWie) (blockSizeWie x blockSizeWie),
+        computed by the library when None (source/GBM3D.m:35)."""
         I = np.asfortranarray(I, dtype=np.float32)
         m, n = I.shape
         psd = np.asfortranarray(sigma_psd, dtype=np.float64)
@@ -447,8 +466,8 @@ def GBM3D(I, sigma, opts: dict | None = None, sigma_wie=None):
     """[U, out] = GBM3D(I, sigma, opts)  (source/GBM3D.m:1-84); out['U1'] is the pilot.
 
     sigma may be a matrix sigmaPSD (coloured noise, :29-36); then sigma_wie =
-    getColoredWie(sigmaPSD, blockSizeWie) must be given (it needs makeGausPSF,
-    which the reference does not contain)."""
+    getColoredWie(sigmaPSD, blockSizeWie), computed here when not given (with
+    a stand-in for makeGausPSF, which the reference does not contain)."""
     if np.ndim(sigma) == 2 and np.size(sigma) > 1:
         return default_context().denoise_colored(I, sigma, sigma_wie, opts)
     return default_context().denoise(I, float(np.asarray(sigma).ravel()[0]), opts)

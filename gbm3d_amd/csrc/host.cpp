@@ -4,8 +4,11 @@
 //   utilities/setBM3Dopts.m:12-86, utilities/checkBM3Dopts.m:10-24,
 //   source/GBM3D.m:15-19, utilities/getTau.m:9-51.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -164,6 +167,151 @@ void colored_tau(int levels, const double* psd, int d1, int d2, double* tau) {
   }
 }
 
+// imresize(A, [out_len ...], 'bilinear') along one axis, as MATLAB's imresize
+// builds it (the documented `contributions` construction): output sample i
+// (1-based) sits at u = i/s + (1 - 1/s)/2 of the input, s = out/in; the
+// triangle kernel, widened to 2/s and scaled by s when shrinking (antialiasing,
+// on by default), weighs input samples floor(u - w/2) + (0 .. ceil(w) + 1),
+// weights normalised to sum 1, indices past either end mirrored
+// ([1:n, n:-1:1]).  Returns, per output sample, its (index, weight) pairs.
+static std::vector<std::vector<std::pair<int, double>>> resize_weights(int in, int out) {
+  const double s = (double)out / in;
+  const bool shrink = s < 1.0;
+  const double kw = shrink ? 2.0 / s : 2.0;
+  auto tri = [](double x) { return (-1.0 <= x && x < 0.0) ? x + 1.0 : ((0.0 <= x && x <= 1.0) ? 1.0 - x : 0.0); };
+  auto h = [&](double x) { return shrink ? s * tri(s * x) : tri(x); };
+  std::vector<std::vector<std::pair<int, double>>> w(out);
+  const int P = (int)std::ceil(kw) + 2;
+  for (int i = 1; i <= out; ++i) {
+    const double u = i / s + 0.5 * (1.0 - 1.0 / s);
+    const long left = (long)std::floor(u - kw / 2.0);
+    double sum = 0.0;
+    std::vector<std::pair<int, double>> c;
+    for (int p = 0; p < P; ++p) {
+      const long idx = left + p;
+      const double wt = h(u - (double)idx);
+      // mirror into 1..in through the period-2n sequence [1:n, n:-1:1]
+      long m = ((idx - 1) % (2L * in) + 2L * in) % (2L * in);
+      const int src = (int)(m < in ? m : 2L * in - 1 - m);  // 0-based
+      c.emplace_back(src, wt);
+      sum += wt;
+    }
+    for (auto& e : c) e.second /= sum;
+    w[i - 1] = std::move(c);
+  }
+  return w;
+}
+
+// getColoredWie (utilities/getColoredWie.m:6-19): the PSD smoothed by a
+// Gaussian (makeGausPSF([d1 d2], 3): not part of the reference; here a
+// circular Gaussian of standard deviation 3 px centred at the origin with
+// unit sum, so that ifft2(ghat .* fft2(PSD)) is that circular convolution),
+// resized to d x d as imresize(., [d d], 'bilinear') does, then
+// sigmaWie(j, i) = sqrt(sum(PSD_d .* |fft2(idct2(e_ji))|^2)) / d.  idct2 of
+// the unit impulse at (j, i) is the outer product of rows j and i of the
+// orthonormal DCT-II matrix T, so |fft2(.)|^2 = A_j(u) A_i(v), A_k = |fft(T(k,:))|^2.
+void colored_wie(const double* psd, int d1, int d2, int d, double* sigmaWie) {
+  const double omega = 3.0;
+  // 1-D circular Gaussians (the 2-D one is their product; unit sum each)
+  auto gauss = [&](int n) {
+    std::vector<double> g(n);
+    double sum = 0.0;
+    for (int x = 0; x < n; ++x) {
+      const double dx = std::min(x, n - x);
+      g[x] = std::exp(-dx * dx / (2.0 * omega * omega));
+      sum += g[x];
+    }
+    for (double& v : g) v /= sum;
+    return g;
+  };
+  const std::vector<double> g1 = gauss(d1), g2 = gauss(d2);
+  // offsets -48..48 on long axes (taps past 16 standard deviations are below
+  // 1e-55 of the centre one), every offset of the circle on the others
+  auto taps = [](const std::vector<double>& g, int n) {
+    std::vector<std::pair<int, double>> t;
+    if (n <= 2 * 48 + 1) {
+      for (int x = 0; x < n; ++x) t.emplace_back(x, g[x]);
+    } else {
+      for (int x = -48; x <= 48; ++x) t.emplace_back(x, g[(x + n) % n]);
+    }
+    return t;
+  };
+  const auto t1 = taps(g1, d1), t2 = taps(g2, d2);
+  std::vector<double> a((size_t)d1 * d2), b((size_t)d1 * d2);
+  for (int c = 0; c < d2; ++c)  // along dim 1
+    for (int rr = 0; rr < d1; ++rr) {
+      double acc = 0.0;
+      for (const auto& [x, w] : t1) acc += w * psd[(size_t)(((rr - x) % d1 + d1) % d1) + (size_t)d1 * c];
+      a[rr + (size_t)d1 * c] = acc;
+    }
+  for (int c = 0; c < d2; ++c)  // along dim 2
+    for (int rr = 0; rr < d1; ++rr) {
+      double acc = 0.0;
+      for (const auto& [y, w] : t2) acc += w * a[rr + (size_t)d1 * (((c - y) % d2 + d2) % d2)];
+      b[rr + (size_t)d1 * c] = acc;
+    }
+  // imresize to d x d: along dim 1 first (equal scale factors, or the smaller
+  // one first, as imresize orders the passes), then dim 2
+  const bool rows_first = (double)d / d1 <= (double)d / d2;
+  std::vector<double> r((size_t)d * d);
+  if (rows_first) {
+    const auto w1 = resize_weights(d1, d), w2 = resize_weights(d2, d);
+    std::vector<double> m1((size_t)d * d2);
+    for (int c = 0; c < d2; ++c)
+      for (int i = 0; i < d; ++i) {
+        double acc = 0.0;
+        for (const auto& [k, w] : w1[i]) acc += w * b[k + (size_t)d1 * c];
+        m1[i + (size_t)d * c] = acc;
+      }
+    for (int j = 0; j < d; ++j)
+      for (int i = 0; i < d; ++i) {
+        double acc = 0.0;
+        for (const auto& [k, w] : w2[j]) acc += w * m1[i + (size_t)d * k];
+        r[i + (size_t)d * j] = acc;
+      }
+  } else {
+    const auto w1 = resize_weights(d1, d), w2 = resize_weights(d2, d);
+    std::vector<double> m2((size_t)d1 * d);
+    for (int j = 0; j < d; ++j)
+      for (int i = 0; i < d1; ++i) {
+        double acc = 0.0;
+        for (const auto& [k, w] : w2[j]) acc += w * b[i + (size_t)d1 * k];
+        m2[i + (size_t)d1 * j] = acc;
+      }
+    for (int j = 0; j < d; ++j)
+      for (int i = 0; i < d; ++i) {
+        double acc = 0.0;
+        for (const auto& [k, w] : w1[i]) acc += w * m2[k + (size_t)d1 * j];
+        r[i + (size_t)d * j] = acc;
+      }
+  }
+  // A_k(u) = |sum_p T(k, p) e^{-2 pi i u p / d}|^2, T(k, p) = c_k cos(pi (2p + 1) k / (2d))
+  std::vector<double> A((size_t)d * d);
+  for (int k = 0; k < d; ++k) {
+    const double ck = k == 0 ? std::sqrt(1.0 / d) : std::sqrt(2.0 / d);
+    for (int u = 0; u < d; ++u) {
+      double re = 0.0, im = 0.0;
+      for (int p = 0; p < d; ++p) {
+        const double t = ck * std::cos(M_PI * (2 * p + 1) * k / (2.0 * d));
+        const double ang = -2.0 * M_PI * (double)((u * p) % d) / d;
+        re += t * std::cos(ang);
+        im += t * std::sin(ang);
+      }
+      A[k + (size_t)d * u] = re * re + im * im;
+    }
+  }
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      double sum = 0.0;
+      for (int v = 0; v < d; ++v) {
+        double col = 0.0;
+        for (int u = 0; u < d; ++u) col += r[u + (size_t)d * v] * A[j + (size_t)d * u];
+        sum += col * A[i + (size_t)d * v];
+      }
+      sigmaWie[j + (size_t)d * i] = std::sqrt(sum) / d;
+    }
+}
+
 }  // namespace gbm3d
 
 extern "C" {
@@ -198,6 +346,13 @@ int gbm3d_get_colored_tau(int levels, const double* sigmaPSD, int d1, int d2, do
   if (levels < 1 || !sigmaPSD || !tau || d1 < 1 || d2 < 1)
     return gbm3d::fail(GBM3D_ERR_ARG, "get_colored_tau: bad argument");
   gbm3d::colored_tau(levels, sigmaPSD, d1, d2, tau);
+  return GBM3D_OK;
+}
+
+int gbm3d_get_colored_wie(const double* sigmaPSD, int d1, int d2, int d, double* sigmaWie) {
+  if (!sigmaPSD || !sigmaWie || d1 < 1 || d2 < 1 || d < 1 || d > 64)
+    return gbm3d::fail(GBM3D_ERR_ARG, "get_colored_wie: bad argument");
+  gbm3d::colored_wie(sigmaPSD, d1, d2, d, sigmaWie);
   return GBM3D_OK;
 }
 

@@ -46,6 +46,7 @@ int resolve(const gbm3d_opts* in, gbm3d_opts* out);
 void tau_table(int levels, int mode, double sigma, double* tau);
 // Coloured-noise thresholds (levels x 7, column-major) from sigma^2 * PSD (d1 x d2).
 void colored_tau(int levels, const double* psd, int d1, int d2, double* tau);
+void colored_wie(const double* psd, int d1, int d2, int d, double* sigmaWie);
 
 // ------------------------------------------------------------ the filters
 // bior1.5 (x, y) x Haar (z) of waveTransforms3D/getWaveFilters3Dbior.m:16-27,49-50.

@@ -417,8 +417,15 @@ int gbm3d_denoise_colored(gbm3d_ctx* c, const float* I, int m, int n, const doub
   if (c->ext.learned) return fail(GBM3D_ERR_UNSUPPORTED, "coloured noise: the learned form is not supported");
   gbm3d_opts o;
   GBM3D_TRY(resolve_checked(opts, &o));
-  if (o.Wiener && !sigmaWie)
-    return fail(GBM3D_ERR_UNSUPPORTED, "coloured noise: sigmaWie (getColoredWie) must be given");
+  // sigmaWie = getColoredWie(sigma, blockSizeWie) when the caller gives none
+  // (source/GBM3D.m:35)
+  std::vector<double> swie;
+  if (o.Wiener && !sigmaWie) {
+    swie.resize((size_t)o.blockSizeWie * o.blockSizeWie);
+    colored_wie(sigmaPSD, d1, d2, o.blockSizeWie, swie.data());
+    sigmaWie = swie.data();
+    wm = wn = o.blockSizeWie;
+  }
   // sigmaWie multiplies the blockSizeWie x blockSizeWie DCT coefficients of
   // every slice (denoiseAndAggregateWieFast.m:49): its size must match
   if (o.Wiener && (wm != o.blockSizeWie || wn != o.blockSizeWie))

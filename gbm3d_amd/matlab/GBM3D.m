@@ -10,11 +10,17 @@ if nargin < 3
 end
 if size(sigma,1) > 1
     % coloured noise (source/GBM3D.m:33-35): the per-coefficient Wiener sigma
-    % comes from the reference's getColoredWie (it needs makeGausPSF)
+    % from the reference's getColoredWie where its makeGausPSF is on the
+    % path, else [] (the library's getColoredWie, whose Gaussian stands in
+    % for makeGausPSF)
     o = opts;
     if isfield(o,'profile'), o = setBM3Dopts(o.profile); end
     o = checkBM3Dopts(o);
-    sigmaWie = getColoredWie(sigma,o.blockSizeWie);
+    if exist('makeGausPSF','file') && exist('getColoredWie','file')
+        sigmaWie = getColoredWie(sigma,o.blockSizeWie);
+    else
+        sigmaWie = [];
+    end
     [U,out] = gbm3d_mex('GBM3D_colored',I,sigma,sigmaWie,opts);
 else
     [U,out] = gbm3d_mex(I,sigma,opts);

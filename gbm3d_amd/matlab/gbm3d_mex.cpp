@@ -195,10 +195,12 @@ void colored(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   }
   ensure_ctx();
   std::vector<float> U(in.size()), U1(in.size());
-  // the library rejects a sigmaWie that is not blockSizeWie x blockSizeWie
-  const int wm = (int)mxGetM(prhs[3]), wn = (int)mxGetN(prhs[3]);
-  int st = gbm3d_denoise_colored(g_ctx, in.data(), m, n, mxGetPr(prhs[2]), d1, d2, mxGetPr(prhs[3]), wm, wn, op,
-                                 U.data(), U1.data());
+  // the library rejects a sigmaWie that is not blockSizeWie x blockSizeWie;
+  // an empty one ([]) is computed by the library (getColoredWie)
+  const bool have_wie = !mxIsEmpty(prhs[3]);
+  const int wm = have_wie ? (int)mxGetM(prhs[3]) : 0, wn = have_wie ? (int)mxGetN(prhs[3]) : 0;
+  int st = gbm3d_denoise_colored(g_ctx, in.data(), m, n, mxGetPr(prhs[2]), d1, d2,
+                                 have_wie ? mxGetPr(prhs[3]) : nullptr, wm, wn, op, U.data(), U1.data());
   if (st) fail(st);
   plhs[0] = to_double(U, m, n);
   if (nlhs > 1) {

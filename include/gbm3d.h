@@ -185,10 +185,9 @@ int gbm3d_denoise_distributed_device(gbm3d_ctx* ctx, const float* dI, int m,
  * source/GBM3D.m:29-36.  sigmaPSD: d1 x d2 (sigma^2 x the noise power
  * spectrum, column-major); thresholds from getColoredTau
  * (utilities/getColoredTau.m); sigmaWie: wm x wn per-coefficient Wiener
- * sigma of getColoredWie (utilities/getColoredWie.m), required when
- * opts.Wiener and then exactly blockSizeWie x blockSizeWie (any other size is
- * GBM3D_ERR_ARG; it needs makeGausPSF, which the reference does not contain,
- * so the caller computes it).  Host buffers.
+ * sigma of getColoredWie (utilities/getColoredWie.m), exactly blockSizeWie x
+ * blockSizeWie when given (any other size is GBM3D_ERR_ARG); NULL computes it
+ * with gbm3d_get_colored_wie, as source/GBM3D.m:35 does.  Host buffers.
  */
 int gbm3d_denoise_colored(gbm3d_ctx* ctx, const float* I, int m, int n,
                           const double* sigmaPSD, int d1, int d2,
@@ -197,6 +196,14 @@ int gbm3d_denoise_colored(gbm3d_ctx* ctx, const float* I, int m, int n,
 /* tau = getColoredTau(levels, sigmaPSD) (levels x 7, column-major). */
 int gbm3d_get_colored_tau(int levels, const double* sigmaPSD, int d1, int d2,
                           double* tau);
+/* sigmaWie = getColoredWie(sigmaPSD, d) -- utilities/getColoredWie.m:6-19
+ * (d x d, column-major; d <= 64): the PSD smoothed by makeGausPSF's Gaussian,
+ * resized to d x d as imresize(., [d d], 'bilinear'), then per DCT-II
+ * coefficient (j, i) sqrt(sum(PSD_d .* |fft2(idct2(e_ji))|^2)) / d.
+ * makeGausPSF is not part of the reference: the Gaussian is taken as the
+ * circular one of standard deviation 3 px at the origin with unit sum. */
+int gbm3d_get_colored_wie(const double* sigmaPSD, int d1, int d2, int d,
+                          double* sigmaWie);
 
 /*
  * gbm3d_denoise_device with no device allocation: no hipMalloc, hipFree or

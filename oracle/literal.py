@@ -657,3 +657,71 @@ def matchBlocksCPU_tol(I, sigma, d, d2, numMax, numMin):
             Volume[(i - 1) * d:i * d, (j - 1) * d:j * d, :s.size] = blocks
             S[i - 1][j - 1] = s
     return S, Volume
+
+
+# ------------------------------------------------------------ coloured noise
+def makeGausPSF(d1, d2, omega):
+    """Stand-in for makeGausPSF([d1 d2], omega), which the reference calls
+    (utilities/getColoredWie.m:6) but does not contain: the circular Gaussian
+    of standard deviation omega pixels centred at the origin, unit sum, and
+    its DFT (used as a blur by multiplication with fft2)."""
+    dx = np.minimum(np.arange(d1), d1 - np.arange(d1))[:, None].astype(float)
+    dy = np.minimum(np.arange(d2), d2 - np.arange(d2))[None, :].astype(float)
+    g = np.exp(-(dx ** 2 + dy ** 2) / (2.0 * omega ** 2))
+    g /= g.sum()
+    return g, np.fft.fft2(g)
+
+
+def _imresize_weights(n_in, n_out):
+    """MATLAB imresize's contributions for the 'bilinear' (triangle) kernel,
+    antialiased when shrinking: (indices 0-based, weights) per output sample."""
+    scale = n_out / n_in
+    kw = 2.0
+    tri = lambda x: (x + 1) * ((-1 <= x) & (x < 0)) + (1 - x) * ((0 <= x) & (x <= 1))
+    if scale < 1:
+        h = lambda x: scale * tri(scale * x)
+        kw = kw / scale
+    else:
+        h = tri
+    x = np.arange(1, n_out + 1, dtype=float)[:, None]
+    u = x / scale + 0.5 * (1 - 1 / scale)
+    left = np.floor(u - kw / 2)
+    P = int(np.ceil(kw)) + 2
+    idx = left + np.arange(P)[None, :]
+    w = h(u - idx)
+    w = w / w.sum(axis=1, keepdims=True)
+    aux = np.concatenate([np.arange(1, n_in + 1), np.arange(n_in, 0, -1)])
+    idx = aux[np.mod(idx.astype(int) - 1, len(aux))] - 1
+    return idx, w
+
+
+def imresize_bilinear(A, out_shape):
+    """imresize(A, out_shape, 'bilinear') for a 2-D double array: the
+    dimension with the smaller scale factor first (rows on a tie)."""
+    A = np.asarray(A, dtype=float)
+    s = (out_shape[0] / A.shape[0], out_shape[1] / A.shape[1])
+    order = (0, 1) if s[0] <= s[1] else (1, 0)
+    for dim in order:
+        idx, w = _imresize_weights(A.shape[dim], out_shape[dim])
+        if dim == 0:
+            A = np.einsum("ip,ipj->ij", w, A[idx, :])
+        else:
+            A = np.einsum("jp,ijp->ij", w, A[:, idx])
+    return A
+
+
+def getColoredWie(sigmaPSD, d):
+    """utilities/getColoredWie.m:6-19 as written (with the makeGausPSF stand-in)."""
+    sigmaPSD = np.asarray(sigmaPSD, dtype=float)
+    _, ghat = makeGausPSF(sigmaPSD.shape[0], sigmaPSD.shape[1], 3)
+    sigmaPSD = np.real(np.fft.ifft2(ghat * np.fft.fft2(sigmaPSD)))
+    sigmaPSD = imresize_bilinear(sigmaPSD, (d, d))
+    sigmaWie = np.zeros((d, d))
+    for i in range(d):
+        for j in range(d):
+            x = np.zeros((d, d))
+            x[j, i] = 1
+            y = scipy.fft.idctn(x, norm="ortho")
+            y = np.abs(np.fft.fft2(y)) ** 2
+            sigmaWie[j, i] = np.sqrt(np.sum(sigmaPSD * y)) / d
+    return sigmaWie

@@ -15,6 +15,7 @@ import os
 import numpy as np
 import pytest
 
+import gbm3d_amd
 import oracle as O
 from gbm3d_amd import synth
 from gbm3d_amd._abi import GBM3DError
@@ -460,5 +461,9 @@ def test_colored_noise_vs_oracle(ctx):
     assert abs(p_gpu - p64) <= abs(p32 - p64) + 0.01
     U2 = O.step2_from_pilot(out["U1"], noisy, 1.0, opts, sigma_wie=sigma_wie)   # chained step 2
     assert np.max(np.abs(U - U2)) <= 1e-3
-    with pytest.raises(GBM3DError, match="sigmaWie"):
-        ctx.denoise_colored(noisy, psd, None, opts)
+    # no sigmaWie: the library computes getColoredWie(sigmaPSD, blockSizeWie)
+    # itself (source/GBM3D.m:35), the same as passing its result
+    sw = gbm3d_amd.getColoredWie(psd, 8)
+    Ua, _ = ctx.denoise_colored(noisy, psd, None, opts)
+    Ub, _ = ctx.denoise_colored(noisy, psd, sw, opts)
+    assert np.array_equal(Ua, Ub)

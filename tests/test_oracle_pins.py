@@ -320,6 +320,52 @@ def test_library_colored_tau_matches_oracle():
     assert np.allclose(tau, O.colored_tau(3, psd), rtol=1e-10)
 
 
+# ---- getColoredWie (utilities/getColoredWie.m) with the makeGausPSF stand-in
+def test_imresize_restatement_halving_is_the_1331_kernel():
+    """imresize(A, size/2, 'bilinear') with antialiasing: the triangle kernel
+    widened to 4 samples gives weights [1 3 3 1]/8 on input rows 2i-2 .. 2i+1
+    (1-based), mirrored at the ends; scale 1 is the identity."""
+    rng = np.random.default_rng(21)
+    A = rng.random((16, 12))
+
+    def half(x):
+        xp = np.concatenate([[x[0]], x, [x[-1]]])         # 1-based rows 0 .. n+1, mirrored
+        return np.array([(xp[2 * i - 2] + 3 * xp[2 * i - 1] + 3 * xp[2 * i] + xp[2 * i + 1]) / 8
+                         for i in range(1, len(x) // 2 + 1)])
+    ref = np.apply_along_axis(half, 1, np.apply_along_axis(half, 0, A))
+    assert np.max(np.abs(L.imresize_bilinear(A, (8, 6)) - ref)) < 1e-15
+    assert np.array_equal(L.imresize_bilinear(A, (16, 12)), A)
+
+
+def test_colored_wie_white_psd_is_sigma():
+    """A flat PSD sigma^2 survives the unit-sum smoothing and the resize, and
+    sum |fft2(idct2(e))|^2 = d^2 (Parseval, unit basis images): sigma everywhere."""
+    import gbm3d_amd
+    for shape, d in (((32, 24), 8), ((40, 40), 16), ((6, 5), 4)):
+        psd = np.full(shape, 625.0)
+        assert np.allclose(L.getColoredWie(psd, d), 25.0, rtol=1e-12)
+        assert np.allclose(gbm3d_amd.getColoredWie(psd, d), 25.0, rtol=1e-12)
+
+
+def test_make_gaus_psf_stand_in_is_a_unit_sum_even_blur():
+    g, ghat = L.makeGausPSF(24, 20, 3)
+    assert abs(g.sum() - 1) < 1e-14 and abs(ghat[0, 0] - 1) < 1e-14
+    assert np.max(np.abs(ghat.imag)) < 1e-14                 # real: even about the origin
+
+
+@pytest.mark.parametrize("shape,d", [((64, 48), 8), ((16, 16), 8), ((6, 5), 8), ((30, 70), 4), ((200, 160), 8),
+                                     ((128, 96), 16)])
+def test_library_colored_wie_matches_literal(shape, d):
+    """The library's separable form (direct circular smoothing, resize weights,
+    |fft(T(k,:))|^2 products) against the literal transcription (fft2 blur,
+    imresize, idct2 of each impulse, fft2)."""
+    import gbm3d_amd
+    rng = np.random.default_rng(sum(shape) + d)
+    psd = rng.uniform(1, 100, shape)
+    ref = L.getColoredWie(psd, d)
+    assert np.max(np.abs(gbm3d_amd.getColoredWie(psd, d) - ref)) <= 1e-12 * np.max(ref)
+
+
 # ---- near-threshold masks: what a changed operation order may flip
 def test_ht_near_mask_covers_every_flip_of_shifted_thresholds():
     """Moving every threshold by delta < tol flips exactly the coefficients
