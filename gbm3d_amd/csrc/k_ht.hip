@@ -24,20 +24,37 @@
 namespace gbm3d {
 
 namespace {
-constexpr float fCa = (float)kCa, fCb = (float)kCb, fCc = (float)kCc;
+constexpr float pCa = (float)(kCa / kCc), pCb = (float)(kCb / kCc);
 
 // analysis lo / hi and synthesis lo / hi (getWaveFilters3Dbior.m:21-23,49-50)
+// in the primed scaling of the register-blocked kernels below: analysis taps
+// divided by Cc, synthesis taps times Cc and a gain g (1, or 1/2 to restore
+// the z axis's factor as well)
 __device__ __forceinline__ float a_lo(int k) {
-  const float t[10] = {fCa, -fCa, -fCb, fCb, fCc, fCc, fCb, -fCb, -fCa, fCa};
+  const float t[10] = {pCa, -pCa, -pCb, pCb, 1.f, 1.f, pCb, -pCb, -pCa, pCa};
   return t[k];
 }
-__device__ __forceinline__ float a_hi(int k) { return k == 4 ? -fCc : (k == 5 ? fCc : 0.f); }
-__device__ __forceinline__ float s_lo(int k) { return (k == 4 || k == 5) ? fCc : 0.f; }
+__device__ __forceinline__ float a_hi(int k) { return k == 4 ? -1.f : (k == 5 ? 1.f : 0.f); }
+template <int GS>
+__device__ __forceinline__ float s_lo(int k) {
+  constexpr float c = GS ? 0.25f : 0.5f;
+  return (k == 4 || k == 5) ? c : 0.f;
+}
+template <int GS>
 __device__ __forceinline__ float s_hi(int k) {
-  const float t[10] = {-fCa, -fCa, fCb, fCb, -fCc, fCc, -fCb, -fCb, fCa, fCa};
+  constexpr double g = GS ? 0.5 : 1.0;
+  constexpr float a = (float)(kCc * kCa * g), b = (float)(kCc * kCb * g), c = (float)(0.5 * g);
+  const float t[10] = {-a, -a, b, b, -c, c, -b, -b, a, a};
   return t[k];
 }
 }  // namespace
+
+// z analysis of the pair (a = plane 2t, b = plane 2t-1) and its synthesis,
+// unscaled (the synthesis factor Cc^2 sits in the in-plane taps, G = 1/2)
+__device__ __forceinline__ float z_sum(float a, float b) { return a + b; }
+__device__ __forceinline__ float z_dif(float a, float b) { return b - a; }
+__device__ __forceinline__ float z_even(float lo, float hi) { return lo - hi; }  // plane 2t
+__device__ __forceinline__ float z_odd(float lo, float hi) { return lo + hi; }   // plane 2t-1
 
 struct Tau7 {
   float t[8];  // t[j] for subband j = 1..7 (t[0] unused)
@@ -75,7 +92,7 @@ __global__ __launch_bounds__(256) void ht_lo_kernel(const float* __restrict__ X,
   for (int e = threadIdx.x; e < LR * LR; e += 256) {
     const int rr = e % LR, cc = e / LR;
     const size_t o = (size_t)pmod_i(rb + rr, p) + (size_t)p * pmod_i(cb + cc, q);
-    Z[cc * LRP + rr] = fCc * X0[o] + fCc * X1[o];
+    Z[cc * LRP + rr] = X0[o] + X1[o];
   }
   __syncthreads();
   const int n1 = min(10, p), n2 = min(10, q);
@@ -157,8 +174,8 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
     const int rr = e % SR, cc = e / SR;
     const size_t o = (size_t)pmod_i(rb + rr, p) + (size_t)p * pmod_i(cb + cc, q);
     const float a = X0[o], b = X1[o];
-    Z[0][cc * SRP + rr] = fCc * a + fCc * b;
-    Z[1][cc * SRP + rr] = -fCc * a + fCc * b;
+    Z[0][cc * SRP + rr] = z_sum(a, b);
+    Z[1][cc * SRP + rr] = z_dif(a, b);
   }
   __syncthreads();
   const int n1 = min(10, p), n2 = min(10, q);
@@ -202,7 +219,7 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
       for (int k = 0; k < 10; ++k) {
         if (k < n1 && ((x + k) & 1) == 0) {
           const int t = (x + k) >> 1;
-          acc += s_lo(k) * B[b2 << 1][c * CPP + t] + s_hi(k) * B[1 | (b2 << 1)][c * CPP + t];
+          acc += s_lo<0>(k) * B[b2 << 1][c * CPP + t] + s_hi<0>(k) * B[1 | (b2 << 1)][c * CPP + t];
         }
       }
       E[b2][c * SPP + x] = acc;
@@ -218,7 +235,7 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
       for (int k = 0; k < 10; ++k) {
         if (k < n2 && ((y + k) & 1) == 0) {
           const int c = (y + k) >> 1;
-          acc += s_lo(k) * E[0][c * SPP + x] + s_hi(k) * E[1][c * SPP + x];
+          acc += s_lo<1>(k) * E[0][c * SPP + x] + s_hi<1>(k) * E[1][c * SPP + x];
         }
       }
       out[zb][i] = acc;
@@ -233,8 +250,8 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
     const int x = e % SP, y = e / SP;
     if (x0 + x >= p || y0 + y >= q) continue;
     const int pr = pmod_i(x0 + x - sh, p), pc = pmod_i(y0 + y - sh, q);
-    put(Y, acc, pr, pc, ze, fCc * out[0][i] - fCc * out[1][i], p, q, r, mode, cs, gd);
-    put(Y, acc, pr, pc, zo, fCc * out[0][i] + fCc * out[1][i], p, q, r, mode, cs, gd);
+    put(Y, acc, pr, pc, ze, z_even(out[0][i], out[1][i]), p, q, r, mode, cs, gd);
+    put(Y, acc, pr, pc, zo, z_odd(out[0][i], out[1][i]), p, q, r, mode, cs, gd);
   }
 }
 
@@ -244,33 +261,38 @@ __global__ __launch_bounds__(256) void ht_syn_kernel(const float* __restrict__ X
 // window, computing the low and the high band from the same loads; zero taps
 // are skipped at compile time; circular indices come from per-tile tables.
 
-// The filters in lifting form.  With the input pairs D_j = v[2j] - v[2j+1],
-// S_j = v[2j] + v[2j+1] (a_lo and s_hi are symmetric up to sign),
+// The filters in lifting form, each axis's analysis without its factor Cc.
+// With the input pairs D_j = v[2j] - v[2j+1], S_j = v[2j] + v[2j+1] (a_lo and
+// s_hi are symmetric up to sign) the reference's coefficients are
 //   lo[i] = Ca (D_i - D_{i+4}) + Cb (D_{i+3} - D_{i+1}) + Cc S_{i+2},
-//   hi[i] = Cc D_{i+2},
-// and with Q_j = Ca (hi[j+4] - hi[j]) + Cb (hi[j+1] - hi[j+3]) the synthesis
-// outputs of coefficient window position j are Q_j + Cc (lo[j+2] - hi[j+2])
-// (even output 2j) and Q_j + Cc (lo[j+2] + hi[j+2]) (odd output 2j-1): 8
-// operations per coefficient (12 taps) and 4 per output (6 taps).  Every
+//   hi[i] = Cc D_{i+2};
+// the kernels compute lo' = lo / Cc and hi' = hi / Cc (taps Ca/Cc, Cb/Cc, 1:
+// 4 operations per low coefficient, none per high one), and along z the sum
+// a + b and difference b - a of the Haar pair.  A level's coefficients are
+// then (1/Cc)^3 = 2 sqrt(2) times the reference's, and its approximation,
+// the next level's input, too: level l (0 = finest) carries (2 sqrt 2)^(l+1),
+// and run_ht scales its thresholds by that.  With Q_j = Ca (hi[j+4] - hi[j])
+// + Cb (hi[j+1] - hi[j+3]), the synthesis outputs of coefficient window
+// position j are Q_j + Cc (lo[j+2] - hi[j+2]) (even output 2j) and Q_j + Cc
+// (lo[j+2] + hi[j+2]) (odd output 2j-1); on the primed coefficients the taps
+// are Cc Ca, Cc Cb and Cc Cc = 1/2, which restore the axis's own factor.  The
+// in-plane synthesis the z synthesis follows takes its taps times a gain G:
+// G = 1/2 restores z's factor Cc^2 as well (the z steps are then one addition
+// or subtraction each), and G = 1/4 also takes the mean over two cycle spins
+// (a power of two: the results are exactly those of G = 1/2, halved).  Every
 // value is a fixed function of its own inputs, whatever the run it is
 // computed in, so kernels that tile the axes differently agree bitwise.
-//
-// The z axis is Haar: lo = Cc (a + b), hi = Cc (b - a) and back.  Its factor
-// Cc is folded into the dim-2 filters next to it (ZF = true: constants Cc Ca,
-// Cc Cb and Cc Cc = 1/2, applied to the unscaled z sums and differences), so
-// the z steps are one addition or subtraction each.
-constexpr float zCa = (float)(kCc * kCa), zCb = (float)(kCc * kCb), zCc = 0.5f;
-
-template <bool ZF>
-struct Taps {
-  static constexpr float a = ZF ? zCa : fCa, b = ZF ? zCb : fCb, c = ZF ? zCc : fCc;
+// synthesis taps on primed coefficients, times the gain 2^-GS
+template <int GS>
+struct STaps {
+  static constexpr double g = GS == 0 ? 1.0 : (GS == 1 ? 0.5 : 0.25);
+  static constexpr float a = (float)(kCc * kCa * g), b = (float)(kCc * kCb * g), c = (float)(0.5 * g);
 };
 
-// lo[i] = sum_k a_lo(k) v[2i+9-k], hi[i] = sum_k a_hi(k) v[2i+9-k], i < R
-template <int R, bool ZF = false, int NV>
+// lo'[i] = sum_k a_lo(k) v[2i+9-k] / Cc, hi'[i] = sum_k a_hi(k) v[2i+9-k] / Cc, i < R
+template <int R, int NV>
 __device__ __forceinline__ void ana_lo_hi(const float (&v)[NV], float (&lo)[R], float (&hi)[R]) {
   static_assert(NV >= 2 * R + 8, "analysis window");
-  using T = Taps<ZF>;
   float D[R + 4], S[R + 4];
 #pragma unroll
   for (int j = 0; j < R + 4; ++j) {
@@ -279,15 +301,14 @@ __device__ __forceinline__ void ana_lo_hi(const float (&v)[NV], float (&lo)[R], 
   }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
-    lo[i] = fmaf(T::a, D[i] - D[i + 4], fmaf(T::b, D[i + 3] - D[i + 1], T::c * S[i + 2]));
-    hi[i] = T::c * D[i + 2];
+    lo[i] = fmaf(pCa, D[i] - D[i + 4], fmaf(pCb, D[i + 3] - D[i + 1], S[i + 2]));
+    hi[i] = D[i + 2];
   }
 }
 
-template <int R, bool ZF = false, int NV>
+template <int R, int NV>
 __device__ __forceinline__ void ana_lo(const float (&v)[NV], float (&lo)[R]) {
   static_assert(NV >= 2 * R + 8, "analysis window");
-  using T = Taps<ZF>;
   float D[R + 4], S[R + 4];
 #pragma unroll
   for (int j = 0; j < R + 4; ++j) {
@@ -295,7 +316,7 @@ __device__ __forceinline__ void ana_lo(const float (&v)[NV], float (&lo)[R]) {
     S[j] = v[2 * j] + v[2 * j + 1];
   }
 #pragma unroll
-  for (int i = 0; i < R; ++i) lo[i] = fmaf(T::a, D[i] - D[i + 4], fmaf(T::b, D[i + 3] - D[i + 1], T::c * S[i + 2]));
+  for (int i = 0; i < R; ++i) lo[i] = fmaf(pCa, D[i] - D[i + 4], fmaf(pCb, D[i + 3] - D[i + 1], S[i + 2]));
 }
 
 // the high band of ana_lo_hi alone (reads v[4 .. 2R+3])
@@ -303,24 +324,24 @@ template <int R, int NV>
 __device__ __forceinline__ void ana_hi(const float (&v)[NV], float (&hi)[R]) {
   static_assert(NV >= 2 * R + 8, "analysis window");
 #pragma unroll
-  for (int i = 0; i < R; ++i) hi[i] = fCc * (v[2 * i + 4] - v[2 * i + 5]);
+  for (int i = 0; i < R; ++i) hi[i] = v[2 * i + 4] - v[2 * i + 5];
 }
 
-template <bool ZF>
+template <int GS>
 __device__ __forceinline__ float syn_q(const float* hi, int j) {
-  using T = Taps<ZF>;
+  using T = STaps<GS>;
   return fmaf(T::a, hi[j + 4] - hi[j], T::b * (hi[j + 1] - hi[j + 3]));
 }
 
 // out[i] = sum_{k: i+k even} s_lo(k) lo[(i+k)/2] + s_hi(k) hi[(i+k)/2], i < R2
 // (the window starts at an even output)
-template <int R2, bool ZF = false, int NC>
+template <int R2, int GS, int NC>
 __device__ __forceinline__ void syn_lo_hi(const float (&lo)[NC], const float (&hi)[NC], float (&out)[R2]) {
   static_assert(NC >= R2 / 2 + 5 && R2 % 2 == 0, "synthesis window");
-  using T = Taps<ZF>;
+  using T = STaps<GS>;
 #pragma unroll
   for (int j = 0; j <= R2 / 2; ++j) {
-    const float q = syn_q<ZF>(hi, j);
+    const float q = syn_q<GS>(hi, j);
     if (j < R2 / 2) out[2 * j] = fmaf(T::c, lo[j + 2] - hi[j + 2], q);
     if (j >= 1) out[2 * j - 1] = fmaf(T::c, lo[j + 2] + hi[j + 2], q);
   }
@@ -329,13 +350,13 @@ __device__ __forceinline__ void syn_lo_hi(const float (&lo)[NC], const float (&h
 // the same synthesis for a window that starts at an odd output 2m-1, its
 // coefficient window starting at m: out[i] = sum_{k: i+k odd} s_lo(k)
 // lo[(i+k-1)/2] + s_hi(k) hi[(i+k-1)/2]
-template <int R2, bool ZF = false, int NC>
+template <int R2, int GS, int NC>
 __device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[NC], const float (&hi)[NC], float (&out)[R2]) {
   static_assert(NC >= R2 / 2 + 4 && R2 % 2 == 0, "synthesis window");
-  using T = Taps<ZF>;
+  using T = STaps<GS>;
 #pragma unroll
   for (int j = 0; j < R2 / 2; ++j) {
-    const float q = syn_q<ZF>(hi, j);
+    const float q = syn_q<GS>(hi, j);
     out[2 * j] = fmaf(T::c, lo[j + 2] + hi[j + 2], q);
     out[2 * j + 1] = fmaf(T::c, lo[j + 2] - hi[j + 2], q);
   }
@@ -346,13 +367,6 @@ __device__ __forceinline__ void syn_lo_hi_odd(const float (&lo)[NC], const float
 // at or below tau becomes +0 instead of a signed zero, which no later sum or
 // the final max(., 0) can tell apart).  A NaN tau arrives as +inf (run_ht).
 __device__ __forceinline__ float hard_thr(float c, float t) { return fabsf(c) <= t ? 0.f : c; }
-
-// z analysis of the pair (a = plane 2t, b = plane 2t-1) and its synthesis,
-// unscaled (the factor Cc sits in the dim-2 filters, ZF = true)
-__device__ __forceinline__ float z_sum(float a, float b) { return a + b; }
-__device__ __forceinline__ float z_dif(float a, float b) { return b - a; }
-__device__ __forceinline__ float z_even(float lo, float hi) { return lo - hi; }  // plane 2t
-__device__ __forceinline__ float z_odd(float lo, float hi) { return lo + hi; }   // plane 2t-1
 
 constexpr int kSynPairsPerBlock = 4;  // z-pairs per block of the register-blocked synthesis
 
@@ -483,7 +497,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
 #pragma unroll
       for (int j = 0; j < 30; ++j) v[j] = Z[zb * R * ZS + (2 * c0 + j) * ZS + rr];
       float lo[11], hi[11];
-      ana_lo_hi<11, true>(v, lo, hi);
+      ana_lo_hi<11>(v, lo, hi);
 #pragma unroll
       for (int i = 0; i < 11; ++i) {
         A[((zb * 2 + 0) * C + c0 + i) * AS + rr] = lo[i];
@@ -528,7 +542,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
         lo[j] = Bl[j];
         hi[j] = Bh[j];
       }
-      syn_lo_hi<16>(lo, hi, out);
+      syn_lo_hi<16, 0>(lo, hi, out);
 #pragma unroll
       for (int i = 0; i < 16; ++i) E[((zb * 2 + b2) * C + c) * ES + xl + i] = out[i];
     }
@@ -545,7 +559,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
           lo[j] = E[((zb * 2 + 0) * C + yq / 2 + j) * ES + x];
           hi[j] = E[((zb * 2 + 1) * C + yq / 2 + j) * ES + x];
         }
-        syn_lo_hi<4, true>(lo, hi, o[zb]);
+        syn_lo_hi<4, 1>(lo, hi, o[zb]);
       }
       const int ze = pmod_i(2 * t3 - sh, r), zo = pmod_i(2 * t3 - 1 - sh, r);
       if (x0 + x < p) {
@@ -731,7 +745,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       if (pc0 + i >= q) continue;
-      const float mean = (u[i] + v[i]) / 2.f;  // mean(Vrec,4), then max(.,0) (denoiseAndAggregateGPU.m:45)
+      // mean(Vrec,4), then max(.,0) (denoiseAndAggregateGPU.m:45): u and v
+      // are the two spins' planes already halved by the synthesis taps
+      const float mean = u[i] + v[i];
       const int so = (i & cm) * cst + (i >> csh) * gst + z * zstride;
       buf_store(ys, 4u * (uint32_t)ob0, mean > 0.f ? mean : 0.f, 4u * (uint32_t)so);
     }
@@ -776,13 +792,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       }
       float lo[7], hi[7];
       float* Aw = A + 7 * ra * AS + wa;
-      ana_lo_hi<7, true>(zl, lo, hi);
+      ana_lo_hi<7>(zl, lo, hi);
 #pragma unroll
       for (int i = 0; i < 7; ++i) {
         Aw[(0 * C + i) * AS] = lo[i];
         Aw[(1 * C + i) * AS] = hi[i];
       }
-      ana_lo_hi<7, true>(zh, lo, hi);
+      ana_lo_hi<7>(zh, lo, hi);
 #pragma unroll
       for (int i = 0; i < 7; ++i) {
         Aw[(2 * C + i) * AS] = lo[i];
@@ -824,9 +840,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       }
       float out[RB];
       if constexpr (s == 0) {
-        syn_lo_hi<RB>(lo, hi, out);
+        syn_lo_hi<RB, 0>(lo, hi, out);
       } else {
-        syn_lo_hi_odd<RB>(lo, hi, out);
+        syn_lo_hi_odd<RB, 0>(lo, hi, out);
       }
       float* Er = E + kbc * ES + xb;
 #pragma unroll
@@ -837,6 +853,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
     // phase C: dim-2 synthesis, z synthesis, spin combine, store
     {
       constexpr int NCC = s == 0 ? 9 : 8;
+      constexpr int GC = CS == 2 ? 2 : 1;  // taps x 1/2 (z), x 1/4 (z and the mean over both spins)
       float o[2][8];
 #pragma unroll
       for (int zb = 0; zb < 2; ++zb) {
@@ -849,9 +866,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
           hi[j] = Eh[j * ES];
         }
         if constexpr (s == 0) {
-          syn_lo_hi<8, true>(lo, hi, o[zb]);
+          syn_lo_hi<8, GC>(lo, hi, o[zb]);
         } else {
-          syn_lo_hi_odd<8, true>(lo, hi, o[zb]);
+          syn_lo_hi_odd<8, GC>(lo, hi, o[zb]);
         }
       }
       float ve[8], vo[8];  // plane az (even tap g[0]) and plane az-1 (odd tap g[1])
@@ -940,7 +957,7 @@ __global__ __launch_bounds__(256) void ht_lo_fast_kernel(const float* __restrict
     float v[24], lo[8];
 #pragma unroll
     for (int j = 0; j < 24; ++j) v[j] = Z[(2 * c0 + j) * ZS + rr];
-    ana_lo<8, true>(v, lo);
+    ana_lo<8>(v, lo);
 #pragma unroll
     for (int i = 0; i < 8; ++i) T2[(c0 + i) * ZS + rr] = lo[i];
   }
@@ -1027,7 +1044,7 @@ __global__ __launch_bounds__(256) void ht_lo0_cs2_kernel(const float* __restrict
       float v[24], lo[8];
 #pragma unroll
       for (int j = 0; j < 24; ++j) v[j] = Z[(2 * c0 + j + off) * ZS + rr + off];
-      ana_lo<8, true>(v, lo);
+      ana_lo<8>(v, lo);
 #pragma unroll
       for (int i = 0; i < 8; ++i) T2[(c0 + i) * ZS + rr] = lo[i];
     }
@@ -1099,12 +1116,15 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
   std::vector<Tau7> tl(levels);
   for (int l = 0; l < levels; ++l) {
     tl[l].t[0] = 0.f;
+    // the kernels' level-l coefficients are (1/Cc)^(3(l+1)) = (2 sqrt 2)^(l+1)
+    // times the reference's (analysis without the factor Cc per axis)
+    const double scale = std::pow(kCc, -3.0 * (l + 1));
     // a NaN threshold (getColoredTau on a level coarser than its PSD) zeroes
     // the band: C .* (abs(C) > NaN) is 0 for every finite C and NaN for a NaN
     // C, which is what +inf gives with the kernels' compare-and-select
     for (int j = 1; j <= 7; ++j) {
       const double t = tau[l + levels * (j - 1)];
-      tl[l].t[j] = std::isnan(t) ? INFINITY : (float)t;
+      tl[l].t[j] = std::isnan(t) ? INFINITY : (float)(t * scale);
     }
   }
   // Two cycle spins (every profile): the finest level of both spins in one
