@@ -620,6 +620,17 @@ constexpr int NYT = (NYC + 255) / 256;
 static_assert(NA <= 256 && ND <= 192 && 192 + NCO <= 256, "thread roles");
 static_assert(42 + 1 + 2 * (NB - 1) + 9 < WX, "phase-B rows inside the window");
 static_assert(14 * 2 + NJ <= WY, "phase-A columns inside the window");
+// LDS-staged planes (p a multiple of 4): the window plane from row x0 - 12 in
+// NQ aligned quads of 4 rows per column, one 128-bit direct-to-LDS load per
+// quad; window row w at staged row w + 3
+constexpr int NQ = 22, PS = 4 * NQ;        // quads per column, staged column stride
+constexpr int NPQ = NQ * WY;               // quads per plane (1,122)
+constexpr int NPC = (NPQ + 63) / 64;       // wave chunks of 64 quads (18)
+static_assert(PS >= WX + 3 + 2 && NPC <= 20, "staged plane");
+// and (ph a multiple of 4) the coarse values of an anchor from coarse row
+// x0/2 in YQ quads per coarse column, one direct-to-LDS load per wave
+constexpr int YQ = 10, YS = 4 * YQ;        // quads per coarse column, staged stride
+static_assert(YS >= JX + 1 && C * YQ <= 256, "staged coarse values");
 }  // namespace f2
 
 // CS = 2: the finest level, both cycle spins fused (above).  CS = 1: one
@@ -628,14 +639,23 @@ static_assert(14 * 2 + NJ <= WY, "phase-A columns inside the window");
 // (Y = Vrec, mosaic layout); grid z selects the spin (X/Ylo0/Y or
 // X1/Ylo1/Y1).  The arithmetic per coefficient and per output is that of
 // ht_syn_fast_kernel, so the two agree bitwise.
-template <int CS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ht_fine_kernel(
+// LP (CS = 2, p a multiple of 8): each plane is staged through LDS by 128-bit
+// direct-to-LDS loads (4.4 per thread instead of 23 32-bit loads, each
+// column's window rows read once instead of by the up to three runs whose
+// coefficients use it); phase A reads its 23 columns from there.  The staged
+// plane is reloaded after the first barrier and has landed at the second, and
+// so have the coarse values of the next anchor (two buffers, one per spin
+// parity).  The staging's LDS (24 KB) leaves room for two workgroups per CU.
+// The values are the same, so both forms agree bitwise.
+template <int CS, bool LP = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3))) void ht_fine_kernel(
     const float* __restrict__ X, const float* __restrict__ X1, const float* __restrict__ Ylo0,
     const float* __restrict__ Ylo1, int p, int q, int r, Tau7 tau, float* __restrict__ Y, float* __restrict__ Y1,
     int gd, const int* __restrict__ run) {
   if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   using namespace f2;
   static_assert(CS == 1 || CS == 2, "cycle spins per launch");
+  static_assert(!LP || CS == 2, "LDS-staged planes: the fused finest level");
   if (CS == 1 && blockIdx.z) {  // the second spin of a coarse level
     X = X1;
     Ylo0 = Ylo1;
@@ -643,9 +663,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   }
   __shared__ float A[4 * C * AS];
   __shared__ float E[4 * C * ES];
-  __shared__ float Yc[NYC];
+  __shared__ float Yc[LP ? 4 : NYC];
   __shared__ int cidxp[WY];             // p x window column (circular)
   __shared__ int yrow[2][JX], ycol[2][C];  // per spin: coarse rows, ph x coarse columns (circular)
+  __shared__ __align__(16) float PB[LP ? WY * PS : 4];  // the staged plane [column][staged row]
+  __shared__ int qrow[NQ];                               // first image row of each staged quad
+  __shared__ __align__(16) float YcL[LP ? 2 * C * YS : 4];  // staged coarse values [az & 1][column][coarse row - x0/2]
+  __shared__ int yqrow[YQ];                              // first coarse row of each staged quad
   const int tid = threadIdx.x;
   __builtin_assume(tid < 256);
   const int nbx = gridDim.x, nby = gridDim.y;
@@ -660,6 +684,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
   } else if (tid < WY + 2 * JX + 2 * C) {
     const int k = tid - WY - 2 * JX, sp = k / C;
     ycol[sp][k % C] = ph * pmod_i((y0 >> 1) + sp + k % C, qh);
+  } else if (LP && tid < WY + 2 * JX + 2 * C + NQ) {
+    // p % 4 == 0: the quad lies inside one column whatever the circular shift
+    const int k = tid - WY - 2 * JX - 2 * C;
+    qrow[k] = pmod_i(x0 - 12 + 4 * k, p);
+  } else if (LP && tid < WY + 2 * JX + 2 * C + NQ + YQ) {
+    const int k = tid - WY - 2 * JX - 2 * C - NQ;
+    yqrow[k] = pmod_i((x0 >> 1) + 4 * k, ph);
   }
   __syncthreads();
   const size_t pq = (size_t)p * q;
@@ -765,17 +796,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       if (e < NYC) yv[i] = buf_load(rs, 4u * (uint32_t)(yrow[s][e % JX] + ycol[s][e / JX]), so);
     }
   };
+  // LP: plane z into PB, quad g = column * NQ + quad at PB + 4 g (the lanes of
+  // a direct-to-LDS load write 16 consecutive bytes each from the wave's base)
+  auto fetch_pb = [&](int z) {
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(X + pq * z, pbytes);
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), ln = tid & 63;
+#pragma unroll
+    for (int it = 0; it < (NPC + 3) / 4; ++it) {
+      const int c = 4 * it + wv;
+      if (c >= NPC) break;
+      const int g = 64 * c + ln;
+      if (g < NPQ) {
+        const uint32_t off = 4u * (uint32_t)(qrow[g % NQ] + cidxp[g / NQ]);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(PB + 256 * c), 16, off,
+                                                 0, 0, 0);
+      }
+    }
+  };
+  // LP: the coarse values of anchor az (spin s = az & 1) into buffer s of YcL,
+  // quad g = column * YQ + quad at 4 g
+  auto fetch_yc = [&](int az) {
+    const int s = az & 1;
+    const int t3 = ((az + s) >> 1) % rh;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(s ? Ylo1 : Ylo0, (uint32_t)(4 * (size_t)ph * qh * rh));
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), g = 64 * wv + (tid & 63);
+    if (g < C * YQ)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(YcL + s * C * YS + 256 * wv), 16,
+                                               4u * (uint32_t)(yqrow[g % YQ] + ycol[s][g / YQ]),
+                                               4u * (uint32_t)(ph * qh * t3), 0, 0);
+  };
+  // LP: the thread's 23 window columns of the staged plane
+  auto read_pb = [&](float (&dst)[NJ]) {
+    const float* Pr = PB + 14 * ra * PS + wa + 3;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) dst[j] = Pr[j * PS];
+  };
   float P0[NJ], P1[NJ];  // planes of even / odd index
 #pragma unroll
   for (int j = 0; j < NJ; ++j) P0[j] = P1[j] = 0.f;
   fetch_plane(P1, r - 1);
-  fetch_plane(P0, 0);
-  fetch_coarse(0);
+  if constexpr (LP) {
+    fetch_pb(0);
+    fetch_yc(0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  } else {
+    fetch_plane(P0, 0);
+    fetch_coarse(0);
+  }
   float first[8], pend[8];
   auto anchor = [&](auto sc, int az, float (&cur)[NJ], float (&prev)[NJ]) {
     constexpr int s = decltype(sc)::value;
     // phase A: z analysis of the pair (az-1, az), dim-2 analysis of both z bands
     if (actA) {
+      if constexpr (LP) read_pb(cur);
       float zl[NJ - 1], zh[NJ - 1];
 #pragma unroll
       for (int i = 0; i < NJ - 1; ++i) {
@@ -785,7 +859,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       // in flight during the phases below: CS = 2, the next plane into the
       // registers of plane az-1; CS = 1, the next pair into both
       if constexpr (CS == 2) {
-        if (az + 1 < r) fetch_plane(prev, az + 1);
+        if (!LP && az + 1 < r) fetch_plane(prev, az + 1);
       } else if (az + 2 < r) {
         fetch_plane(prev, az + 1);
         fetch_plane(cur, az + 2);
@@ -805,14 +879,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
         Aw[(3 * C + i) * AS] = hi[i];
       }
     }
+    if constexpr (!LP) {
 #pragma unroll
-    for (int i = 0; i < NYT; ++i) {
-      const int e = tid + 256 * i;
-      if (e < NYC) Yc[e] = yv[i];
+      for (int i = 0; i < NYT; ++i) {
+        const int e = tid + 256 * i;
+        if (e < NYC) Yc[e] = yv[i];
+      }
+      constexpr int step = CS == 2 ? 1 : 2;  // the next anchor: both spins alternate, or one spin's pairs
+      if (az + step < r) fetch_coarse(az + step);
     }
-    constexpr int step = CS == 2 ? 1 : 2;  // the next anchor: both spins alternate, or one spin's pairs
-    if (az + step < r) fetch_coarse(az + step);
     __syncthreads();
+    // LP: every thread has read PB; the next plane lands there during phase B,
+    // the next anchor's coarse values in the other buffer (last read in the
+    // previous anchor's phase B)
+    if constexpr (LP) {
+      if (az + 1 < r) {
+        fetch_pb(az + 1);
+        fetch_yc(az + 1);
+      }
+    }
     // phase B: dim-1 analysis, threshold, dim-1 synthesis
     if (actB) {
       const float* Ar = A + kbc * AS + xb + s;
@@ -822,7 +907,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
 #pragma unroll
         for (int j = 4; j < 2 * NB + 4; ++j) v[j] = Ar[j];
         ana_hi<NB>(v, hi);
-        const float* Yr = Yc + cB * JX + (xb >> 1);
+        const float* Yr = LP ? YcL + (s * C + cB) * YS + s + (xb >> 1) : Yc + cB * JX + (xb >> 1);
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
           lo[i] = Yr[i];
@@ -849,6 +934,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void h
       for (int i = 0; i < RB; ++i)
         if (i >= 2 || xr != 2) Er[i] = out[i];
     }
+    if constexpr (LP) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's share of the next plane
     __syncthreads();
     // phase C: dim-2 synthesis, z synthesis, spin combine, store
     {
@@ -1225,8 +1311,12 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
     const float* y1 = levels == 1 ? w.lo[0][1] : w.rec[0][1];
     dim3 grid((M + f2::TX - 1) / f2::TX, (N + f2::TY - 1) / f2::TY, 1);
     mark(GBM3D_STAGE_HT_SYN0, 1);
-    hipLaunchKernelGGL(ht_fine_kernel<2>, grid, dim3(256), 0, st, vol, nullptr, y0, y1, M, N, K, tl[0], vavg, nullptr,
-                       gd, run);
+    if (M % 8 == 0)
+      hipLaunchKernelGGL((ht_fine_kernel<2, true>), grid, dim3(256), 0, st, vol, nullptr, y0, y1, M, N, K, tl[0], vavg,
+                         nullptr, gd, run);
+    else
+      hipLaunchKernelGGL((ht_fine_kernel<2, false>), grid, dim3(256), 0, st, vol, nullptr, y0, y1, M, N, K, tl[0], vavg,
+                         nullptr, gd, run);
     GBM3D_TRY(check_launch("ht_fine (finest level, both spins)"));
     mark(GBM3D_STAGE_HT_SYN0, 0);
   }

@@ -112,7 +112,7 @@ def test_ht_filter_vs_fp64_oracle(ctx, m, n, cs, seed):
 
 
 @pytest.mark.parametrize("m,n,K,levels,seed", [(64, 64, 16, 3, 51), (96, 160, 16, 3, 52), (48, 80, 8, 2, 53),
-                                                (16, 16, 2, 1, 54), (256, 200, 16, 3, 55)])
+                                                (16, 16, 2, 1, 54), (256, 200, 16, 3, 55), (18, 22, 2, 1, 56)])
 def test_ht_fused_cycle_spins_equal_per_spin_kernels(ctx, monkeypatch, m, n, K, levels, seed):
     """The finest level of both cycle spins in one pass (anchors of parity s in
     original coordinates) is the per-spin computation with its operations in
