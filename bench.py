@@ -448,7 +448,7 @@ def global_match_report(args):
 
 # kernel symbol prefix -> roofline key; the largest dispatch of a kernel is the
 # one measured (the finest level for the wavelet synthesis)
-TRAFFIC_KERNELS = {"void gbm3d::ht_fine_kernel<2>": "ht_syn0", "void gbm3d::match_kernel": "match2",
+TRAFFIC_KERNELS = {"void gbm3d::ht_fine_kernel<2": "ht_syn0", "void gbm3d::match_kernel": "match2",
                    "void gbm3d::wiener_kernel": "wiener", "void gbm3d::aggregate_batched_kernel<1": "agg1",
                    "void gbm3d::aggregate_batched_kernel<2": "agg2", "void gbm3d::agg1_prep_wave_kernel": "agg1_prep"}
 
@@ -573,14 +573,21 @@ def main():
         return
     ro = gbm3d_amd.resolve_opts(OPTS)
 
-    def staged():
-        """One more step with the stage timers on (same work, outside the timed region)."""
+    def staged(reps=5):
+        """Steps with the stage timers on (same work, outside the timed region):
+        one to warm the timers, then per-stage ms per step and launches per
+        step averaged over `reps` steps."""
         ctx.set_timing(True)
         step()
-        st = ctx.stage_times()
+        tot_ms, tot_n = {}, {}
+        for _ in range(reps):
+            step()
+            for k, (v, n) in ctx.stage_times().items():
+                tot_ms[k] = tot_ms.get(k, 0.0) + v
+                tot_n[k] = tot_n.get(k, 0) + n
         ctx.set_timing(False)
-        stage_ms = {k: v[0] for k, v in st.items()}
-        stage_n = {k: v[1] for k, v in st.items()}
+        stage_ms = {k: v / reps for k, v in tot_ms.items()}
+        stage_n = {k: v // reps if v % reps == 0 else v / reps for k, v in tot_n.items()}
         rl = kernel_rooflines(stage_ms, stage_n, SIZE, SIZE, ro["numMax"], ro["blockSize"], ro["matchSize"],
                               len(ro["matchSpins"]), len(ro["matchSpinsWie"]), ro["blockSizeWie"])
         return stage_ms, rl
