@@ -63,8 +63,9 @@ def kernel_rooflines(stage_ms, stage_n, M, N, K, d1, d2, n1, n2, bw=8):
         out["agg2"] = dict(bound="hbm", unit="GB/s", work=float(vol_bytes + G2 * K * 4 + G2 * 4 + 2 * MN * 4),
                            launches=stage_n["agg2"], ms=stage_ms["agg2"] / stage_n["agg2"], peak=PEAK_HBM_GBS)
     if stage_n["ht_syn0"]:
-        # finest-level threshold + synthesis, one cycle spin per launch: read the
-        # mosaic volume, write the rebuilt volume (the coarse level read is 1/8 of it)
+        # finest-level threshold + synthesis of both cycle spins (at 8x8 blocks
+        # with the step-1 weights, which read nothing more): read the mosaic
+        # volume, write Vavg (the coarse level read is 1/8 of it)
         out["ht_syn0"] = dict(bound="hbm", unit="GB/s", work=2.0 * vol_bytes, launches=stage_n["ht_syn0"],
                               ms=stage_ms["ht_syn0"] / stage_n["ht_syn0"], peak=PEAK_HBM_GBS)
     if stage_n["agg1_prep"]:

@@ -192,15 +192,17 @@ int run_distributed(gbm3d_ctx* c, const float* dI, int m, int n, float sigma, co
     }
     GBM3D_TRY(do_match(c, b.pad1, b.bn, b.smap, th, tw, k, o.matchSize, o.numMax, b.S1, b.vol, ls,
                        GBM3D_STAGE_MATCH1));
+    bool wdone = false;  // the step-1 weights written with Vavg
     if (o.filtType == GBM3D_FILT_MEDIAN) {
       StageTimer tm(c, GBM3D_STAGE_HT_SYN0);
       GBM3D_TRY(launch_median3(b.vol, th, tw, o.numMax, k, b.vavg, ls));
     } else {
       HtWork hw;
       ht_work_bind(b.htw, th, tw, o.numMax, o.levels, o.cycleSpin, &hw);
-      GBM3D_TRY(run_ht(b.vol, th, tw, o.numMax, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, k, ls, ht_hook, c));
+      GBM3D_TRY(run_ht(b.vol, th, tw, o.numMax, o.levels, o.cycleSpin, tau.data(), hw, b.vavg, k, ls, ht_hook, c,
+                       nullptr, b.w1, &wdone));
     }
-    {
+    if (!wdone) {
       StageTimer tm(c, GBM3D_STAGE_AGG1_PREP);
       GBM3D_TRY(launch_agg1_prep(b.vavg, th, tw, o.numMax, k, b.w1, ls));
     }

@@ -134,9 +134,13 @@ typedef void (*StageHook)(void* user, int stage, int begin);
 // mosaic layout (gd == 0) or group-contiguous with gd x gd blocks
 // (Vavg[u + gd*(v + gd*(k + K*g))], the aggregation's layout).
 // run (nullable, device): when it holds 0 every kernel returns at once.
+// wgt (nullable, device, (M/gd)(N/gd) floats): where the fused finest level
+// can (gd == 8), it also writes the step-1 weights of the gd x gd groups
+// (launch_agg1_prep's values, bit for bit) and sets *wgt_done; otherwise
+// *wgt_done is false and the caller runs launch_agg1_prep.
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau,
            const HtWork& w, float* vavg, int gd, hipStream_t st, StageHook hook = nullptr, void* user = nullptr,
-           const int* run = nullptr);
+           const int* run = nullptr, float* wgt = nullptr, bool* wgt_done = nullptr);
 
 // The 'median' step-1 filter: Vavg = medfilt3(vol, [3 3 3]) (edge values
 // repeated), mosaic (gd == 0) or group-contiguous output.

@@ -23,11 +23,6 @@ namespace gbm3d {
 // One 256-thread workgroup per group (tile) g: Vavg = max(mean_s vrec_s, 0)
 // over the tile's d x d x K stack, then w_g = 1/(D + eps) with
 // D = sum|diff3_v|/4 + sum|diff3_u|/4 + sum|diff1_k| (getBM3DAggWeights(3,1)).
-__device__ __forceinline__ float third_diff(float x0, float x1, float x2, float x3) {
-  const float a = x1 - x0, b = x2 - x1, c = x3 - x2;
-  const float e = b - a, f = c - b;
-  return f - e;
-}
 
 // 8 x 8 blocks, 16 slices: the group's 4 KB stack is read with 16-byte loads
 // (four per lane), staged in LDS with one exchange, and each lane then takes

@@ -647,15 +647,22 @@ static_assert(YS >= JX + 1 && C * YQ <= 256, "staged coarse values");
 // so have the coarse values of the next anchor (two buffers, one per spin
 // parity).  The staging's LDS (24 KB) leaves room for two workgroups per CU.
 // The values are the same, so both forms agree bitwise.
-template <int CS, bool LP = false>
+// WG (LP, gd = 8): the kernel also writes the step-1 weight of every 8x8
+// group of its tile, wgt[g] = 1/(D + eps) with D the sums of
+// getBM3DAggWeights(3,1) over the group's stack (the output tile holds whole
+// groups over all planes): per pixel the terms are added plane by plane as
+// the planes are stored, then summed over the group in agg1_prep's order, so
+// the weights are those of agg1_prep_wave_kernel bit for bit.
+template <int CS, bool LP = false, bool WG = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3))) void ht_fine_kernel(
     const float* __restrict__ X, const float* __restrict__ X1, const float* __restrict__ Ylo0,
     const float* __restrict__ Ylo1, int p, int q, int r, Tau7 tau, float* __restrict__ Y, float* __restrict__ Y1,
-    int gd, const int* __restrict__ run) {
+    int gd, const int* __restrict__ run, float* __restrict__ wgt) {
   if (run && *run == 0) return;  // gated launch (the learned form's fallback)
   using namespace f2;
   static_assert(CS == 1 || CS == 2, "cycle spins per launch");
   static_assert(!LP || CS == 2, "LDS-staged planes: the fused finest level");
+  static_assert(!WG || LP, "step-1 weights: the LDS-staged fused finest level (gd = 8)");
   if (CS == 1 && blockIdx.z) {  // the second spin of a coarse level
     X = X1;
     Ylo0 = Ylo1;
@@ -771,17 +778,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3)
       buf_store(ys, 4u * (uint32_t)ob0, v[i], 4u * (uint32_t)so);
     }
   };
-  auto store_plane = [&](int z, const float (&u)[8], const float (&v)[8]) {
-    if (!rowC) return;
+  // WG: per pixel (row xo & 7 of its group, column i), the weight terms of
+  // the planes stored so far: |third diff| down the rows, along the columns,
+  // |diff| across the planes (agg1_prep_wave_kernel's s1, s2, s3)
+  float ws1[8], ws2[8], ws3[8], wprev[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) ws1[i] = ws2[i] = ws3[i] = wprev[i] = 0.f;
+  auto weigh_plane = [&](int z, const float (&x)[8]) {
+    const bool down = (xo & 7) + 3 < 8;  // the row's group holds rows +1 .. +3
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      if (pc0 + i >= q) continue;
+      if (z > 0) ws3[i] += fabsf(x[i] - wprev[i]);
+      wprev[i] = x[i];
+      if (i + 3 < 8) ws2[i] += fabsf(third_diff(x[i], x[i + 1], x[i + 2], x[i + 3]));
+      // rows +1, +2, +3 of the same column from lanes +1, +2, +3 (a group's
+      // 8 rows are 8 consecutive lanes of a 16-lane DPP row)
+      const int xi = __float_as_int(x[i]);
+      const float x1 = __int_as_float(__builtin_amdgcn_update_dpp(0, xi, 0x101, 0xF, 0xF, true));
+      const float x2 = __int_as_float(__builtin_amdgcn_update_dpp(0, xi, 0x102, 0xF, 0xF, true));
+      const float x3 = __int_as_float(__builtin_amdgcn_update_dpp(0, xi, 0x103, 0xF, 0xF, true));
+      if (down) ws1[i] += fabsf(third_diff(x[i], x1, x2, x3));
+    }
+  };
+  auto store_plane = [&](int z, const float (&u)[8], const float (&v)[8]) {
+    if (!rowC) return;
+    float val[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
       // mean(Vrec,4), then max(.,0) (denoiseAndAggregateGPU.m:45): u and v
       // are the two spins' planes already halved by the synthesis taps
       const float mean = u[i] + v[i];
-      const int so = (i & cm) * cst + (i >> csh) * gst + z * zstride;
-      buf_store(ys, 4u * (uint32_t)ob0, mean > 0.f ? mean : 0.f, 4u * (uint32_t)so);
+      val[i] = mean > 0.f ? mean : 0.f;
     }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (pc0 + i >= q) continue;
+      const int so = (i & cm) * cst + (i >> csh) * gst + z * zstride;
+      buf_store(ys, 4u * (uint32_t)ob0, val[i], 4u * (uint32_t)so);
+    }
+    if constexpr (WG) weigh_plane(z, val);
   };
   // ---- coarse (rebuilt level-1) approximation of anchor az, staged through registers
   float yv[NYT];
@@ -985,6 +1020,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3)
   for (int az = 0; az < r; az += 2) {  // r is even: anchors in spin pairs, plane az in P0, az+1 in P1
     anchor(std::integral_constant<int, 0>{}, az, P0, P1);
     if constexpr (CS == 2) anchor(std::integral_constant<int, 1>{}, az + 1, P1, P0);
+  }
+  if constexpr (WG) {
+    // sums over the group in the order of agg1_prep's wave sum over lanes
+    // u + 8 v (xor 32, 16, 8 pair the columns v, xor 4, 2, 1 the rows u)
+    auto cols = [](const float (&a)[8]) { return ((a[0] + a[4]) + (a[2] + a[6])) + ((a[1] + a[5]) + (a[3] + a[7])); };
+    float s1 = cols(ws1), s2 = cols(ws2), s3 = cols(ws3);
+#pragma unroll
+    for (int o = 4; o >= 1; o >>= 1) {
+      s1 += __uint_as_float(shfl_xor_u32(__float_as_uint(s1), o));
+      s2 += __uint_as_float(shfl_xor_u32(__float_as_uint(s2), o));
+      s3 += __uint_as_float(shfl_xor_u32(__float_as_uint(s3), o));
+    }
+    if ((xo & 7) == 0 && rowC && pc0 < q) {
+      float Dv = s2 * 0.25f;
+      Dv = Dv + s1 * 0.25f;
+      Dv = Dv + s3;
+      wgt[(x0 + xo) / 8 + (p / 8) * (pc0 / 8)] = 1.0f / (Dv + 2.220446049250313e-16f);  // 1/(D(blocks)+eps)
+    }
   }
 }
 
@@ -1190,7 +1243,8 @@ void ht_work_bind(float* base, int M, int N, int K, int levels, int cycleSpin, H
 }
 
 int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, const double* tau, const HtWork& w,
-           float* vavg, int gd, hipStream_t st, StageHook hook, void* user, const int* run) {
+           float* vavg, int gd, hipStream_t st, StageHook hook, void* user, const int* run, float* wgt, bool* wgt_done) {
+  if (wgt_done) *wgt_done = false;
   if (gd != 0 && (M % gd || N % gd)) return fail(GBM3D_ERR_ARG, "ht: group layout block size");
   auto mark = [&](int stage, int begin) {
     if (hook) hook(user, stage, begin);
@@ -1252,7 +1306,7 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
         // both spins of the level in one launch (grid z)
         dim3 gf((p + f2::TX - 1) / f2::TX, (q + f2::TY - 1) / f2::TY, 2);
         hipLaunchKernelGGL(ht_fine_kernel<1>, gf, dim3(256), 0, st, w.lo[l - 1][0], w.lo[l - 1][1], ylo0, ylo1, p, q,
-                           r, tl[l], w.rec[l - 1][0], w.rec[l - 1][1], 0, run);
+                           r, tl[l], w.rec[l - 1][0], w.rec[l - 1][1], 0, run, nullptr);
       } else {
         dim3 grid((p + SP - 1) / SP, (q + SP - 1) / SP, r / 2);
         for (int s = 0; s < 2; ++s)
@@ -1311,12 +1365,17 @@ int run_ht(const float* vol, int M, int N, int K, int levels, int cycleSpin, con
     const float* y1 = levels == 1 ? w.lo[0][1] : w.rec[0][1];
     dim3 grid((M + f2::TX - 1) / f2::TX, (N + f2::TY - 1) / f2::TY, 1);
     mark(GBM3D_STAGE_HT_SYN0, 1);
-    if (M % 8 == 0)
+    if (wgt && gd == 8) {  // M % 8 == 0
+      hipLaunchKernelGGL((ht_fine_kernel<2, true, true>), grid, dim3(256), 0, st, vol, nullptr, y0, y1, M, N, K, tl[0],
+                         vavg, nullptr, gd, run, wgt);
+      if (wgt_done) *wgt_done = true;
+    } else if (M % 8 == 0) {
       hipLaunchKernelGGL((ht_fine_kernel<2, true>), grid, dim3(256), 0, st, vol, nullptr, y0, y1, M, N, K, tl[0], vavg,
-                         nullptr, gd, run);
-    else
+                         nullptr, gd, run, nullptr);
+    } else {
       hipLaunchKernelGGL((ht_fine_kernel<2, false>), grid, dim3(256), 0, st, vol, nullptr, y0, y1, M, N, K, tl[0], vavg,
-                         nullptr, gd, run);
+                         nullptr, gd, run, nullptr);
+    }
     GBM3D_TRY(check_launch("ht_fine (finest level, both spins)"));
     mark(GBM3D_STAGE_HT_SYN0, 0);
   }

@@ -241,6 +241,7 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
       GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad2, ls));
       noisy_spin[ii % l1] = sp;
     }
+    bool wdone = false;  // the step-1 weights written with Vavg
     if (c->ext.learned) {
       // variant: learned gains on the DCT-Haar coefficients of each group,
       // gathered straight from the search image (no mosaic volume); then the
@@ -275,10 +276,12 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
         StageTimer t(c, GBM3D_STAGE_HT_SYN0);
         GBM3D_TRY(launch_median3(sb.vol, p.M, p.N, p.K, p.bs, sb.vavg, ls));  // denoiseAndAggregateGPU.m:49-52
       } else {
-        GBM3D_TRY(run_ht(sb.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau, hw, sb.vavg, p.bs, ls, ht_hook, c));
+        // the fused finest level writes the step-1 weights too where it can
+        GBM3D_TRY(run_ht(sb.vol, p.M, p.N, p.K, o.levels, o.cycleSpin, tau, hw, sb.vavg, p.bs, ls, ht_hook, c,
+                         nullptr, sb.w1, &wdone));
       }
     }
-    {
+    if (!wdone) {
       StageTimer t(c, GBM3D_STAGE_AGG1_PREP);
       GBM3D_TRY(launch_agg1_prep(sb.vavg, p.M, p.N, p.K, p.bs, sb.w1, ls));
     }

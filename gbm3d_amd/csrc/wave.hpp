@@ -61,6 +61,15 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off_bytes, soff_bytes, 0);
 }
 
+// diff(x, 3) of four consecutive samples in the order diff() takes it
+// (getBM3DAggWeights.m:15-19), shared by the step-1 weights wherever they
+// are computed so that every place gives the same bits
+__device__ __forceinline__ float third_diff(float x0, float x1, float x2, float x3) {
+  const float a = x1 - x0, b = x2 - x1, c = x3 - x2;
+  const float e = b - a, f = c - b;
+  return f - e;
+}
+
 // a + b in a scalar register, opaque to the optimiser (which would otherwise
 // hoist every multiple of a loop-invariant stride and keep them all live)
 __device__ __forceinline__ uint32_t sadd_u32(uint32_t a, uint32_t b) {
