@@ -784,6 +784,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3)
   float ws1[8], ws2[8], ws3[8], wprev[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) ws1[i] = ws2[i] = ws3[i] = wprev[i] = 0.f;
+  // the value of the lane below in the same 16-lane row (0 past its end)
+  auto below = [](float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xF, 0xF, true));
+  };
   auto weigh_plane = [&](int z, const float (&x)[8]) {
     const bool down = (xo & 7) + 3 < 8;  // the row's group holds rows +1 .. +3
 #pragma unroll
@@ -791,13 +795,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3)
       if (z > 0) ws3[i] += fabsf(x[i] - wprev[i]);
       wprev[i] = x[i];
       if (i + 3 < 8) ws2[i] += fabsf(third_diff(x[i], x[i + 1], x[i + 2], x[i + 3]));
-      // rows +1, +2, +3 of the same column from lanes +1, +2, +3 (a group's
-      // 8 rows are 8 consecutive lanes of a 16-lane DPP row)
-      const int xi = __float_as_int(x[i]);
-      const float x1 = __int_as_float(__builtin_amdgcn_update_dpp(0, xi, 0x101, 0xF, 0xF, true));
-      const float x2 = __int_as_float(__builtin_amdgcn_update_dpp(0, xi, 0x102, 0xF, 0xF, true));
-      const float x3 = __int_as_float(__builtin_amdgcn_update_dpp(0, xi, 0x103, 0xF, 0xF, true));
-      if (down) ws1[i] += fabsf(third_diff(x[i], x1, x2, x3));
+      // down the rows: the differences of the differences, each from the lane
+      // below (a group's 8 rows are 8 consecutive lanes of a 16-lane DPP row,
+      // and row u <= 4 reaches rows up to u + 3): the steps of third_diff on
+      // rows u .. u + 3, the same operations on the same values
+      const float d = below(x[i]) - x[i];
+      const float e = below(d) - d;
+      const float f = below(e) - e;
+      if (down) ws1[i] += fabsf(f);
     }
   };
   auto store_plane = [&](int z, const float (&u)[8], const float (&v)[8]) {
