@@ -129,8 +129,9 @@ size_t pipeline_bytes(const Plan& p, const gbm3d_opts& o, int lanes = 1);
 int spin_lanes(const gbm3d_ctx* c, const Plan& p);
 
 // Block matching on a padded image with the context's distance variant.
+// bn_ready: bn already holds the block norms of pad (launch_search_pad_norm).
 int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int N, int d, int d2, int K, uint32_t* S,
-             float* vol, hipStream_t st, int stage, const TolArgs* ta = nullptr);
+             float* vol, hipStream_t st, int stage, const TolArgs* ta = nullptr, bool bn_ready = false);
 
 // Fork `lanes` lane streams from st and join them back (no-ops for lanes <= 1).
 // A fork is always joined, also after an error between the two, so that a

@@ -72,6 +72,9 @@ bool wiener_supported(int d, int K);       // step-2 Wiener: d in {4, 8, 16}, K 
 int launch_mirror_pad(const float* src, int m, int n, int M, int N, float* dst, hipStream_t st);
 int launch_search_pad(const float* img, int M, int N, int spin, int d2, float* pad, hipStream_t st);
 int launch_norm_map(const float* pad, int P, int Q, int d, float* bn, hipStream_t st);
+// launch_search_pad and launch_norm_map (block size d) of its result in one pass
+int launch_search_pad_norm(const float* img, int M, int N, int spin, int d2, int d, float* pad, float* bn,
+                           hipStream_t st);
 // Block matching: S (K x Mt x Nt, 1-based window indices) and optionally the
 // mosaic volume (M x N x K).
 int launch_match(const float* pad, const float* bn, int M, int N, int d, int d2, int K, uint32_t* S,

@@ -121,6 +121,76 @@ __global__ __launch_bounds__(256) void norm_map_kernel(const float* __restrict__
   }
 }
 
+// The search image and its block norms in one pass: the norm tile's window
+// comes from the image through the search-pad map (the values search_pad
+// writes), each workgroup writes the pad elements of its tile's rows and
+// columns (the last tile row and column also the D - 1 beyond them), and the
+// norms are norm_map_kernel's arithmetic on the same window.
+template <int D>
+__global__ __launch_bounds__(256) void search_pad_norm_kernel(const float* __restrict__ img, int M, int N, int spinR,
+                                                              int spinC, int d2, float* __restrict__ pad,
+                                                              float* __restrict__ bn) {
+  constexpr int TR = 64, TC = 16, RB = 4;
+  constexpr int LR = TR + D - 1, LC = TC + D - 1, LDT = LR + 2;
+  static_assert(LDT % 2 == 1, "tile stride");
+  __shared__ float tile[LC * LDT];
+  const int P = M + d2, Q = N + d2, s = d2 / 2;
+  const int Pm = P - D + 1, Qm = Q - D + 1;
+  const int r0 = blockIdx.x * TR, c0 = blockIdx.y * TC;
+  for (int e = threadIdx.x; e < LR * LC; e += 256) {
+    const int rr = e % LR, cc = e / LR;
+    const int gr = min(r0 + rr, P - 1), gc = min(c0 + cc, Q - 1);
+    tile[cc * LDT + rr] = img[padmap_spin(gr, M, s, spinR) + (size_t)M * padmap_spin(gc, N, s, spinC)];
+  }
+  __syncthreads();
+  {
+    const int nr = (r0 + TR >= Pm ? P : r0 + TR) - r0, nc = (c0 + TC >= Qm ? Q : c0 + TC) - c0;
+    for (int e = threadIdx.x; e < nr * nc; e += 256) {
+      const int rr = e % nr, cc = e / nr;
+      pad[(r0 + rr) + (size_t)P * (c0 + cc)] = tile[cc * LDT + rr];
+    }
+  }
+  const int tr = (threadIdx.x & 15) * RB, tc = threadIdx.x >> 4;
+  float acc[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) acc[i] = 0.f;
+#pragma unroll 1
+  for (int v = 0; v < D; ++v) {
+    const float* col = &tile[(tc + v) * LDT + tr];
+#pragma unroll
+    for (int t = 0; t < RB + D - 1; ++t) {
+      float y = col[t];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        const int u = t - i;
+        if (u >= 0 && u < D) acc[i] = fmaf(y, y, acc[i]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    int p = r0 + tr + i, q = c0 + tc;
+    if (p < Pm && q < Qm) bn[p + (size_t)Pm * q] = acc[i];
+  }
+}
+
+int launch_search_pad_norm(const float* img, int M, int N, int spin, int d2, int d, float* pad, float* bn,
+                           hipStream_t st) {
+  if (M < d2 || N < d2 || (d2 & 1)) return fail(GBM3D_ERR_ARG, "search pad: image smaller than matchSize");
+  const int P = M + d2, Q = N + d2, Pm = P - d + 1, Qm = Q - d + 1;
+  if (Pm < 1 || Qm < 1) return fail(GBM3D_ERR_ARG, "norm map: size");
+  const int spinR = ((spin % M) + M) % M, spinC = ((spin % N) + N) % N;
+  dim3 grid((Pm + 63) / 64, (Qm + 15) / 16);
+  switch (d) {
+    case 4: hipLaunchKernelGGL(search_pad_norm_kernel<4>, grid, dim3(256), 0, st, img, M, N, spinR, spinC, d2, pad, bn); break;
+    case 8: hipLaunchKernelGGL(search_pad_norm_kernel<8>, grid, dim3(256), 0, st, img, M, N, spinR, spinC, d2, pad, bn); break;
+    case 16: hipLaunchKernelGGL(search_pad_norm_kernel<16>, grid, dim3(256), 0, st, img, M, N, spinR, spinC, d2, pad, bn); break;
+    case 32: hipLaunchKernelGGL(search_pad_norm_kernel<32>, grid, dim3(256), 0, st, img, M, N, spinR, spinC, d2, pad, bn); break;
+    default: return fail(GBM3D_ERR_UNSUPPORTED, "block size must be 4, 8, 16 or 32");
+  }
+  return check_launch("search_pad_norm");
+}
+
 int launch_sum_map(const float* pad, int P, int Q, int d, float* sm, hipStream_t st) {
   const int Pm = P - d + 1, Qm = Q - d + 1;
   if (Pm < 1 || Qm < 1) return fail(GBM3D_ERR_ARG, "sum map: size");

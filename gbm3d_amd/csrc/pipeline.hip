@@ -155,7 +155,7 @@ int spin_lanes(const gbm3d_ctx* c, const Plan& p) {
 }
 
 int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int N, int d, int d2, int K, uint32_t* S,
-             float* vol, hipStream_t st, int stage, const TolArgs* ta) {
+             float* vol, hipStream_t st, int stage, const TolArgs* ta, bool bn_ready) {
   const int P = M + d2, Q = N + d2;
   // the BF16 form is built for 8 x 8 and 16 x 16 blocks and the windowed
   // mink; other block sizes and the tolerance branch take the FP32 search,
@@ -163,7 +163,7 @@ int do_match(gbm3d_ctx* c, const float* pad, float* bn, float* smap, int M, int 
   const bool bf16 = c->ext.match_bf16 && match_bf16_supported(d, d2) && !(ta && ta->Kmin);
   {
     StageTimer t(c, GBM3D_STAGE_PAD);
-    GBM3D_TRY(launch_norm_map(pad, P, Q, d, bn, st));
+    if (!bn_ready) GBM3D_TRY(launch_norm_map(pad, P, Q, d, bn, st));
     if (bf16) GBM3D_TRY(launch_sum_map(pad, P, Q, d, smap, st));
   }
   StageTimer t(c, stage);
@@ -238,7 +238,8 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
     SpinBufs& sb = b.sb[ii % l1];
     {
       StageTimer t(c, GBM3D_STAGE_PAD);
-      GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad2, ls));
+      // the search image and the block norms its matching takes
+      GBM3D_TRY(launch_search_pad_norm(b.Ip, p.M, p.N, sp, p.d2, p.bs, sb.pad2, sb.bn, ls));
       noisy_spin[ii % l1] = sp;
     }
     bool wdone = false;  // the step-1 weights written with Vavg
@@ -250,7 +251,7 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
       // overwrite vavg exactly as the reference path computes them)
       if (!lslot) return fail(GBM3D_ERR_ARG, "learned shrinkage: no device state for this image");
       GBM3D_TRY(do_match(c, sb.pad2, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, nullptr, ls,
-                         GBM3D_STAGE_MATCH1));
+                         GBM3D_STAGE_MATCH1, nullptr, true));
       {
         StageTimer t(c, GBM3D_STAGE_SHRINK);
         const long G1 = (long)(p.M / p.bs) * (p.N / p.bs);
@@ -271,7 +272,7 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
       HtWork hw;
       ht_work_bind(sb.htw, p.M, p.N, p.K, o.levels, o.cycleSpin, &hw);
       GBM3D_TRY(do_match(c, sb.pad2, sb.bn, sb.smap, p.M, p.N, p.bs, p.d2, p.K, sb.S1, sb.vol, ls,
-                         GBM3D_STAGE_MATCH1, tol.Kmin ? &tol : nullptr));
+                         GBM3D_STAGE_MATCH1, tol.Kmin ? &tol : nullptr, true));
       if (o.filtType == GBM3D_FILT_MEDIAN) {
         StageTimer t(c, GBM3D_STAGE_HT_SYN0);
         GBM3D_TRY(launch_median3(sb.vol, p.M, p.N, p.K, p.bs, sb.vavg, ls));  // denoiseAndAggregateGPU.m:49-52
@@ -308,14 +309,14 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
     SpinBufs& sb = b.sb[ii % l2];
     {
       StageTimer t(c, GBM3D_STAGE_PAD);
-      GBM3D_TRY(launch_search_pad(b.pilot, p.M, p.N, sp, p.d2, sb.pad1, ls));
+      GBM3D_TRY(launch_search_pad_norm(b.pilot, p.M, p.N, sp, p.d2, p.bw, sb.pad1, sb.bn, ls));
       if (noisy_spin[ii % l2] != sp) {  // else step 1 left this search image in pad2 (the steps are joined)
         GBM3D_TRY(launch_search_pad(b.Ip, p.M, p.N, sp, p.d2, sb.pad2, ls));
         noisy_spin[ii % l2] = sp;
       }
     }
     GBM3D_TRY(do_match(c, sb.pad1, sb.bn, sb.smap, p.M, p.N, p.bw, p.d2, p.K, sb.S2, nullptr, ls,
-                       GBM3D_STAGE_MATCH2));
+                       GBM3D_STAGE_MATCH2, nullptr, true));
     {
       StageTimer t(c, GBM3D_STAGE_WIENER);
       GBM3D_TRY(launch_wiener(sb.pad1, sb.pad2, sb.S2, nullptr, nullptr, p.M, p.N, p.bw, p.d2, p.K, G2, sigma,
