@@ -210,7 +210,11 @@ int run_pipeline_on(gbm3d_ctx* c, const float* dI, const Plan& p, float sigma, c
   // spin ii of a step with `nl` lanes runs on lane ii % nl
   auto lane_stream = [&](int ii, int nl) { return nl > 1 ? c->bstreams[ii % nl] : st; };
 
-  {
+  if (p.M == m && p.N == n && dI != dU1 && dI != dU) {
+    // already a block multiple: the padded image is the input itself (read
+    // only; not when an output overwrites it before step 2 reads it)
+    b.Ip = const_cast<float*>(dI);
+  } else {
     StageTimer t(c, GBM3D_STAGE_PAD);
     GBM3D_TRY(launch_mirror_pad(dI, m, n, p.M, p.N, b.Ip, st));  // source/GBM3D.m:22-25
   }
