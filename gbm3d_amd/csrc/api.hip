@@ -220,8 +220,9 @@ int gbm3d_match(gbm3d_ctx* c, const float* I, int m, int n, int d, int d2, int K
   Carve k2{c->arena};
   lay(k2, &img, &pad, &bn, &sm, &dS, &vol);
   GBM3D_TRY(upload(c, img, I, mn));
-  GBM3D_TRY(launch_search_pad(img, m, n, 0, d2, pad, c->stream));
-  GBM3D_TRY(do_match(c, pad, bn, sm, m, n, d, d2, K, dS, Volume ? vol : nullptr, c->stream, GBM3D_STAGE_MATCH1));
+  GBM3D_TRY(launch_search_pad_norm(img, m, n, 0, d2, d, pad, bn, c->stream));  // the pipeline's search image
+  GBM3D_TRY(do_match(c, pad, bn, sm, m, n, d, d2, K, dS, Volume ? vol : nullptr, c->stream, GBM3D_STAGE_MATCH1,
+                     nullptr, true));
   GBM3D_TRY(download(c, S, dS, G * K));
   if (Volume) GBM3D_TRY(download(c, Volume, vol, mn * K));
   GBM3D_HIP(hipStreamSynchronize(c->stream));
@@ -256,8 +257,7 @@ int gbm3d_match_tol(gbm3d_ctx* c, const float* I, int m, int n, int d, int d2, i
   const TolArgs ta{numMin == numMax ? 0 : numMin, tu, c->tolflag};
   GBM3D_TRY(upload(c, img, I, mn));
   GBM3D_TRY(launch_set_i32(c->tolflag, 0, c->stream));
-  GBM3D_TRY(launch_search_pad(img, m, n, 0, d2, pad, c->stream));
-  GBM3D_TRY(launch_norm_map(pad, (int)P, (int)Q, d, bn, c->stream));
+  GBM3D_TRY(launch_search_pad_norm(img, m, n, 0, d2, d, pad, bn, c->stream));
   GBM3D_TRY(launch_match_w(pad, bn, m, n, d, d2, numMax, dS, Volume ? vol : nullptr, 4, c->stream,
                            ta.Kmin ? &ta : nullptr));
   int short_tile = 0;
