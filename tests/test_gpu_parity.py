@@ -117,7 +117,10 @@ def test_ht_fused_cycle_spins_equal_per_spin_kernels(ctx, monkeypatch, m, n, K, 
     """The finest level of both cycle spins in one pass (anchors of parity s in
     original coordinates) is the per-spin computation with its operations in
     the same order: bitwise equal, in the mosaic layout and, end to end, in the
-    group-contiguous layout the aggregation reads."""
+    group-contiguous layout the aggregation reads, with the step-1 weights the
+    fused kernel writes at 8x8 blocks (the per-spin path runs the weights
+    kernel); end to end on 136x104 images too, whose last output tiles are
+    partial (64x32 tiles)."""
     rng = np.random.default_rng(seed)
     V = (rng.standard_normal((m, n, K)) * 40 + 100).astype(np.float32)
     tau = O.get_tau(levels, 1, 25.0)
@@ -125,7 +128,7 @@ def test_ht_fused_cycle_spins_equal_per_spin_kernels(ctx, monkeypatch, m, n, K, 
     monkeypatch.setenv("GBM3D_HT_UNFUSED", "1")
     b = ctx.ht_filter(V, levels, 2, tau)
     np.testing.assert_array_equal(a, b)
-    clean, noisy = synth.make_pair(seed, 128, 96, 25.0)
+    clean, noisy = synth.make_pair(seed, *((136, 104) if seed % 2 else (128, 96)), 25.0)
     Ub, ob = ctx.denoise(noisy, 25.0, {"blockSize": 8, "matchSpinsWie": [0, 4]})
     monkeypatch.delenv("GBM3D_HT_UNFUSED")
     Ua, oa = ctx.denoise(noisy, 25.0, {"blockSize": 8, "matchSpinsWie": [0, 4]})
