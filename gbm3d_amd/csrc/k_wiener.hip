@@ -59,13 +59,30 @@ __global__ __launch_bounds__(256) void wiener_kernel(const float* __restrict__ p
       const uint32_t pbytes = (uint32_t)(4 * P * ((size_t)N + d2));
       const __amdgpu_buffer_rsrc_t rP = buf_rsrc(padP, pbytes), rN = buf_rsrc(padN, pbytes);
       const uint32_t lo = 4u * (uint32_t)(u + (int)P * v);
+      if constexpr (TPG == 64) {
+        // lane k < K: slot k's block origin (its window index divided by d2
+        // once, all slots in parallel), then each slot's offset from lane k
+        uint32_t sol = 0;
+        if (t < K) {
+          const uint32_t idx = S[(size_t)K * g + t] - 1u;
+          const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
+          sol = 4u * (uint32_t)((ti * D + a) + (int)P * (tj * D + b));
+        }
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const uint32_t idx = S[(size_t)K * g + k] - 1u;
-        const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
-        const uint32_t so = 4u * (uint32_t)((ti * D + a) + (int)P * (tj * D + b));
-        c1[k] = buf_load(rP, lo, so);
-        c2[k] = buf_load(rN, lo, so);
+        for (int k = 0; k < K; ++k) {
+          const uint32_t so = (uint32_t)__builtin_amdgcn_readlane((int)sol, k);
+          c1[k] = buf_load(rP, lo, so);
+          c2[k] = buf_load(rN, lo, so);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const uint32_t idx = S[(size_t)K * g + k] - 1u;
+          const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
+          const uint32_t so = 4u * (uint32_t)((ti * D + a) + (int)P * (tj * D + b));
+          c1[k] = buf_load(rP, lo, so);
+          c2[k] = buf_load(rN, lo, so);
+        }
       }
     } else {
 #pragma unroll
