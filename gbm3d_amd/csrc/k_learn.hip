@@ -93,11 +93,20 @@ __device__ __forceinline__ void load_group(const float* __restrict__ src, const 
     const int Mt = M / 8;
     const size_t P = (size_t)M + d2;
     const int ti = (int)(g % Mt), tj = (int)(g / Mt);
+    // lane k < POS: slot k's block origin (its window index divided by d2
+    // once, all slots in parallel), then each slot's origin from lane k
+    size_t org = 0;
+    if (lane < POS) {
+      const uint32_t idx = S[(size_t)POS * g + lane] - 1u;
+      const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
+      org = (size_t)(ti * 8 + a) + P * (size_t)(tj * 8 + b);
+    }
+    const size_t off = (size_t)u + P * (size_t)v;
 #pragma unroll
     for (int k = 0; k < POS; ++k) {
-      const uint32_t idx = S[(size_t)POS * g + k] - 1u;
-      const int a = (int)(idx % (uint32_t)d2), b = (int)(idx / (uint32_t)d2);
-      x[k] = src[(size_t)(ti * 8 + a + u) + P * (size_t)(tj * 8 + b + v)];
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)org, k);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(org >> 32), k);
+      x[k] = src[(((size_t)hi << 32) | lo) + off];
     }
   } else {
 #pragma unroll
