@@ -329,22 +329,26 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
     if (nh == 0) continue;
     if (hit) {
       const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-      hl[wave][rank] = make_int4(blk, (int)brc, __float_as_int(wv), bob);
+      // BUF: the block's byte offset in the blocks array (below 2^31)
+      hl[wave][rank] = make_int4(BUF ? (int)(4u * (uint32_t)DD * (uint32_t)blk) : blk, (int)brc, __float_as_int(wv), bob);
     }
     wave_sync();
     const int hblk = hl[wave][lane].x;
     for (int j0 = 0; j0 < nh; j0 += NB) {
-      // all NB loads issued (past the last hit: the first hit's block again),
-      // so each hit waits only for its own block
+      // all NB loads issued (past the last hit: some block, not used), so
+      // each hit waits only for its own block
       float val[NB][PER];
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
-        const int b = __builtin_amdgcn_readlane(hblk, j0 + j < nh ? j0 + j : 0);
         if constexpr (BUF) {
+          // past the last hit: an entry of an earlier round or never written;
+          // through the range-checked resource any offset reads safely
+          const uint32_t bo = (uint32_t)__builtin_amdgcn_readlane(hblk, (j0 + j) & 63);
 #pragma unroll
           for (int i = 0; i < PER; ++i)
-            val[j][i] = (lane + 64 * i < DD) ? buf_load(rB, 4u * (uint32_t)(lane + 64 * i), 4u * (uint32_t)(DD * b)) : 0.f;
+            val[j][i] = (lane + 64 * i < DD) ? buf_load(rB, 4u * (uint32_t)(lane + 64 * i), bo) : 0.f;
         } else {
+          const int b = __builtin_amdgcn_readlane(hblk, j0 + j < nh ? j0 + j : 0);
           const float* bp = blocks + (size_t)DD * b;
 #pragma unroll
           for (int i = 0; i < PER; ++i) val[j][i] = (lane + 64 * i < DD) ? bp[lane + 64 * i] : 0.f;
