@@ -681,7 +681,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3)
   __builtin_assume(tid < 256);
   const int nbx = gridDim.x, nby = gridDim.y;
   const int lb = xcd_remap(blockIdx.x + nbx * blockIdx.y, nbx * nby);
-  const int x0 = (lb % nbx) * TX, y0 = (lb / nbx) * TY;
+  int bx, by;
+  tile_order8(lb, nbx, nby, bx, by);
+  const int x0 = bx * TX, y0 = by * TY;
   const int ph = p >> 1, qh = q >> 1, rh = r >> 1;
   if (tid < WY) {
     cidxp[tid] = p * pmod_i(y0 - 9 + tid, q);

@@ -21,6 +21,22 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// Tile (bx, by) of an nbx x nby tile grid for logical index lb, in 8 x 8
+// super-tiles (bx fastest inside one, super-tiles down the bx axis first) when
+// both counts are multiples of 8, else bx fastest.  The 64 workgroups an XCD
+// runs at once then cover a square-ish region whose halos they share in its
+// L2, instead of two full tile columns.  A bijection on [0, nbx nby).
+__device__ __forceinline__ void tile_order8(int lb, int nbx, int nby, int& bx, int& by) {
+  if ((nbx & 7) == 0 && (nby & 7) == 0) {
+    const int st = lb >> 6, w = lb & 63, sx = nbx >> 3;
+    bx = (st % sx) * 8 + (w & 7);
+    by = (st / sx) * 8 + (w >> 3);
+  } else {
+    bx = lb % nbx;
+    by = lb / nbx;
+  }
+}
+
 // Make LDS writes of this wave visible to its other lanes (a wave executes in
 // lockstep; the fences stop the compiler from moving LDS accesses across).
 __device__ __forceinline__ void wave_sync() {
