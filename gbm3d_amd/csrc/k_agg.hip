@@ -327,10 +327,14 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
     const uint64_t mask = __ballot(hit);
     const int nh = __popcll(mask);
     if (nh == 0) continue;
-    if (hit) {
+    {
+      // BUF: the block's byte offset in the blocks array (below 2^31).  The
+      // other lanes fill entries nh..63 with the first hit's block, which a
+      // batch's loads past the last hit then read again from the cache.
+      const int bx = BUF ? (int)(4u * (uint32_t)DD * (uint32_t)blk) : blk;
+      const int b0 = __builtin_amdgcn_readlane(bx, __builtin_ctzll(mask));
       const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-      // BUF: the block's byte offset in the blocks array (below 2^31)
-      hl[wave][rank] = make_int4(BUF ? (int)(4u * (uint32_t)DD * (uint32_t)blk) : blk, (int)brc, __float_as_int(wv), bob);
+      hl[wave][hit ? rank : nh + (lane - rank)] = make_int4(hit ? bx : b0, (int)brc, __float_as_int(wv), bob);
     }
     wave_sync();
     const int hblk = hl[wave][lane].x;
@@ -341,14 +345,14 @@ __global__ __launch_bounds__(256) void aggregate_batched_kernel(const float* __r
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         if constexpr (BUF) {
-          // past the last hit: an entry of an earlier round or never written;
-          // through the range-checked resource any offset reads safely
+          // past the last hit: the first hit's block (entries nh..63) or,
+          // wrapped, a hit of this round
           const uint32_t bo = (uint32_t)__builtin_amdgcn_readlane(hblk, (j0 + j) & 63);
 #pragma unroll
           for (int i = 0; i < PER; ++i)
             val[j][i] = (lane + 64 * i < DD) ? buf_load(rB, 4u * (uint32_t)(lane + 64 * i), bo) : 0.f;
         } else {
-          const int b = __builtin_amdgcn_readlane(hblk, j0 + j < nh ? j0 + j : 0);
+          const int b = __builtin_amdgcn_readlane(hblk, (j0 + j) & 63);
           const float* bp = blocks + (size_t)DD * b;
 #pragma unroll
           for (int i = 0; i < PER; ++i) val[j][i] = (lane + 64 * i < DD) ? bp[lane + 64 * i] : 0.f;
