@@ -807,6 +807,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3)
       if (down) ws1[i] += fabsf(f);
     }
   };
+  // 16-byte stores: with p a multiple of 4 the four rows of a lane quad are
+  // all inside the volume or all outside, never straddle a group, and one
+  // column's four are contiguous and 16-byte aligned in every layout once Y is
+  const bool st16 = (p & 3) == 0 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0;
+  // the value of lane ql ^ 1 (quad_perm [1,0,3,2]) / ql ^ 2 ([2,3,0,1]) of the quad
+  auto xdpp = [](float x, int b) {
+    return __int_as_float(b ? __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, true)
+                            : __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, true));
+  };
   auto store_plane = [&](int z, const float (&u)[8], const float (&v)[8]) {
     if (!rowC) return;
     float val[8];
@@ -817,13 +826,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP ? 2 : 3)
       const float mean = u[i] + v[i];
       val[i] = mean > 0.f ? mean : 0.f;
     }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      if (pc0 + i >= q) continue;
-      const int so = (i & cm) * cst + (i >> csh) * gst + z * zstride;
-      buf_store(ys, 4u * (uint32_t)ob0, val[i], 4u * (uint32_t)so);
-    }
     if constexpr (WG) weigh_plane(z, val);
+    if (st16) {
+      // the quad's 4 x 8 values transposed by column pairs (two butterfly
+      // exchanges within the quad), so that lane ql holds columns 2 ql and
+      // 2 ql + 1 of the quad's 4 rows: t[2 r + h] = row 4k + r, column 2 ql + h
+      const int ql = xo & 3;
+      float t[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) t[i] = val[i];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const bool hi = (ql >> b) & 1;
+#pragma unroll
+        for (int c0 = 0; c0 < 4; ++c0) {
+          if ((c0 >> b) & 1) continue;
+          const int c1 = c0 | (1 << b);
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const float a0 = t[2 * c0 + h], a1 = t[2 * c1 + h];
+            const float from1 = xdpp(a1, b), from0 = xdpp(a0, b);
+            t[2 * c0 + h] = hi ? from1 : a0;
+            t[2 * c1 + h] = hi ? a1 : from0;
+          }
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = 2 * ql + h;
+        if (pc0 + i >= q) continue;
+        const int so = (i & cm) * cst + (i >> csh) * gst + z * zstride;
+        typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
+        const u32x4 w = {__float_as_uint(t[h]), __float_as_uint(t[2 + h]), __float_as_uint(t[4 + h]),
+                         __float_as_uint(t[6 + h])};
+        __builtin_amdgcn_raw_buffer_store_b128(w, ys, 4u * (uint32_t)(ob0 - ql + so), 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        if (pc0 + i >= q) continue;
+        const int so = (i & cm) * cst + (i >> csh) * gst + z * zstride;
+        buf_store(ys, 4u * (uint32_t)ob0, val[i], 4u * (uint32_t)so);
+      }
+    }
   };
   // ---- coarse (rebuilt level-1) approximation of anchor az, staged through registers
   float yv[NYT];

@@ -112,7 +112,8 @@ def test_ht_filter_vs_fp64_oracle(ctx, m, n, cs, seed):
 
 
 @pytest.mark.parametrize("m,n,K,levels,seed", [(64, 64, 16, 3, 51), (96, 160, 16, 3, 52), (48, 80, 8, 2, 53),
-                                                (16, 16, 2, 1, 54), (256, 200, 16, 3, 55), (18, 22, 2, 1, 56)])
+                                                (16, 16, 2, 1, 54), (256, 200, 16, 3, 55), (18, 22, 2, 1, 56),
+                                                (20, 28, 2, 1, 58)])
 def test_ht_fused_cycle_spins_equal_per_spin_kernels(ctx, monkeypatch, m, n, K, levels, seed):
     """The finest level of both cycle spins in one pass (anchors of parity s in
     original coordinates) is the per-spin computation with its operations in
@@ -120,7 +121,8 @@ def test_ht_fused_cycle_spins_equal_per_spin_kernels(ctx, monkeypatch, m, n, K, 
     group-contiguous layout the aggregation reads, with the step-1 weights the
     fused kernel writes at 8x8 blocks (the per-spin path runs the weights
     kernel); end to end on 136x104 images too, whose last output tiles are
-    partial (64x32 tiles)."""
+    partial (64x32 tiles).  18x22 has row quads that straddle the volume's
+    edge (32-bit stores), 20x28 a last run of 4 columns (16-byte stores)."""
     rng = np.random.default_rng(seed)
     V = (rng.standard_normal((m, n, K)) * 40 + 100).astype(np.float32)
     tau = O.get_tau(levels, 1, 25.0)
