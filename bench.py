@@ -574,12 +574,14 @@ def main():
         return
     ro = gbm3d_amd.resolve_opts(OPTS)
 
-    def staged(reps=5):
+    def staged(reps=8, warm=8):
         """Steps with the stage timers on (same work, outside the timed region):
-        one to warm the timers, then per-stage ms per step and launches per
-        step averaged over `reps` steps."""
+        `warm` steps first (the serial schedule's kernels take ~8 steps to
+        settle: the finest-level kernel runs 15 % slower in the first), then
+        per-stage ms per step and launches per step averaged over `reps` steps."""
         ctx.set_timing(True)
-        step()
+        for _ in range(warm):
+            step()
         tot_ms, tot_n = {}, {}
         for _ in range(reps):
             step()
